@@ -1,0 +1,175 @@
+/*
+ * rtiling.h -- C ABI of the MI355X random-tiling engine (librtiling.so).
+ *
+ * Plain pointers and sizes only.  Every call returns an int status
+ * (RT_OK = 0, negative on error) and never throws; rt_last_error() gives
+ * the message of the last failing call on a context.  Calls on one context
+ * are synchronous and must come from one host thread at a time.
+ *
+ * Tilings are int32 N*N row-major arrays, exactly the reference's `tiling`
+ * (std::vector<int>, src/common/common.h:50-52).  Walk calls mutate the
+ * tiling in place and return after the device work has finished, like the
+ * reference's blocking cl::copy (src/Domino/DominoTiler.cpp:100-114).
+ */
+#ifndef RTILING_H
+#define RTILING_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+#define RT_OK 0
+#define RT_EINVAL (-1)      /* bad argument: size, shape, null pointer */
+#define RT_EPARAMS (-2)     /* TinyMT parameter table missing, short, malformed */
+#define RT_EIO (-3)         /* file cannot be opened */
+#define RT_ENOCOUPLE (-4)   /* CFTP did not coalesce within max_rounds */
+#define RT_EDEVICE (-5)     /* HIP runtime error */
+#define RT_ENOMEM (-6)      /* device or host allocation failed */
+#define RT_EUNTILEABLE (-7) /* Thurston found the region not tileable */
+#define RT_EADMIT (-8)      /* serving: request exceeds the HBM budget */
+#define RT_ECANCELLED (-9)  /* serving: request cancelled */
+#define RT_EAGAIN (-10)     /* serving: result not ready yet */
+
+typedef struct rt_ctx rt_ctx;
+
+/* Context: one HIP device, one stream, the TinyMT parameter table and the
+ * device arenas.  Replaces the reference's (cl::Context, cl::CommandQueue,
+ * devices, kernel source) constructor arguments (DominoTiler.h:70-76,
+ * LozengeTiler.h:55-61, RectTriangleTiler.h:54-61). */
+int rt_create(rt_ctx** out, int device_ordinal);
+void rt_destroy(rt_ctx* ctx);
+const char* rt_last_error(const rt_ctx* ctx);
+/* Name of the device the context runs on (e.g. "gfx950"). */
+const char* rt_device_name(const rt_ctx* ctx);
+
+/* TinyMT parameter table: `count` (mat1, mat2, tmat) triples, interleaved.
+ * Replaces LoadTinyMT -> get_params_buffer (src/TinyMT/file_reader.cpp:22-38). */
+int rt_set_tinymt_params(rt_ctx* ctx, const uint32_t* mat1_mat2_tmat,
+                         size_t count);
+/* Parse `count` parameter lines in the tinymt32dc text format (hex fields
+ * after the 3rd/4th/5th comma, '#' lines skipped) and load them.
+ * Mirrors DominoTiler::LoadTinyMT(path, size) (DominoTiler.cpp:235-237,
+ * file_reader.cpp:51-70,139-166). */
+int rt_load_tinymt_file(rt_ctx* ctx, const char* path, size_t count);
+/* Parse only (host), into out3[3*count]. */
+int rt_parse_tinymt_file(const char* path, size_t count, uint32_t* out3);
+
+/* Device time (ms, HIP events on the context's stream) and kernel launch
+ * count of the last walk / CFTP / batch call on this context. */
+typedef struct {
+  double device_ms;
+  long launches;
+  long steps;         /* colour steps run, per chain */
+  long attempts;      /* flip attempts (TinyMT draws) */
+} rt_timing;
+int rt_last_timing(const rt_ctx* ctx, rt_timing* out);
+
+/* Kernel selection for walks: 0 = auto (fused LDS sweep blocks where the
+ * shape allows), 1 = one launch per colour step, 2 = fused only. */
+int rt_set_engine(rt_ctx* ctx, int engine);
+
+/* ---------------------------------------------------------------- domino */
+/* Flip rules: the four reference kernel files. */
+#define RT_DOMINO_TOGGLE08 0    /* dominokernel.cl:4-18 */
+#define RT_DOMINO_HEATBATH 1    /* dominokernelCFTP.cl:11-31 */
+#define RT_DOMINO_METRO2P 2     /* dominokerneltwoperiodic.cl:7-35 */
+#define RT_DOMINO_HEATBATH2P 3  /* dominokernelCFTPtwoperiodic.cl:15-44 */
+
+typedef struct {
+  int kind;   /* RT_DOMINO_* */
+  double a;   /* two-periodic weights (reference: #define a 0.7, b 1.) */
+  double b;
+} rt_domino_rule;
+
+/* DominoTiler::Walk(tiling&, long steps, long seed) (DominoTiler.cpp:18-122,
+ * DominoTiler.h:89). Needs >= N*N/2 TinyMT parameter sets. */
+int rt_domino_walk(rt_ctx* ctx, int32_t* tiling, int N, long steps,
+                   long seed, const rt_domino_rule* rule);
+/* DominoTiler::Walk(tiling&, vector<long> steps, vector<long> seeds)
+ * (DominoTiler.cpp:125-231, DominoTiler.h:92). */
+int rt_domino_walk_segments(rt_ctx* ctx, int32_t* tiling, int N,
+                            const long* steps, const long* seeds, int nseg,
+                            const rt_domino_rule* rule);
+
+/* CFTP (Examples/AztecDiamondCftpEx.cpp:51-77): master mt19937 -> segment
+ * seeds; top from tmax, bottom from tmin; both chains share every TinyMT
+ * draw on the device. */
+#define RT_SCHED_REFERENCE 0  /* prepend 2*t0 per round (AztecDiamondCftpEx.cpp:76) */
+#define RT_SCHED_DOUBLING 1   /* prepend the current total: look-back doubles */
+
+typedef struct {
+  int rounds;        /* rounds run until coalescence */
+  long total_steps;  /* look-back of the coalesced round (sum of segments) */
+  long steps_run;    /* colour steps run over all rounds, per chain */
+  double device_ms;  /* device time over all rounds */
+} rt_cftp_stats;
+
+int rt_domino_cftp(rt_ctx* ctx, const int32_t* tmax, const int32_t* tmin,
+                   int N, uint32_t master_seed, int sched, long t0,
+                   int max_rounds, const rt_domino_rule* rule, int32_t* out,
+                   rt_cftp_stats* stats);
+
+/* --------------------------------------------------------------- lozenge */
+/* LozengeTiler::Walk(tiling&, int steps, long seed) (LozengeTiler.cpp:27-159).
+ * Needs >= N*(N/3) parameter sets. */
+int rt_lozenge_walk(rt_ctx* ctx, int32_t* tiling, int N, long steps,
+                    long seed);
+/* Segmented lozenge walk: per segment, TinyMT and the colour engine are
+ * both seeded with seeds[k] (LozengeTiler.cpp:69,86-88).  The reference
+ * has it for domino only (DominoTiler.cpp:125-231). */
+int rt_lozenge_walk_segments(rt_ctx* ctx, int32_t* tiling, int N,
+                             const long* steps, const long* seeds, int nseg);
+int rt_lozenge_cftp(rt_ctx* ctx, const int32_t* tmax, const int32_t* tmin,
+                    int N, uint32_t master_seed, int sched, long t0,
+                    int max_rounds, int32_t* out, rt_cftp_stats* stats);
+
+/* ---------------------------------------------------- rectangle-triangle */
+/* RectTriangleTiler::Walk (RectTriangleTiler.cpp:33-171); weights16 is the
+ * kernel's weight table (recttrianglekernel.cl:3-13; NULL = all 1.0 with
+ * weights[0] = 0). Needs >= N*N/3 parameter sets. */
+int rt_recttri_walk(rt_ctx* ctx, int32_t* tiling, int N, long steps,
+                    long seed, const float* weights16);
+
+/* ---------------------------------------------- host-side domain helpers */
+/* An owned int32 vector returned by the helpers below. */
+typedef struct rt_vec rt_vec;
+const int32_t* rt_vec_data(const rt_vec* v);
+size_t rt_vec_size(const rt_vec* v);
+void rt_vec_free(rt_vec* v);
+
+/* DominoTiler statics (DominoTiler.h:99-123). */
+rt_vec* rt_domino_rectangle(int M, int N);               /* DominoTiler.cpp:249-260 */
+rt_vec* rt_domino_aztec_diamond(int n);                  /* DominoTiler.cpp:262-265 */
+rt_vec* rt_domino_aztec_rectangle(int n, const int* remove, int k); /* 267-324 */
+int rt_domino_max_tiling(const int32_t* d, size_t dsize, rt_vec** out);   /* 345-438 */
+int rt_domino_min_tiling(const int32_t* d, size_t dsize, rt_vec** out);   /* 440-531 */
+rt_vec* rt_domino_heightfunc_to_tiling(const int32_t* hf, size_t n,
+                                       const int32_t* d, size_t dsize);   /* 534-553 */
+rt_vec* rt_domino_tiling_to_heightfunc(const int32_t* t, size_t n,
+                                       const int32_t* d, size_t dsize);   /* 555-593 */
+rt_vec* rt_domino_tiling_to_domain(const int32_t* t, size_t n);           /* 595-609 */
+rt_vec* rt_domino_domain_to_vertices(const int32_t* d, size_t dsize);     /* 611-631 */
+
+/* LozengeTiler statics (LozengeTiler.h:77-92). */
+rt_vec* rt_lozenge_almost_hex_domain(int n, int m);      /* LozengeTiler.cpp:422-467 */
+int rt_lozenge_max_tiling(const int32_t* d, size_t dsize, rt_vec** out);  /* 308-396 */
+int rt_lozenge_min_tiling(const int32_t* d, size_t dsize, rt_vec** out);  /* 218-306 */
+rt_vec* rt_lozenge_heightfunc_to_tiling(const int32_t* hf, size_t n,
+                                        const int32_t* d, size_t dsize);  /* 399-420 */
+rt_vec* rt_lozenge_tiling_to_heightfunc(const int32_t* t, size_t n,
+                                        const int32_t* d, size_t dsize);  /* 173-216 */
+rt_vec* rt_lozenge_tiling_to_domain(const int32_t* t, size_t n);          /* 514-540 */
+rt_vec* rt_lozenge_domain_to_vertices(const int32_t* d, size_t dsize);    /* 483-512 */
+
+/* RectTriangleTiler statics (RectTriangleTiler.h:77-82). */
+rt_vec* rt_recttri_max_hex(int n);                       /* RectTriangleTiler.cpp:480-538 */
+rt_vec* rt_recttri_min_hex(int n);                       /* 540-600 */
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,80 @@
+// ref_probe.cpp -- extern "C" entry points over the reference's own host
+// statics, compiled together with the reference's sources into
+// oracle/_ref/libref_host.so (see oracle/Makefile).  Test infrastructure:
+// only tests/golden scripts and CPU tests load it, to pin the engine's
+// host helpers against the reference implementation itself.
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "Domino/DominoTiler.h"
+#include "Lozenge/LozengeTiler.h"
+#include "RectTriangle/RectTriangleTiler.h"
+#include "TriangleDimer/TriangleDimerTiler.h"
+
+namespace {
+// Results are returned through a caller buffer; the call returns the
+// element count (or, when cap is too small, the negated required count).
+long give(const std::vector<int>& v, int32_t* out, long cap) {
+  if ((long)v.size() > cap) return -(long)v.size();
+  for (size_t k = 0; k < v.size(); k++) out[k] = v[k];
+  return (long)v.size();
+}
+std::vector<int> vec(const int32_t* p, long n) { return std::vector<int>(p, p + n); }
+}  // namespace
+
+extern "C" {
+long ref_domino_aztec_diamond(int n, int32_t* out, long cap) {
+  return give(DominoTiler::AztecDiamond(n), out, cap);
+}
+long ref_domino_aztec_rectangle(int n, const int* rem, int k, int32_t* out, long cap) {
+  return give(DominoTiler::AztecRectangle(n, std::vector<int>(rem, rem + k)), out, cap);
+}
+long ref_domino_rectangle(int m, int n, int32_t* out, long cap) {
+  return give(DominoTiler::Rectangle(m, n), out, cap);
+}
+long ref_domino_max_tiling(const int32_t* d, long nd, int32_t* out, long cap) {
+  return give(DominoTiler::MaxTiling(vec(d, nd)), out, cap);
+}
+long ref_domino_min_tiling(const int32_t* d, long nd, int32_t* out, long cap) {
+  return give(DominoTiler::MinTiling(vec(d, nd)), out, cap);
+}
+long ref_domino_tiling_to_heightfunc(const int32_t* t, long nt, const int32_t* d, long nd,
+                                     int32_t* out, long cap) {
+  return give(DominoTiler::TilingToHeightfunc(vec(t, nt), vec(d, nd)), out, cap);
+}
+long ref_domino_tiling_to_domain(const int32_t* t, long nt, int32_t* out, long cap) {
+  return give(DominoTiler::TilingToDomain(vec(t, nt)), out, cap);
+}
+long ref_domino_domain_to_vertices(const int32_t* d, long nd, int32_t* out, long cap) {
+  return give(DominoTiler::DomainToVertices(vec(d, nd)), out, cap);
+}
+long ref_lozenge_almost_hex_domain(int n, int m, int32_t* out, long cap) {
+  return give(LozengeTiler::AlmostHexDomain(n, m), out, cap);
+}
+long ref_lozenge_max_tiling(const int32_t* d, long nd, int32_t* out, long cap) {
+  return give(LozengeTiler::MaxTiling(vec(d, nd)), out, cap);
+}
+long ref_lozenge_min_tiling(const int32_t* d, long nd, int32_t* out, long cap) {
+  return give(LozengeTiler::MinTiling(vec(d, nd)), out, cap);
+}
+long ref_lozenge_tiling_to_heightfunc(const int32_t* t, long nt, const int32_t* d, long nd,
+                                      int32_t* out, long cap) {
+  return give(LozengeTiler::TilingToHeightfunc(vec(t, nt), vec(d, nd)), out, cap);
+}
+long ref_lozenge_tiling_to_domain(const int32_t* t, long nt, int32_t* out, long cap) {
+  return give(LozengeTiler::TilingToDomain(vec(t, nt)), out, cap);
+}
+long ref_recttri_max_hex(int n, int32_t* out, long cap) {
+  return give(RectTriangleTiler::maxHex(n), out, cap);
+}
+long ref_recttri_min_hex(int n, int32_t* out, long cap) {
+  return give(RectTriangleTiler::minHex(n), out, cap);
+}
+long ref_recttri_slope_hex(int n, int32_t* out, long cap) {
+  return give(RectTriangleTiler::slopeHex(n), out, cap);
+}
+long ref_bibone_ice_cream_cone(int n, int m, int32_t* out, long cap) {
+  return give(TriangleDimerTiler::IceCreamCone(n, m), out, cap);
+}
+}

@@ -1,0 +1,385 @@
+"""MI355X random-tiling MCMC engine: Python view of librtiling.so.
+
+The engine is C++/HIP (randomtilingspublic_amd/csrc) behind the C ABI in
+include/rtiling.h; this module binds that ABI with ctypes so tests and the
+benchmark can drive it.  It never computes a tiling itself and has no CPU
+fallback: if librtiling.so is missing, every call raises.
+
+Names follow the reference's per-model classes (DominoTiler, LozengeTiler,
+RectTriangleTiler): Walk, LoadTinyMT, AztecDiamond, MinTiling, ...
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librtiling.so")
+
+RT_OK = 0
+ERRORS = {
+    -1: "EINVAL", -2: "EPARAMS", -3: "EIO", -4: "ENOCOUPLE", -5: "EDEVICE",
+    -6: "ENOMEM", -7: "EUNTILEABLE", -8: "EADMIT", -9: "ECANCELLED", -10: "EAGAIN",
+}
+
+DOMINO_TOGGLE08, DOMINO_HEATBATH, DOMINO_METRO2P, DOMINO_HEATBATH2P = 0, 1, 2, 3
+SCHED_REFERENCE, SCHED_DOUBLING = 0, 1
+ENGINE_AUTO, ENGINE_STEP, ENGINE_FUSED = 0, 1, 2
+
+
+class RTError(RuntimeError):
+    def __init__(self, code, msg=""):
+        self.code = code
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+
+
+class _Timing(C.Structure):
+    _fields_ = [("device_ms", C.c_double), ("launches", C.c_long),
+                ("steps", C.c_long), ("attempts", C.c_long)]
+
+
+class DominoRule(C.Structure):
+    _fields_ = [("kind", C.c_int), ("a", C.c_double), ("b", C.c_double)]
+
+
+class CftpStats(C.Structure):
+    _fields_ = [("rounds", C.c_int), ("total_steps", C.c_long),
+                ("steps_run", C.c_long), ("device_ms", C.c_double)]
+
+
+_lib = None
+_I32P = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_U32P = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_LONGP = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+
+
+def lib():
+    """Load librtiling.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run python -m randomtilingspublic_amd.build")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    L.rt_create.argtypes = [C.POINTER(vp), C.c_int]
+    L.rt_destroy.argtypes = [vp]
+    L.rt_destroy.restype = None
+    L.rt_last_error.argtypes = [vp]
+    L.rt_last_error.restype = C.c_char_p
+    L.rt_device_name.argtypes = [vp]
+    L.rt_device_name.restype = C.c_char_p
+    L.rt_set_tinymt_params.argtypes = [vp, _U32P, C.c_size_t]
+    L.rt_load_tinymt_file.argtypes = [vp, C.c_char_p, C.c_size_t]
+    L.rt_parse_tinymt_file.argtypes = [C.c_char_p, C.c_size_t, _U32P]
+    L.rt_last_timing.argtypes = [vp, C.POINTER(_Timing)]
+    L.rt_set_engine.argtypes = [vp, C.c_int]
+    L.rt_domino_walk.argtypes = [vp, _I32P, C.c_int, C.c_long, C.c_long, C.POINTER(DominoRule)]
+    L.rt_domino_walk_segments.argtypes = [vp, _I32P, C.c_int, _LONGP, _LONGP, C.c_int,
+                                          C.POINTER(DominoRule)]
+    L.rt_domino_cftp.argtypes = [vp, _I32P, _I32P, C.c_int, C.c_uint32, C.c_int, C.c_long,
+                                 C.c_int, C.POINTER(DominoRule), _I32P, C.POINTER(CftpStats)]
+    L.rt_lozenge_walk.argtypes = [vp, _I32P, C.c_int, C.c_long, C.c_long]
+    L.rt_lozenge_walk_segments.argtypes = [vp, _I32P, C.c_int, _LONGP, _LONGP, C.c_int]
+    L.rt_lozenge_cftp.argtypes = [vp, _I32P, _I32P, C.c_int, C.c_uint32, C.c_int, C.c_long,
+                                  C.c_int, _I32P, C.POINTER(CftpStats)]
+    L.rt_recttri_walk.argtypes = [vp, _I32P, C.c_int, C.c_long, C.c_long, C.c_void_p]
+    # host helpers returning rt_vec*
+    L.rt_vec_data.argtypes = [vp]
+    L.rt_vec_data.restype = C.POINTER(C.c_int32)
+    L.rt_vec_size.argtypes = [vp]
+    L.rt_vec_size.restype = C.c_size_t
+    L.rt_vec_free.argtypes = [vp]
+    L.rt_vec_free.restype = None
+    for name, args in [
+        ("rt_domino_rectangle", [C.c_int, C.c_int]),
+        ("rt_domino_aztec_diamond", [C.c_int]),
+        ("rt_domino_aztec_rectangle", [C.c_int, C.c_void_p, C.c_int]),
+        ("rt_domino_heightfunc_to_tiling", [_I32P, C.c_size_t, _I32P, C.c_size_t]),
+        ("rt_domino_tiling_to_heightfunc", [_I32P, C.c_size_t, _I32P, C.c_size_t]),
+        ("rt_domino_tiling_to_domain", [_I32P, C.c_size_t]),
+        ("rt_domino_domain_to_vertices", [_I32P, C.c_size_t]),
+        ("rt_lozenge_almost_hex_domain", [C.c_int, C.c_int]),
+        ("rt_lozenge_heightfunc_to_tiling", [_I32P, C.c_size_t, _I32P, C.c_size_t]),
+        ("rt_lozenge_tiling_to_heightfunc", [_I32P, C.c_size_t, _I32P, C.c_size_t]),
+        ("rt_lozenge_tiling_to_domain", [_I32P, C.c_size_t]),
+        ("rt_lozenge_domain_to_vertices", [_I32P, C.c_size_t]),
+        ("rt_recttri_max_hex", [C.c_int]),
+        ("rt_recttri_min_hex", [C.c_int]),
+    ]:
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = vp
+    for name in ["rt_domino_max_tiling", "rt_domino_min_tiling",
+                 "rt_lozenge_max_tiling", "rt_lozenge_min_tiling"]:
+        getattr(L, name).argtypes = [_I32P, C.c_size_t, C.POINTER(vp)]
+    _lib = L
+    return L
+
+
+def _take_vec(ptr):
+    if not ptr:
+        raise RTError(-1, "helper rejected its arguments")
+    L = lib()
+    n = L.rt_vec_size(ptr)
+    out = np.ctypeslib.as_array(L.rt_vec_data(ptr), shape=(n,)).copy() if n else np.zeros(0, np.int32)
+    L.rt_vec_free(ptr)
+    return out.astype(np.int32)
+
+
+def _i32(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.int32).ravel())
+
+
+def side(a):
+    n = int(round(np.sqrt(len(a))))
+    if n * n != len(a):
+        raise ValueError("array is not square")
+    return n
+
+
+class Context:
+    """One device, one stream, one TinyMT parameter table (rt_ctx)."""
+
+    def __init__(self, device=0):
+        L = lib()
+        h = C.c_void_p()
+        rc = L.rt_create(C.byref(h), device)
+        if rc != RT_OK:
+            raise RTError(rc, "rt_create failed (no HIP device?)")
+        self._h = h
+        self._L = L
+
+    def close(self):
+        if self._h:
+            self._L.rt_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc != RT_OK:
+            raise RTError(rc, self._L.rt_last_error(self._h).decode())
+
+    @property
+    def device_name(self):
+        return self._L.rt_device_name(self._h).decode()
+
+    def set_engine(self, engine):
+        self._chk(self._L.rt_set_engine(self._h, engine))
+
+    def set_tinymt_params(self, params3):
+        p = np.ascontiguousarray(np.asarray(params3, dtype=np.uint32).reshape(-1, 3).ravel())
+        self._chk(self._L.rt_set_tinymt_params(self._h, p, len(p) // 3))
+
+    def load_tinymt_file(self, path, count):
+        self._chk(self._L.rt_load_tinymt_file(self._h, str(path).encode(), count))
+
+    def last_timing(self):
+        t = _Timing()
+        self._chk(self._L.rt_last_timing(self._h, C.byref(t)))
+        return {"device_ms": t.device_ms, "launches": t.launches, "steps": t.steps,
+                "attempts": t.attempts}
+
+    # ---- domino ----
+    def domino_walk(self, t, steps, seed, rule=DOMINO_TOGGLE08, a=1.0, b=1.0):
+        t = _i32(t).copy()
+        r = DominoRule(rule, a, b)
+        self._chk(self._L.rt_domino_walk(self._h, t, side(t), steps, seed, C.byref(r)))
+        return t
+
+    def domino_walk_segments(self, t, steps, seeds, rule=DOMINO_TOGGLE08, a=1.0, b=1.0):
+        t = _i32(t).copy()
+        st = np.ascontiguousarray(steps, dtype=np.int64)
+        se = np.ascontiguousarray(seeds, dtype=np.int64)
+        r = DominoRule(rule, a, b)
+        self._chk(self._L.rt_domino_walk_segments(self._h, t, side(t), st, se, len(st),
+                                                  C.byref(r)))
+        return t
+
+    def domino_cftp(self, tmax, tmin, master_seed, sched=SCHED_REFERENCE, t0=128,
+                    max_rounds=40, rule=DOMINO_HEATBATH, a=1.0, b=1.0):
+        tmax, tmin = _i32(tmax), _i32(tmin)
+        out = np.zeros_like(tmax)
+        st = CftpStats()
+        r = DominoRule(rule, a, b)
+        self._chk(self._L.rt_domino_cftp(self._h, tmax, tmin, side(tmax), master_seed, sched,
+                                         t0, max_rounds, C.byref(r), out, C.byref(st)))
+        return out, {"rounds": st.rounds, "total_steps": st.total_steps,
+                     "steps_run": st.steps_run, "device_ms": st.device_ms}
+
+    # ---- lozenge ----
+    def lozenge_walk(self, t, steps, seed):
+        t = _i32(t).copy()
+        self._chk(self._L.rt_lozenge_walk(self._h, t, side(t), steps, seed))
+        return t
+
+    def lozenge_walk_segments(self, t, steps, seeds):
+        t = _i32(t).copy()
+        st = np.ascontiguousarray(steps, dtype=np.int64)
+        se = np.ascontiguousarray(seeds, dtype=np.int64)
+        self._chk(self._L.rt_lozenge_walk_segments(self._h, t, side(t), st, se, len(st)))
+        return t
+
+    def lozenge_cftp(self, tmax, tmin, master_seed, sched=SCHED_REFERENCE, t0=128,
+                     max_rounds=40):
+        tmax, tmin = _i32(tmax), _i32(tmin)
+        out = np.zeros_like(tmax)
+        st = CftpStats()
+        self._chk(self._L.rt_lozenge_cftp(self._h, tmax, tmin, side(tmax), master_seed, sched,
+                                          t0, max_rounds, out, C.byref(st)))
+        return out, {"rounds": st.rounds, "total_steps": st.total_steps,
+                     "steps_run": st.steps_run, "device_ms": st.device_ms}
+
+    # ---- rectangle-triangle ----
+    def recttri_walk(self, t, steps, seed, weights16=None):
+        t = _i32(t).copy()
+        w = None
+        if weights16 is not None:
+            warr = np.ascontiguousarray(weights16, dtype=np.float32)
+            assert warr.size == 16
+            w = warr.ctypes.data_as(C.c_void_p)
+        self._chk(self._L.rt_recttri_walk(self._h, t, side(t), steps, seed, w))
+        return t
+
+
+def parse_tinymt_file(path, count):
+    out = np.zeros(3 * count, dtype=np.uint32)
+    rc = lib().rt_parse_tinymt_file(str(path).encode(), count, out)
+    if rc != RT_OK:
+        raise RTError(rc, f"cannot parse {path}")
+    return out.reshape(count, 3)
+
+
+# ---------------------------------------------------------------- statics
+def _thurston(fn, d):
+    d = _i32(d)
+    h = C.c_void_p()
+    rc = getattr(lib(), fn)(d, len(d), C.byref(h))
+    if rc != RT_OK:
+        raise RTError(rc, fn)
+    return _take_vec(h.value)
+
+
+class DominoTiler:
+    """Static helpers of the reference's DominoTiler (DominoTiler.h:99-130)."""
+
+    @staticmethod
+    def Rectangle(M, N):
+        return _take_vec(lib().rt_domino_rectangle(M, N))
+
+    @staticmethod
+    def AztecDiamond(n):
+        return _take_vec(lib().rt_domino_aztec_diamond(n))
+
+    @staticmethod
+    def AztecRectangle(n, remove):
+        r = np.ascontiguousarray(remove, dtype=np.int32)
+        ptr = r.ctypes.data_as(C.c_void_p) if len(r) else None
+        return _take_vec(lib().rt_domino_aztec_rectangle(n, ptr, len(r)))
+
+    @staticmethod
+    def MaxTiling(d):
+        return _thurston("rt_domino_max_tiling", d)
+
+    @staticmethod
+    def MinTiling(d):
+        return _thurston("rt_domino_min_tiling", d)
+
+    @staticmethod
+    def HeightfuncToTiling(hf, d):
+        hf, d = _i32(hf), _i32(d)
+        return _take_vec(lib().rt_domino_heightfunc_to_tiling(hf, len(hf), d, len(d)))
+
+    @staticmethod
+    def TilingToHeightfunc(t, d):
+        t, d = _i32(t), _i32(d)
+        return _take_vec(lib().rt_domino_tiling_to_heightfunc(t, len(t), d, len(d)))
+
+    @staticmethod
+    def TilingToDomain(t):
+        t = _i32(t)
+        return _take_vec(lib().rt_domino_tiling_to_domain(t, len(t)))
+
+    @staticmethod
+    def DomainToVertices(d):
+        d = _i32(d)
+        return _take_vec(lib().rt_domino_domain_to_vertices(d, len(d)))
+
+
+class LozengeTiler:
+    """Static helpers of the reference's LozengeTiler (LozengeTiler.h:77-92)."""
+
+    @staticmethod
+    def AlmostHexDomain(n, m):
+        return _take_vec(lib().rt_lozenge_almost_hex_domain(n, m))
+
+    @staticmethod
+    def MaxTiling(d):
+        return _thurston("rt_lozenge_max_tiling", d)
+
+    @staticmethod
+    def MinTiling(d):
+        return _thurston("rt_lozenge_min_tiling", d)
+
+    @staticmethod
+    def HeightfuncToTiling(hf, d):
+        hf, d = _i32(hf), _i32(d)
+        return _take_vec(lib().rt_lozenge_heightfunc_to_tiling(hf, len(hf), d, len(d)))
+
+    @staticmethod
+    def TilingToHeightfunc(t, d):
+        t, d = _i32(t), _i32(d)
+        return _take_vec(lib().rt_lozenge_tiling_to_heightfunc(t, len(t), d, len(d)))
+
+    @staticmethod
+    def TilingToDomain(t):
+        t = _i32(t)
+        return _take_vec(lib().rt_lozenge_tiling_to_domain(t, len(t)))
+
+    @staticmethod
+    def DomainToVertices(d):
+        d = _i32(d)
+        return _take_vec(lib().rt_lozenge_domain_to_vertices(d, len(d)))
+
+
+class RectTriangleTiler:
+    """Static helpers of the reference's RectTriangleTiler (RectTriangleTiler.h:77-82)."""
+
+    @staticmethod
+    def maxHex(n):
+        return _take_vec(lib().rt_recttri_max_hex(n))
+
+    @staticmethod
+    def minHex(n):
+        return _take_vec(lib().rt_recttri_min_hex(n))
+
+
+def synthetic_params(count, seed=0x5eed):
+    """Synthetic TinyMT parameter table (the reference's 2^20-line table is
+    not in its checkout): per id a (mat1, mat2, tmat) triple from a
+    splitmix64 stream of `seed`; seed=None gives the TINYMT32J constants for
+    every id (tinymt32def.clh:48-50)."""
+    if seed is None:
+        p = np.empty((count, 3), dtype=np.uint32)
+        p[:, 0], p[:, 1], p[:, 2] = 0x8F7011EE, 0xFC78FF1F, 0x3793FDFF
+        return p
+    idx = np.arange(3 * count, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = (np.uint64(seed) + (idx + np.uint64(1)) * np.uint64(0x9E3779B97F4A7C15))
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return (z & np.uint64(0xFFFFFFFF)).astype(np.uint32).reshape(count, 3)
+
+
+def write_params_file(path, params3):
+    """Write a table in the tinymt32dc text layout the reference parses."""
+    with open(path, "w") as f:
+        f.write("# synthetic TinyMT32 parameter table\n")
+        for k, (m1, m2, tm) in enumerate(np.asarray(params3, dtype=np.uint32)):
+            f.write(f"synthetic,127,{k},{int(m1):08x},{int(m2):08x},{int(tm):08x},0,0\n")

@@ -1,0 +1,191 @@
+// models_dev.hpp -- per-cell local moves and neighbour rebuilds of the four
+// reference models, as device functions shared by every kernel shape
+// (one-step, fused LDS sweep block, CFTP pair, batched chains).
+//
+// Cell states and bit layouts are the reference's:
+//   domino vertex bits 1 up, 2 down, 4 left, 8 right (DominoTiler.h:28-32,
+//     DominoTiler.cpp:337-341: pws = {2,1,8,4} for down, up, right, left);
+//   lozenge vertex bits 1,2,4,8,16,32 (LozengeTiler.h:16-19);
+//   rectangle-triangle: six 4-bit face codes d5..d0 (RectTriangleTiler.h:19-24).
+//
+// All comparisons against the reference's double literals (.8, .4, 0.5,
+// 0.75, 0.25) are done in float: the draws lie on the 2^-23 grid and no
+// grid value falls between such a literal and its float rounding (checked
+// exhaustively in tests/test_oracle.py), and 0.5/0.75/0.25 are exact.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rt {
+
+// ------------------------------------------------------------ domino ----
+enum DominoRule { D_TOGGLE08 = 0, D_HEATBATH = 1, D_METRO2P = 2, D_HEATBATH2P = 3 };
+
+struct DominoWeights {
+  float wa2;  // (float)(a*a), a*a evaluated in double like the reference's
+  float wb2;  // (float)(b*b)  `#define a 0.7` literals
+};
+
+// Local move at a vertex of colour t in compressed row i
+// (dominokernel.cl:14-17, dominokernelCFTP.cl:65-69,
+//  dominokerneltwoperiodic.cl:108-123, dominokernelCFTPtwoperiodic.cl:175-190).
+__device__ __forceinline__ uint32_t domino_move(uint32_t v, float rd, int rule,
+                                                int i, int t, DominoWeights w) {
+  if (rule == D_TOGGLE08) {
+    if (rd < 0.8f) v = (v == 3u) ? 12u : ((v == 12u) ? 3u : v);
+  } else if (rule == D_HEATBATH) {
+    if (rd < 0.5f) v = (v == 3u) ? 12u : v;
+    else v = (v == 12u) ? 3u : v;
+  } else {
+    // Horizontal pair weight: row parity i; vertical pair weight: (i+t).
+    const float wrow = (i & 1) ? w.wa2 : w.wb2;
+    const float wcol = ((i + t) & 1) ? w.wa2 : w.wb2;
+    if (rule == D_METRO2P) {
+      if (rd < 0.8f) {
+        const float th = rd / 0.8f;
+        if (v == 3u) {
+          if (wrow / wcol > th) v = 12u;
+        } else if (v == 12u) {
+          if (wcol / wrow > th) v = 3u;
+        }
+      }
+    } else {  // D_HEATBATH2P: thresholds rd/0.5 and (rd-0.5)/0.5 are exact
+      if (rd < 0.5f) {
+        const float th = rd * 2.0f;
+        if (v == 3u && wrow / wcol > th) v = 12u;
+      } else {
+        const float th = (rd - 0.5f) * 2.0f;
+        if (v == 12u && wcol / wrow > th) v = 3u;
+      }
+    }
+  }
+  return v;
+}
+
+// Rebuild of a vertex from its four neighbours of the other colour, where
+// `tr` is the colour that was rotated (dominokernel.cl:27-30).  `ref` points
+// at the rotated array, row stride H.
+__device__ __forceinline__ uint32_t domino_rebuild(const uint8_t* ref, int H, int i,
+                                                   int j, int tr) {
+  const uint32_t up = ref[(i - 1) * H + j];
+  const uint32_t dn = ref[(i + 1) * H + j];
+  const uint32_t lf = ref[i * H + j - ((i + tr + 1) & 1)];
+  const uint32_t rg = ref[i * H + j + ((i + tr) & 1)];
+  return ((up >> 1) & 1u) | ((dn & 1u) << 1) | ((lf >> 1) & 4u) | ((rg & 4u) << 1);
+}
+
+// ----------------------------------------------------------- lozenge ----
+// Cube add (25 -> 38) with rd < .4, remove (38 -> 25) with .4 <= rd < .8
+// (lozengekernel.cl:20-25).
+__device__ __forceinline__ uint32_t lozenge_move(uint32_t v, float rd) {
+  if (rd < 0.4f) {
+    if (v == 25u) v = 38u;
+  } else if (rd < 0.8f) {
+    if (v == 38u) v = 25u;
+  }
+  return v;
+}
+
+// Bits of a colour-x vertex that face colour x-1 ("next" of the rotated
+// colour in lozengekernel.cl:49-50) are {2,4,32}; bits facing colour x+1
+// ("next-next", 51-52) are {1,8,16}.
+// Refresh the {2,4,32} bits of vertex (i,j) of colour x from the array of
+// colour tr = x-1 (mod 3).
+__device__ __forceinline__ uint32_t lozenge_pull_prev(uint32_t v, const uint8_t* a,
+                                                      int M, int i, int j, int tr) {
+  const int p2 = (i % 3) == ((2 * tr + 1) % 3);
+  const int p4 = (i % 3) == ((2 * tr + 2) % 3);
+  v &= ~(2u | 4u | 32u);
+  v |= ((uint32_t)a[(i - 1) * M + j + p2] & 16u) >> 3;
+  v |= ((uint32_t)a[i * M + j - p4] & 8u) >> 1;
+  v |= ((uint32_t)a[(i + 1) * M + j] & 1u) << 5;
+  return v & 0xffu;
+}
+
+// Refresh the {1,8,16} bits of vertex (i,j) of colour x from colour
+// tr = x-2 (mod 3), i.e. x is the "next-next" colour of tr.
+__device__ __forceinline__ uint32_t lozenge_pull_prev2(uint32_t v, const uint8_t* a,
+                                                       int M, int i, int j, int tr) {
+  const int p8 = (i % 3) == ((2 * tr) % 3);
+  const int p16 = (i % 3) == ((2 * tr + 1) % 3);
+  v &= ~(1u | 8u | 16u);
+  v |= ((uint32_t)a[(i - 1) * M + j] & 32u) >> 5;
+  v |= ((uint32_t)a[i * M + j + p8] & 4u) << 1;
+  v |= ((uint32_t)a[(i + 1) * M + j - p16] & 2u) << 3;
+  return v & 0xffu;
+}
+
+// ------------------------------------------------- rectangle-triangle ----
+struct RTWeights {
+  float w[16];  // recttrianglekernel.cl:13: {0, wt1..3, wr1..3, wr1..3, wm1..3, wm1..3}
+};
+
+__device__ __forceinline__ float rt_weight(const RTWeights& W, uint32_t h) {
+  return W.w[(h >> 20) & 15u] * W.w[(h >> 16) & 15u] * W.w[(h >> 12) & 15u] *
+         W.w[(h >> 8) & 15u] * W.w[(h >> 4) & 15u] * W.w[h & 15u];
+}
+
+// Hexagon move at one vertex (recttrianglekernel.cl:37-117).  Returns the
+// new 24-bit state.  The five "all-triangle / single-rectangle" states form
+// a 4-way choice by quarter of rd/0.9; the twelve "rectangle pair" states
+// form fixed 2-cycles.
+__device__ __forceinline__ uint32_t recttri_move(uint32_t h, float rd, const RTWeights& W) {
+  if (!(rd < 0.9f)) return h;
+  rd = rd / 0.9f;
+  const float q = (rd < 0.25f) ? 0.0f : (rd < 0.5f) ? 0.25f : (rd < 0.75f) ? 0.5f : 0.75f;
+  // Targets for quarters [0,.25), [.25,.5), [.5,.75), [.75,1).
+  uint32_t nt = h;
+  bool pair = false;
+  switch (h) {
+    case 0x122133u: nt = rd < 0.25f ? 0xa7dd7au : rd < 0.5f ? 0xfc99cfu : rd < 0.75f ? 0x8ebbe8u : 0x331221u; break;
+    case 0x331221u: nt = rd < 0.25f ? 0xa7dd7au : rd < 0.5f ? 0xfc99cfu : rd < 0.75f ? 0x8ebbe8u : 0x122133u; break;
+    case 0x8ebbe8u: nt = rd < 0.25f ? 0xa7dd7au : rd < 0.5f ? 0xfc99cfu : rd < 0.75f ? 0x122133u : 0x331221u; break;
+    case 0xfc99cfu: nt = rd < 0.25f ? 0xa7dd7au : rd < 0.5f ? 0x122133u : rd < 0.75f ? 0x8ebbe8u : 0x331221u; break;
+    case 0xa7dd7au: nt = rd < 0.25f ? 0x122133u : rd < 0.5f ? 0xfc99cfu : rd < 0.75f ? 0x8ebbe8u : 0x331221u; break;
+    case 0x47d47au: nt = 0xd22a33u; pair = true; break;
+    case 0xd22a33u: nt = 0x47d47au; pair = true; break;
+    case 0xa74d74u: nt = 0x33a22du; pair = true; break;
+    case 0x33a22du: nt = 0xa74d74u; pair = true; break;
+    case 0x8eb558u: nt = 0x331eb1u; pair = true; break;
+    case 0x331eb1u: nt = 0x8eb558u; pair = true; break;
+    case 0x855be8u: nt = 0x1be133u; pair = true; break;
+    case 0x1be133u: nt = 0x855be8u; pair = true; break;
+    case 0x6699cfu: nt = 0xcf1221u; pair = true; break;
+    case 0xcf1221u: nt = 0x6699cfu; pair = true; break;
+    case 0xfc9966u: nt = 0x1221fcu; pair = true; break;
+    case 0x1221fcu: nt = 0xfc9966u; pair = true; break;
+    default: return h;
+  }
+  // Accept if w(new)/w(old) >= rescaled rd; 2-cycles compare against rd itself.
+  const float r2 = pair ? rd : (rd - q) * 4.0f;
+  const float ratio = rt_weight(W, nt) / rt_weight(W, h);
+  return (ratio >= r2) ? nt : h;
+}
+
+// updateTiles1 (recttrianglekernel.cl:120-132): rebuild a vertex of colour
+// tr+1 from the rotated colour tr.
+__device__ __forceinline__ uint32_t recttri_rebuild1(const int32_t* a, int M, int i, int j,
+                                                     int tr) {
+  const int p1 = (i % 3) == tr;
+  const int p2 = (i % 3) == ((tr + 1) % 3);
+  const uint32_t u = (uint32_t)a[(i - 1) * M + j];
+  const uint32_t r = (uint32_t)a[i * M + j + p1];
+  const uint32_t d = (uint32_t)a[(i + 1) * M + j - p2];
+  return ((u & 0xffu) << 16) | ((r & 0xf00000u) >> 8) | ((d & 0x0ff000u) >> 8) |
+         ((r & 0x000f00u) >> 8);
+}
+
+// updateTiles2 (recttrianglekernel.cl:134-146): rebuild a vertex of colour
+// tr+2 from the rotated colour tr.
+__device__ __forceinline__ uint32_t recttri_rebuild2(const int32_t* a, int M, int i, int j,
+                                                     int tr) {
+  const int p1 = (i % 3) == ((tr + 2) % 3);
+  const int p2 = (i % 3) == ((tr + 1) % 3);
+  const uint32_t l = (uint32_t)a[i * M + j - p1];
+  const uint32_t u = (uint32_t)a[(i - 1) * M + j + p2];
+  const uint32_t d = (uint32_t)a[(i + 1) * M + j];
+  return ((l & 0x00f000u) << 8) | ((u & 0x000ff0u) << 8) | ((l & 0x00000fu) << 8) |
+         ((d & 0xff0000u) >> 16);
+}
+
+}  // namespace rt

@@ -1,0 +1,56 @@
+// rt_internal.hpp -- host-side declarations shared by the translation units
+// of librtiling.so (not part of the public C ABI).
+#pragma once
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/rtiling.h"
+
+struct rt_vec {
+  std::vector<int32_t> v;
+};
+
+namespace rt {
+
+using tiling = std::vector<int>;
+using domain = std::vector<int>;
+using heightfunc = std::vector<int>;
+
+// Colour schedules of the reference's host loops.
+// Domino: std::mt19937(seed); single-seed Walk spends the first output on
+// the TinyMT base seed (DominoTiler.cpp:51-53); colour r = out % 2, r == 1
+// is black (70-79).  Colour index 0 = black, 1 = white.
+void domino_schedule(uint32_t seed, long steps, bool skip_first,
+                     uint32_t* base_out, std::vector<uint8_t>& colours);
+// Lozenge and rectangle-triangle: std::default_random_engine(seed) with
+// uniform_int_distribution<int>(0,2) (LozengeTiler.cpp:86-91,
+// RectTriangleTiler.cpp:69-71,92).
+void tri_schedule(long seed, long steps, std::vector<uint8_t>& colours);
+
+int parse_tinymt_file(const char* path, size_t count, uint32_t* out3,
+                      std::string* err);
+
+rt_vec* make_vec(std::vector<int32_t>&& v);
+
+// Host statics (rt_host.cpp)
+domain domino_aztec_rectangle(int n, const std::vector<int>& remove);
+domain domino_rectangle(int M, int N);
+std::vector<int> domino_domain_to_vertices(const domain& d);
+int domino_thurston(const domain& d, bool max, tiling& out);
+tiling domino_heightfunc_to_tiling(const heightfunc& hf, const domain& d);
+heightfunc domino_tiling_to_heightfunc(const tiling& t, const domain& d);
+domain domino_tiling_to_domain(const tiling& t);
+
+domain lozenge_almost_hex_domain(int n, int m);
+std::vector<int> lozenge_domain_to_vertices(const domain& d);
+int lozenge_thurston(const domain& d, bool max, tiling& out);
+tiling lozenge_heightfunc_to_tiling(const heightfunc& hf, const domain& d);
+heightfunc lozenge_tiling_to_heightfunc(const tiling& t, const domain& d);
+domain lozenge_tiling_to_domain(const tiling& t);
+
+tiling recttri_max_hex(int n);
+tiling recttri_min_hex(int n);
+
+}  // namespace rt

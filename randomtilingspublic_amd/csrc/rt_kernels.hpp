@@ -1,0 +1,60 @@
+// rt_kernels.hpp -- device-side types and the host drivers shared by the
+// walk, CFTP, batching and serving translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "models_dev.hpp"
+#include "rt_internal.hpp"
+#include "tinymt_dev.hpp"
+
+struct rt_ctx {
+  int device = 0;
+  int num_cu = 256;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::string err;
+  std::string devname;
+  uint32_t* d_param = nullptr;  // SoA: mat1[n], mat2[n], tmat[n]
+  size_t nparam = 0;
+  void* arena = nullptr;
+  size_t arena_bytes = 0;
+  uint32_t* d_flag = nullptr;
+  uint32_t* h_flag = nullptr;
+  rt_timing last{};
+  int engine = 0;
+
+  int reserve(size_t bytes);
+  int check_params(size_t need);
+  ~rt_ctx();
+};
+
+namespace rt {
+
+struct DominoArrays {
+  uint8_t* col[2];  // 0 = black ((i+j) even), 1 = white; each N x N/2
+};
+struct TriArrays8 {
+  uint8_t* col[3];  // lozenge R, B, G; each N x N/3
+};
+struct TriArrays32 {
+  int32_t* col[3];  // rectangle-triangle R, B, G; each N x N/3
+};
+
+int domino_fused_walk(rt_ctx* ctx, DominoArrays* a, int nc, int N, int rule, DominoWeights w,
+                      const std::vector<long>& steps, const std::vector<long>& seeds, bool single,
+                      uint32_t* state);
+int lozenge_device_walk(rt_ctx* ctx, TriArrays8* a, int nc, int N, const std::vector<long>& steps,
+                        const std::vector<long>& seeds, uint32_t* state);
+int lozenge_device_sync(rt_ctx* ctx, TriArrays8* a, int nc, int N, const long* last);
+int recttri_device_walk(rt_ctx* ctx, TriArrays32* a, int nc, int N,
+                        const std::vector<long>& steps, const std::vector<long>& seeds,
+                        const RTWeights& W, uint32_t* state, int* last_out);
+RTWeights recttri_weights(const float* w16);
+
+}  // namespace rt

@@ -1,0 +1,88 @@
+// tinymt_dev.hpp -- TinyMT32 on gfx950, state held in VGPRs.
+//
+// Same generator as the reference's per-work-item TinyMT
+// (src/TinyMT/tinymt32.clh: next_state 39-55, temper12 81-93, single01
+// 125-129, init 182-203), written for registers: the conditional xors are
+// lane-wise selects, so a wavefront never diverges on the low state bit.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rt {
+
+struct Tiny {
+  uint32_t s0, s1, s2, s3;
+  uint32_t mat1, mat2, tmat;
+};
+
+__device__ __forceinline__ void tiny_next(Tiny& t) {
+  uint32_t x = (t.s0 & 0x7fffffffu) ^ t.s1 ^ t.s2;
+  uint32_t y = t.s3;
+  x ^= x << 1;
+  y ^= (y >> 1) ^ x;
+  const uint32_t m = 0u - (y & 1u);
+  t.s0 = t.s1;
+  t.s1 = t.s2 ^ (m & t.mat1);
+  t.s2 = x ^ (y << 10) ^ (m & t.mat2);
+  t.s3 = y;
+}
+
+// Float in [0,1) on the 2^-23 grid: ((temper >> 9) | 1.0f bits) - 1.0f.
+__device__ __forceinline__ float tiny_float01(Tiny& t) {
+  tiny_next(t);
+  uint32_t t1 = t.s0 + (t.s2 >> 8);
+  uint32_t t0 = t.s3 ^ t1 ^ ((0u - (t1 & 1u)) & t.tmat);
+  return __uint_as_float((t0 >> 9) | 0x3f800000u) - 1.0f;
+}
+
+// tinymt32_init(params, seed): 7 mixing rounds, period certification,
+// 8 discarded outputs.
+__device__ __forceinline__ void tiny_init(Tiny& t, uint32_t seed) {
+  uint32_t st[4] = {seed, t.mat1, t.mat2, t.tmat};
+#pragma unroll
+  for (int i = 1; i < 8; i++)
+    st[i & 3] ^= (uint32_t)i + 1812433253u * (st[(i - 1) & 3] ^ (st[(i - 1) & 3] >> 30));
+  t.s0 = st[0];
+  t.s1 = st[1];
+  t.s2 = st[2];
+  t.s3 = st[3];
+  if ((t.s0 & 0x7fffffffu) == 0 && t.s1 == 0 && t.s2 == 0 && t.s3 == 0) {
+    t.s0 = 'T';
+    t.s1 = 'I';
+    t.s2 = 'N';
+    t.s3 = 'Y';
+  }
+#pragma unroll
+  for (int i = 0; i < 8; i++) tiny_next(t);
+}
+
+// Stream table in HBM, structure-of-arrays so that consecutive work items
+// read consecutive words: state[k*stride + id] for k = 0..3 (s0..s3),
+// param[k*pstride + id] for k = 0..2 (mat1, mat2, tmat).
+struct StreamTable {
+  uint32_t* state;
+  const uint32_t* param;
+  long stride;
+  long pstride;
+};
+
+__device__ __forceinline__ Tiny tiny_load(const StreamTable& S, long id) {
+  Tiny t;
+  t.s0 = S.state[id];
+  t.s1 = S.state[S.stride + id];
+  t.s2 = S.state[2 * S.stride + id];
+  t.s3 = S.state[3 * S.stride + id];
+  t.mat1 = S.param[id];
+  t.mat2 = S.param[S.pstride + id];
+  t.tmat = S.param[2 * S.pstride + id];
+  return t;
+}
+
+__device__ __forceinline__ void tiny_store(const StreamTable& S, long id, const Tiny& t) {
+  S.state[id] = t.s0;
+  S.state[S.stride + id] = t.s1;
+  S.state[2 * S.stride + id] = t.s2;
+  S.state[3 * S.stride + id] = t.s3;
+}
+
+}  // namespace rt

@@ -1,0 +1,176 @@
+"""Device path vs the CPU oracle, bit-exact, through the C ABI.
+
+Every case prints what it runs before it launches anything, so a failing
+device run names its case.
+"""
+import numpy as np
+import pytest
+
+import randomtilingspublic_amd as rt
+from tests import oracle_lib as o
+
+pytestmark = pytest.mark.gpu
+
+RULES = [rt.DOMINO_TOGGLE08, rt.DOMINO_HEATBATH, rt.DOMINO_METRO2P, rt.DOMINO_HEATBATH2P]
+
+
+def _say(*a):
+    print("[case]", *a, flush=True)
+
+
+def _domino(n):
+    d = rt.DominoTiler.AztecDiamond(n)
+    return d, rt.DominoTiler.MinTiling(d), rt.DominoTiler.MaxTiling(d)
+
+
+def _params(count, seed=0x5eed):
+    return rt.synthetic_params(count, seed)
+
+
+@pytest.mark.parametrize("rule", RULES)
+@pytest.mark.parametrize("n,steps,seed", [(2, 1, 1), (5, 2, 7), (5, 257, 34893), (16, 1000, 3)])
+def test_domino_walk_matches_oracle(gpu_ctx, rule, n, steps, seed):
+    d, tmin, tmax = _domino(n)
+    N = rt.side(tmin)
+    P = _params(N * N // 2)
+    gpu_ctx.set_tinymt_params(P)
+    _say("domino walk", rule, n, steps, seed)
+    got = gpu_ctx.domino_walk(tmin, steps, seed, rule, 0.7, 1.0)
+    want = o.domino_walk(tmin, steps, seed, P, rule, 0.7, 1.0)
+    assert np.array_equal(got, want)
+
+
+def test_domino_walk_zero_steps_identity(gpu_ctx):
+    d, tmin, _ = _domino(4)
+    N = rt.side(tmin)
+    gpu_ctx.set_tinymt_params(_params(N * N // 2))
+    _say("domino walk 0 steps")
+    assert np.array_equal(gpu_ctx.domino_walk(tmin, 0, 5), tmin)
+
+
+@pytest.mark.parametrize("rule", [rt.DOMINO_TOGGLE08, rt.DOMINO_HEATBATH2P])
+def test_domino_walk_segments_matches_oracle(gpu_ctx, rule):
+    d, tmin, tmax = _domino(7)
+    N = rt.side(tmax)
+    P = _params(N * N // 2, seed=11)
+    gpu_ctx.set_tinymt_params(P)
+    steps, seeds = [256, 128, 33], [4000000001, 17, 2**31 + 5]
+    _say("domino segments", rule)
+    got = gpu_ctx.domino_walk_segments(tmax, steps, seeds, rule, 0.5, 1.0)
+    want = o.domino_walk_segments(tmax, steps, seeds, P, rule, 0.5, 1.0)
+    assert np.array_equal(got, want)
+
+
+def test_domino_weighted_config_rule(gpu_ctx):
+    """Two-periodic a=1.0, b=0.5 (the weighted Aztec configuration's
+    weights), Metropolis rule, on an order-24 diamond."""
+    d, tmin, _ = _domino(24)
+    N = rt.side(tmin)
+    P = _params(N * N // 2, seed=2)
+    gpu_ctx.set_tinymt_params(P)
+    _say("domino 2-periodic a=1 b=.5")
+    got = gpu_ctx.domino_walk(tmin, 3000, 9, rt.DOMINO_METRO2P, 1.0, 0.5)
+    want = o.domino_walk(tmin, 3000, 9, P, rt.DOMINO_METRO2P, 1.0, 0.5)
+    assert np.array_equal(got, want)
+
+
+def test_domino_aztec100_1e4_steps(gpu_ctx):
+    """Minimal slice: Walk(MinTiling(AztecDiamond(100)), 10^4, seed=1) with
+    both the 0.8 toggle and the 1/2 heat-bath rule, byte for byte."""
+    d, tmin, _ = _domino(100)
+    N = rt.side(tmin)
+    P = _params(N * N // 2)
+    gpu_ctx.set_tinymt_params(P)
+    o.set_threads(8)
+    for rule in (rt.DOMINO_HEATBATH, rt.DOMINO_TOGGLE08):
+        _say("aztec-100 1e4", rule)
+        got = gpu_ctx.domino_walk(tmin, 10000, 1, rule)
+        want = o.domino_walk(tmin, 10000, 1, P, rule)
+        assert np.array_equal(got, want)
+    o.set_threads(1)
+
+
+@pytest.mark.parametrize("sched", [rt.SCHED_REFERENCE, rt.SCHED_DOUBLING])
+def test_domino_cftp_matches_oracle(gpu_ctx, sched):
+    d, tmin, tmax = _domino(5)
+    N = rt.side(tmin)
+    P = _params(N * N // 2)
+    gpu_ctx.set_tinymt_params(P)
+    for master in (345903, 1, 2):
+        _say("domino cftp", sched, master)
+        got, gs = gpu_ctx.domino_cftp(tmax, tmin, master, sched, 128, 40)
+        want, ws = o.domino_cftp(tmax, tmin, master, P, sched, 128, 40)
+        assert np.array_equal(got, want)
+        assert gs["rounds"] == ws["rounds"] and gs["total_steps"] == ws["total_steps"]
+
+
+def _lozenge(n, m=0):
+    d = rt.LozengeTiler.AlmostHexDomain(n, m)
+    return d, rt.LozengeTiler.MinTiling(d), rt.LozengeTiler.MaxTiling(d)
+
+
+@pytest.mark.parametrize("n,m,steps,seed", [(2, 0, 1, 1), (4, 0, 100, 5), (10, 2, 1000, 77),
+                                            (20, 0, 2000, 12345)])
+def test_lozenge_walk_matches_oracle(gpu_ctx, n, m, steps, seed):
+    d, tmin, tmax = _lozenge(n, m)
+    N = rt.side(tmin)
+    P = _params(N * N)
+    gpu_ctx.set_tinymt_params(P)
+    _say("lozenge walk", n, m, steps, seed)
+    for start in (tmin, tmax):
+        got = gpu_ctx.lozenge_walk(start, steps, seed)
+        want = o.lozenge_walk(start, steps, seed, P)
+        assert np.array_equal(got, want)
+
+
+def test_lozenge_segments_and_cftp(gpu_ctx):
+    d, tmin, tmax = _lozenge(5)
+    N = rt.side(tmin)
+    P = _params(N * N, seed=3)
+    gpu_ctx.set_tinymt_params(P)
+    _say("lozenge segments")
+    got = gpu_ctx.lozenge_walk_segments(tmin, [64, 10, 300], [9, 8, 7])
+    want = o.lozenge_walk_segments(tmin, [64, 10, 300], [9, 8, 7], P)
+    assert np.array_equal(got, want)
+    for sched in (rt.SCHED_REFERENCE, rt.SCHED_DOUBLING):
+        for master in (1, 2, 3):
+            _say("lozenge cftp", sched, master)
+            g, gs = gpu_ctx.lozenge_cftp(tmax, tmin, master, sched, 64, 40)
+            w, ws = o.lozenge_cftp(tmax, tmin, master, P, sched, 64, 40)
+            assert np.array_equal(g, w)
+            assert gs["rounds"] == ws["rounds"]
+
+
+@pytest.mark.parametrize("n,steps,seed", [(2, 1, 1), (3, 50, 4), (8, 700, 21)])
+def test_recttri_walk_matches_oracle(gpu_ctx, n, steps, seed):
+    for start in (rt.RectTriangleTiler.minHex(n), rt.RectTriangleTiler.maxHex(n)):
+        N = rt.side(start)
+        P = _params(N * N, seed=5)
+        gpu_ctx.set_tinymt_params(P)
+        _say("recttri walk", n, steps, seed)
+        got = gpu_ctx.recttri_walk(start, steps, seed)
+        want = o.recttri_walk(start, steps, seed, P)
+        assert np.array_equal(got, want)
+
+
+def test_recttri_weighted_matches_oracle(gpu_ctx):
+    start = rt.RectTriangleTiler.minHex(6)
+    N = rt.side(start)
+    P = _params(N * N, seed=6)
+    gpu_ctx.set_tinymt_params(P)
+    w = np.array([0.0, 1.0, 0.7, 1.3, 0.5, 2.0, 0.9, 0.5, 2.0, 0.9, 1.1, 0.3, 1.7, 1.1, 0.3, 1.7],
+                 dtype=np.float32)
+    _say("recttri weighted")
+    got = gpu_ctx.recttri_walk(start, 900, 3, w)
+    want = o.recttri_walk(start, 900, 3, P, w)
+    assert np.array_equal(got, want)
+
+
+def test_params_too_small_is_rejected(gpu_ctx):
+    d, tmin, _ = _domino(5)
+    N = rt.side(tmin)
+    gpu_ctx.set_tinymt_params(_params(N * N // 2 - 1))
+    _say("short param table")
+    with pytest.raises(rt.RTError) as e:
+        gpu_ctx.domino_walk(tmin, 10, 1)
+    assert e.value.code == -2
