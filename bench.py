@@ -1,0 +1,182 @@
+#!/usr/bin/env python3
+"""Benchmark of the random-tiling engine on one MI355X.
+
+Default workload (the headline configuration): uniform lozenge tilings of
+the hexagon with sides a=b=c=300, exact samples by coupling from the past
+(top = MaxTiling, bottom = MinTiling, look-back doubling from 2^10 steps),
+one sample per master seed, seeds 1..K, all K samples advanced together by
+the batched fused kernel.  A "step" of this benchmark is one exact sample
+(one master seed); --steps K seeds are timed, --warmup W short walks run
+first.  Synthetic TinyMT parameter table (the TINYMT32J set for every
+stream; the reference's 2^20-line table is not in its checkout).
+
+Prints ONE JSON line: value = CFTP samples/s; also flip attempts/s (one
+attempt = one chain's flip attempt at one site, as the reference counts
+them: each CFTP round is two reference Walks), the plain one-chain-per-call
+path on the same inputs, the roofline of the sweep kernel and the CPU
+baseline (the oracle, OpenMP over the sites, on this host's cores).
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(N, tmin, params, units_attempts, nsamples, threads, budget_s=12.0):
+    """Oracle (CPU restatement of the reference kernels, OpenMP over the
+    NDRange) timed on a bounded sample: plain walks of the same hexagon
+    until ~budget_s of CPU work.  Scaled to samples/s with the attempts
+    per sample the device run needed."""
+    from tests import oracle_lib as o
+    o.set_threads(threads)
+    steps = 32
+    done_attempts = 0
+    t0 = time.time()
+    P = N * (N // 3)
+    while time.time() - t0 < budget_s:
+        o.lozenge_walk(tmin, steps, 1 + done_attempts % 97, params)
+        done_attempts += steps * P
+    dt = time.time() - t0
+    o.set_threads(1)
+    att_s = done_attempts / dt
+    return {
+        "value": att_s * nsamples / units_attempts if units_attempts else None,
+        "unit": "CFTP samples/s",
+        "cores": threads,
+        "kind": "port",
+        "attempts_per_s": att_s,
+        "sample": f"oracle lozenge walks on the side-300 hexagon, {done_attempts // P} steps "
+                  f"in {dt:.1f} s, scaled by the device run's attempts per sample; cpu {cpu_model()}",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=32, help="CFTP samples (master seeds 1..K)")
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--side", type=int, default=300)
+    ap.add_argument("--t0", type=int, default=1024)
+    ap.add_argument("--max-rounds", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import randomtilingspublic_amd as rt
+
+    side = args.side
+    d = rt.LozengeTiler.AlmostHexDomain(side, 0)
+    tmin, tmax = rt.LozengeTiler.MinTiling(d), rt.LozengeTiler.MaxTiling(d)
+    N = rt.side(tmin)
+    P = N * (N // 3)
+    params = rt.synthetic_params(N * N, None)
+    ctx = rt.Context(0)
+    ctx.set_tinymt_params(params)
+    print(f"[bench] device {ctx.device_name} hexagon {side} N={N} sites/step={P}", file=sys.stderr,
+          flush=True)
+    for w in range(args.warmup):
+        ctx.lozenge_walk(tmin, 2048, 1000 + w)
+    masters = list(range(1, args.steps + 1))
+    t0 = time.time()
+    outs, stats = ctx.lozenge_cftp_batch(tmax, tmin, masters, rt.SCHED_DOUBLING, args.t0,
+                                         args.max_rounds)
+    wall = time.time() - t0
+    tm = ctx.last_timing()
+    dev_s = tm["device_ms"] / 1e3
+    draws = tm["attempts"]                  # one TinyMT draw per site per step per sample
+    attempts = 2 * draws                    # two chains (top, bottom) per draw
+    samples_s = len(masters) / wall
+    print(f"[bench] batched CFTP {len(masters)} samples wall {wall:.2f}s device {dev_s:.2f}s "
+          f"rounds {[s['rounds'] for s in stats]}", file=sys.stderr, flush=True)
+    # exactness guard: every sample is a tiling of the hexagon
+    dom = rt.LozengeTiler.TilingToDomain(tmin)
+    for k in range(len(masters)):
+        assert np.array_equal(rt.LozengeTiler.TilingToDomain(outs[k]), dom)
+
+    # Plain path: the reference's shape, one sample per call on one chain pair.
+    plain = None
+    if not args.no_plain:
+        t1 = time.time()
+        o1, s1 = ctx.lozenge_cftp(tmax, tmin, masters[0], rt.SCHED_DOUBLING, args.t0,
+                                  args.max_rounds)
+        pw = time.time() - t1
+        assert np.array_equal(o1, outs[0])
+        plain = {"samples_per_s": 1.0 / pw, "seed": masters[0], "wall_s": pw,
+                 "rounds": s1["rounds"]}
+        print(f"[bench] plain CFTP seed {masters[0]} wall {pw:.2f}s", file=sys.stderr, flush=True)
+
+    # Roofline of the sweep kernel: algorithmic bytes per launch = for every
+    # sample's sites, TinyMT state (16 B) + parameters (12 B) read, state
+    # written (16 B), and 3 colour planes x 2 chains read and written (12 B).
+    klaunch = max(tm["kernel_launches"], 1)
+    kms = tm["kernel_ms"]
+    avg_launch_s = kms / 1e3 / klaunch
+    sites_per_launch = tm["kernel_sites"] / klaunch
+    alg_bytes = 56.0 * sites_per_launch
+    achieved_gbs = alg_bytes / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
+    roof = {"bound": "hbm", "achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "k_fused<lozenge, 2 chains>", "avg_launch_us": round(avg_launch_s * 1e6, 2),
+            "launches": klaunch, "steps_per_launch": round(tm["steps"] / klaunch, 2),
+            "sites_per_launch": int(sites_per_launch), "alg_bytes_per_launch": int(alg_bytes),
+            "flip_attempts_per_s_in_kernel": round(2 * draws / (kms / 1e3), 1) if kms else None}
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(N, tmin, params, attempts, len(masters), os.cpu_count() or 1)
+        print(f"[bench] cpu baseline {cpu}", file=sys.stderr, flush=True)
+
+    line = {
+        "metric": "CFTP samples/s (flip attempts/s alongside), batched vs plain, 1 GPU",
+        "value": round(samples_s, 4),
+        "unit": "samples/s",
+        "n_gpus": 1,
+        "steps": len(masters),
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * wall / len(masters), 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic (hexagon domain built in-process; synthetic TinyMT parameter table)",
+        "config": {"workload": f"lozenge hexagon a=b=c={side}, CFTP from min/max, doubling from "
+                               f"2^{int(np.log2(args.t0))}, master seeds 1..{len(masters)}, "
+                               "one exact sample per seed, batched",
+                   "N": N, "sites_per_step": P},
+        "flip_attempts_per_s": round(attempts / wall, 1),
+        "flip_attempts_per_s_device": round(attempts / dev_s, 1),
+        "device_s": round(dev_s, 3),
+        "rounds": [s["rounds"] for s in stats],
+        "plain": plain,
+        "batched_vs_plain": round(samples_s / plain["samples_per_s"], 2) if plain else None,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

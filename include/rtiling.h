@@ -65,6 +65,9 @@ typedef struct {
   long launches;
   long steps;         /* colour steps run, per chain */
   long attempts;      /* flip attempts (TinyMT draws) */
+  double kernel_ms;   /* time inside the sweep kernels alone (batched calls) */
+  long kernel_launches;
+  long kernel_sites;  /* sum over those launches of the TinyMT sites they own */
 } rt_timing;
 int rt_last_timing(const rt_ctx* ctx, rt_timing* out);
 
@@ -126,6 +129,29 @@ int rt_lozenge_walk_segments(rt_ctx* ctx, int32_t* tiling, int N,
 int rt_lozenge_cftp(rt_ctx* ctx, const int32_t* tmax, const int32_t* tmin,
                     int N, uint32_t master_seed, int sched, long t0,
                     int max_rounds, int32_t* out, rt_cftp_stats* stats);
+
+/* ------------------------------------------------------------- batching */
+/* Batched CFTP: one exact sample per master seed, all samples of the batch
+ * advanced by the same fused launches (one launch covers every tile of
+ * every unfinished sample).  Sample k is bit-identical to
+ * rt_*_cftp(master_seeds[k]) run alone.  out: nsamples*N*N, stats:
+ * nsamples entries.  No reference counterpart: the reference draws one
+ * sample per driver run (AztecDiamondCftpEx.cpp:58-77). */
+int rt_lozenge_cftp_batch(rt_ctx* ctx, const int32_t* tmax, const int32_t* tmin, int N,
+                          const uint32_t* master_seeds, int nsamples, int sched, long t0,
+                          int max_rounds, int32_t* out, rt_cftp_stats* stats);
+int rt_domino_cftp_batch(rt_ctx* ctx, const int32_t* tmax, const int32_t* tmin, int N,
+                         const uint32_t* master_seeds, int nsamples, int sched, long t0,
+                         int max_rounds, const rt_domino_rule* rule, int32_t* out,
+                         rt_cftp_stats* stats);
+
+/* Batched domino walks on domains of uneven size: chain k is
+ * DominoTiler::Walk(tilings[k], steps, seeds[k]) with side Ns[k]; all
+ * chains advance together.  use_graph != 0 captures the launch sequence in
+ * a hipGraph and replays it. */
+int rt_domino_walk_batch(rt_ctx* ctx, int32_t* const* tilings, const int* Ns, int nchains,
+                         long steps, const long* seeds, const rt_domino_rule* rule,
+                         int use_graph);
 
 /* ---------------------------------------------------- rectangle-triangle */
 /* RectTriangleTiler::Walk (RectTriangleTiler.cpp:33-171); weights16 is the

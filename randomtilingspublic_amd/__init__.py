@@ -35,7 +35,9 @@ class RTError(RuntimeError):
 
 class _Timing(C.Structure):
     _fields_ = [("device_ms", C.c_double), ("launches", C.c_long),
-                ("steps", C.c_long), ("attempts", C.c_long)]
+                ("steps", C.c_long), ("attempts", C.c_long),
+                ("kernel_ms", C.c_double), ("kernel_launches", C.c_long),
+                ("kernel_sites", C.c_long)]
 
 
 class DominoRule(C.Structure):
@@ -84,6 +86,12 @@ def lib():
     L.rt_lozenge_cftp.argtypes = [vp, _I32P, _I32P, C.c_int, C.c_uint32, C.c_int, C.c_long,
                                   C.c_int, _I32P, C.POINTER(CftpStats)]
     L.rt_recttri_walk.argtypes = [vp, _I32P, C.c_int, C.c_long, C.c_long, C.c_void_p]
+    L.rt_lozenge_cftp_batch.argtypes = [vp, _I32P, _I32P, C.c_int, _U32P, C.c_int, C.c_int,
+                                        C.c_long, C.c_int, _I32P, C.c_void_p]
+    L.rt_domino_cftp_batch.argtypes = [vp, _I32P, _I32P, C.c_int, _U32P, C.c_int, C.c_int,
+                                       C.c_long, C.c_int, C.POINTER(DominoRule), _I32P, C.c_void_p]
+    L.rt_domino_walk_batch.argtypes = [vp, C.c_void_p, C.c_void_p, C.c_int, C.c_long, _LONGP,
+                                       C.POINTER(DominoRule), C.c_int]
     # host helpers returning rt_vec*
     L.rt_vec_data.argtypes = [vp]
     L.rt_vec_data.restype = C.POINTER(C.c_int32)
@@ -183,7 +191,8 @@ class Context:
         t = _Timing()
         self._chk(self._L.rt_last_timing(self._h, C.byref(t)))
         return {"device_ms": t.device_ms, "launches": t.launches, "steps": t.steps,
-                "attempts": t.attempts}
+                "attempts": t.attempts, "kernel_ms": t.kernel_ms,
+                "kernel_launches": t.kernel_launches, "kernel_sites": t.kernel_sites}
 
     # ---- domino ----
     def domino_walk(self, t, steps, seed, rule=DOMINO_TOGGLE08, a=1.0, b=1.0):
@@ -234,6 +243,45 @@ class Context:
                                           t0, max_rounds, out, C.byref(st)))
         return out, {"rounds": st.rounds, "total_steps": st.total_steps,
                      "steps_run": st.steps_run, "device_ms": st.device_ms}
+
+    # ---- batching ----
+    def _cftp_batch(self, fn, tmax, tmin, masters, sched, t0, max_rounds, *rule):
+        tmax, tmin = _i32(tmax), _i32(tmin)
+        masters = np.ascontiguousarray(masters, dtype=np.uint32)
+        n = len(masters)
+        out = np.zeros(n * len(tmax), dtype=np.int32)
+        st = (CftpStats * n)()
+        args = [self._h, tmax, tmin, side(tmax), masters, n, sched, t0, max_rounds]
+        args += list(rule)
+        args += [out, C.cast(st, C.c_void_p)]
+        rc = fn(*args)
+        stats = [{"rounds": s.rounds, "total_steps": s.total_steps, "steps_run": s.steps_run,
+                  "device_ms": s.device_ms} for s in st]
+        self._chk(rc)
+        return out.reshape(n, len(tmax)), stats
+
+    def lozenge_cftp_batch(self, tmax, tmin, masters, sched=SCHED_DOUBLING, t0=1024,
+                           max_rounds=40):
+        return self._cftp_batch(self._L.rt_lozenge_cftp_batch, tmax, tmin, masters, sched, t0,
+                                max_rounds)
+
+    def domino_cftp_batch(self, tmax, tmin, masters, sched=SCHED_DOUBLING, t0=128,
+                          max_rounds=40, rule=DOMINO_HEATBATH, a=1.0, b=1.0):
+        r = DominoRule(rule, a, b)
+        return self._cftp_batch(self._L.rt_domino_cftp_batch, tmax, tmin, masters, sched, t0,
+                                max_rounds, C.byref(r))
+
+    def domino_walk_batch(self, tilings, steps, seeds, rule=DOMINO_TOGGLE08, a=1.0, b=1.0,
+                          use_graph=False):
+        ts = [_i32(t).copy() for t in tilings]
+        Ns = np.array([side(t) for t in ts], dtype=np.int32)
+        ptrs = (C.c_void_p * len(ts))(*[t.ctypes.data for t in ts])
+        se = np.ascontiguousarray(seeds, dtype=np.int64)
+        r = DominoRule(rule, a, b)
+        self._chk(self._L.rt_domino_walk_batch(self._h, C.cast(ptrs, C.c_void_p),
+                                               Ns.ctypes.data_as(C.c_void_p), len(ts), steps,
+                                               se, C.byref(r), 1 if use_graph else 0))
+        return ts
 
     # ---- rectangle-triangle ----
     def recttri_walk(self, t, steps, seed, weights16=None):

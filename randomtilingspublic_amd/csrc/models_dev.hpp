@@ -16,6 +16,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "tinymt_dev.hpp"
+
 namespace rt {
 
 // ------------------------------------------------------------ domino ----
@@ -29,14 +31,16 @@ struct DominoWeights {
 // Local move at a vertex of colour t in compressed row i
 // (dominokernel.cl:14-17, dominokernelCFTP.cl:65-69,
 //  dominokerneltwoperiodic.cl:108-123, dominokernelCFTPtwoperiodic.cl:175-190).
-__device__ __forceinline__ uint32_t domino_move(uint32_t v, float rd, int rule,
+// `b` is the tempered draw; rd = (b>>9)*2^-23.
+__device__ __forceinline__ uint32_t domino_move(uint32_t v, uint32_t b, int rule,
                                                 int i, int t, DominoWeights w) {
   if (rule == D_TOGGLE08) {
-    if (rd < 0.8f) v = (v == 3u) ? 12u : ((v == 12u) ? 3u : v);
+    if (b < kBelow08) v = (v == 3u) ? 12u : ((v == 12u) ? 3u : v);
   } else if (rule == D_HEATBATH) {
-    if (rd < 0.5f) v = (v == 3u) ? 12u : v;
+    if (b < kBelow05) v = (v == 3u) ? 12u : v;
     else v = (v == 12u) ? 3u : v;
   } else {
+    const float rd = tiny_to_float01(b);
     // Horizontal pair weight: row parity i; vertical pair weight: (i+t).
     const float wrow = (i & 1) ? w.wa2 : w.wb2;
     const float wcol = ((i + t) & 1) ? w.wa2 : w.wb2;
@@ -63,24 +67,25 @@ __device__ __forceinline__ uint32_t domino_move(uint32_t v, float rd, int rule,
 }
 
 // Rebuild of a vertex from its four neighbours of the other colour, where
-// `tr` is the colour that was rotated (dominokernel.cl:27-30).  `ref` points
-// at the rotated array, row stride H.
-__device__ __forceinline__ uint32_t domino_rebuild(const uint8_t* ref, int H, int i,
-                                                   int j, int tr) {
-  const uint32_t up = ref[(i - 1) * H + j];
-  const uint32_t dn = ref[(i + 1) * H + j];
-  const uint32_t lf = ref[i * H + j - ((i + tr + 1) & 1)];
-  const uint32_t rg = ref[i * H + j + ((i + tr) & 1)];
+// `tr` is the colour that was rotated (dominokernel.cl:27-30).  `p` points
+// at the rotated array's cell with the same compressed index (i,j), `rs` is
+// that array's row stride; i is the global row (it fixes the parities).
+template <class P>
+__device__ __forceinline__ uint32_t domino_rebuild(const P* p, int rs, int i, int tr) {
+  const uint32_t up = (uint32_t)p[-rs] & 0xffu;
+  const uint32_t dn = (uint32_t)p[rs] & 0xffu;
+  const uint32_t lf = (uint32_t)p[-((i + tr + 1) & 1)] & 0xffu;
+  const uint32_t rg = (uint32_t)p[(i + tr) & 1] & 0xffu;
   return ((up >> 1) & 1u) | ((dn & 1u) << 1) | ((lf >> 1) & 4u) | ((rg & 4u) << 1);
 }
 
 // ----------------------------------------------------------- lozenge ----
 // Cube add (25 -> 38) with rd < .4, remove (38 -> 25) with .4 <= rd < .8
-// (lozengekernel.cl:20-25).
-__device__ __forceinline__ uint32_t lozenge_move(uint32_t v, float rd) {
-  if (rd < 0.4f) {
+// (lozengekernel.cl:20-25).  `b` is the tempered draw (rd = (b>>9)*2^-23).
+__device__ __forceinline__ uint32_t lozenge_move(uint32_t v, uint32_t b) {
+  if (b < kBelow04) {
     if (v == 25u) v = 38u;
-  } else if (rd < 0.8f) {
+  } else if (b < kBelow08) {
     if (v == 38u) v = 25u;
   }
   return v;
@@ -88,30 +93,34 @@ __device__ __forceinline__ uint32_t lozenge_move(uint32_t v, float rd) {
 
 // Bits of a colour-x vertex that face colour x-1 ("next" of the rotated
 // colour in lozengekernel.cl:49-50) are {2,4,32}; bits facing colour x+1
-// ("next-next", 51-52) are {1,8,16}.
-// Refresh the {2,4,32} bits of vertex (i,j) of colour x from the array of
-// colour tr = x-1 (mod 3).
-__device__ __forceinline__ uint32_t lozenge_pull_prev(uint32_t v, const uint8_t* a,
-                                                      int M, int i, int j, int tr) {
-  const int p2 = (i % 3) == ((2 * tr + 1) % 3);
-  const int p4 = (i % 3) == ((2 * tr + 2) % 3);
+// ("next-next", 51-52) are {1,8,16}.  `p` points at cell (i,j) of the
+// source colour's array (row stride rs); i is the global row.
+// Refresh the {2,4,32} bits of v from the array of colour tr = x-1 (mod 3).
+template <class P>
+__device__ __forceinline__ uint32_t lozenge_pull_prev(uint32_t v, const P* p, int rs, int i,
+                                                      int tr) {
+  const int im = i % 3;
+  const int p2 = im == ((2 * tr + 1) % 3);
+  const int p4 = im == ((2 * tr + 2) % 3);
   v &= ~(2u | 4u | 32u);
-  v |= ((uint32_t)a[(i - 1) * M + j + p2] & 16u) >> 3;
-  v |= ((uint32_t)a[i * M + j - p4] & 8u) >> 1;
-  v |= ((uint32_t)a[(i + 1) * M + j] & 1u) << 5;
+  v |= ((uint32_t)p[-rs + p2] & 16u) >> 3;
+  v |= ((uint32_t)p[-p4] & 8u) >> 1;
+  v |= ((uint32_t)p[rs] & 1u) << 5;
   return v & 0xffu;
 }
 
-// Refresh the {1,8,16} bits of vertex (i,j) of colour x from colour
-// tr = x-2 (mod 3), i.e. x is the "next-next" colour of tr.
-__device__ __forceinline__ uint32_t lozenge_pull_prev2(uint32_t v, const uint8_t* a,
-                                                       int M, int i, int j, int tr) {
-  const int p8 = (i % 3) == ((2 * tr) % 3);
-  const int p16 = (i % 3) == ((2 * tr + 1) % 3);
+// Refresh the {1,8,16} bits of v from colour tr = x-2 (mod 3), i.e. x is
+// the "next-next" colour of tr.
+template <class P>
+__device__ __forceinline__ uint32_t lozenge_pull_prev2(uint32_t v, const P* p, int rs, int i,
+                                                       int tr) {
+  const int im = i % 3;
+  const int p8 = im == ((2 * tr) % 3);
+  const int p16 = im == ((2 * tr + 1) % 3);
   v &= ~(1u | 8u | 16u);
-  v |= ((uint32_t)a[(i - 1) * M + j] & 32u) >> 5;
-  v |= ((uint32_t)a[i * M + j + p8] & 4u) << 1;
-  v |= ((uint32_t)a[(i + 1) * M + j - p16] & 2u) << 3;
+  v |= ((uint32_t)p[-rs] & 32u) >> 5;
+  v |= ((uint32_t)p[p8] & 4u) << 1;
+  v |= ((uint32_t)p[rs - p16] & 2u) << 3;
   return v & 0xffu;
 }
 
@@ -129,7 +138,8 @@ __device__ __forceinline__ float rt_weight(const RTWeights& W, uint32_t h) {
 // new 24-bit state.  The five "all-triangle / single-rectangle" states form
 // a 4-way choice by quarter of rd/0.9; the twelve "rectangle pair" states
 // form fixed 2-cycles.
-__device__ __forceinline__ uint32_t recttri_move(uint32_t h, float rd, const RTWeights& W) {
+__device__ __forceinline__ uint32_t recttri_move(uint32_t h, uint32_t b, const RTWeights& W) {
+  float rd = tiny_to_float01(b);
   if (!(rd < 0.9f)) return h;
   rd = rd / 0.9f;
   const float q = (rd < 0.25f) ? 0.0f : (rd < 0.5f) ? 0.25f : (rd < 0.75f) ? 0.5f : 0.75f;
@@ -163,27 +173,25 @@ __device__ __forceinline__ uint32_t recttri_move(uint32_t h, float rd, const RTW
 }
 
 // updateTiles1 (recttrianglekernel.cl:120-132): rebuild a vertex of colour
-// tr+1 from the rotated colour tr.
-__device__ __forceinline__ uint32_t recttri_rebuild1(const int32_t* a, int M, int i, int j,
-                                                     int tr) {
+// tr+1 from the rotated colour tr; p points at cell (i,j) of colour tr.
+__device__ __forceinline__ uint32_t recttri_rebuild1(const int32_t* p, int rs, int i, int tr) {
   const int p1 = (i % 3) == tr;
   const int p2 = (i % 3) == ((tr + 1) % 3);
-  const uint32_t u = (uint32_t)a[(i - 1) * M + j];
-  const uint32_t r = (uint32_t)a[i * M + j + p1];
-  const uint32_t d = (uint32_t)a[(i + 1) * M + j - p2];
+  const uint32_t u = (uint32_t)p[-rs];
+  const uint32_t r = (uint32_t)p[p1];
+  const uint32_t d = (uint32_t)p[rs - p2];
   return ((u & 0xffu) << 16) | ((r & 0xf00000u) >> 8) | ((d & 0x0ff000u) >> 8) |
          ((r & 0x000f00u) >> 8);
 }
 
 // updateTiles2 (recttrianglekernel.cl:134-146): rebuild a vertex of colour
 // tr+2 from the rotated colour tr.
-__device__ __forceinline__ uint32_t recttri_rebuild2(const int32_t* a, int M, int i, int j,
-                                                     int tr) {
+__device__ __forceinline__ uint32_t recttri_rebuild2(const int32_t* p, int rs, int i, int tr) {
   const int p1 = (i % 3) == ((tr + 2) % 3);
   const int p2 = (i % 3) == ((tr + 1) % 3);
-  const uint32_t l = (uint32_t)a[i * M + j - p1];
-  const uint32_t u = (uint32_t)a[(i - 1) * M + j + p2];
-  const uint32_t d = (uint32_t)a[(i + 1) * M + j];
+  const uint32_t l = (uint32_t)p[-p1];
+  const uint32_t u = (uint32_t)p[-rs + p2];
+  const uint32_t d = (uint32_t)p[rs];
   return ((l & 0x00f000u) << 8) | ((u & 0x000ff0u) << 8) | ((l & 0x00000fu) << 8) |
          ((d & 0xff0000u) >> 16);
 }

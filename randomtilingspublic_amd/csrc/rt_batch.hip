@@ -1,0 +1,468 @@
+// rt_batch.hip -- batched CFTP (many master seeds per launch sequence) and
+// batched domino walks on uneven domains, on top of the fused engine.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <random>
+#include <thread>
+#include <vector>
+
+#include "rt_fused.hpp"
+
+namespace rt {
+
+// Per-unit coalescence test: flags[u] |= 1 if chain 0 and chain 1 of unit u
+// differ anywhere in their `bytes`-long plane blocks.
+__global__ void k_differ_units(const uint8_t* const* tops, const uint8_t* const* bots, long bytes,
+                               uint32_t* flags) {
+  const int u = blockIdx.y;
+  const uint8_t* a = tops[u];
+  const uint8_t* b = bots[u];
+  bool diff = false;
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < bytes; k += stride)
+    diff |= a[k] != b[k];
+  if (__any(diff) && (threadIdx.x & (warpSize - 1)) == 0) atomicOr(&flags[u], 1u);
+}
+
+__global__ void k_zero_u32(uint32_t* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+template <class F>
+static void parallel_for(int n, F f) {
+  const int nt = std::max(1, std::min<int>(n, (int)std::thread::hardware_concurrency()));
+  if (nt <= 1) {
+    for (int i = 0; i < n; i++) f(i);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (int t = 0; t < nt; t++)
+    th.emplace_back([&, t] {
+      for (int i = t; i < n; i += nt) f(i);
+    });
+  for (auto& x : th) x.join();
+}
+
+namespace {
+
+// Host split / merge between the int32 N x N tiling and the per-colour
+// byte planes (domino: DominoTiler.cpp:24-33, lozenge: LozengeTiler.cpp:38-48).
+void split_planes(int model, const int32_t* t, int N, uint8_t* planes) {
+  if (model == M_DOMINO) {
+    const int H = N / 2;
+    for (int i = 0; i < N; i++)
+      for (int j = 0; j < N; j++)
+        planes[(size_t)((i + j) % 2) * N * H + (size_t)i * H + j / 2] = (uint8_t)t[(size_t)i * N + j];
+  } else {
+    const int M = N / 3;
+    for (int i = 0; i < N; i++)
+      for (int j = 0; j < N; j++)
+        planes[(size_t)(((j - i) % 3 + 3) % 3) * N * M + (size_t)i * M + j / 3] =
+            (uint8_t)t[(size_t)i * N + j];
+  }
+}
+
+void merge_planes(int model, int32_t* t, int N, const uint8_t* planes) {
+  if (model == M_DOMINO) {
+    const int H = N / 2;
+    for (int i = 0; i < N; i++)
+      for (int j = 0; j < N; j++)
+        t[(size_t)i * N + j] =
+            (int8_t)planes[(size_t)((i + j) % 2) * N * H + (size_t)i * H + j / 2];
+  } else {
+    const int M = N / 3;
+    for (int i = 0; i < N; i++)
+      for (int j = 0; j < N; j++)
+        t[(size_t)i * N + j] =
+            (int8_t)planes[(size_t)(((j - i) % 3 + 3) % 3) * N * M + (size_t)i * M + j / 3];
+  }
+}
+
+struct Geometry {
+  int ncol;
+  long cols;
+  long plane;     // cells per colour plane
+  long nstream;   // TinyMT streams
+};
+
+Geometry geometry_of(int model, int N) {
+  Geometry g;
+  if (model == M_DOMINO) {
+    g.ncol = 2;
+    g.cols = N / 2;
+    g.nstream = (long)(N - 2) * (N / 2 - 2);
+  } else {
+    g.ncol = 3;
+    g.cols = N / 3;
+    g.nstream = (long)N * (N / 3);
+  }
+  g.plane = (long)N * g.cols;
+  return g;
+}
+
+struct Sample {
+  std::mt19937 master;
+  std::vector<long> seeds;                    // newest first, like the segment list
+  std::vector<std::vector<uint8_t>> colours;  // per segment, oldest first
+  bool done = false;
+};
+
+}  // namespace
+
+static int cftp_batch(rt_ctx* ctx, int model, const int32_t* tmax, const int32_t* tmin, int N,
+                      const uint32_t* masters, int ns, int sched, long t0, int max_rounds,
+                      const ModelParams& mp, int32_t* out, rt_cftp_stats* stats) {
+  if (!tmax || !tmin || !out || !masters || ns <= 0) {
+    ctx->err = "bad batch arguments";
+    return RT_EINVAL;
+  }
+  if (model == M_DOMINO ? (N < 6 || N % 2) : (N < 9 || N % 3)) {
+    ctx->err = "bad tiling side for the model";
+    return RT_EINVAL;
+  }
+  if (t0 <= 0 || max_rounds <= 0 || (sched != RT_SCHED_REFERENCE && sched != RT_SCHED_DOUBLING)) {
+    ctx->err = "bad CFTP schedule";
+    return RT_EINVAL;
+  }
+  const Geometry g = geometry_of(model, N);
+  int rc = ctx->check_params(model == M_DOMINO ? (size_t)N * N / 2 : (size_t)g.plane);
+  if (rc) return rc;
+  HIPCHK_RET(ctx, hipSetDevice(ctx->device));
+  const size_t pbytes = (size_t)g.ncol * g.plane;          // one chain's planes
+  const size_t sbytes = 4 * (size_t)g.nstream * 4;         // one sample's TinyMT state
+  const size_t per = sbytes + 2 * pbytes;
+  const size_t head = 2 * pbytes + 64 * 1024 + (size_t)ns * 4 * sizeof(void*) + (size_t)ns * 4;
+  rc = ctx->reserve(head + (size_t)ns * ((per + 255) & ~(size_t)255) + 4096);
+  if (rc) return rc;
+  uint8_t* base = (uint8_t*)ctx->arena;
+  uint8_t* init_top = base;
+  uint8_t* init_bot = base + pbytes;
+  uint8_t* cur = base + 2 * pbytes;
+  cur = (uint8_t*)(((uintptr_t)cur + 255) & ~(uintptr_t)255);
+  uint32_t* d_flags = (uint32_t*)cur;
+  cur += ((size_t)ns * 4 + 255) & ~(size_t)255;
+  const uint8_t** d_tops = (const uint8_t**)cur;
+  cur += ((size_t)ns * sizeof(void*) + 255) & ~(size_t)255;
+  const uint8_t** d_bots = (const uint8_t**)cur;
+  cur += ((size_t)ns * sizeof(void*) + 255) & ~(size_t)255;
+  std::vector<uint32_t*> state(ns);
+  std::vector<uint8_t*> top(ns), bot(ns);
+  for (int k = 0; k < ns; k++) {
+    state[k] = (uint32_t*)cur;
+    top[k] = cur + sbytes;
+    bot[k] = cur + sbytes + pbytes;
+    cur += (per + 255) & ~(size_t)255;
+  }
+  {
+    std::vector<uint8_t> h(pbytes, 0);
+    split_planes(model, tmax, N, h.data());
+    HIPCHK_RET(ctx, hipMemcpy(init_top, h.data(), pbytes, hipMemcpyHostToDevice));
+    std::fill(h.begin(), h.end(), 0);
+    split_planes(model, tmin, N, h.data());
+    HIPCHK_RET(ctx, hipMemcpy(init_bot, h.data(), pbytes, hipMemcpyHostToDevice));
+  }
+  std::vector<Sample> S(ns);
+  for (int k = 0; k < ns; k++) {
+    S[k].master.seed(masters[k]);
+    S[k].seeds.push_back((long)S[k].master());
+  }
+  std::vector<long> steps{t0};
+  auto gen = [&](int k, long seed, long n) {
+    std::vector<uint8_t> c;
+    if (model == M_DOMINO) domino_schedule((uint32_t)seed, n, false, nullptr, c);
+    else tri_schedule(seed, n, c);
+    S[k].colours.push_back(std::move(c));
+  };
+  parallel_for(ns, [&](int k) { gen(k, S[k].seeds[0], t0); });
+
+  rt_timing acc{};
+  double ms_total = 0;
+  long run = 0;
+  int remaining = ns;
+  for (int r = 1; r <= max_rounds && remaining > 0; r++) {
+    long tot = 0;
+    for (long s : steps) tot += s;
+    std::vector<int> act;
+    for (int k = 0; k < ns; k++)
+      if (!S[k].done) act.push_back(k);
+    FusedJob job;
+    job.model = model;
+    job.nc = 2;
+    job.mp = mp;
+    job.seg_steps = steps;
+    job.units.resize(act.size());
+    job.seg_seeds.resize(act.size());
+    job.ops.resize(act.size());
+    job.hist.resize(act.size());
+    parallel_for((int)act.size(), [&](int u) {
+      const int k = act[u];
+      UnitDesc d{};
+      for (int c = 0; c < g.ncol; c++) {
+        d.plane[0][c] = top[k] + (size_t)c * g.plane;
+        d.plane[1][c] = bot[k] + (size_t)c * g.plane;
+      }
+      d.state = state[k];
+      d.param = ctx->d_param;
+      d.nstream = g.nstream;
+      d.pstride = (long)ctx->nparam;
+      d.rows = N;
+      d.cols = (int)g.cols;
+      job.units[u] = d;
+      job.hist[u].model = model;
+      job.ops[u].reserve(tot);
+      // segments newest first; colours are stored oldest first
+      const int nseg = (int)S[k].seeds.size();
+      for (int s = 0; s < nseg; s++) {
+        job.seg_seeds[u].push_back((uint32_t)S[k].seeds[s]);
+        job.hist[u].append(S[k].colours[nseg - 1 - s], job.ops[u]);
+      }
+    });
+    rc = ctx->reserve_aux(fused_scratch_bytes(job));
+    if (rc) return rc;
+    ctx->last = rt_timing{};
+    HIPCHK_RET(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    for (int k : act) {
+      HIPCHK_RET(ctx, hipMemcpyAsync(top[k], init_top, pbytes, hipMemcpyDeviceToDevice, ctx->stream));
+      HIPCHK_RET(ctx, hipMemcpyAsync(bot[k], init_bot, pbytes, hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    FusedPlan plan;
+    rc = fused_upload(ctx, job, ctx->aux, &plan);
+    if (rc) return rc;
+    HIPCHK_RET(ctx, hipEventRecord(ctx->evk0, ctx->stream));
+    rc = fused_launch(ctx, job, plan);
+    if (rc) return rc;
+    HIPCHK_RET(ctx, hipEventRecord(ctx->evk1, ctx->stream));
+    rc = fused_sync(ctx, job);
+    if (rc) return rc;
+    std::vector<const uint8_t*> ht(act.size()), hb(act.size());
+    for (size_t u = 0; u < act.size(); u++) {
+      ht[u] = top[act[u]];
+      hb[u] = bot[act[u]];
+    }
+    HIPCHK_RET(ctx, hipMemcpyAsync(d_tops, ht.data(), ht.size() * sizeof(void*),
+                                   hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK_RET(ctx, hipMemcpyAsync(d_bots, hb.data(), hb.size() * sizeof(void*),
+                                   hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)((act.size() + 255) / 256)), dim3(256), 0,
+                       ctx->stream, d_flags, (int)act.size());
+    const unsigned bx = (unsigned)std::min<long>((long)(pbytes + 1023) / 1024, 64);
+    hipLaunchKernelGGL(k_differ_units, dim3(bx, (unsigned)act.size()), dim3(256), 0, ctx->stream,
+                       d_tops, d_bots, (long)pbytes, d_flags);
+    HIPCHK_RET(ctx, hipGetLastError());
+    std::vector<uint32_t> flags(act.size());
+    HIPCHK_RET(ctx, hipMemcpyAsync(flags.data(), d_flags, flags.size() * 4, hipMemcpyDeviceToHost,
+                                   ctx->stream));
+    HIPCHK_RET(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    HIPCHK_RET(ctx, hipEventSynchronize(ctx->ev1));
+    float ms = 0;
+    HIPCHK_RET(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ms_total += ms;
+    float kms = 0;
+    if (!plan.windows.empty()) HIPCHK_RET(ctx, hipEventElapsedTime(&kms, ctx->evk0, ctx->evk1));
+    acc.kernel_ms += kms;
+    acc.kernel_launches += (long)plan.windows.size();
+    acc.kernel_sites += (long)plan.windows.size() * plan.streams;
+    acc.launches += ctx->last.launches;
+    acc.steps += ctx->last.steps;
+    acc.attempts += ctx->last.attempts;
+    run += tot;
+    std::vector<uint8_t> h(pbytes);
+    for (size_t u = 0; u < act.size(); u++) {
+      const int k = act[u];
+      if (flags[u]) continue;
+      HIPCHK_RET(ctx, hipMemcpy(h.data(), top[k], pbytes, hipMemcpyDeviceToHost));
+      merge_planes(model, out + (size_t)k * N * N, N, h.data());
+      S[k].done = true;
+      remaining--;
+      if (stats) {
+        stats[k].rounds = r;
+        stats[k].total_steps = tot;
+        stats[k].steps_run = run;
+        stats[k].device_ms = ms_total;
+      }
+    }
+    if (remaining == 0 || r == max_rounds) break;
+    const long ns_len = sched == RT_SCHED_REFERENCE ? steps.back() * 2 : tot;
+    steps.insert(steps.begin(), ns_len);
+    for (int k = 0; k < ns; k++)
+      if (!S[k].done) S[k].seeds.insert(S[k].seeds.begin(), (long)S[k].master());
+    std::vector<int> still;
+    for (int k = 0; k < ns; k++)
+      if (!S[k].done) still.push_back(k);
+    parallel_for((int)still.size(), [&](int i) {
+      const int k = still[i];
+      gen(k, S[k].seeds[0], ns_len);
+    });
+  }
+  acc.device_ms = ms_total;
+  ctx->last = acc;
+  if (remaining > 0) {
+    for (int k = 0; k < ns; k++)
+      if (!S[k].done && stats) {
+        stats[k].rounds = max_rounds;
+        stats[k].total_steps = 0;
+        stats[k].steps_run = run;
+        stats[k].device_ms = ms_total;
+      }
+    ctx->err = "CFTP did not coalesce within max_rounds for every sample";
+    return RT_ENOCOUPLE;
+  }
+  return RT_OK;
+}
+
+}  // namespace rt
+
+using namespace rt;
+
+extern "C" int rt_lozenge_cftp_batch(rt_ctx* ctx, const int32_t* tmax, const int32_t* tmin, int N,
+                                     const uint32_t* masters, int nsamples, int sched, long t0,
+                                     int max_rounds, int32_t* out, rt_cftp_stats* stats) {
+  if (!ctx) return RT_EINVAL;
+  ModelParams mp{};
+  return cftp_batch(ctx, M_LOZENGE, tmax, tmin, N, masters, nsamples, sched, t0, max_rounds, mp,
+                    out, stats);
+}
+
+extern "C" int rt_domino_cftp_batch(rt_ctx* ctx, const int32_t* tmax, const int32_t* tmin, int N,
+                                    const uint32_t* masters, int nsamples, int sched, long t0,
+                                    int max_rounds, const rt_domino_rule* rule, int32_t* out,
+                                    rt_cftp_stats* stats) {
+  if (!ctx) return RT_EINVAL;
+  if (rule && (rule->kind < 0 || rule->kind > 3)) {
+    ctx->err = "unknown domino rule";
+    return RT_EINVAL;
+  }
+  ModelParams mp{};
+  mp.rule = rule ? rule->kind : RT_DOMINO_HEATBATH;
+  const double a = rule ? rule->a : 1.0, b = rule ? rule->b : 1.0;
+  mp.dw.wa2 = (float)(a * a);
+  mp.dw.wb2 = (float)(b * b);
+  return cftp_batch(ctx, M_DOMINO, tmax, tmin, N, masters, nsamples, sched, t0, max_rounds, mp, out,
+                    stats);
+}
+
+extern "C" int rt_domino_walk_batch(rt_ctx* ctx, int32_t* const* tilings, const int* Ns,
+                                    int nchains, long steps, const long* seeds,
+                                    const rt_domino_rule* rule, int use_graph) {
+  if (!ctx) return RT_EINVAL;
+  if (!tilings || !Ns || !seeds || nchains <= 0 || steps < 0) {
+    ctx->err = "bad batch arguments";
+    return RT_EINVAL;
+  }
+  if (rule && (rule->kind < 0 || rule->kind > 3)) {
+    ctx->err = "unknown domino rule";
+    return RT_EINVAL;
+  }
+  size_t need = 0, bytes = 0;
+  for (int k = 0; k < nchains; k++) {
+    const int N = Ns[k];
+    if (!tilings[k] || N < 6 || N % 2) {
+      ctx->err = "chain " + std::to_string(k) + ": side must be even and >= 6";
+      return RT_EINVAL;
+    }
+    need = std::max(need, (size_t)N * N / 2);
+    const Geometry g = geometry_of(M_DOMINO, N);
+    bytes += ((4 * (size_t)g.nstream * 4 + 2 * (size_t)g.plane) + 255) & ~(size_t)255;
+  }
+  int rc = ctx->check_params(need);
+  if (rc) return rc;
+  HIPCHK_RET(ctx, hipSetDevice(ctx->device));
+  rc = ctx->reserve(bytes + 4096);
+  if (rc) return rc;
+  FusedJob job;
+  job.model = M_DOMINO;
+  job.nc = 1;
+  job.mp.rule = rule ? rule->kind : RT_DOMINO_TOGGLE08;
+  const double a = rule ? rule->a : 1.0, b = rule ? rule->b : 1.0;
+  job.mp.dw.wa2 = (float)(a * a);
+  job.mp.dw.wb2 = (float)(b * b);
+  job.seg_steps = {steps};
+  job.units.resize(nchains);
+  job.seg_seeds.resize(nchains);
+  job.ops.resize(nchains);
+  job.hist.resize(nchains);
+  std::vector<uint8_t*> planes(nchains);
+  uint8_t* cur = (uint8_t*)ctx->arena;
+  for (int k = 0; k < nchains; k++) {
+    const Geometry g = geometry_of(M_DOMINO, Ns[k]);
+    UnitDesc d{};
+    d.state = (uint32_t*)cur;
+    planes[k] = cur + 4 * (size_t)g.nstream * 4;
+    d.plane[0][0] = planes[k];
+    d.plane[0][1] = planes[k] + g.plane;
+    d.param = ctx->d_param;
+    d.nstream = g.nstream;
+    d.pstride = (long)ctx->nparam;
+    d.rows = Ns[k];
+    d.cols = (int)g.cols;
+    job.units[k] = d;
+    cur += ((4 * (size_t)g.nstream * 4 + 2 * (size_t)g.plane) + 255) & ~(size_t)255;
+  }
+  parallel_for(nchains, [&](int k) {
+    std::vector<uint8_t> colours;
+    uint32_t base = 0;
+    domino_schedule((uint32_t)seeds[k], steps, true, &base, colours);
+    job.seg_seeds[k] = {base};
+    job.hist[k].model = M_DOMINO;
+    job.hist[k].append(colours, job.ops[k]);
+  });
+  for (int k = 0; k < nchains; k++) {
+    const Geometry g = geometry_of(M_DOMINO, Ns[k]);
+    std::vector<uint8_t> h(2 * (size_t)g.plane);
+    split_planes(M_DOMINO, tilings[k], Ns[k], h.data());
+    HIPCHK_RET(ctx, hipMemcpy(planes[k], h.data(), h.size(), hipMemcpyHostToDevice));
+  }
+  rc = ctx->reserve_aux(fused_scratch_bytes(job));
+  if (rc) return rc;
+  FusedPlan plan;
+  rc = fused_upload(ctx, job, ctx->aux, &plan);
+  if (rc) return rc;
+  ctx->last = rt_timing{};
+  if (use_graph) {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    HIPCHK_RET(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    int lrc = fused_launch(ctx, job, plan);
+    if (!lrc) lrc = fused_sync(ctx, job);
+    hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+    if (lrc) {
+      if (graph) hipGraphDestroy(graph);
+      return lrc;
+    }
+    HIPCHK_RET(ctx, e);
+    HIPCHK_RET(ctx, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    HIPCHK_RET(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    HIPCHK_RET(ctx, hipGraphLaunch(exec, ctx->stream));
+    HIPCHK_RET(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    HIPCHK_RET(ctx, hipEventSynchronize(ctx->ev1));
+    hipGraphExecDestroy(exec);
+    hipGraphDestroy(graph);
+  } else {
+    HIPCHK_RET(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    rc = fused_launch(ctx, job, plan);
+    if (rc) return rc;
+    HIPCHK_RET(ctx, hipEventRecord(ctx->evk1, ctx->stream));
+    rc = fused_sync(ctx, job);
+    if (rc) return rc;
+    HIPCHK_RET(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    HIPCHK_RET(ctx, hipEventSynchronize(ctx->ev1));
+    float kms = 0;
+    HIPCHK_RET(ctx, hipEventElapsedTime(&kms, ctx->ev0, ctx->evk1));
+    ctx->last.kernel_ms = kms;
+    ctx->last.kernel_launches = (long)plan.windows.size();
+    ctx->last.kernel_sites = (long)plan.windows.size() * plan.streams;
+  }
+  float ms = 0;
+  HIPCHK_RET(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  ctx->last.device_ms = ms;
+  for (int k = 0; k < nchains; k++) {
+    const Geometry g = geometry_of(M_DOMINO, Ns[k]);
+    std::vector<uint8_t> h(2 * (size_t)g.plane);
+    HIPCHK_RET(ctx, hipMemcpy(h.data(), planes[k], h.size(), hipMemcpyDeviceToHost));
+    merge_planes(M_DOMINO, tilings[k], Ns[k], h.data());
+  }
+  return RT_OK;
+}

@@ -24,6 +24,7 @@
 
 #include "models_dev.hpp"
 #include "rt_internal.hpp"
+#include "rt_fused.hpp"
 #include "rt_kernels.hpp"
 #include "tinymt_dev.hpp"
 
@@ -69,14 +70,14 @@ __global__ void k_domino_step(DominoArrays c0, DominoArrays c1, StreamTable S, i
   if (gid >= P) return;
   const int i = (int)(gid / Wd) + 1, j = (int)(gid % Wd) + 1;
   Tiny t = tiny_load(S, gid);
-  const float rd = tiny_float01(t);
+  const uint32_t rd = tiny_bits(t);
   tiny_store(S, gid, t);
   const int tpar = 1 - colour;
 #pragma unroll
   for (int c = 0; c < NC; c++) {
     const DominoArrays& A = c == 0 ? c0 : c1;
     uint8_t* X = A.col[colour];
-    uint32_t v = pull ? domino_rebuild(A.col[1 - colour], H, i, j, 1 - tpar)
+    uint32_t v = pull ? domino_rebuild(A.col[1 - colour] + i * H + j, H, i, 1 - tpar)
                       : (uint32_t)X[i * H + j];
     X[i * H + j] = (uint8_t)domino_move(v, rd, rule, i, tpar, w);
   }
@@ -92,7 +93,7 @@ __global__ void k_domino_sync(DominoArrays c0, DominoArrays c1, int N, int last)
 #pragma unroll
   for (int c = 0; c < NC; c++) {
     const DominoArrays& A = c == 0 ? c0 : c1;
-    A.col[1 - last][i * H + j] = (uint8_t)domino_rebuild(A.col[last], H, i, j, 1 - last);
+    A.col[1 - last][i * H + j] = (uint8_t)domino_rebuild(A.col[last] + i * H + j, H, i, 1 - last);
   }
 }
 
@@ -108,7 +109,7 @@ __global__ void k_lozenge_step(TriArrays8 c0, TriArrays8 c1, StreamTable S, int 
   if (gid >= P) return;
   const int i = (int)(gid / M), j = (int)(gid % M);
   Tiny t = tiny_load(S, gid);
-  const float rd = tiny_float01(t);
+  const uint32_t rd = tiny_bits(t);
   tiny_store(S, gid, t);
   const bool interior = i >= 1 && i <= N - 2 && j >= 1 && j <= M - 2;
   const int ca = (colour + 2) % 3, cb = (colour + 1) % 3;
@@ -118,8 +119,8 @@ __global__ void k_lozenge_step(TriArrays8 c0, TriArrays8 c1, StreamTable S, int 
     uint8_t* X = A.col[colour];
     uint32_t v = X[gid];
     if (interior) {
-      if (pull_a) v = lozenge_pull_prev(v, A.col[ca], M, i, j, ca);
-      if (pull_b) v = lozenge_pull_prev2(v, A.col[cb], M, i, j, cb);
+      if (pull_a) v = lozenge_pull_prev(v, A.col[ca] + gid, M, i, ca);
+      if (pull_b) v = lozenge_pull_prev2(v, A.col[cb] + gid, M, i, cb);
     }
     X[gid] = (uint8_t)lozenge_move(v, rd);
   }
@@ -139,8 +140,8 @@ __global__ void k_lozenge_sync(TriArrays8 c0, TriArrays8 c1, int N, int colour, 
   for (int c = 0; c < NC; c++) {
     const TriArrays8& A = c == 0 ? c0 : c1;
     uint32_t v = A.col[colour][gid];
-    if (pull_a) v = lozenge_pull_prev(v, A.col[ca], M, i, j, ca);
-    if (pull_b) v = lozenge_pull_prev2(v, A.col[cb], M, i, j, cb);
+    if (pull_a) v = lozenge_pull_prev(v, A.col[ca] + gid, M, i, ca);
+    if (pull_b) v = lozenge_pull_prev2(v, A.col[cb] + gid, M, i, cb);
     A.col[colour][gid] = (uint8_t)v;
   }
 }
@@ -157,7 +158,7 @@ __global__ void k_recttri_step(TriArrays32 c0, TriArrays32 c1, StreamTable S, in
   if (gid >= P) return;
   const int i = (int)(gid / M), j = (int)(gid % M);
   Tiny t = tiny_load(S, gid);
-  const float rd = tiny_float01(t);
+  const uint32_t rd = tiny_bits(t);
   tiny_store(S, gid, t);
   const bool interior = i >= 1 && i <= N - 2 && j >= 1 && j <= M - 2;
 #pragma unroll
@@ -166,8 +167,8 @@ __global__ void k_recttri_step(TriArrays32 c0, TriArrays32 c1, StreamTable S, in
     int32_t* X = A.col[colour];
     uint32_t v = (uint32_t)X[gid];
     if (interior && prev >= 0 && prev != colour)
-      v = colour == (prev + 1) % 3 ? recttri_rebuild1(A.col[prev], M, i, j, prev)
-                                   : recttri_rebuild2(A.col[prev], M, i, j, prev);
+      v = colour == (prev + 1) % 3 ? recttri_rebuild1(A.col[prev] + gid, M, i, prev)
+                                   : recttri_rebuild2(A.col[prev] + gid, M, i, prev);
     X[gid] = (int32_t)recttri_move(v, rd, W);
   }
 }
@@ -183,22 +184,15 @@ __global__ void k_recttri_sync(TriArrays32 c0, TriArrays32 c1, int N, int last) 
 #pragma unroll
   for (int c = 0; c < NC; c++) {
     const TriArrays32& A = c == 0 ? c0 : c1;
-    A.col[(last + 1) % 3][gid] = (int32_t)recttri_rebuild1(A.col[last], M, i, j, last);
-    A.col[(last + 2) % 3][gid] = (int32_t)recttri_rebuild2(A.col[last], M, i, j, last);
+    A.col[(last + 1) % 3][gid] = (int32_t)recttri_rebuild1(A.col[last] + gid, M, i, last);
+    A.col[(last + 2) % 3][gid] = (int32_t)recttri_rebuild2(A.col[last] + gid, M, i, last);
   }
 }
 
 }  // namespace rt
 
 // ------------------------------------------------------------------ context
-#define HIPCHK(ctx, expr)                                                   \
-  do {                                                                      \
-    hipError_t e_ = (expr);                                                 \
-    if (e_ != hipSuccess) {                                                 \
-      (ctx)->err = std::string(#expr) + ": " + hipGetErrorString(e_);        \
-      return RT_EDEVICE;                                                    \
-    }                                                                       \
-  } while (0)
+#define HIPCHK HIPCHK_RET
 
 static inline unsigned blocks_for(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
@@ -214,6 +208,21 @@ int rt_ctx::reserve(size_t bytes) {
     return RT_ENOMEM;
   }
   arena_bytes = want;
+  return RT_OK;
+}
+
+int rt_ctx::reserve_aux(size_t bytes) {
+  if (bytes <= aux_bytes) return RT_OK;
+  if (aux) hipFree(aux);
+  aux = nullptr;
+  aux_bytes = 0;
+  size_t want = bytes + bytes / 4 + (1u << 20);
+  hipError_t e = hipMalloc(&aux, want);
+  if (e != hipSuccess) {
+    err = std::string("hipMalloc aux: ") + hipGetErrorString(e);
+    return RT_ENOMEM;
+  }
+  aux_bytes = want;
   return RT_OK;
 }
 
@@ -236,6 +245,7 @@ extern "C" int rt_create(rt_ctx** out, int device_ordinal) {
   c->device = device_ordinal;
   if (hipSetDevice(device_ordinal) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+      hipEventCreate(&c->evk0) != hipSuccess || hipEventCreate(&c->evk1) != hipSuccess ||
       hipMalloc(&c->d_flag, 64 * sizeof(uint32_t)) != hipSuccess ||
       hipHostMalloc(&c->h_flag, 64 * sizeof(uint32_t)) != hipSuccess) {
     delete c;
@@ -253,11 +263,14 @@ extern "C" int rt_create(rt_ctx** out, int device_ordinal) {
 rt_ctx::~rt_ctx() {
   if (stream) hipStreamSynchronize(stream);
   if (arena) hipFree(arena);
+  if (aux) hipFree(aux);
   if (d_param) hipFree(d_param);
   if (d_flag) hipFree(d_flag);
   if (h_flag) hipHostFree(h_flag);
   if (ev0) hipEventDestroy(ev0);
   if (ev1) hipEventDestroy(ev1);
+  if (evk0) hipEventDestroy(evk0);
+  if (evk1) hipEventDestroy(evk1);
   if (stream) hipStreamDestroy(stream);
 }
 
@@ -486,24 +499,82 @@ int domino_walk_impl(rt_ctx* ctx, int32_t* tiling, int N, const Segments& seg,
 
 }  // namespace
 
+int rt::launch_domino_sync(rt_ctx* ctx, DominoArrays* a, int nc, int N, int last) {
+  const long P = (long)(N - 2) * (N / 2 - 2);
+  if (nc == 1)
+    hipLaunchKernelGGL(k_domino_sync<1>, dim3(blocks_for(P, 256)), dim3(256), 0, ctx->stream,
+                       a[0], a[0], N, last);
+  else
+    hipLaunchKernelGGL(k_domino_sync<2>, dim3(blocks_for(P, 256)), dim3(256), 0, ctx->stream,
+                       a[0], a[1], N, last);
+  ctx->last.launches += 1;
+  HIPCHK(ctx, hipGetLastError());
+  return RT_OK;
+}
+
+int rt::launch_recttri_sync(rt_ctx* ctx, TriArrays32* a, int nc, int N, int last) {
+  const long P = (long)N * (N / 3);
+  if (nc == 1)
+    hipLaunchKernelGGL(k_recttri_sync<1>, dim3(blocks_for(P, 256)), dim3(256), 0, ctx->stream,
+                       a[0], a[0], N, last);
+  else
+    hipLaunchKernelGGL(k_recttri_sync<2>, dim3(blocks_for(P, 256)), dim3(256), 0, ctx->stream,
+                       a[0], a[1], N, last);
+  ctx->last.launches += 1;
+  HIPCHK(ctx, hipGetLastError());
+  return RT_OK;
+}
+
 int rt::domino_fused_walk(rt_ctx* ctx, DominoArrays* a, int nc, int N, int rule, DominoWeights w,
                           const std::vector<long>& steps, const std::vector<long>& seeds,
                           bool single, uint32_t* state) {
-  // Engine 0/2 entry point; falls back to the per-step kernels for shapes
-  // the fused kernel does not cover.
-  DominoRun R;
-  R.ctx = ctx;
-  R.N = N;
-  R.a[0] = a[0];
-  R.a[1] = nc > 1 ? a[1] : a[0];
-  R.nc = nc;
-  R.rule = rule;
-  R.w = w;
-  Segments s;
-  s.steps = steps;
-  s.seeds = seeds;
-  s.single = single;
-  return domino_device_walk(R, s, state);
+  if (ctx->engine == 1) {
+    DominoRun R;
+    R.ctx = ctx;
+    R.N = N;
+    R.a[0] = a[0];
+    R.a[1] = nc > 1 ? a[1] : a[0];
+    R.nc = nc;
+    R.rule = rule;
+    R.w = w;
+    Segments s;
+    s.steps = steps;
+    s.seeds = seeds;
+    s.single = single;
+    return domino_device_walk(R, s, state);
+  }
+  FusedJob job;
+  job.model = M_DOMINO;
+  job.nc = nc;
+  job.mp.rule = rule;
+  job.mp.dw = w;
+  UnitDesc u{};
+  for (int ch = 0; ch < nc; ch++)
+    for (int k = 0; k < 2; k++) u.plane[ch][k] = a[ch].col[k];
+  u.state = state;
+  u.param = ctx->d_param;
+  u.nstream = (long)(N - 2) * (N / 2 - 2);
+  u.pstride = (long)ctx->nparam;
+  u.rows = N;
+  u.cols = N / 2;
+  job.units.push_back(u);
+  job.seg_steps = steps;
+  job.seg_seeds.resize(1);
+  job.ops.resize(1);
+  job.hist.resize(1);
+  job.hist[0].model = M_DOMINO;
+  std::vector<uint8_t> colours;
+  for (size_t k = 0; k < steps.size(); k++) {
+    uint32_t base = 0;
+    domino_schedule((uint32_t)seeds[k], steps[k], single, &base, colours);
+    job.seg_seeds[0].push_back(single ? base : (uint32_t)seeds[k]);
+    job.hist[0].append(colours, job.ops[0]);
+  }
+  int rc = ctx->reserve_aux(fused_scratch_bytes(job));
+  if (rc) return rc;
+  rc = fused_run(ctx, job, ctx->aux);
+  if (rc) return rc;
+  return fused_sync(ctx, job);
 }
 
 extern "C" int rt_domino_walk(rt_ctx* ctx, int32_t* tiling, int N, long steps, long seed,
@@ -568,6 +639,37 @@ void tri_merge8(int32_t* t, int N, const uint8_t* host) {
 int rt::lozenge_device_walk(rt_ctx* ctx, TriArrays8* a, int nc, int N,
                             const std::vector<long>& steps, const std::vector<long>& seeds,
                             uint32_t* state) {
+  if (ctx->engine != 1) {
+    FusedJob job;
+    job.model = M_LOZENGE;
+    job.nc = nc;
+    UnitDesc u{};
+    for (int ch = 0; ch < nc; ch++)
+      for (int k = 0; k < 3; k++) u.plane[ch][k] = a[ch].col[k];
+    u.state = state;
+    u.param = ctx->d_param;
+    u.nstream = (long)N * (N / 3);
+    u.pstride = (long)ctx->nparam;
+    u.rows = N;
+    u.cols = N / 3;
+    job.units.push_back(u);
+    job.seg_steps = steps;
+    job.seg_seeds.resize(1);
+    job.ops.resize(1);
+    job.hist.resize(1);
+    job.hist[0].model = M_LOZENGE;
+    std::vector<uint8_t> colours;
+    for (size_t k = 0; k < steps.size(); k++) {
+      tri_schedule(seeds[k], steps[k], colours);
+      job.seg_seeds[0].push_back((uint32_t)seeds[k]);
+      job.hist[0].append(colours, job.ops[0]);
+    }
+    int rc = ctx->reserve_aux(fused_scratch_bytes(job));
+    if (rc) return rc;
+    rc = fused_run(ctx, job, ctx->aux);
+    if (rc) return rc;
+    return fused_sync(ctx, job);
+  }
   const long P = (long)N * (N / 3);
   StreamTable S = stream_table(ctx, state, P);
   long last[3] = {-1, -1, -1};
@@ -676,6 +778,39 @@ extern "C" int rt_lozenge_walk_segments(rt_ctx* ctx, int32_t* tiling, int N, con
 int rt::recttri_device_walk(rt_ctx* ctx, TriArrays32* a, int nc, int N,
                             const std::vector<long>& steps, const std::vector<long>& seeds,
                             const RTWeights& W, uint32_t* state, int* last_out) {
+  if (ctx->engine != 1) {
+    FusedJob job;
+    job.model = M_RECTTRI;
+    job.nc = nc;
+    job.mp.rw = W;
+    UnitDesc u{};
+    for (int ch = 0; ch < nc; ch++)
+      for (int k = 0; k < 3; k++) u.plane[ch][k] = a[ch].col[k];
+    u.state = state;
+    u.param = ctx->d_param;
+    u.nstream = (long)N * (N / 3);
+    u.pstride = (long)ctx->nparam;
+    u.rows = N;
+    u.cols = N / 3;
+    job.units.push_back(u);
+    job.seg_steps = steps;
+    job.seg_seeds.resize(1);
+    job.ops.resize(1);
+    job.hist.resize(1);
+    job.hist[0].model = M_RECTTRI;
+    std::vector<uint8_t> colours;
+    for (size_t k = 0; k < steps.size(); k++) {
+      tri_schedule(seeds[k], steps[k], colours);
+      job.seg_seeds[0].push_back((uint32_t)seeds[k]);
+      job.hist[0].append(colours, job.ops[0]);
+    }
+    int rc = ctx->reserve_aux(fused_scratch_bytes(job));
+    if (rc) return rc;
+    rc = fused_run(ctx, job, ctx->aux);
+    if (rc) return rc;
+    if (last_out) *last_out = job.hist[0].prev;
+    return fused_sync(ctx, job);
+  }
   const long P = (long)N * (N / 3);
   StreamTable S = stream_table(ctx, state, P);
   int prev = -1;

@@ -1,0 +1,364 @@
+// rt_fused.hip -- fused sweep-block kernel (K colour steps per launch,
+// cells in LDS, TinyMT states in VGPRs) and its host driver.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "rt_fused.hpp"
+
+namespace rt {
+
+constexpr int kLW = 64;        // loaded tile width: one column per lane
+constexpr int kKMax = 64;      // steps per window
+
+template <int MODEL>
+struct CellOf {
+  using type = uint8_t;
+};
+template <>
+struct CellOf<M_RECTTRI> {
+  using type = int32_t;
+};
+
+template <int MODEL>
+__device__ __forceinline__ bool has_stream(int i, int j, int rows, int cols) {
+  if (MODEL == M_DOMINO) return i >= 1 && i <= rows - 2 && j >= 1 && j <= cols - 2;
+  return i >= 0 && i < rows && j >= 0 && j < cols;
+}
+
+// TinyMT stream index = the reference's NDRange sequential id
+// (tinymt.clh:23-30): domino NDRange (N-2, N/2-2) offset by one,
+// lozenge / rect-triangle NDRange (N, N/3).
+template <int MODEL>
+__device__ __forceinline__ long stream_id(int i, int j, int rows, int cols) {
+  if (MODEL == M_DOMINO) return (long)(j - 1) + (long)(cols - 2) * (i - 1);
+  return (long)j + (long)cols * i;
+}
+
+// W wave64s per workgroup; each lane owns one tile column and R rows
+// (rows wv, wv+W, ...), so the loaded tile is (W*R) x 64 cells.
+template <int MODEL, int NC, int W, int R>
+__global__ __launch_bounds__(64 * W) void k_fused(const UnitDesc* __restrict__ units,
+                                                    const WorkItem* __restrict__ work, long s0,
+                                                    int K, int init, int seg, int H,
+                                                    ModelParams mp) {
+  using Cell = typename CellOf<MODEL>::type;
+  constexpr int NCOL = MODEL == M_DOMINO ? 2 : 3;
+  constexpr int kThreads = 64 * W;
+  constexpr int LH = W * R;
+  constexpr int RS = kLW + 2;
+  constexpr int PL = (LH + 2) * RS;
+  __shared__ Cell lds[NC * NCOL * PL];
+  __shared__ uint8_t sops[kKMax];
+
+  const WorkItem wi = work[blockIdx.x];
+  const UnitDesc U = units[wi.unit];
+  const int rows = U.rows, cols = U.cols;
+  const int TH = LH - 2 * H, TW = kLW - 2 * H;
+  const int tr = wi.tile / U.tiles_c, tc = wi.tile % U.tiles_c;
+  const int r0 = tr * TH - H, c0 = tc * TW - H;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int gc = c0 + lane;
+  const bool col_in = gc >= 0 && gc < cols;
+
+  for (int k = threadIdx.x; k < NC * NCOL * PL; k += kThreads) lds[k] = 0;
+  if (threadIdx.x < K) sops[threadIdx.x] = U.ops[s0 + threadIdx.x];
+  __syncthreads();
+
+  Tiny st[R];
+  bool hs[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int lr = wv + W * r, gr = r0 + lr;
+    const bool in = col_in && gr >= 0 && gr < rows;
+    if (in) {
+#pragma unroll
+      for (int ch = 0; ch < NC; ch++)
+#pragma unroll
+        for (int k = 0; k < NCOL; k++)
+          lds[(ch * NCOL + k) * PL + (lr + 1) * RS + lane + 1] =
+              ((const Cell*)U.plane[ch][k])[(long)gr * cols + gc];
+    }
+    hs[r] = in && has_stream<MODEL>(gr, gc, rows, cols);
+    if (hs[r]) {
+      const long id = stream_id<MODEL>(gr, gc, rows, cols);
+      st[r].mat1 = U.param[id];
+      st[r].mat2 = U.param[U.pstride + id];
+      st[r].tmat = U.param[2 * U.pstride + id];
+      if (init) {
+        tiny_init(st[r], U.seeds[seg] + (uint32_t)id);
+      } else {
+        st[r].s0 = U.state[id];
+        st[r].s1 = U.state[U.nstream + id];
+        st[r].s2 = U.state[2 * U.nstream + id];
+        st[r].s3 = U.state[3 * U.nstream + id];
+      }
+    }
+  }
+  __syncthreads();
+
+  for (int s = 0; s < K; s++) {
+    const uint32_t op = sops[s];
+    const int c = op & OP_COLOUR;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int lr = wv + W * r, gr = r0 + lr;
+      uint32_t rd = 0;
+      if (hs[r]) rd = tiny_bits(st[r]);
+      const bool inter = col_in && gr >= 1 && gr <= rows - 2 && gc >= 1 && gc <= cols - 2;
+      const int li = (lr + 1) * RS + lane + 1;
+#pragma unroll
+      for (int ch = 0; ch < NC; ch++) {
+        Cell* P = lds + ch * NCOL * PL;
+        uint32_t v = (uint32_t)P[c * PL + li];
+        if (MODEL == M_DOMINO) {
+          if ((op & OP_PULL_A) && inter) v = domino_rebuild(P + (1 - c) * PL + li, RS, gr, c);
+          if (hs[r]) v = domino_move(v, rd, mp.rule, gr, 1 - c, mp.dw);
+        } else if (MODEL == M_LOZENGE) {
+          if (inter) {
+            const int ca = (c + 2) % 3, cb = (c + 1) % 3;
+            if (op & OP_PULL_A) v = lozenge_pull_prev(v, P + ca * PL + li, RS, gr, ca);
+            if (op & OP_PULL_B) v = lozenge_pull_prev2(v, P + cb * PL + li, RS, gr, cb);
+          }
+          if (hs[r]) v = lozenge_move(v, rd);
+        } else {
+          if ((op & OP_PULL_A) && inter) {
+            const int pv = (op >> 4) & 3;
+            const int32_t* src = (const int32_t*)(P + pv * PL + li);
+            v = c == (pv + 1) % 3 ? recttri_rebuild1(src, RS, gr, pv)
+                                  : recttri_rebuild2(src, RS, gr, pv);
+          }
+          if (hs[r]) v = recttri_move(v, rd, mp.rw);
+        }
+        P[c * PL + li] = (Cell)v;
+      }
+    }
+    __syncthreads();
+  }
+
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int lr = wv + W * r, gr = r0 + lr;
+    const bool inner = col_in && gr >= 0 && gr < rows && lr >= H && lr < H + TH && lane >= H &&
+                       lane < H + TW;
+    if (!inner) continue;
+#pragma unroll
+    for (int ch = 0; ch < NC; ch++)
+#pragma unroll
+      for (int k = 0; k < NCOL; k++)
+        ((Cell*)U.plane[ch][k])[(long)gr * cols + gc] =
+            lds[(ch * NCOL + k) * PL + (lr + 1) * RS + lane + 1];
+    if (hs[r]) {
+      const long id = stream_id<MODEL>(gr, gc, rows, cols);
+      U.state[id] = st[r].s0;
+      U.state[U.nstream + id] = st[r].s1;
+      U.state[2 * U.nstream + id] = st[r].s2;
+      U.state[3 * U.nstream + id] = st[r].s3;
+    }
+  }
+}
+
+// ------------------------------------------------------------- host side
+long OpBuilder::append(const std::vector<uint8_t>& colours, std::vector<uint8_t>& ops) {
+  long pulls = 0;
+  for (uint8_t c : colours) {
+    uint8_t op = c;
+    if (model == M_DOMINO) {
+      if (prev >= 0 && prev != c) op |= OP_PULL_A;
+    } else if (model == M_LOZENGE) {
+      const int ca = (c + 2) % 3, cb = (c + 1) % 3;
+      if (last[ca] > last[c]) op |= OP_PULL_A;
+      if (last[cb] > last[c]) op |= OP_PULL_B;
+      last[c] = g;
+    } else {
+      if (prev >= 0 && prev != c) op |= OP_PULL_A | (uint8_t)(prev << 4);
+    }
+    if (op & (OP_PULL_A | OP_PULL_B)) pulls++;
+    prev = c;
+    g++;
+    ops.push_back(op);
+  }
+  return pulls;
+}
+
+// Tile geometry per (model, chains): waves W, rows per thread R, halo H.
+// 64 x 64 tiles for byte cells; int32 rect-triangle pairs use 32 x 64 to
+// keep two chains' three planes within 64 KB of LDS.
+struct Geo {
+  int W, R, H;
+};
+
+static Geo geometry(int model, int nc) {
+  if (model == M_RECTTRI && nc == 2) return Geo{8, 4, 4};
+  return Geo{8, 8, 6};
+}
+
+int fused_rows_per_thread(const FusedJob& job) {
+  const Geo g = geometry(job.model, job.nc);
+  return g.W * g.R;
+}
+
+int fused_halo(const FusedJob& job) { return geometry(job.model, job.nc).H; }
+
+static void tile_counts(const UnitDesc& u, int LH, int H, int* tr, int* tc) {
+  const int TH = LH - 2 * H, TW = kLW - 2 * H;
+  *tr = (u.rows + TH - 1) / TH;
+  *tc = (u.cols + TW - 1) / TW;
+}
+
+size_t fused_scratch_bytes(const FusedJob& job) {
+  const int LH = fused_rows_per_thread(job), H = fused_halo(job);
+  size_t work = 0;
+  for (const UnitDesc& u : job.units) {
+    int tr, tc;
+    tile_counts(u, LH, H, &tr, &tc);
+    work += (size_t)tr * tc;
+  }
+  long T = 0;
+  for (long s : job.seg_steps) T += s;
+  const size_t nu = job.units.size();
+  return nu * sizeof(UnitDesc) + work * sizeof(WorkItem) + nu * (size_t)T +
+         nu * job.seg_steps.size() * sizeof(uint32_t) + 1024;
+}
+
+template <int MODEL, int NC, int W, int R>
+static void launch(rt_ctx* ctx, unsigned nblocks, const UnitDesc* units, const WorkItem* work,
+                   long s0, int K, int init, int seg, int H, const ModelParams& mp) {
+  hipLaunchKernelGGL((k_fused<MODEL, NC, W, R>), dim3(nblocks), dim3(64 * W), 0, ctx->stream,
+                     units, work, s0, K, init, seg, H, mp);
+}
+
+typedef void (*LaunchFn)(rt_ctx*, unsigned, const UnitDesc*, const WorkItem*, long, int, int, int,
+                         int, const ModelParams&);
+
+static LaunchFn pick(int model, int nc) {
+  if (model == M_DOMINO) return nc == 1 ? launch<M_DOMINO, 1, 8, 8> : launch<M_DOMINO, 2, 8, 8>;
+  if (model == M_LOZENGE)
+    return nc == 1 ? launch<M_LOZENGE, 1, 8, 8> : launch<M_LOZENGE, 2, 8, 8>;
+  return nc == 1 ? launch<M_RECTTRI, 1, 8, 8> : launch<M_RECTTRI, 2, 8, 4>;
+}
+
+int fused_upload(rt_ctx* ctx, FusedJob& job, void* scratch, FusedPlan* plan) {
+  const int LH = fused_rows_per_thread(job), H = fused_halo(job);
+  const size_t nu = job.units.size();
+  *plan = FusedPlan();
+  long T = 0;
+  for (long s : job.seg_steps) T += s;
+  plan->total_steps = T;
+  for (const UnitDesc& u : job.units) plan->streams += u.nstream;
+  if (nu == 0 || T == 0) return RT_OK;
+  // Device layout of the scratch block: units | work | ops | seeds.
+  uint8_t* p = (uint8_t*)scratch;
+  UnitDesc* d_units = (UnitDesc*)p;
+  p += nu * sizeof(UnitDesc);
+  std::vector<WorkItem> work;
+  for (size_t u = 0; u < nu; u++) {
+    int tr, tc;
+    tile_counts(job.units[u], LH, H, &tr, &tc);
+    job.units[u].tiles_c = tc;
+    for (int t = 0; t < tr * tc; t++) work.push_back(WorkItem{(int)u, t});
+  }
+  WorkItem* d_work = (WorkItem*)p;
+  p += work.size() * sizeof(WorkItem);
+  uint8_t* d_ops = p;
+  p += nu * (size_t)T;
+  p = (uint8_t*)(((uintptr_t)p + 15) & ~(uintptr_t)15);
+  uint32_t* d_seeds = (uint32_t*)p;
+  const size_t nseg = job.seg_steps.size();
+  std::vector<uint32_t> seeds(nu * nseg);
+  std::vector<uint8_t> anypull(T, 0);
+  for (size_t u = 0; u < nu; u++) {
+    for (size_t k = 0; k < nseg; k++) seeds[u * nseg + k] = job.seg_seeds[u][k];
+    job.units[u].ops = d_ops + u * (size_t)T;
+    job.units[u].seeds = d_seeds + u * nseg;
+    const std::vector<uint8_t>& o = job.ops[u];
+    if ((long)o.size() != T) {
+      ctx->err = "fused job: op stream length differs from the segment total";
+      return RT_EINVAL;
+    }
+    for (long s = 0; s < T; s++) anypull[s] |= (o[s] & (OP_PULL_A | OP_PULL_B)) ? 1 : 0;
+    HIPCHK_RET(ctx, hipMemcpyAsync(d_ops + u * (size_t)T, o.data(), T, hipMemcpyHostToDevice,
+                                   ctx->stream));
+  }
+  HIPCHK_RET(ctx, hipMemcpyAsync(d_seeds, seeds.data(), seeds.size() * 4, hipMemcpyHostToDevice,
+                                 ctx->stream));
+  HIPCHK_RET(ctx, hipMemcpyAsync(d_units, job.units.data(), nu * sizeof(UnitDesc),
+                                 hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK_RET(ctx, hipMemcpyAsync(d_work, work.data(), work.size() * sizeof(WorkItem),
+                                 hipMemcpyHostToDevice, ctx->stream));
+  // Pageable host sources: the copies above may still read them after the
+  // calls return, so wait before the vectors go out of scope.
+  HIPCHK_RET(ctx, hipStreamSynchronize(ctx->stream));
+  plan->d_units = d_units;
+  plan->d_work = d_work;
+  plan->nwork = (unsigned)work.size();
+  long s0 = 0;
+  for (size_t k = 0; k < nseg; k++) {
+    const long end = s0 + job.seg_steps[k];
+    bool first = true;
+    while (s0 < end) {
+      int K = 0, pulls = 0;
+      while (s0 + K < end && K < kKMax) {
+        const int p1 = anypull[s0 + K];
+        if (pulls + p1 > H) break;
+        pulls += p1;
+        K++;
+      }
+      plan->windows.push_back(FusedPlan::Window{s0, K, first ? 1 : 0, (int)k});
+      first = false;
+      s0 += K;
+    }
+  }
+  return RT_OK;
+}
+
+int fused_launch(rt_ctx* ctx, const FusedJob& job, const FusedPlan& plan) {
+  if (plan.windows.empty()) return RT_OK;
+  LaunchFn fn = pick(job.model, job.nc);
+  const int H = fused_halo(job);
+  for (const FusedPlan::Window& w : plan.windows)
+    fn(ctx, plan.nwork, plan.d_units, plan.d_work, w.s0, w.K, w.init, w.seg, H, job.mp);
+  HIPCHK_RET(ctx, hipGetLastError());
+  ctx->last.launches += (long)plan.windows.size();
+  ctx->last.steps += plan.total_steps;
+  ctx->last.attempts += plan.total_steps * plan.streams;
+  return RT_OK;
+}
+
+int fused_run(rt_ctx* ctx, FusedJob& job, void* scratch) {
+  FusedPlan plan;
+  int rc = fused_upload(ctx, job, scratch, &plan);
+  if (rc) return rc;
+  return fused_launch(ctx, job, plan);
+}
+
+int fused_sync(rt_ctx* ctx, const FusedJob& job) {
+  for (size_t u = 0; u < job.units.size(); u++) {
+    const UnitDesc& U = job.units[u];
+    const OpBuilder& h = job.hist[u];
+    if (h.prev < 0) continue;  // no step ran: planes are as loaded
+    int rc = RT_OK;
+    if (job.model == M_DOMINO) {
+      DominoArrays a[2];
+      for (int ch = 0; ch < 2; ch++)
+        for (int k = 0; k < 2; k++) a[ch].col[k] = (uint8_t*)U.plane[ch < job.nc ? ch : 0][k];
+      rc = launch_domino_sync(ctx, a, job.nc, U.rows, h.prev);
+    } else if (job.model == M_LOZENGE) {
+      TriArrays8 a[2];
+      for (int ch = 0; ch < 2; ch++)
+        for (int k = 0; k < 3; k++) a[ch].col[k] = (uint8_t*)U.plane[ch < job.nc ? ch : 0][k];
+      rc = lozenge_device_sync(ctx, a, job.nc, U.rows, h.last);
+    } else {
+      TriArrays32 a[2];
+      for (int ch = 0; ch < 2; ch++)
+        for (int k = 0; k < 3; k++) a[ch].col[k] = (int32_t*)U.plane[ch < job.nc ? ch : 0][k];
+      rc = launch_recttri_sync(ctx, a, job.nc, U.rows, h.prev);
+    }
+    if (rc) return rc;
+  }
+  return RT_OK;
+}
+
+}  // namespace rt

@@ -1,0 +1,114 @@
+// rt_fused.hpp -- the fused sweep-block engine shared by plain walks, CFTP
+// and batched chains.
+//
+// A "unit" is one TinyMT stream table driving NC chains (NC = 1 for a walk,
+// 2 for a CFTP top/bottom pair that share every draw) on one domain.  A
+// launch advances every unit of a job by one window of K colour steps.
+// Each workgroup owns a tile of the unit's compressed cell grid, loads the
+// tile plus an H-cell halo into LDS (all colour planes of all NC chains)
+// and the TinyMT states of those cells into VGPRs, runs the K steps with one
+// workgroup barrier per step, and writes back the tile interior.
+//
+// Exactness of the halo: a step only reads neighbours when it refreshes the
+// rotated colour from a colour rotated since (a "pull"); the local move is
+// pointwise.  Cells whose neighbourhood reaches past the loaded region go
+// stale, one cell deeper per pull step, so a window with at most H pull
+// steps leaves the tile interior exact.  Halo cells also advance their
+// (owners') TinyMT streams, redundantly, so every draw is the reference's.
+#pragma once
+#include <vector>
+
+#include "rt_kernels.hpp"
+
+namespace rt {
+
+enum ModelKind { M_DOMINO = 0, M_LOZENGE = 1, M_RECTTRI = 2 };
+
+// One op byte per colour step:
+//   bits 0-1 colour rotated;
+//   bit 2    domino: pull the rotated colour from the other one;
+//            lozenge: refresh bits facing colour-1 ("pull A");
+//            rect-tri: rebuild from the previous step's colour;
+//   bit 3    lozenge: refresh bits facing colour-2 ("pull B");
+//   bits 4-5 rect-tri: previous step's colour.
+enum : uint8_t { OP_COLOUR = 3u, OP_PULL_A = 4u, OP_PULL_B = 8u };
+
+struct UnitDesc {
+  void* plane[2][3];       // [chain][colour] -> rows x cols cells (uint8 or int32)
+  uint32_t* state;         // SoA s0..s3, stride = nstream
+  const uint32_t* param;   // SoA mat1, mat2, tmat, stride = pstride
+  long nstream;
+  long pstride;
+  const uint8_t* ops;      // op byte per global step of this unit
+  const uint32_t* seeds;   // TinyMT seed per segment
+  int rows, cols;          // compressed grid
+  int tiles_c;             // tile columns
+  int pad;
+};
+
+struct WorkItem {
+  int unit;
+  int tile;  // tile index within the unit (row-major over tiles_c)
+};
+
+struct ModelParams {
+  int rule;
+  DominoWeights dw;
+  RTWeights rw;
+};
+
+// Unit colour-step history used to build op bytes and the final sync.
+struct OpBuilder {
+  int model;
+  long last[3] = {-1, -1, -1};
+  int prev = -1;
+  long g = 0;
+  // Append ops for `colours`; returns the number of pull steps.
+  long append(const std::vector<uint8_t>& colours, std::vector<uint8_t>& ops);
+};
+
+// Host job: units on one device advancing through a shared segment list.
+struct FusedJob {
+  int model = M_DOMINO;
+  int nc = 1;
+  ModelParams mp{};
+  std::vector<UnitDesc> units;        // host copy (ops/seeds filled by the driver)
+  std::vector<long> seg_steps;        // shared segment lengths
+  std::vector<std::vector<uint32_t>> seg_seeds;   // [unit][segment]
+  std::vector<std::vector<uint8_t>> ops;          // [unit] concatenated op bytes
+  std::vector<OpBuilder> hist;        // [unit]
+  bool adaptive = true;               // single unit: stretch windows to H pulls
+};
+
+// Run all windows of the job on ctx->stream (no final sync).  `scratch`
+// must hold fused_scratch_bytes(job) bytes of device memory.
+size_t fused_scratch_bytes(const FusedJob& job);
+int fused_run(rt_ctx* ctx, FusedJob& job, void* scratch);
+
+// The same in two phases, so the launch sequence can be captured in a
+// hipGraph: fused_upload copies unit tables, work list, ops and seeds to
+// the device; fused_launch enqueues only kernels.
+struct FusedPlan {
+  const UnitDesc* d_units = nullptr;
+  const WorkItem* d_work = nullptr;
+  unsigned nwork = 0;
+  struct Window {
+    long s0;
+    int K;
+    int init;
+    int seg;
+  };
+  std::vector<Window> windows;
+  long total_steps = 0;
+  long streams = 0;
+};
+int fused_upload(rt_ctx* ctx, FusedJob& job, void* scratch, FusedPlan* plan);
+int fused_launch(rt_ctx* ctx, const FusedJob& job, const FusedPlan& plan);
+// Bring every colour plane of every unit up to date (end of a walk).
+int fused_sync(rt_ctx* ctx, const FusedJob& job);
+
+// Tile geometry (256 threads, 64 columns, 4*R rows per tile incl. halo).
+int fused_rows_per_thread(const FusedJob& job);
+int fused_halo(const FusedJob& job);
+
+}  // namespace rt

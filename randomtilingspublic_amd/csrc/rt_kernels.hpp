@@ -13,23 +13,36 @@
 #include "rt_internal.hpp"
 #include "tinymt_dev.hpp"
 
+#define HIPCHK_RET(ctx, expr)                                              \
+  do {                                                                      \
+    hipError_t e_ = (expr);                                                 \
+    if (e_ != hipSuccess) {                                                 \
+      (ctx)->err = std::string(#expr) + ": " + hipGetErrorString(e_);        \
+      return RT_EDEVICE;                                                    \
+    }                                                                       \
+  } while (0)
+
 struct rt_ctx {
   int device = 0;
   int num_cu = 256;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t evk0 = nullptr, evk1 = nullptr;  // around sweep-kernel sequences
   std::string err;
   std::string devname;
   uint32_t* d_param = nullptr;  // SoA: mat1[n], mat2[n], tmat[n]
   size_t nparam = 0;
   void* arena = nullptr;
   size_t arena_bytes = 0;
+  void* aux = nullptr;  // schedules, unit tables, work lists
+  size_t aux_bytes = 0;
   uint32_t* d_flag = nullptr;
   uint32_t* h_flag = nullptr;
   rt_timing last{};
   int engine = 0;
 
   int reserve(size_t bytes);
+  int reserve_aux(size_t bytes);
   int check_params(size_t need);
   ~rt_ctx();
 };
@@ -56,5 +69,9 @@ int recttri_device_walk(rt_ctx* ctx, TriArrays32* a, int nc, int N,
                         const std::vector<long>& steps, const std::vector<long>& seeds,
                         const RTWeights& W, uint32_t* state, int* last_out);
 RTWeights recttri_weights(const float* w16);
+
+// Final lazy sync of the colour planes (step-path kernels).
+int launch_domino_sync(rt_ctx* ctx, DominoArrays* a, int nc, int N, int last);
+int launch_recttri_sync(rt_ctx* ctx, TriArrays32* a, int nc, int N, int last);
 
 }  // namespace rt

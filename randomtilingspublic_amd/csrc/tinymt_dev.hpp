@@ -27,13 +27,27 @@ __device__ __forceinline__ void tiny_next(Tiny& t) {
   t.s3 = y;
 }
 
-// Float in [0,1) on the 2^-23 grid: ((temper >> 9) | 1.0f bits) - 1.0f.
-__device__ __forceinline__ float tiny_float01(Tiny& t) {
+// Tempered output of the next state (tinymt32_temper).
+__device__ __forceinline__ uint32_t tiny_bits(Tiny& t) {
   tiny_next(t);
-  uint32_t t1 = t.s0 + (t.s2 >> 8);
-  uint32_t t0 = t.s3 ^ t1 ^ ((0u - (t1 & 1u)) & t.tmat);
-  return __uint_as_float((t0 >> 9) | 0x3f800000u) - 1.0f;
+  const uint32_t t1 = t.s0 + (t.s2 >> 8);
+  return t.s3 ^ t1 ^ ((0u - (t1 & 1u)) & t.tmat);
 }
+
+// The float the reference draws from tempered bits: ((b >> 9) | 1.0f) - 1.0f,
+// i.e. exactly (b >> 9) * 2^-23.
+__device__ __forceinline__ float tiny_to_float01(uint32_t b) {
+  return __uint_as_float((b >> 9) | 0x3f800000u) - 1.0f;
+}
+
+__device__ __forceinline__ float tiny_float01(Tiny& t) { return tiny_to_float01(tiny_bits(t)); }
+
+// Integer forms of the reference's threshold tests on rd = (b>>9)*2^-23:
+// rd < 0.8f <=> (b>>9) <= 6710886; rd < 0.4f <=> (b>>9) <= 3355443;
+// rd < 0.5f <=> b < 2^31.
+constexpr uint32_t kBelow08 = 6710887u << 9;
+constexpr uint32_t kBelow04 = 3355444u << 9;
+constexpr uint32_t kBelow05 = 1u << 31;
 
 // tinymt32_init(params, seed): 7 mixing rounds, period certification,
 // 8 discarded outputs.

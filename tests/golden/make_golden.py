@@ -1,0 +1,91 @@
+"""Regenerates the golden fixtures under tests/golden/.
+
+* schedules.json: colour schedules from libstdc++'s own mt19937 and
+  default_random_engine + uniform_int_distribution (schedule_golden.cpp,
+  compiled with the image's g++).
+* reference_host.npz: outputs of the reference's own host statics
+  (MinTiling / MaxTiling / AlmostHexDomain / maxHex / minHex ...), taken
+  from oracle/_ref/libref_host.so, which oracle/Makefile builds from the
+  reference sources.  Skipped when that library is not built.
+* reference_outputs/*.txt: SaveMatrix files the reference checkout ships
+  in Examples/ExampleOuts (copied data, not code).
+"""
+import ctypes as C
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+
+
+def schedules():
+    with tempfile.TemporaryDirectory() as d:
+        exe = os.path.join(d, "sched")
+        subprocess.run(["g++", "-O2", "-std=c++11", "-o", exe,
+                        os.path.join(HERE, "schedule_golden.cpp")], check=True)
+        out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout
+    data = json.loads(out)
+    with open(os.path.join(HERE, "schedules.json"), "w") as f:
+        json.dump(data, f)
+
+
+def ref_host():
+    lib = os.path.join(ROOT, "oracle", "_ref", "libref_host.so")
+    if not os.path.exists(lib):
+        print("oracle/_ref/libref_host.so not built; skipping reference_host.npz")
+        return
+    R = C.CDLL(lib)
+
+    def call(fn, *args):
+        cap = 1 << 22
+        out = np.zeros(cap, np.int32)
+        f = getattr(R, fn)
+        f.restype = C.c_long
+        conv = []
+        for a in args:
+            if isinstance(a, np.ndarray):
+                conv += [a.ctypes.data_as(C.c_void_p), C.c_long(len(a))]
+            else:
+                conv.append(C.c_int(a))
+        n = f(*conv, out.ctypes.data_as(C.c_void_p), C.c_long(cap))
+        assert n >= 0, fn
+        return out[:n].copy()
+
+    arrs = {}
+    for n in (1, 2, 3, 7, 16, 33):
+        d = call("ref_domino_aztec_diamond", n)
+        arrs[f"domino_aztec{n}_domain"] = d
+        arrs[f"domino_aztec{n}_min"] = call("ref_domino_min_tiling", d)
+        arrs[f"domino_aztec{n}_max"] = call("ref_domino_max_tiling", d)
+        arrs[f"domino_aztec{n}_min_hf"] = call("ref_domino_tiling_to_heightfunc",
+                                               arrs[f"domino_aztec{n}_min"], d)
+    rem = np.array([0, 2], np.int32)
+    d = call("ref_domino_aztec_rectangle", 6, rem)  # (ptr, k) from the array
+    arrs["domino_aztecrect6_02_domain"] = d
+    arrs["domino_aztecrect6_02_min"] = call("ref_domino_min_tiling", d)
+    for m, n in ((4, 6), (7, 10)):
+        d = call("ref_domino_rectangle", m, n)
+        arrs[f"domino_rect{m}x{n}_domain"] = d
+        arrs[f"domino_rect{m}x{n}_max"] = call("ref_domino_max_tiling", d)
+    for n, m in ((2, 0), (5, 1), (10, 2), (13, 0)):
+        d = call("ref_lozenge_almost_hex_domain", n, m)
+        arrs[f"lozenge_hex{n}_{m}_domain"] = d
+        arrs[f"lozenge_hex{n}_{m}_min"] = call("ref_lozenge_min_tiling", d)
+        arrs[f"lozenge_hex{n}_{m}_max"] = call("ref_lozenge_max_tiling", d)
+        arrs[f"lozenge_hex{n}_{m}_max_hf"] = call("ref_lozenge_tiling_to_heightfunc",
+                                                  arrs[f"lozenge_hex{n}_{m}_max"], d)
+    for n in (1, 2, 5, 8):
+        arrs[f"recttri_max{n}"] = call("ref_recttri_max_hex", n)
+        arrs[f"recttri_min{n}"] = call("ref_recttri_min_hex", n)
+    np.savez_compressed(os.path.join(HERE, "reference_host.npz"), **arrs)
+
+
+if __name__ == "__main__":
+    schedules()
+    ref_host()
+    print("golden fixtures written", file=sys.stderr)

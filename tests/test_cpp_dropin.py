@@ -1,0 +1,98 @@
+"""The C++ drop-in header (include/rtiling_tilers.hpp) compiles against
+librtiling.so and its static helpers reproduce the reference's outputs;
+on a GPU box the same program also runs a Walk and a CFTP through it."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import randomtilingspublic_amd as rt
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+PROGRAM = r'''
+#include <cstdio>
+#include "rtiling_tilers.hpp"
+int main(int argc, char** argv) {
+  domain d = DominoTiler::AztecDiamond(16);
+  tiling t = DominoTiler::MinTiling(d);
+  long s = 0;
+  for (size_t k = 0; k < t.size(); k++) s += (long)(k % 97 + 1) * t[k];
+  std::printf("min %zu %ld\n", t.size(), s);
+  domain dl = LozengeTiler::AlmostHexDomain(7, 2);
+  tiling l = LozengeTiler::MaxTiling(dl);
+  s = 0;
+  for (size_t k = 0; k < l.size(); k++) s += (long)(k % 89 + 1) * l[k];
+  std::printf("loz %zu %ld\n", l.size(), s);
+  try {
+    domain bad = d;
+    for (size_t k = 0; k < bad.size(); k++) if (bad[k]) { bad[k] = 0; break; }
+    DominoTiler::MinTiling(bad);
+    std::printf("untileable not reported\n");
+  } catch (const rt::Error& e) {
+    std::printf("untileable %d\n", e.code);
+  }
+  if (argc > 1) {  // device part
+    DominoTiler D(0, "dominokernelCFTP.cl");
+    std::vector<uint32_t> p(3 * t.size() / 2);
+    for (size_t k = 0; k < p.size(); k++) p[k] = (uint32_t)(k * 2654435761u + 12345u);
+    D.LoadTinyMT(p);
+    tiling w = t;
+    D.Walk(w, 1000, 1);
+    s = 0;
+    for (size_t k = 0; k < w.size(); k++) s += (long)(k % 97 + 1) * w[k];
+    std::printf("walk %ld\n", s);
+    tiling c = D.CFTP(DominoTiler::MaxTiling(d), t, 345903);
+    s = 0;
+    for (size_t k = 0; k < c.size(); k++) s += (long)(k % 97 + 1) * c[k];
+    std::printf("cftp %ld\n", s);
+  }
+  return 0;
+}
+'''
+
+
+def _checksum(t, m):
+    k = np.arange(len(t))
+    return int(((k % m + 1) * t.astype(np.int64)).sum())
+
+
+def _build(tmp_path):
+    src = tmp_path / "dropin.cpp"
+    src.write_text(PROGRAM)
+    exe = tmp_path / "dropin"
+    lib = os.path.join(ROOT, "randomtilingspublic_amd")
+    subprocess.run(["g++", "-std=c++17", "-O1", "-I", os.path.join(ROOT, "include"), str(src),
+                    "-L", lib, "-lrtiling", f"-Wl,-rpath,{lib}", "-o", str(exe)], check=True)
+    return exe
+
+
+def test_cpp_dropin_statics(tmp_path):
+    exe = _build(tmp_path)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    d = rt.DominoTiler.AztecDiamond(16)
+    t = rt.DominoTiler.MinTiling(d)
+    assert out[0] == f"min {len(t)} {_checksum(t, 97)}"
+    dl = rt.LozengeTiler.AlmostHexDomain(7, 2)
+    l = rt.LozengeTiler.MaxTiling(dl)
+    assert out[1] == f"loz {len(l)} {_checksum(l, 89)}"
+    assert out[2] == "untileable -7"
+
+
+@pytest.mark.gpu
+def test_cpp_dropin_walk_and_cftp(tmp_path):
+    from tests import oracle_lib as o
+    exe = _build(tmp_path)
+    print("[case] C++ drop-in walk + cftp", flush=True)
+    out = subprocess.run([str(exe), "gpu"], check=True, capture_output=True,
+                         text=True).stdout.split("\n")
+    d = rt.DominoTiler.AztecDiamond(16)
+    t = rt.DominoTiler.MinTiling(d)
+    n = len(t) // 2
+    p = ((np.arange(3 * n, dtype=np.uint64) * np.uint64(2654435761) + np.uint64(12345))
+         & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+    w = o.domino_walk(t, 1000, 1, p, 1)
+    assert out[3] == f"walk {_checksum(w, 97)}"
+    c, _ = o.domino_cftp(rt.DominoTiler.MaxTiling(d), t, 345903, p, 0, 128, 64, 1)
+    assert out[4] == f"cftp {_checksum(c, 97)}"

@@ -174,3 +174,34 @@ def test_params_too_small_is_rejected(gpu_ctx):
     with pytest.raises(rt.RTError) as e:
         gpu_ctx.domino_walk(tmin, 10, 1)
     assert e.value.code == -2
+
+
+@pytest.mark.parametrize("engine", [rt.ENGINE_STEP, rt.ENGINE_FUSED])
+def test_engines_agree_with_oracle(gpu_ctx, engine):
+    """Both kernel shapes (one launch per step; fused LDS sweep blocks) on
+    domains wider and taller than one fused tile."""
+    gpu_ctx.set_engine(engine)
+    try:
+        d, tmin, tmax = _domino(60)
+        N = rt.side(tmin)
+        P = _params(N * N // 2, seed=8)
+        gpu_ctx.set_tinymt_params(P)
+        for rule in RULES:
+            _say("engine", engine, "domino-60 rule", rule)
+            got = gpu_ctx.domino_walk(tmax, 777, 4, rule, 0.7, 1.0)
+            want = o.domino_walk(tmax, 777, 4, P, rule, 0.7, 1.0)
+            assert np.array_equal(got, want)
+        dl, lmin, lmax = _lozenge(40, 3)
+        NL = rt.side(lmin)
+        PL = _params(NL * NL, seed=9)
+        gpu_ctx.set_tinymt_params(PL)
+        _say("engine", engine, "lozenge-40")
+        assert np.array_equal(gpu_ctx.lozenge_walk(lmin, 999, 6), o.lozenge_walk(lmin, 999, 6, PL))
+        r = rt.RectTriangleTiler.maxHex(30)
+        NR = rt.side(r)
+        PR = _params(NR * NR, seed=10)
+        gpu_ctx.set_tinymt_params(PR)
+        _say("engine", engine, "recttri-30")
+        assert np.array_equal(gpu_ctx.recttri_walk(r, 555, 2), o.recttri_walk(r, 555, 2, PR))
+    finally:
+        gpu_ctx.set_engine(rt.ENGINE_AUTO)

@@ -1,0 +1,160 @@
+"""Host-side drop-ins (domains, Thurston min/max, height functions) against
+the reference's own outputs: the SaveMatrix files its checkout ships, the
+vectors taken from its compiled host statics (tests/golden), and, when
+oracle/_ref is built, the reference library itself at other sizes."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+import randomtilingspublic_amd as rt
+
+HERE = os.path.dirname(__file__)
+GOLD = os.path.join(HERE, "golden")
+REF_LIB = os.path.join(os.path.dirname(HERE), "oracle", "_ref", "libref_host.so")
+
+
+def load_matrix(name):
+    """LoadMatrix format (common.cpp:160-179): whitespace-separated ints."""
+    with open(os.path.join(GOLD, "reference_outputs", name)) as f:
+        return np.array([int(x) for x in f.read().split()], dtype=np.int32)
+
+
+def test_aztec16_min_tiling_matches_example_output():
+    # WeightedAztecDiamondEx.cpp:36-41: SaveMatrix(MinTiling(AztecDiamond(16)))
+    t = rt.DominoTiler.MinTiling(rt.DominoTiler.AztecDiamond(16))
+    assert np.array_equal(t, load_matrix("aztec16_min_tiling.txt"))
+
+
+@pytest.mark.parametrize("kind", ["max", "min"])
+def test_recttri_hex5_matches_example_output(kind):
+    # RectTriangleEx.cpp:29-33
+    f = rt.RectTriangleTiler.maxHex if kind == "max" else rt.RectTriangleTiler.minHex
+    assert np.array_equal(f(5), load_matrix(f"recttri5_{kind}.txt"))
+
+
+def test_random_end_state_is_a_tiling_of_the_same_domain():
+    end = load_matrix("aztec16_random_end.txt")
+    d = rt.DominoTiler.AztecDiamond(16)
+    assert np.array_equal(rt.DominoTiler.TilingToDomain(end),
+                          rt.DominoTiler.TilingToDomain(rt.DominoTiler.MinTiling(d)))
+
+
+def test_golden_reference_host_vectors():
+    z = np.load(os.path.join(GOLD, "reference_host.npz"))
+    seen = 0
+    for k in z.files:
+        if k.startswith("domino_aztec") and k.endswith("_domain") and "rect" not in k:
+            n = int(k[len("domino_aztec"):-len("_domain")])
+            assert np.array_equal(rt.DominoTiler.AztecDiamond(n), z[k])
+        if k.startswith("lozenge_hex") and k.endswith("_domain"):
+            n, m = map(int, k[len("lozenge_hex"):-len("_domain")].split("_"))
+            assert np.array_equal(rt.LozengeTiler.AlmostHexDomain(n, m), z[k])
+        for suf, dom_fn in (("_min", "MinTiling"), ("_max", "MaxTiling")):
+            if not k.endswith(suf):
+                continue
+            base = k[: -len(suf)]
+            d = z[base + "_domain"]
+            cls = rt.DominoTiler if k.startswith("domino") else rt.LozengeTiler
+            assert np.array_equal(getattr(cls, dom_fn)(d), z[k]), k
+            seen += 1
+        if k.endswith("_min_hf") or k.endswith("_max_hf"):
+            base = k[:-3]
+            d = z[base.rsplit("_", 1)[0] + "_domain"]
+            cls = rt.DominoTiler if k.startswith("domino") else rt.LozengeTiler
+            assert np.array_equal(cls.TilingToHeightfunc(z[base], d), z[k]), k
+        if k.startswith("recttri_max") or k.startswith("recttri_min"):
+            n = int(k[len("recttri_max"):])
+            f = rt.RectTriangleTiler.maxHex if "max" in k else rt.RectTriangleTiler.minHex
+            assert np.array_equal(f(n), z[k]), k
+            seen += 1
+    assert seen >= 30
+
+
+def test_aztec_rectangle_with_removed_points():
+    z = np.load(os.path.join(GOLD, "reference_host.npz"))
+    d = rt.DominoTiler.AztecRectangle(6, [0, 2])
+    assert np.array_equal(d, z["domino_aztecrect6_02_domain"])
+    assert np.array_equal(rt.DominoTiler.MinTiling(d), z["domino_aztecrect6_02_min"])
+
+
+@pytest.mark.parametrize("n", [2, 5, 11])
+def test_domino_invariants(n):
+    d = rt.DominoTiler.AztecDiamond(n)
+    lo, hi = rt.DominoTiler.MinTiling(d), rt.DominoTiler.MaxTiling(d)
+    for t in (lo, hi):
+        # every vertex state is 0 or a single-dimer-pair pattern, and the
+        # tiling covers exactly the domain's faces
+        dd = rt.DominoTiler.TilingToDomain(t)
+        M = int(round(np.sqrt(len(dd))))
+        assert dd.sum() == 2 * n * (n + 1)
+        hf = rt.DominoTiler.TilingToHeightfunc(t, d)
+        assert np.array_equal(rt.DominoTiler.HeightfuncToTiling(hf, d), t)
+    hlo = rt.DominoTiler.TilingToHeightfunc(lo, d)
+    hhi = rt.DominoTiler.TilingToHeightfunc(hi, d)
+    inside = hlo != np.iinfo(np.int32).max
+    assert (hlo[inside] <= hhi[inside]).all()
+    assert (hlo[inside] < hhi[inside]).any()
+
+
+@pytest.mark.parametrize("n,m", [(2, 0), (6, 2), (9, 0)])
+def test_lozenge_invariants(n, m):
+    d = rt.LozengeTiler.AlmostHexDomain(n, m)
+    lo, hi = rt.LozengeTiler.MinTiling(d), rt.LozengeTiler.MaxTiling(d)
+    N = rt.side(lo)
+    assert N % 3 == 0
+    hlo = rt.LozengeTiler.TilingToHeightfunc(lo, d)
+    hhi = rt.LozengeTiler.TilingToHeightfunc(hi, d)
+    inside = hlo != np.iinfo(np.int32).max
+    # extremal height functions agree on the boundary, differ inside
+    assert (hlo[inside] - hlo[inside].min() <= hhi[inside] - hhi[inside].min() + 3 * n).all()
+    assert np.array_equal(rt.LozengeTiler.TilingToDomain(lo), rt.LozengeTiler.TilingToDomain(hi))
+
+
+def test_untileable_domain_reports_status():
+    d = rt.DominoTiler.AztecDiamond(4).copy()
+    M = int(round(np.sqrt(len(d))))
+    ones = np.flatnonzero(d)
+    d[ones[0]] = 0  # remove one square: odd area
+    with pytest.raises(rt.RTError) as e:
+        rt.DominoTiler.MinTiling(d)
+    assert e.value.code == -7
+
+
+def test_helpers_reject_bad_shapes():
+    with pytest.raises(rt.RTError):
+        rt.DominoTiler.MinTiling(np.zeros(10, np.int32))
+    with pytest.raises(rt.RTError):
+        rt.DominoTiler.AztecDiamond(0)
+
+
+@pytest.mark.skipif(not os.path.exists(REF_LIB), reason="oracle/_ref not built")
+@pytest.mark.parametrize("n", [4, 23, 61])
+def test_against_reference_library(n):
+    R = C.CDLL(REF_LIB)
+    cap = 1 << 22
+
+    def call(fn, *args):
+        out = np.zeros(cap, np.int32)
+        f = getattr(R, fn)
+        f.restype = C.c_long
+        conv = []
+        for a in args:
+            if isinstance(a, np.ndarray):
+                conv += [a.ctypes.data_as(C.c_void_p), C.c_long(len(a))]
+            else:
+                conv.append(C.c_int(a))
+        k = f(*conv, out.ctypes.data_as(C.c_void_p), C.c_long(cap))
+        return out[:k].copy()
+
+    d = rt.DominoTiler.AztecDiamond(n)
+    assert np.array_equal(d, call("ref_domino_aztec_diamond", n))
+    assert np.array_equal(rt.DominoTiler.MaxTiling(d), call("ref_domino_max_tiling", d))
+    assert np.array_equal(rt.DominoTiler.MinTiling(d), call("ref_domino_min_tiling", d))
+    dl = rt.LozengeTiler.AlmostHexDomain(n, n // 3)
+    assert np.array_equal(dl, call("ref_lozenge_almost_hex_domain", n, n // 3))
+    assert np.array_equal(rt.LozengeTiler.MaxTiling(dl), call("ref_lozenge_max_tiling", dl))
+    assert np.array_equal(rt.LozengeTiler.MinTiling(dl), call("ref_lozenge_min_tiling", dl))
+    assert np.array_equal(rt.RectTriangleTiler.maxHex(n), call("ref_recttri_max_hex", n))
+    assert np.array_equal(rt.RectTriangleTiler.minHex(n), call("ref_recttri_min_hex", n))

@@ -1,0 +1,129 @@
+"""Pins the CPU oracle before anything is compared against it: standard
+known answers, libstdc++'s own generators, the second TinyMT formulation,
+and exhaustive checks of the threshold arithmetic."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import randomtilingspublic_amd as rt
+from tests import oracle_lib as o
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def test_mt19937_standard_known_answers():
+    # C++ standard [rand.predef]: 10000th output of a default-constructed
+    # mt19937 (seed 5489) is 4123659995.
+    assert int(o.mt19937(5489, 10000)[-1]) == 4123659995
+    # DominoTiler::Walk(t, steps, seed=1) seeds TinyMT with mt19937(1)'s
+    # first output.
+    assert int(o.mt19937(1, 1)[0]) == 1791095845
+
+
+def test_minstd_rand0_standard_known_answer():
+    # C++ standard [rand.predef]: 10000th output of minstd_rand0 is 1043618065.
+    assert int(o.minstd(1, 10000)[-1]) == 1043618065
+
+
+def test_schedules_match_libstdcxx():
+    with open(os.path.join(GOLDEN, "schedules.json")) as f:
+        g = json.load(f)
+    for seed, vals in g["mt19937"].items():
+        s = int(seed) & 0xFFFFFFFF
+        assert o.mt19937(s, len(vals)).tolist() == vals
+    for seed, vals in g["uniform012"].items():
+        assert o.uniform012(int(seed), len(vals)).tolist() == vals
+
+
+def test_uniform012_is_unbiased():
+    v = o.uniform012(12345, 30000)
+    counts = np.bincount(v, minlength=3)
+    chi2 = (((counts - 10000.0) ** 2) / 10000.0).sum()
+    assert chi2 < 13.8  # chi^2_2 at p = 0.001
+
+
+@pytest.mark.parametrize("params", [(0x8F7011EE, 0xFC78FF1F, 0x3793FDFF),
+                                    (0x12345678, 0x9ABCDEF0, 0x0F0F0F0F),
+                                    (0, 0, 0)])
+def test_tinymt_two_formulations_agree(params):
+    """tinymt32.clh (branches) vs tinymt32.h (sign masks) on many seeds."""
+    for seed in [0, 1, 2, 0xFFFFFFFF, 12345, 0x80000000]:
+        a = o.tiny(*params, seed)
+        b = o.tiny(*params, seed)
+        fa = o.tiny_floats(a, 300)
+        fb = o.tiny_floats(b, 300, h=True)
+        assert np.array_equal(fa.view(np.uint32), fb.view(np.uint32))
+
+
+def test_tinymt_output_grid():
+    t = o.tiny(0x8F7011EE, 0xFC78FF1F, 0x3793FDFF, 7)
+    f = o.tiny_floats(t, 5000).astype(np.float64)
+    assert (f >= 0).all() and (f < 1).all()
+    k = f * 2.0 ** 23
+    assert np.array_equal(k, np.floor(k))
+    # roughly uniform
+    assert abs(f.mean() - 0.5) < 0.02
+
+
+def test_tinymt_init_changes_with_seed_and_params():
+    a = o.tiny_floats(o.tiny(1, 2, 3, 10), 4)
+    b = o.tiny_floats(o.tiny(1, 2, 3, 11), 4)
+    c = o.tiny_floats(o.tiny(1, 2, 4, 10), 4)
+    assert not np.array_equal(a, b) and not np.array_equal(a, c)
+
+
+def test_threshold_literals_exhaustive():
+    """Every draw is k*2^-23.  The reference compares with double literals
+    (.8, .4, 0.5) and float ones (0.8f, 0.9f); the device compares the
+    tempered bits with integer cut-offs.  All agree on the whole grid."""
+    k = np.arange(1 << 23, dtype=np.int64)
+    rd32 = (k.astype(np.float64) * 2.0 ** -23).astype(np.float32)
+    rd64 = rd32.astype(np.float64)
+    assert np.array_equal(rd64 < 0.8, rd32 < np.float32(0.8))
+    assert np.array_equal(rd64 < 0.8, k < 6710887)
+    assert np.array_equal(rd64 < 0.4, rd32 < np.float32(0.4))
+    assert np.array_equal(rd64 < 0.4, k < 3355444)
+    assert np.array_equal(rd64 < 0.5, k < (1 << 22))
+    # heat-bath two-periodic thresholds rd/0.5 and (rd-0.5)/0.5 are exact
+    lo = rd32[rd32 < 0.5]
+    assert np.array_equal((lo.astype(np.float64) / 0.5).astype(np.float32), lo * np.float32(2))
+    hi = rd32[rd32 >= 0.5]
+    assert np.array_equal(((hi.astype(np.float64) - 0.5) / 0.5).astype(np.float32),
+                          (hi - np.float32(0.5)) * np.float32(2))
+
+
+def test_param_file_parse(tmp_path):
+    p = rt.synthetic_params(257, 99)
+    path = tmp_path / "params.txt"
+    rt.write_params_file(path, p)
+    rc, got = o.parse_params(path, 257)
+    assert rc == 0 and np.array_equal(got, p)
+    assert np.array_equal(rt.parse_tinymt_file(path, 257), p)
+    # a prefix is fine, a longer request is a format error (short file)
+    rc, got = o.parse_params(path, 10)
+    assert rc == 0 and np.array_equal(got, p[:10])
+    rc, _ = o.parse_params(path, 258)
+    assert rc == -2
+    with pytest.raises(rt.RTError) as e:
+        rt.parse_tinymt_file(path, 258)
+    assert e.value.code == -2
+
+
+def test_param_file_malformed(tmp_path):
+    bad = tmp_path / "bad.txt"
+    bad.write_text("# header\nname,127,0,8f7011ee,fc78ff1f,3793fdff,0,0\nname,127,1,8f7011ee\n")
+    rc, _ = o.parse_params(bad, 2)
+    assert rc == -2
+    with pytest.raises(rt.RTError):
+        rt.parse_tinymt_file(bad, 2)
+    blank = tmp_path / "blank.txt"
+    blank.write_text("name,127,0,1,2,3,0,0\n\nname,127,1,4,5,6,0,0\n")
+    rc, _ = o.parse_params(blank, 2)
+    assert rc == -2
+    with pytest.raises(rt.RTError):
+        rt.parse_tinymt_file(blank, 2)
+    with pytest.raises(rt.RTError) as e:
+        rt.parse_tinymt_file(tmp_path / "missing.txt", 1)
+    assert e.value.code == -3
