@@ -153,6 +153,55 @@ int rt_domino_walk_batch(rt_ctx* ctx, int32_t* const* tilings, const int* Ns, in
                          long steps, const long* seeds, const rt_domino_rule* rule,
                          int use_graph);
 
+/* -------------------------------------------------------------- serving */
+/* Continuous-batching CFTP server on one context.  A device loop thread
+ * admits requests by HBM footprint into slots of an arena of
+ * `hbm_budget` bytes (at most `max_active` at once), keeps their chains
+ * resident and advances all of them in shared launches, each at its own
+ * point of its own schedule.  Request k's sample equals rt_*_cftp on the
+ * same inputs, alone or mixed with any others.  While a server exists, its
+ * context must not be used by other calls.  No reference counterpart. */
+#define RT_MODEL_DOMINO 0
+#define RT_MODEL_LOZENGE 1
+
+typedef struct {
+  int model;               /* RT_MODEL_DOMINO or RT_MODEL_LOZENGE */
+  int N;                   /* tiling side */
+  const int32_t* tmax;     /* N*N, copied at submit */
+  const int32_t* tmin;
+  uint32_t master_seed;
+  int sched;               /* RT_SCHED_* */
+  long t0;
+  int max_rounds;
+  rt_domino_rule rule;     /* domino only */
+} rt_request;
+
+typedef struct {
+  long completed;
+  long queued;
+  long ticks;              /* fused launches issued per model */
+  double p50_latency_s;    /* submit -> result, completed requests */
+  double p99_latency_s;
+  double span_s;           /* first submit -> last completion */
+  double samples_per_s;    /* completed / span_s */
+  int status;              /* RT_OK unless the device loop failed */
+} rt_server_stats;
+
+typedef struct rt_server rt_server;
+int rt_server_create(rt_ctx* ctx, size_t hbm_budget, int max_active, rt_server** out);
+void rt_server_destroy(rt_server* srv);
+/* RT_EADMIT: larger than the whole budget; RT_EPARAMS: parameter table too
+ * small for the request; RT_EINVAL: bad shape or model. */
+int rt_server_submit(rt_server* srv, const rt_request* req, uint64_t* id);
+/* RT_EAGAIN until done; then the request's status (RT_OK, RT_ENOCOUPLE,
+ * RT_ECANCELLED, ...), its sample in out[N*N] and stats; forgets the id. */
+int rt_server_poll(rt_server* srv, uint64_t id, int32_t* out, rt_cftp_stats* stats,
+                   double* latency_s);
+/* Block until request `id` is done (timeout_s < 0: no limit). */
+int rt_server_wait(rt_server* srv, uint64_t id, double timeout_s);
+int rt_server_cancel(rt_server* srv, uint64_t id);
+int rt_server_get_stats(rt_server* srv, rt_server_stats* out);
+
 /* ---------------------------------------------------- rectangle-triangle */
 /* RectTriangleTiler::Walk (RectTriangleTiler.cpp:33-171); weights16 is the
  * kernel's weight table (recttrianglekernel.cl:3-13; NULL = all 1.0 with

@@ -19,6 +19,7 @@
 #pragma once
 
 #include <cstdint>
+#include <cstdio>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -83,6 +84,60 @@ class Device {
 };
 
 }  // namespace rt
+
+// ------------------------------------------------------- matrix text I/O
+// The reference's only on-disk tiling format (src/common/common.cpp:115-179):
+// rows of space-separated integers, each row ending in " \n"; the loader
+// reads every maximal run of decimal digits as one non-negative integer.
+inline void SaveMatrix(const std::vector<int>& m, int M, int N, const std::string& filename) {
+  FILE* f = std::fopen(filename.c_str(), "w");
+  if (!f) throw rt::Error(RT_EIO, "cannot open " + filename);
+  for (int i = 0; i < M; i++) {
+    for (int j = 0; j < N; j++) std::fprintf(f, "%d ", m[(size_t)i * N + j]);
+    std::fputc('\n', f);
+  }
+  std::fclose(f);
+}
+
+inline void SaveMatrix(const std::vector<int>& m, const std::string& filename) {
+  const int n = rt::side_of(m);
+  SaveMatrix(m, n, n, filename);
+}
+
+inline std::vector<int> LoadMatrix(const std::string& filename) {
+  FILE* f = std::fopen(filename.c_str(), "r");
+  if (!f) throw rt::Error(RT_EIO, "Cannot open file: " + filename);
+  std::vector<int> out;
+  int c, v = 0;
+  bool in = false;
+  while ((c = std::fgetc(f)) != EOF) {
+    if (c >= '0' && c <= '9') {
+      v = in ? 10 * v + (c - '0') : (c - '0');
+      in = true;
+    } else if (in) {
+      out.push_back(v);
+      in = false;
+    }
+  }
+  if (in) out.push_back(v);
+  std::fclose(f);
+  return out;
+}
+
+// Zero padding around an M x N matrix (common.cpp:4-38).
+inline void PadMatrix(std::vector<int>& v, int M, int N) {
+  std::vector<int> p((size_t)(M + 2) * (N + 2), 0);
+  for (int i = 0; i < M; i++)
+    for (int j = 0; j < N; j++) p[(size_t)(i + 1) * (N + 2) + j + 1] = v[(size_t)i * N + j];
+  v.swap(p);
+}
+
+inline void UnPadMatrix(std::vector<int>& v, int M, int N) {
+  std::vector<int> u((size_t)(M - 2) * (N - 2));
+  for (int i = 0; i < M - 2; i++)
+    for (int j = 0; j < N - 2; j++) u[(size_t)i * (N - 2) + j] = v[(size_t)(i + 1) * N + j + 1];
+  v.swap(u);
+}
 
 // ------------------------------------------------------------ DominoTiler
 class DominoTiler : public rt::Device {

@@ -49,6 +49,20 @@ class CftpStats(C.Structure):
                 ("steps_run", C.c_long), ("device_ms", C.c_double)]
 
 
+class Request(C.Structure):
+    _fields_ = [("model", C.c_int), ("N", C.c_int), ("tmax", C.c_void_p), ("tmin", C.c_void_p),
+                ("master_seed", C.c_uint32), ("sched", C.c_int), ("t0", C.c_long),
+                ("max_rounds", C.c_int), ("rule", DominoRule)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("completed", C.c_long), ("queued", C.c_long), ("ticks", C.c_long),
+                ("p50_latency_s", C.c_double), ("p99_latency_s", C.c_double),
+                ("span_s", C.c_double), ("samples_per_s", C.c_double), ("status", C.c_int)]
+
+
+MODEL_DOMINO, MODEL_LOZENGE = 0, 1
+
 _lib = None
 _I32P = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _U32P = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
@@ -92,6 +106,15 @@ def lib():
                                        C.c_long, C.c_int, C.POINTER(DominoRule), _I32P, C.c_void_p]
     L.rt_domino_walk_batch.argtypes = [vp, C.c_void_p, C.c_void_p, C.c_int, C.c_long, _LONGP,
                                        C.POINTER(DominoRule), C.c_int]
+    L.rt_server_create.argtypes = [vp, C.c_size_t, C.c_int, C.POINTER(vp)]
+    L.rt_server_destroy.argtypes = [vp]
+    L.rt_server_destroy.restype = None
+    L.rt_server_submit.argtypes = [vp, C.POINTER(Request), C.POINTER(C.c_uint64)]
+    L.rt_server_poll.argtypes = [vp, C.c_uint64, C.c_void_p, C.POINTER(CftpStats),
+                                 C.POINTER(C.c_double)]
+    L.rt_server_wait.argtypes = [vp, C.c_uint64, C.c_double]
+    L.rt_server_cancel.argtypes = [vp, C.c_uint64]
+    L.rt_server_get_stats.argtypes = [vp, C.POINTER(ServerStats)]
     # host helpers returning rt_vec*
     L.rt_vec_data.argtypes = [vp]
     L.rt_vec_data.restype = C.POINTER(C.c_int32)
@@ -293,6 +316,68 @@ class Context:
             w = warr.ctypes.data_as(C.c_void_p)
         self._chk(self._L.rt_recttri_walk(self._h, t, side(t), steps, seed, w))
         return t
+
+
+class Server:
+    """Continuous-batching CFTP server (rt_server) on a Context."""
+
+    def __init__(self, ctx, hbm_budget, max_active=64):
+        self._L = lib()
+        self.ctx = ctx
+        h = C.c_void_p()
+        rc = self._L.rt_server_create(ctx._h, hbm_budget, max_active, C.byref(h))
+        if rc != RT_OK:
+            raise RTError(rc, "rt_server_create failed")
+        self._h = h
+        self._n = {}
+
+    def close(self):
+        if self._h:
+            self._L.rt_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, model, tmax, tmin, master_seed, sched=SCHED_DOUBLING, t0=1024,
+               max_rounds=32, rule=DOMINO_HEATBATH, a=1.0, b=1.0):
+        tmax, tmin = _i32(tmax), _i32(tmin)
+        q = Request(model, side(tmax), tmax.ctypes.data, tmin.ctypes.data, master_seed, sched, t0,
+                    max_rounds, DominoRule(rule, a, b))
+        rid = C.c_uint64()
+        rc = self._L.rt_server_submit(self._h, C.byref(q), C.byref(rid))
+        if rc != RT_OK:
+            raise RTError(rc, "submit rejected")
+        self._n[rid.value] = len(tmax)
+        return rid.value
+
+    def poll(self, rid):
+        """None while running; else (status, sample or None, stats, latency_s)."""
+        out = np.zeros(self._n[rid], dtype=np.int32)
+        st = CftpStats()
+        lat = C.c_double()
+        rc = self._L.rt_server_poll(self._h, rid, out.ctypes.data, C.byref(st), C.byref(lat))
+        if rc == -10:
+            return None
+        del self._n[rid]
+        stats = {"rounds": st.rounds, "total_steps": st.total_steps, "steps_run": st.steps_run}
+        return rc, (out if rc == RT_OK else None), stats, lat.value
+
+    def wait(self, rid, timeout_s=-1.0):
+        return self._L.rt_server_wait(self._h, rid, timeout_s) == RT_OK
+
+    def cancel(self, rid):
+        rc = self._L.rt_server_cancel(self._h, rid)
+        if rc != RT_OK:
+            raise RTError(rc, "cancel")
+
+    def stats(self):
+        st = ServerStats()
+        self._L.rt_server_get_stats(self._h, C.byref(st))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
 
 
 def parse_tinymt_file(path, count):

@@ -12,7 +12,7 @@ CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "librtiling.so")
 
-HIP_SOURCES = ["rt_device.hip", "rt_fused.hip", "rt_batch.hip"]
+HIP_SOURCES = ["rt_device.hip", "rt_fused.hip", "rt_batch.hip", "rt_serve.hip"]
 CXX_SOURCES = ["rt_host.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -35,10 +35,12 @@ struct DominoWeights {
 __device__ __forceinline__ uint32_t domino_move(uint32_t v, uint32_t b, int rule,
                                                 int i, int t, DominoWeights w) {
   if (rule == D_TOGGLE08) {
-    if (b < kBelow08) v = (v == 3u) ? 12u : ((v == 12u) ? 3u : v);
+    const uint32_t t = (v == 3u) ? 12u : ((v == 12u) ? 3u : v);
+    v = b < kBelow08 ? t : v;
   } else if (rule == D_HEATBATH) {
-    if (b < kBelow05) v = (v == 3u) ? 12u : v;
-    else v = (v == 12u) ? 3u : v;
+    // a flippable vertex ends in 12 if b < 2^31, else in 3
+    const bool flippable = v == 3u || v == 12u;
+    v = flippable ? (b < kBelow05 ? 12u : 3u) : v;
   } else {
     const float rd = tiny_to_float01(b);
     // Horizontal pair weight: row parity i; vertical pair weight: (i+t).
@@ -83,12 +85,9 @@ __device__ __forceinline__ uint32_t domino_rebuild(const P* p, int rs, int i, in
 // Cube add (25 -> 38) with rd < .4, remove (38 -> 25) with .4 <= rd < .8
 // (lozengekernel.cl:20-25).  `b` is the tempered draw (rd = (b>>9)*2^-23).
 __device__ __forceinline__ uint32_t lozenge_move(uint32_t v, uint32_t b) {
-  if (b < kBelow04) {
-    if (v == 25u) v = 38u;
-  } else if (b < kBelow08) {
-    if (v == 38u) v = 25u;
-  }
-  return v;
+  const bool up = b < kBelow04;
+  const bool down = !up && b < kBelow08;
+  return (up && v == 25u) ? 38u : ((down && v == 38u) ? 25u : v);
 }
 
 // Bits of a colour-x vertex that face colour x-1 ("next" of the rotated
@@ -107,6 +106,25 @@ __device__ __forceinline__ uint32_t lozenge_pull_prev(uint32_t v, const P* p, in
   v |= ((uint32_t)p[-p4] & 8u) >> 1;
   v |= ((uint32_t)p[rs] & 1u) << 5;
   return v & 0xffu;
+}
+
+// Same, with im = i % 3 precomputed.
+template <class P>
+__device__ __forceinline__ uint32_t lozenge_pull_prev_m(uint32_t v, const P* p, int rs, int im,
+                                                        int tr) {
+  const int p2 = im == ((2 * tr + 1) % 3);
+  const int p4 = im == ((2 * tr + 2) % 3);
+  const uint32_t a = p[-rs + p2], b = p[-p4], c = p[rs];
+  return (v & ~(2u | 4u | 32u) & 0xffu) | ((a & 16u) >> 3) | ((b & 8u) >> 1) | ((c & 1u) << 5);
+}
+
+template <class P>
+__device__ __forceinline__ uint32_t lozenge_pull_prev2_m(uint32_t v, const P* p, int rs, int im,
+                                                         int tr) {
+  const int p8 = im == ((2 * tr) % 3);
+  const int p16 = im == ((2 * tr + 1) % 3);
+  const uint32_t a = p[-rs], b = p[p8], c = p[rs - p16];
+  return (v & ~(1u | 8u | 16u) & 0xffu) | ((a & 32u) >> 5) | ((b & 4u) << 1) | ((c & 2u) << 3);
 }
 
 // Refresh the {1,8,16} bits of v from colour tr = x-2 (mod 3), i.e. x is

@@ -31,6 +31,24 @@ __global__ void k_zero_u32(uint32_t* p, int n) {
   if (i < n) p[i] = 0;
 }
 
+__global__ void k_differ_pair(const uint8_t* a, const uint8_t* b, long n, uint32_t* flag) {
+  bool diff = false;
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride)
+    diff |= a[k] != b[k];
+  if (__any(diff) && (threadIdx.x & (warpSize - 1)) == 0) atomicOr(flag, 1u);
+}
+
+// Sets *flag to 0, then to 1 if the two byte blocks differ anywhere.
+int launch_differ_pair(rt_ctx* ctx, const uint8_t* a, const uint8_t* b, long bytes,
+                       uint32_t* flag) {
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, flag, 1);
+  const unsigned nb = (unsigned)std::max<long>(1, std::min<long>((bytes + 4095) / 4096, 512));
+  hipLaunchKernelGGL(k_differ_pair, dim3(nb), dim3(256), 0, ctx->stream, a, b, bytes, flag);
+  HIPCHK_RET(ctx, hipGetLastError());
+  return RT_OK;
+}
+
 template <class F>
 static void parallel_for(int n, F f) {
   const int nt = std::max(1, std::min<int>(n, (int)std::thread::hardware_concurrency()));

@@ -37,13 +37,84 @@ __device__ __forceinline__ long stream_id(int i, int j, int rows, int cols) {
   return (long)j + (long)cols * i;
 }
 
+// Per-lane constants of a tile, fixed for a whole window.  Row r of a
+// thread is global row gr0 + W*r and LDS index li0 + W*r*RS, so every LDS
+// address of the step is li0 plus a compile-time offset.
+struct Lanes {
+  int li0;          // LDS index of the thread's row-0 cell
+  int gr0;          // its global row
+  int im0;          // gr0 mod 3
+  uint32_t hsm;     // bit r: row r owns a TinyMT stream (and moves)
+  uint32_t interm;  // bit r: row r is interior (pulls from neighbours)
+};
+
+__device__ __forceinline__ uint32_t pick(uint32_t mask_bits, int r, uint32_t yes, uint32_t no) {
+  const uint32_t m = 0u - ((mask_bits >> r) & 1u);
+  return (yes & m) | (no & ~m);
+}
+
+// One colour step on the LDS tile, specialised on the (workgroup-uniform)
+// pull pattern so the body is branch-free: every draw, neighbour read and
+// select of the R rows and NC chains can be scheduled together.
+template <int MODEL, int NC, int W, int R, int PULL, class Cell, int RS, int PL>
+__device__ __forceinline__ void sweep_step(Cell* lds, const Lanes& L, Tiny* st, uint32_t op,
+                                           const UnitDesc& U, const ModelParams& mp) {
+  constexpr int NCOL = MODEL == M_DOMINO ? 2 : 3;
+  const int c = op & OP_COLOUR;
+  uint32_t b[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) b[r] = tiny_bits(st[r]);
+#pragma unroll
+  for (int ch = 0; ch < NC; ch++) {
+    Cell* P = lds + ch * NCOL * PL + L.li0;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int off = r * W * RS;
+      const int gr = L.gr0 + W * r;
+      uint32_t v = (uint32_t)P[c * PL + off];
+      uint32_t nv;
+      if (MODEL == M_DOMINO) {
+        if (PULL & 1) {
+          const uint32_t pv = domino_rebuild(P + (1 - c) * PL + off, RS, gr, c);
+          v = pick(L.interm, r, pv, v);
+        }
+        nv = domino_move(v, b[r], U.rule, gr, 1 - c, U.dw);
+      } else if (MODEL == M_LOZENGE) {
+        const int im = (L.im0 + (W % 3) * r) % 3;
+        if (PULL & 1) {
+          const int ca = (c + 2) % 3;
+          const uint32_t pv = lozenge_pull_prev_m(v, P + ca * PL + off, RS, im, ca);
+          v = pick(L.interm, r, pv, v);
+        }
+        if (PULL & 2) {
+          const int cb = (c + 1) % 3;
+          const uint32_t pv = lozenge_pull_prev2_m(v, P + cb * PL + off, RS, im, cb);
+          v = pick(L.interm, r, pv, v);
+        }
+        nv = lozenge_move(v, b[r]);
+      } else {
+        if (PULL & 1) {
+          const int pv = (op >> 4) & 3;
+          const int32_t* src = (const int32_t*)(P + pv * PL + off);
+          const uint32_t w = c == (pv + 1) % 3 ? recttri_rebuild1(src, RS, gr, pv)
+                                               : recttri_rebuild2(src, RS, gr, pv);
+          v = pick(L.interm, r, w, v);
+        }
+        nv = ((L.hsm >> r) & 1u) ? recttri_move(v, b[r], mp.rw) : v;
+      }
+      P[c * PL + off] = (Cell)pick(L.hsm, r, nv, v);
+    }
+  }
+}
+
 // W wave64s per workgroup; each lane owns one tile column and R rows
 // (rows wv, wv+W, ...), so the loaded tile is (W*R) x 64 cells.
 template <int MODEL, int NC, int W, int R>
 __global__ __launch_bounds__(64 * W) void k_fused(const UnitDesc* __restrict__ units,
                                                     const WorkItem* __restrict__ work, long s0,
                                                     int K, int init, int seg, int H,
-                                                    ModelParams mp) {
+                                                    ModelParams mp,
+                                                    const WindowRec* __restrict__ wins) {
   using Cell = typename CellOf<MODEL>::type;
   constexpr int NCOL = MODEL == M_DOMINO ? 2 : 3;
   constexpr int kThreads = 64 * W;
@@ -55,6 +126,17 @@ __global__ __launch_bounds__(64 * W) void k_fused(const UnitDesc* __restrict__ u
 
   const WorkItem wi = work[blockIdx.x];
   const UnitDesc U = units[wi.unit];
+  uint32_t seed = 0;
+  if (wins) {
+    const WindowRec w = wins[wi.unit];
+    s0 = w.s0;
+    K = w.K;
+    init = w.init;
+    seed = w.seed;
+    if (K <= 0 && !init) return;  // unit idle in this launch
+  } else if (init) {
+    seed = U.seeds[seg];
+  }
   const int rows = U.rows, cols = U.cols;
   const int TH = LH - 2 * H, TW = kLW - 2 * H;
   const int tr = wi.tile / U.tiles_c, tc = wi.tile % U.tiles_c;
@@ -68,27 +150,35 @@ __global__ __launch_bounds__(64 * W) void k_fused(const UnitDesc* __restrict__ u
   __syncthreads();
 
   Tiny st[R];
-  bool hs[R];
+  Lanes L;
+  L.li0 = (wv + 1) * RS + lane + 1;
+  L.gr0 = r0 + wv;
+  L.im0 = ((L.gr0 % 3) + 3) % 3;
+  L.hsm = 0;
+  L.interm = 0;
 #pragma unroll
   for (int r = 0; r < R; r++) {
     const int lr = wv + W * r, gr = r0 + lr;
     const bool in = col_in && gr >= 0 && gr < rows;
+    const int li = L.li0 + r * W * RS;
+    if (col_in && gr >= 1 && gr <= rows - 2 && gc >= 1 && gc <= cols - 2) L.interm |= 1u << r;
     if (in) {
 #pragma unroll
       for (int ch = 0; ch < NC; ch++)
 #pragma unroll
         for (int k = 0; k < NCOL; k++)
-          lds[(ch * NCOL + k) * PL + (lr + 1) * RS + lane + 1] =
-              ((const Cell*)U.plane[ch][k])[(long)gr * cols + gc];
+          lds[(ch * NCOL + k) * PL + li] = ((const Cell*)U.plane[ch][k])[(long)gr * cols + gc];
     }
-    hs[r] = in && has_stream<MODEL>(gr, gc, rows, cols);
-    if (hs[r]) {
+    const bool hs = in && has_stream<MODEL>(gr, gc, rows, cols);
+    if (hs) L.hsm |= 1u << r;
+    st[r] = Tiny{0, 0, 0, 0, 0, 0, 0};
+    if (hs) {
       const long id = stream_id<MODEL>(gr, gc, rows, cols);
       st[r].mat1 = U.param[id];
       st[r].mat2 = U.param[U.pstride + id];
       st[r].tmat = U.param[2 * U.pstride + id];
       if (init) {
-        tiny_init(st[r], U.seeds[seg] + (uint32_t)id);
+        tiny_init(st[r], seed + (uint32_t)id);
       } else {
         st[r].s0 = U.state[id];
         st[r].s1 = U.state[U.nstream + id];
@@ -101,39 +191,17 @@ __global__ __launch_bounds__(64 * W) void k_fused(const UnitDesc* __restrict__ u
 
   for (int s = 0; s < K; s++) {
     const uint32_t op = sops[s];
-    const int c = op & OP_COLOUR;
-#pragma unroll
-    for (int r = 0; r < R; r++) {
-      const int lr = wv + W * r, gr = r0 + lr;
-      uint32_t rd = 0;
-      if (hs[r]) rd = tiny_bits(st[r]);
-      const bool inter = col_in && gr >= 1 && gr <= rows - 2 && gc >= 1 && gc <= cols - 2;
-      const int li = (lr + 1) * RS + lane + 1;
-#pragma unroll
-      for (int ch = 0; ch < NC; ch++) {
-        Cell* P = lds + ch * NCOL * PL;
-        uint32_t v = (uint32_t)P[c * PL + li];
-        if (MODEL == M_DOMINO) {
-          if ((op & OP_PULL_A) && inter) v = domino_rebuild(P + (1 - c) * PL + li, RS, gr, c);
-          if (hs[r]) v = domino_move(v, rd, mp.rule, gr, 1 - c, mp.dw);
-        } else if (MODEL == M_LOZENGE) {
-          if (inter) {
-            const int ca = (c + 2) % 3, cb = (c + 1) % 3;
-            if (op & OP_PULL_A) v = lozenge_pull_prev(v, P + ca * PL + li, RS, gr, ca);
-            if (op & OP_PULL_B) v = lozenge_pull_prev2(v, P + cb * PL + li, RS, gr, cb);
-          }
-          if (hs[r]) v = lozenge_move(v, rd);
-        } else {
-          if ((op & OP_PULL_A) && inter) {
-            const int pv = (op >> 4) & 3;
-            const int32_t* src = (const int32_t*)(P + pv * PL + li);
-            v = c == (pv + 1) % 3 ? recttri_rebuild1(src, RS, gr, pv)
-                                  : recttri_rebuild2(src, RS, gr, pv);
-          }
-          if (hs[r]) v = recttri_move(v, rd, mp.rw);
-        }
-        P[c * PL + li] = (Cell)v;
+    const int pull = (op >> 2) & 3;
+    if (MODEL == M_LOZENGE) {
+      switch (pull) {
+        case 0: sweep_step<MODEL, NC, W, R, 0, Cell, RS, PL>(lds, L, st, op, U, mp); break;
+        case 1: sweep_step<MODEL, NC, W, R, 1, Cell, RS, PL>(lds, L, st, op, U, mp); break;
+        case 2: sweep_step<MODEL, NC, W, R, 2, Cell, RS, PL>(lds, L, st, op, U, mp); break;
+        default: sweep_step<MODEL, NC, W, R, 3, Cell, RS, PL>(lds, L, st, op, U, mp); break;
       }
+    } else {
+      if (pull & 1) sweep_step<MODEL, NC, W, R, 1, Cell, RS, PL>(lds, L, st, op, U, mp);
+      else sweep_step<MODEL, NC, W, R, 0, Cell, RS, PL>(lds, L, st, op, U, mp);
     }
     __syncthreads();
   }
@@ -149,8 +217,8 @@ __global__ __launch_bounds__(64 * W) void k_fused(const UnitDesc* __restrict__ u
 #pragma unroll
       for (int k = 0; k < NCOL; k++)
         ((Cell*)U.plane[ch][k])[(long)gr * cols + gc] =
-            lds[(ch * NCOL + k) * PL + (lr + 1) * RS + lane + 1];
-    if (hs[r]) {
+            lds[(ch * NCOL + k) * PL + L.li0 + r * W * RS];
+    if ((L.hsm >> r) & 1u) {
       const long id = stream_id<MODEL>(gr, gc, rows, cols);
       U.state[id] = st[r].s0;
       U.state[U.nstream + id] = st[r].s1;
@@ -225,13 +293,14 @@ size_t fused_scratch_bytes(const FusedJob& job) {
 
 template <int MODEL, int NC, int W, int R>
 static void launch(rt_ctx* ctx, unsigned nblocks, const UnitDesc* units, const WorkItem* work,
-                   long s0, int K, int init, int seg, int H, const ModelParams& mp) {
+                   long s0, int K, int init, int seg, int H, const ModelParams& mp,
+                   const WindowRec* wins) {
   hipLaunchKernelGGL((k_fused<MODEL, NC, W, R>), dim3(nblocks), dim3(64 * W), 0, ctx->stream,
-                     units, work, s0, K, init, seg, H, mp);
+                     units, work, s0, K, init, seg, H, mp, wins);
 }
 
 typedef void (*LaunchFn)(rt_ctx*, unsigned, const UnitDesc*, const WorkItem*, long, int, int, int,
-                         int, const ModelParams&);
+                         int, const ModelParams&, const WindowRec*);
 
 static LaunchFn pick(int model, int nc) {
   if (model == M_DOMINO) return nc == 1 ? launch<M_DOMINO, 1, 8, 8> : launch<M_DOMINO, 2, 8, 8>;
@@ -258,6 +327,8 @@ int fused_upload(rt_ctx* ctx, FusedJob& job, void* scratch, FusedPlan* plan) {
     int tr, tc;
     tile_counts(job.units[u], LH, H, &tr, &tc);
     job.units[u].tiles_c = tc;
+    job.units[u].rule = job.mp.rule;
+    job.units[u].dw = job.mp.dw;
     for (int t = 0; t < tr * tc; t++) work.push_back(WorkItem{(int)u, t});
   }
   WorkItem* d_work = (WorkItem*)p;
@@ -319,12 +390,31 @@ int fused_launch(rt_ctx* ctx, const FusedJob& job, const FusedPlan& plan) {
   LaunchFn fn = pick(job.model, job.nc);
   const int H = fused_halo(job);
   for (const FusedPlan::Window& w : plan.windows)
-    fn(ctx, plan.nwork, plan.d_units, plan.d_work, w.s0, w.K, w.init, w.seg, H, job.mp);
+    fn(ctx, plan.nwork, plan.d_units, plan.d_work, w.s0, w.K, w.init, w.seg, H, job.mp, nullptr);
   HIPCHK_RET(ctx, hipGetLastError());
   ctx->last.launches += (long)plan.windows.size();
   ctx->last.steps += plan.total_steps;
   ctx->last.attempts += plan.total_steps * plan.streams;
   return RT_OK;
+}
+
+int fused_launch_windows(rt_ctx* ctx, int model, int nc, const ModelParams& mp,
+                         const UnitDesc* d_units, const WorkItem* d_work, unsigned nwork,
+                         const WindowRec* d_wins) {
+  if (nwork == 0) return RT_OK;
+  const Geo g = geometry(model, nc);
+  pick(model, nc)(ctx, nwork, d_units, d_work, 0, 0, 0, 0, g.H, mp, d_wins);
+  HIPCHK_RET(ctx, hipGetLastError());
+  ctx->last.launches += 1;
+  return RT_OK;
+}
+
+void fused_tiles(int model, int nc, int rows, int cols, int* tiles_r, int* tiles_c) {
+  const Geo g = geometry(model, nc);
+  UnitDesc u{};
+  u.rows = rows;
+  u.cols = cols;
+  tile_counts(u, g.W * g.R, g.H, tiles_r, tiles_c);
 }
 
 int fused_run(rt_ctx* ctx, FusedJob& job, void* scratch) {

@@ -43,6 +43,18 @@ struct UnitDesc {
   const uint32_t* seeds;   // TinyMT seed per segment
   int rows, cols;          // compressed grid
   int tiles_c;             // tile columns
+  int rule;                // domino flip rule of this unit
+  DominoWeights dw;        // domino two-periodic weights of this unit
+};
+
+// Per-unit window for launches where every unit is at its own step (the
+// serving loop): ops are read from unit.ops[s0 .. s0+K); init re-seeds the
+// unit's TinyMT streams with `seed` before the first step.
+struct WindowRec {
+  long s0;
+  int K;
+  int init;
+  uint32_t seed;
   int pad;
 };
 
@@ -104,6 +116,14 @@ struct FusedPlan {
 };
 int fused_upload(rt_ctx* ctx, FusedJob& job, void* scratch, FusedPlan* plan);
 int fused_launch(rt_ctx* ctx, const FusedJob& job, const FusedPlan& plan);
+
+// One launch over `nwork` work items where each unit follows its own
+// window (wins[unit]); model / chains / params as in `job` (units unused).
+int fused_launch_windows(rt_ctx* ctx, int model, int nc, const ModelParams& mp,
+                         const UnitDesc* d_units, const WorkItem* d_work, unsigned nwork,
+                         const WindowRec* d_wins);
+// Tile counts of a unit under the job geometry of (model, nc).
+void fused_tiles(int model, int nc, int rows, int cols, int* tiles_r, int* tiles_c);
 // Bring every colour plane of every unit up to date (end of a walk).
 int fused_sync(rt_ctx* ctx, const FusedJob& job);
 

@@ -25,6 +25,9 @@ int main(int argc, char** argv) {
   s = 0;
   for (size_t k = 0; k < l.size(); k++) s += (long)(k % 89 + 1) * l[k];
   std::printf("loz %zu %ld\n", l.size(), s);
+  SaveMatrix(t, "saved.txt");
+  tiling back = LoadMatrix("saved.txt");
+  std::printf("io %d\n", back == t ? 1 : 0);
   try {
     domain bad = d;
     for (size_t k = 0; k < bad.size(); k++) if (bad[k]) { bad[k] = 0; break; }
@@ -70,14 +73,16 @@ def _build(tmp_path):
 
 def test_cpp_dropin_statics(tmp_path):
     exe = _build(tmp_path)
-    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True,
+                         cwd=tmp_path).stdout.split("\n")
     d = rt.DominoTiler.AztecDiamond(16)
     t = rt.DominoTiler.MinTiling(d)
     assert out[0] == f"min {len(t)} {_checksum(t, 97)}"
     dl = rt.LozengeTiler.AlmostHexDomain(7, 2)
     l = rt.LozengeTiler.MaxTiling(dl)
     assert out[1] == f"loz {len(l)} {_checksum(l, 89)}"
-    assert out[2] == "untileable -7"
+    assert out[2] == "io 1"
+    assert out[3] == "untileable -7"
 
 
 @pytest.mark.gpu
@@ -85,14 +90,33 @@ def test_cpp_dropin_walk_and_cftp(tmp_path):
     from tests import oracle_lib as o
     exe = _build(tmp_path)
     print("[case] C++ drop-in walk + cftp", flush=True)
-    out = subprocess.run([str(exe), "gpu"], check=True, capture_output=True,
-                         text=True).stdout.split("\n")
+    out = subprocess.run([str(exe), "gpu"], check=True, capture_output=True, text=True,
+                         cwd=tmp_path).stdout.split("\n")
     d = rt.DominoTiler.AztecDiamond(16)
     t = rt.DominoTiler.MinTiling(d)
     n = len(t) // 2
     p = ((np.arange(3 * n, dtype=np.uint64) * np.uint64(2654435761) + np.uint64(12345))
          & np.uint64(0xFFFFFFFF)).astype(np.uint32)
     w = o.domino_walk(t, 1000, 1, p, 1)
-    assert out[3] == f"walk {_checksum(w, 97)}"
+    assert out[4] == f"walk {_checksum(w, 97)}"
     c, _ = o.domino_cftp(rt.DominoTiler.MaxTiling(d), t, 345903, p, 0, 128, 64, 1)
-    assert out[4] == f"cftp {_checksum(c, 97)}"
+    assert out[5] == f"cftp {_checksum(c, 97)}"
+
+
+def test_savematrix_format_matches_reference_file(tmp_path):
+    """SaveMatrix(MinTiling(AztecDiamond(16))) from the C++ drop-in equals the
+    file the reference ships (WeightedAztecDiamondEx.cpp:41), byte for byte."""
+    prog = tmp_path / "save.cpp"
+    prog.write_text('''
+#include "rtiling_tilers.hpp"
+int main() { SaveMatrix(DominoTiler::MinTiling(DominoTiler::AztecDiamond(16)), "TilingStart.txt"); }
+''')
+    exe = tmp_path / "save"
+    lib = os.path.join(ROOT, "randomtilingspublic_amd")
+    subprocess.run(["g++", "-std=c++17", "-I", os.path.join(ROOT, "include"), str(prog), "-L", lib,
+                    "-lrtiling", f"-Wl,-rpath,{lib}", "-o", str(exe)], check=True)
+    subprocess.run([str(exe)], check=True, cwd=tmp_path)
+    ours = (tmp_path / "TilingStart.txt").read_bytes()
+    ref = open(os.path.join(ROOT, "tests", "golden", "reference_outputs", "aztec16_min_tiling.txt"),
+               "rb").read()
+    assert ours == ref

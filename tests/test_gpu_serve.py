@@ -1,0 +1,112 @@
+"""Serving: every request's sample equals the same request run alone (and
+the oracle on small cases), whatever else shares the device with it."""
+import time
+
+import numpy as np
+import pytest
+
+import randomtilingspublic_amd as rt
+from tests import oracle_lib as o
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print("[case]", *a, flush=True)
+
+
+def _domino(n):
+    d = rt.DominoTiler.AztecDiamond(n)
+    return rt.DominoTiler.MaxTiling(d), rt.DominoTiler.MinTiling(d)
+
+
+def _lozenge(n, m=0):
+    d = rt.LozengeTiler.AlmostHexDomain(n, m)
+    return rt.LozengeTiler.MaxTiling(d), rt.LozengeTiler.MinTiling(d)
+
+
+def _mix():
+    reqs = []
+    for k, n in enumerate([3, 6, 10, 5]):
+        tmax, tmin = _domino(n)
+        reqs.append(dict(model=rt.MODEL_DOMINO, tmax=tmax, tmin=tmin, master_seed=100 + k,
+                         sched=rt.SCHED_DOUBLING, t0=128, rule=rt.DOMINO_HEATBATH))
+    tmax, tmin = _domino(7)
+    reqs.append(dict(model=rt.MODEL_DOMINO, tmax=tmax, tmin=tmin, master_seed=7,
+                     sched=rt.SCHED_REFERENCE, t0=128, rule=rt.DOMINO_HEATBATH2P, a=1.0, b=0.5))
+    for k, (n, m) in enumerate([(4, 0), (7, 2), (12, 0)]):
+        tmax, tmin = _lozenge(n, m)
+        reqs.append(dict(model=rt.MODEL_LOZENGE, tmax=tmax, tmin=tmin, master_seed=200 + k,
+                         sched=rt.SCHED_DOUBLING, t0=64))
+    return reqs
+
+
+def _alone(ctx, q):
+    if q["model"] == rt.MODEL_DOMINO:
+        out, st = ctx.domino_cftp(q["tmax"], q["tmin"], q["master_seed"], q["sched"], q["t0"], 32,
+                                  q["rule"], q.get("a", 1.0), q.get("b", 1.0))
+    else:
+        out, st = ctx.lozenge_cftp(q["tmax"], q["tmin"], q["master_seed"], q["sched"], q["t0"],
+                                   32)
+    return out, st
+
+
+def test_served_equals_alone(gpu_ctx):
+    reqs = _mix()
+    big = max(rt.side(q["tmax"]) for q in reqs)
+    P = rt.synthetic_params(big * big, 31)
+    gpu_ctx.set_tinymt_params(P)
+    alone = [_alone(gpu_ctx, q) for q in reqs]
+    _say("server mix of", len(reqs))
+    srv = rt.Server(gpu_ctx, 256 << 20, 64)
+    try:
+        ids = []
+        for k, q in enumerate(reqs):
+            ids.append(srv.submit(max_rounds=32, **q))
+            if k % 3 == 2:
+                time.sleep(0.01)  # later requests arrive while earlier ones run
+        for rid, q, (want, wst) in zip(ids, reqs, alone):
+            assert srv.wait(rid, 120.0)
+            rc, got, st, lat = srv.poll(rid)
+            assert rc == 0
+            assert np.array_equal(got, want)
+            assert st["rounds"] == wst["rounds"]
+        s = srv.stats()
+        assert s["completed"] == len(reqs) and s["status"] == 0
+    finally:
+        srv.close()
+    # and the small domino ones against the oracle directly
+    q = reqs[0]
+    w, _ = o.domino_cftp(q["tmax"], q["tmin"], q["master_seed"], P, q["sched"], q["t0"], 32,
+                         q["rule"])
+    assert np.array_equal(alone[0][0], w)
+
+
+def test_tight_budget_queues_and_cancel(gpu_ctx):
+    reqs = _mix()[:4]
+    big = max(rt.side(q["tmax"]) for q in reqs)
+    gpu_ctx.set_tinymt_params(rt.synthetic_params(big * big, 32))
+    alone = [_alone(gpu_ctx, q) for q in reqs]
+    # a budget that holds only one of these at a time
+    tmax, _ = _domino(10)
+    N = rt.side(tmax)
+    one = 16 * (N - 2) * (N // 2 - 2) + 4 * 2 * N * (N // 2) + 4096
+    _say("server tight budget", one)
+    srv = rt.Server(gpu_ctx, one, 8)
+    try:
+        with pytest.raises(rt.RTError) as e:
+            tm, tn = _domino(40)
+            srv.submit(rt.MODEL_DOMINO, tm, tn, 1)
+        assert e.value.code == -8  # larger than the whole budget
+        ids = [srv.submit(max_rounds=32, **q) for q in reqs]
+        victim = srv.submit(rt.MODEL_DOMINO, reqs[1]["tmax"], reqs[1]["tmin"], 999, t0=128)
+        srv.cancel(victim)
+        for rid, (want, _) in zip(ids, alone):
+            assert srv.wait(rid, 120.0)
+            rc, got, _, _ = srv.poll(rid)
+            assert rc == 0 and np.array_equal(got, want)
+        assert srv.wait(victim, 60.0)
+        rc, got, _, _ = srv.poll(victim)
+        assert rc in (-9, 0)  # cancelled unless it had already finished
+    finally:
+        srv.close()
