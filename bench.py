@@ -70,21 +70,68 @@ def cpu_baseline(N, tmin, params, units_attempts, nsamples, threads, budget_s=12
     }
 
 
+def copy_probe(mib):
+    """Plain device-to-device copy of `mib` MiB, five times: a kernel whose
+    HBM bytes are known (mib MiB read, mib MiB written per launch), to scale
+    the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = mib * (1 << 20) // 4
+    x = torch.arange(n, dtype=torch.int32, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev[0].record()
+    for _ in range(5):
+        y.copy_(x)
+    ev[1].record()
+    torch.cuda.synchronize()
+    ms = ev[0].elapsed_time(ev[1]) / 5
+    assert torch.equal(x, y)
+    print(json.dumps({"copy_probe_mib": mib, "bytes_read": n * 4, "bytes_written": n * 4,
+                      "avg_ms": round(ms, 4), "GBs": round(2 * n * 4 / ms / 1e6, 1)}), flush=True)
+
+
+def profile_rounds(ctx, rt, tmax, tmin, masters, t0, rounds):
+    """The benchmark's batch, stopped after `rounds` doubling rounds: every
+    launch has the timed run's shape, so per-launch kernel time matches."""
+    w = time.time()
+    try:
+        ctx.lozenge_cftp_batch(tmax, tmin, masters, rt.SCHED_DOUBLING, t0, rounds)
+    except rt.RTError:
+        pass  # no coalescence within the cap is expected here
+    tm = ctx.last_timing()
+    kl = max(tm["kernel_launches"], 1)
+    print(json.dumps({"profile_rounds": rounds, "samples": len(masters), "wall_s": time.time() - w,
+                      "kernel_ms": tm["kernel_ms"], "kernel_launches": tm["kernel_launches"],
+                      "avg_launch_us": round(tm["kernel_ms"] * 1e3 / kl, 2),
+                      "sites_per_launch": int(tm["kernel_sites"] / kl),
+                      "steps": tm["steps"]}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=32, help="CFTP samples (master seeds 1..K)")
+    ap.add_argument("--steps", type=int, default=4, help="CFTP samples (master seeds 1..K)")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--side", type=int, default=300)
     ap.add_argument("--t0", type=int, default=1024)
     ap.add_argument("--max-rounds", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--profile-rounds", type=int, default=0,
+                    help="profiling run: the same batch capped at R doubling rounds (no sample "
+                         "needs to finish); prints the sweep kernel's timing only")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="MiB: time a plain device copy of that size (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
 
     import randomtilingspublic_amd as rt
 
@@ -101,6 +148,9 @@ def main():
     for w in range(args.warmup):
         ctx.lozenge_walk(tmin, 2048, 1000 + w)
     masters = list(range(1, args.steps + 1))
+    if args.profile_rounds:
+        profile_rounds(ctx, rt, tmax, tmin, masters, args.t0, args.profile_rounds)
+        return
     t0 = time.time()
     outs, stats = ctx.lozenge_cftp_batch(tmax, tmin, masters, rt.SCHED_DOUBLING, args.t0,
                                          args.max_rounds)
@@ -138,15 +188,23 @@ def main():
     sites_per_launch = tm["kernel_sites"] / klaunch
     alg_bytes = 56.0 * sites_per_launch
     achieved_gbs = alg_bytes / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
+    traffic = None
+    counters = os.path.join(ROOT, "profiles", "k_fused_lozenge2_hex300_hbm.json")
+    if os.path.exists(counters) and side == 300:
+        # FETCH_SIZE / WRITE_SIZE per dispatch of this kernel on this workload,
+        # scaled by a plain copy's readings, as bytes per site: times this
+        # run's sites per launch.
+        with open(counters) as f:
+            traffic = int(json.load(f)["hbm_bytes_per_site"] * sites_per_launch)
     roof = {"bound": "hbm", "achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+            "unit": "GB/s", "frac": round(achieved_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
             "kernel": "k_fused<lozenge, 2 chains>", "avg_launch_us": round(avg_launch_s * 1e6, 2),
             "launches": klaunch, "steps_per_launch": round(tm["steps"] / klaunch, 2),
             "sites_per_launch": int(sites_per_launch), "alg_bytes_per_launch": int(alg_bytes),
             "flip_attempts_per_s_in_kernel": round(2 * draws / (kms / 1e3), 1) if kms else None}
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(N, tmin, params, attempts, len(masters), os.cpu_count() or 1)
+        cpu = cpu_baseline(N, tmin, params, attempts, len(masters), min(os.cpu_count() or 1, 16))
         print(f"[bench] cpu baseline {cpu}", file=sys.stderr, flush=True)
 
     line = {

@@ -209,6 +209,15 @@ int rt_server_get_stats(rt_server* srv, rt_server_stats* out);
 int rt_recttri_walk(rt_ctx* ctx, int32_t* tiling, int N, long steps,
                     long seed, const float* weights16);
 
+/* ------------------------------------------------ bibone (triangle dimer) */
+/* TriangleDimerTiler::Walk (TriangleDimerTiler.cpp:101-695): in place on
+ * the N*N lozenge-type tiling (N even); the triangle- and butterfly-type
+ * states are rebuilt from it on entry, as the reference does.  Same
+ * schedule (default_random_engine(seed): move type, r1, r2 and, for
+ * butterfly steps, p1, p2) and TinyMT streams (id j + (N/2)*i, seeded
+ * seed + id).  Needs >= (N/2)^2 parameter sets. */
+int rt_bibone_walk(rt_ctx* ctx, int32_t* tiling, int N, long steps, long seed);
+
 /* ---------------------------------------------- host-side domain helpers */
 /* An owned int32 vector returned by the helpers below. */
 typedef struct rt_vec rt_vec;
@@ -243,6 +252,15 @@ rt_vec* rt_lozenge_domain_to_vertices(const int32_t* d, size_t dsize);    /* 483
 /* RectTriangleTiler statics (RectTriangleTiler.h:77-82). */
 rt_vec* rt_recttri_max_hex(int n);                       /* RectTriangleTiler.cpp:480-538 */
 rt_vec* rt_recttri_min_hex(int n);                       /* 540-600 */
+
+/* TriangleDimerTiler statics (TriangleDimerTiler.h:194-209). */
+rt_vec* rt_bibone_ice_cream_cone(int N, int M);          /* TriangleDimerTiler.cpp:697-793 */
+rt_vec* rt_bibone_test_butterfly(int type);              /* 795-844 */
+rt_vec* rt_bibone_triangle_flips(const int32_t* t, size_t n);    /* 846-873 */
+rt_vec* rt_bibone_butterfly_flips(const int32_t* t, size_t n);   /* 875-921 */
+rt_vec* rt_bibone_tiling_to_vertices(const int32_t* t, size_t n); /* 923-949 */
+rt_vec* rt_bibone_tiling_to_domain(const int32_t* t, size_t n);  /* 951-982 */
+rt_vec* rt_bibone_dimer_indicator(const int32_t* t, size_t n, int c); /* 35-95 */
 
 #ifdef __cplusplus
 }

@@ -258,3 +258,33 @@ class RectTriangleTiler : public rt::Device {
  private:
   std::vector<float> w_;
 };
+
+// ----------------------------------------------------- TriangleDimerTiler
+// Bibone (triangular-lattice dimer) model (TriangleDimerTiler.h:124-209).
+class TriangleDimerTiler : public rt::Device {
+ public:
+  explicit TriangleDimerTiler(int device = 0) : rt::Device(device) {}
+
+  // TriangleDimerTiler.cpp:101-695.  LoadTinyMT's size must be >= (N/2)^2.
+  void Walk(tiling& t, int steps, long seed) {
+    check(rt_bibone_walk(ctx(), t.data(), rt::side_of(t), steps, seed));
+  }
+
+  static tiling IceCreamCone(int N, int M) { return rt::take(rt_bibone_ice_cream_cone(N, M)); }
+  static tiling TestButterfly(int t) { return rt::take(rt_bibone_test_butterfly(t)); }
+  static std::vector<int> TilingToTriangleFlips(tiling& t) {
+    return rt::take(rt_bibone_triangle_flips(t.data(), t.size()));
+  }
+  static std::vector<int> TilingToButterflyFlips(tiling& t) {
+    return rt::take(rt_bibone_butterfly_flips(t.data(), t.size()));
+  }
+  static std::vector<int> TilingToVertices(tiling& t) {
+    return rt::take(rt_bibone_tiling_to_vertices(t.data(), t.size()));
+  }
+  static std::vector<int> DimerIndicator(tiling& t, int c) {
+    return rt::take(rt_bibone_dimer_indicator(t.data(), t.size(), c));
+  }
+  static domain TilingToDomain(tiling& t) {
+    return rt::take(rt_bibone_tiling_to_domain(t.data(), t.size()));
+  }
+};

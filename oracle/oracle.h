@@ -118,6 +118,19 @@ int orc_recttri_walk(int32_t* t, int N, long steps, long seed,
                      const uint32_t* params3, size_t nparams,
                      const float* weights16);
 
+/* ---- bibone (triangledimerkernel.cl, TriangleDimerTiler.cpp) ---- */
+/* Derived flip-state arrays of an N x N tiling (TriangleDimerTiler.cpp:
+ * 846-921): tri is (N/2) x N, bf is N x N. */
+void orc_bibone_triangle_flips(const int32_t* t, int N, int32_t* tri);
+void orc_bibone_butterfly_flips(const int32_t* t, int N, int32_t* bf);
+/* Move schedule: five ints per step (type, r1, r2, p1, p2; -1 = none). */
+void orc_bibone_schedule(long seed, long steps, int32_t* out5);
+/* Walk (TriangleDimerTiler.cpp:101-695), in place.  With `planes` non-NULL
+ * the eight final M x M planes (eH, eL, eR, tU, tD, bH, bL, bR; M = N/2)
+ * are copied out as well. */
+int orc_bibone_walk(int32_t* t, int N, long steps, long seed,
+                    const uint32_t* params3, size_t nparams, int32_t* planes);
+
 /* Number of OpenMP threads the oracle uses (1 = serial). */
 void orc_set_threads(int n);
 int orc_get_threads(void);

@@ -77,4 +77,27 @@ long ref_recttri_slope_hex(int n, int32_t* out, long cap) {
 long ref_bibone_ice_cream_cone(int n, int m, int32_t* out, long cap) {
   return give(TriangleDimerTiler::IceCreamCone(n, m), out, cap);
 }
+long ref_bibone_test_butterfly(int type, int32_t* out, long cap) {
+  return give(TriangleDimerTiler::TestButterfly(type), out, cap);
+}
+long ref_bibone_triangle_flips(const int32_t* t, long nt, int32_t* out, long cap) {
+  std::vector<int> v = vec(t, nt);
+  return give(TriangleDimerTiler::TilingToTriangleFlips(v), out, cap);
+}
+long ref_bibone_butterfly_flips(const int32_t* t, long nt, int32_t* out, long cap) {
+  std::vector<int> v = vec(t, nt);
+  return give(TriangleDimerTiler::TilingToButterflyFlips(v), out, cap);
+}
+long ref_bibone_tiling_to_vertices(const int32_t* t, long nt, int32_t* out, long cap) {
+  std::vector<int> v = vec(t, nt);
+  return give(TriangleDimerTiler::TilingToVertices(v), out, cap);
+}
+long ref_bibone_tiling_to_domain(const int32_t* t, long nt, int32_t* out, long cap) {
+  std::vector<int> v = vec(t, nt);
+  return give(TriangleDimerTiler::TilingToDomain(v), out, cap);
+}
+long ref_bibone_dimer_indicator(const int32_t* t, long nt, int c, int32_t* out, long cap) {
+  std::vector<int> v = vec(t, nt);
+  return give(TriangleDimerTiler::DimerIndicator(v, c), out, cap);
+}
 }

@@ -100,6 +100,7 @@ def lib():
     L.rt_lozenge_cftp.argtypes = [vp, _I32P, _I32P, C.c_int, C.c_uint32, C.c_int, C.c_long,
                                   C.c_int, _I32P, C.POINTER(CftpStats)]
     L.rt_recttri_walk.argtypes = [vp, _I32P, C.c_int, C.c_long, C.c_long, C.c_void_p]
+    L.rt_bibone_walk.argtypes = [vp, _I32P, C.c_int, C.c_long, C.c_long]
     L.rt_lozenge_cftp_batch.argtypes = [vp, _I32P, _I32P, C.c_int, _U32P, C.c_int, C.c_int,
                                         C.c_long, C.c_int, _I32P, C.c_void_p]
     L.rt_domino_cftp_batch.argtypes = [vp, _I32P, _I32P, C.c_int, _U32P, C.c_int, C.c_int,
@@ -137,6 +138,13 @@ def lib():
         ("rt_lozenge_domain_to_vertices", [_I32P, C.c_size_t]),
         ("rt_recttri_max_hex", [C.c_int]),
         ("rt_recttri_min_hex", [C.c_int]),
+        ("rt_bibone_ice_cream_cone", [C.c_int, C.c_int]),
+        ("rt_bibone_test_butterfly", [C.c_int]),
+        ("rt_bibone_triangle_flips", [_I32P, C.c_size_t]),
+        ("rt_bibone_butterfly_flips", [_I32P, C.c_size_t]),
+        ("rt_bibone_tiling_to_vertices", [_I32P, C.c_size_t]),
+        ("rt_bibone_tiling_to_domain", [_I32P, C.c_size_t]),
+        ("rt_bibone_dimer_indicator", [_I32P, C.c_size_t, C.c_int]),
     ]:
         f = getattr(L, name)
         f.argtypes = args
@@ -317,6 +325,12 @@ class Context:
         self._chk(self._L.rt_recttri_walk(self._h, t, side(t), steps, seed, w))
         return t
 
+    def bibone_walk(self, t, steps, seed):
+        """TriangleDimerTiler::Walk: returns the tiling after `steps` moves."""
+        t = _i32(t).copy()
+        self._chk(self._L.rt_bibone_walk(self._h, t, side(t), steps, seed))
+        return t
+
 
 class Server:
     """Continuous-batching CFTP server (rt_server) on a Context."""
@@ -490,6 +504,44 @@ class RectTriangleTiler:
     @staticmethod
     def minHex(n):
         return _take_vec(lib().rt_recttri_min_hex(n))
+
+
+class TriangleDimerTiler:
+    """Static helpers of the reference's TriangleDimerTiler (bibone model,
+    TriangleDimerTiler.h:194-209)."""
+
+    @staticmethod
+    def IceCreamCone(N, M):
+        return _take_vec(lib().rt_bibone_ice_cream_cone(N, M))
+
+    @staticmethod
+    def TestButterfly(kind):
+        return _take_vec(lib().rt_bibone_test_butterfly(kind))
+
+    @staticmethod
+    def TilingToTriangleFlips(t):
+        t = _i32(t)
+        return _take_vec(lib().rt_bibone_triangle_flips(t, len(t)))
+
+    @staticmethod
+    def TilingToButterflyFlips(t):
+        t = _i32(t)
+        return _take_vec(lib().rt_bibone_butterfly_flips(t, len(t)))
+
+    @staticmethod
+    def TilingToVertices(t):
+        t = _i32(t)
+        return _take_vec(lib().rt_bibone_tiling_to_vertices(t, len(t)))
+
+    @staticmethod
+    def TilingToDomain(t):
+        t = _i32(t)
+        return _take_vec(lib().rt_bibone_tiling_to_domain(t, len(t)))
+
+    @staticmethod
+    def DimerIndicator(t, c):
+        t = _i32(t)
+        return _take_vec(lib().rt_bibone_dimer_indicator(t, len(t), c))
 
 
 def synthetic_params(count, seed=0x5eed):

@@ -1,0 +1,227 @@
+// rt_bibone_host.cpp -- host statics of the bibone (triangular-lattice
+// dimer) model: start tilings, the derived flip-state arrays, vertices,
+// domain and dimer indicators (TriangleDimerTiler.cpp:35-95, 697-982).
+//
+// Tiling format (the reference's): an N x N int array of 4-bit cell
+// states.  Cells at (even row, odd col) are "H" lozenge cells, (odd, even)
+// "L" cells and (odd, odd) "R" cells; bit k says whether the k-th of the
+// cell's four dimer slots holds a dimer.  The derived arrays are
+//   triangle flips: (N/2) x N, 6-bit states (three 2-bit fields);
+//   butterfly flips: N x N, 9-bit states (four 2-bit fields + a flag).
+#include <cmath>
+#include <vector>
+
+#include "rt_internal.hpp"
+
+namespace rt {
+
+namespace {
+
+inline int bit(int x, int k) { return (x >> k) & 1; }
+
+// Three dimer-edge families on an E x E grid: h (horizontal), l (up-left),
+// r (up-right).  Vertex-state cells are read off the edges around them
+// (TriangleDimerTiler.cpp:738-753, 838-852).
+struct Edges {
+  int E;
+  std::vector<int> h, l, r;
+  explicit Edges(int e) : E(e), h(e * e, 0), l(e * e, 0), r(e * e, 0) {}
+  int H(int i, int j) const { return h[i * E + j]; }
+  int L(int i, int j) const { return l[i * E + j]; }
+  int R(int i, int j) const { return r[i * E + j]; }
+};
+
+tiling edges_to_tiling(const Edges& e) {
+  const int T = 2 * e.E;
+  tiling t((size_t)T * T, 0);
+  for (int i = 3; i < T - 1; i++) {
+    const int a = i / 2;
+    for (int j = 3; j < T - 1; j++) {
+      const int b = j / 2;
+      int v = 0;
+      if ((i & 1) == 0) {
+        if (j & 1) v = e.R(a - 1, b) | e.L(a - 1, b + 1) << 1 | e.R(a, b) << 2 | e.L(a, b) << 3;
+      } else if ((j & 1) == 0) {
+        v = e.H(a, b) | e.R(a, b) << 1 | e.H(a + 1, b - 1) << 2 | e.R(a, b - 1) << 3;
+      } else {
+        v = e.H(a, b) | e.L(a, b + 1) << 1 | e.H(a + 1, b) << 2 | e.L(a, b) << 3;
+      }
+      t[(size_t)i * T + j] = v;
+    }
+  }
+  return t;
+}
+
+}  // namespace
+
+// IceCreamCone(N, M) (TriangleDimerTiler.cpp:697-757): downward triangle of
+// side N with a side-M triangle cut from its top-left corner, covered by
+// horizontal dimers on alternate vertices of each row and a fringe of l / r
+// dimers along the right edge.  Untileable sizes are built the same way
+// (the reference only prints a warning).
+tiling bibone_ice_cream_cone(int N, int M) {
+  const int E = N + 4;
+  const int k = ((N + 1) * (N + 2) / 2) % 2;
+  Edges e(E);
+  for (int i = 2; i < E - 2; i++) {
+    for (int j = 2; j < E - i; j++) {
+      const bool right_edge = j == E - i - 1;
+      const int at = i * E + j;
+      if (i < M + 2) {
+        if (M != 0 && (i + j) % 2 == M % 2 && j > M + 3 - i) e.h[at] = 1;
+        if ((N - M) % 2 == 0 && i % 2 == 0 && right_edge) e.r[at] = 1;
+      } else {
+        if (j % 2 == 0) e.h[at] = 1;
+        if (right_edge && i % 4 == (3 + N % 2 + 2 * k) % 4) {
+          e.h[at] = 0;
+          e.l[at] = 1;
+        }
+        if (right_edge && i % 4 == (2 + N % 2 + 2 * k) % 4) e.r[at] = 1;
+      }
+    }
+  }
+  return edges_to_tiling(e);
+}
+
+// TestButterfly(type) (TriangleDimerTiler.cpp:795-844): one butterfly of
+// orientation 0 (H), 1 (L) or 2 (R) on a 16 x 16 tiling; any other type
+// gives the all-zero tiling.
+tiling bibone_test_butterfly(int type) {
+  Edges e(8);
+  auto set = [&](std::vector<int>& f, int i, int j) { f[i * 8 + j] = 1; };
+  if (type == 0) {
+    set(e.h, 2, 3), set(e.h, 4, 3), set(e.r, 2, 4), set(e.r, 3, 2);
+  } else if (type == 1) {
+    set(e.r, 2, 3), set(e.r, 4, 2), set(e.l, 3, 2), set(e.l, 3, 4);
+  } else if (type == 2) {
+    set(e.l, 2, 4), set(e.l, 4, 3), set(e.r, 3, 2), set(e.r, 3, 4);
+  } else {
+    return tiling(256, 0);
+  }
+  return edges_to_tiling(e);
+}
+
+// TilingToTriangleFlips (TriangleDimerTiler.cpp:846-873).  Output (N/2) x N;
+// even columns are down triangles, odd columns up triangles; each is three
+// 2-bit fields, field k = which of its 3 edges (1..3) the k-th corner's
+// dimer uses, 0 = none.  The outer ring stays 0 (the tiling is padded).
+std::vector<int> bibone_triangle_flips(const tiling& t, int N) {
+  std::vector<int> out((size_t)(N / 2) * N, 0);
+  auto T = [&](int i, int j) { return t[(size_t)i * N + j]; };
+  for (int i = 1; i < N / 2 - 1; i++) {
+    const int r0 = 2 * i + 1;
+    for (int j = 1; j < N - 1; j++) {
+      const int a = T(r0, j), b = T(r0, j + 1);
+      int f0, f1, f2;
+      if (j % 2 == 0) {
+        const int u = T(2 * i, j + 1);
+        f0 = bit(a, 2) + 2 * bit(a, 3) + 3 * bit(b, 3);
+        f1 = bit(b, 2) + 2 * bit(a, 1) + 3 * bit(b, 1);
+        f2 = 2 * bit(u, 0) + bit(a, 0) + 3 * bit(u, 1);
+      } else {
+        const int d = T(2 * i + 2, j);
+        f0 = bit(a, 0) + 2 * bit(a, 3) + 3 * bit(b, 3);
+        f1 = bit(b, 0) + 2 * bit(a, 1) + 3 * bit(b, 1);
+        f2 = 2 * bit(d, 3) + 3 * bit(d, 2) + bit(a, 2);
+      }
+      out[(size_t)i * N + j] = f0 + 4 * f1 + 16 * f2;  // sums, as the reference packs
+    }
+  }
+  return out;
+}
+
+// TilingToButterflyFlips (TriangleDimerTiler.cpp:875-921).  Output N x N,
+// filled at the H, L and R cells at distance >= 2 from the border: four
+// 2-bit wing fields plus bit 8.
+std::vector<int> bibone_butterfly_flips(const tiling& t, int N) {
+  std::vector<int> out((size_t)N * N, 0);
+  auto T = [&](int i, int j) { return t[(size_t)i * N + j]; };
+  // wing(x, a, b, own, k) = bit a of x + 2 * bit b of x + 3 * bit k of own
+  auto wing = [](int x, int a, int b, int own, int k) {
+    return bit(x, a) + 2 * bit(x, b) + 3 * bit(own, k);
+  };
+  for (int i = 2; i < N - 2; i++) {
+    for (int j = 2; j < N - 2; j++) {
+      const int o = T(i, j);
+      int f0, f1, f2, f3, c;
+      if (i % 2 == 0) {
+        if (j % 2 == 0) continue;
+        const int up = T(i - 1, j), upr = T(i - 1, j + 1), dn = T(i + 1, j), dnl = T(i + 1, j - 1);
+        f0 = wing(up, 3, 0, o, 0);
+        f1 = wing(upr, 0, 1, o, 1);
+        f2 = wing(dn, 1, 2, o, 2);
+        f3 = wing(dnl, 2, 3, o, 3);
+        c = bit(upr, 2);
+      } else if (j % 2 == 0) {
+        const int upr = T(i - 1, j + 1), rt = T(i, j + 1), dnl = T(i + 1, j - 1), lf = T(i, j - 1);
+        f0 = wing(upr, 0, 1, o, 0);
+        f1 = wing(rt, 1, 2, o, 1);
+        f2 = wing(dnl, 2, 3, o, 2);
+        f3 = wing(lf, 3, 0, o, 3);
+        c = bit(rt, 3);
+      } else {
+        const int up = T(i - 1, j), rt = T(i, j + 1), dn = T(i + 1, j), lf = T(i, j - 1);
+        f0 = wing(up, 0, 1, o, 0);
+        f1 = wing(rt, 0, 1, o, 1);
+        f2 = wing(dn, 2, 3, o, 2);
+        f3 = wing(lf, 2, 3, o, 3);
+        c = bit(rt, 3);
+      }
+      out[(size_t)i * N + j] = f0 + 4 * f1 + 16 * f2 + 64 * f3 + 256 * c;
+    }
+  }
+  return out;
+}
+
+// TilingToVertices (TriangleDimerTiler.cpp:923-949): (N/2+1)^2 flags, 1 for
+// a lattice vertex touched by one of its six dimer slots.
+std::vector<int> bibone_tiling_to_vertices(const tiling& t, int N) {
+  const int M = N / 2 + 1;
+  std::vector<int> v((size_t)M * M, 0);
+  auto T = [&](int i, int j) { return t[(size_t)i * N + j]; };
+  for (int i = 2; i < M - 2; i++) {
+    for (int j = 2; j < M - 2; j++) {
+      const int a = T(2 * (i - 1), 2 * j + 1), b = T(2 * (i + 1), 2 * j - 1);
+      const int c = T(2 * i - 1, 2 * j), d = T(2 * i - 1, 2 * j + 1);
+      const bool touched = (a & 12) || (b & 3) || (c & 4) || (d & 4);
+      v[(size_t)i * M + j] = touched ? 1 : 0;
+    }
+  }
+  return v;
+}
+
+// TilingToDomain (TriangleDimerTiler.cpp:951-982): (M-1) x 2(M-1) triangle
+// faces, M = N/2+1; a face is in the domain when its three corners are.
+domain bibone_tiling_to_domain(const tiling& t, int N) {
+  const int M = N / 2 + 1;
+  const std::vector<int> v = bibone_tiling_to_vertices(t, N);
+  auto V = [&](int i, int j) { return v[(size_t)i * M + j] == 1; };
+  const int W = 2 * (M - 1);
+  domain d((size_t)(M - 1) * W, 0);
+  for (int i = 0; i < M - 1; i++) {
+    for (int j = 0; j < W; j++) {
+      const int b = j / 2;
+      const bool in = (j % 2 == 0) ? (V(i, b) && V(i, b + 1) && V(i + 1, b))
+                                   : (V(i, b + 1) && V(i + 1, b) && V(i + 1, b + 1));
+      d[(size_t)i * W + j] = in ? 1 : 0;
+    }
+  }
+  return d;
+}
+
+// DimerIndicator(t, c) (TriangleDimerTiler.cpp:35-95): (N/2)^2 flags.  c = 0
+// reads bit 0 of the (odd, odd) cells; c = 1 and c = 2 both read the
+// (even, odd) cells, bit 3 and bit 2 respectively (as the reference does);
+// any other c gives all zeros.
+std::vector<int> bibone_dimer_indicator(const tiling& t, int N, int c) {
+  const int M = N / 2;
+  std::vector<int> out((size_t)M * M, 0);
+  if (c < 0 || c > 2) return out;
+  const int row_par = c == 0 ? 1 : 0;
+  const int b = c == 0 ? 0 : (c == 1 ? 3 : 2);
+  for (int i = row_par; i < N; i += 2)
+    for (int j = 1; j < N; j += 2) out[(size_t)(i / 2) * M + j / 2] = bit(t[(size_t)i * N + j], b);
+  return out;
+}
+
+}  // namespace rt

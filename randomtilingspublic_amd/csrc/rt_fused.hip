@@ -54,8 +54,11 @@ __device__ __forceinline__ uint32_t pick(uint32_t mask_bits, int r, uint32_t yes
 }
 
 // One colour step on the LDS tile, specialised on the (workgroup-uniform)
-// pull pattern so the body is branch-free: every draw, neighbour read and
-// select of the R rows and NC chains can be scheduled together.
+// pull pattern so the body is branch-free.  All reads of the step (own cell
+// and neighbours, every row and chain) come before any write: the planes
+// read and the plane written differ, but the compiler cannot see that from
+// runtime colour offsets, and stores issued between reads would serialise
+// every row behind an LDS round trip.
 template <int MODEL, int NC, int W, int R, int PULL, class Cell, int RS, int PL>
 __device__ __forceinline__ void sweep_step(Cell* lds, const Lanes& L, Tiny* st, uint32_t op,
                                            const UnitDesc& U, const ModelParams& mp) {
@@ -64,9 +67,10 @@ __device__ __forceinline__ void sweep_step(Cell* lds, const Lanes& L, Tiny* st, 
   uint32_t b[R];
 #pragma unroll
   for (int r = 0; r < R; r++) b[r] = tiny_bits(st[r]);
+  uint32_t out[NC][R];
 #pragma unroll
   for (int ch = 0; ch < NC; ch++) {
-    Cell* P = lds + ch * NCOL * PL + L.li0;
+    const Cell* P = lds + ch * NCOL * PL + L.li0;
 #pragma unroll
     for (int r = 0; r < R; r++) {
       const int off = r * W * RS;
@@ -102,15 +106,20 @@ __device__ __forceinline__ void sweep_step(Cell* lds, const Lanes& L, Tiny* st, 
         }
         nv = ((L.hsm >> r) & 1u) ? recttri_move(v, b[r], mp.rw) : v;
       }
-      P[c * PL + off] = (Cell)pick(L.hsm, r, nv, v);
+      out[ch][r] = pick(L.hsm, r, nv, v);
     }
   }
+#pragma unroll
+  for (int ch = 0; ch < NC; ch++)
+#pragma unroll
+    for (int r = 0; r < R; r++)
+      lds[ch * NCOL * PL + L.li0 + c * PL + r * W * RS] = (Cell)out[ch][r];
 }
 
 // W wave64s per workgroup; each lane owns one tile column and R rows
 // (rows wv, wv+W, ...), so the loaded tile is (W*R) x 64 cells.
 template <int MODEL, int NC, int W, int R>
-__global__ __launch_bounds__(64 * W) void k_fused(const UnitDesc* __restrict__ units,
+__global__ __launch_bounds__(64 * W, 2) void k_fused(const UnitDesc* __restrict__ units,
                                                     const WorkItem* __restrict__ work, long s0,
                                                     int K, int init, int seg, int H,
                                                     ModelParams mp,
@@ -141,7 +150,8 @@ __global__ __launch_bounds__(64 * W) void k_fused(const UnitDesc* __restrict__ u
   const int TH = LH - 2 * H, TW = kLW - 2 * H;
   const int tr = wi.tile / U.tiles_c, tc = wi.tile % U.tiles_c;
   const int r0 = tr * TH - H, c0 = tc * TW - H;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform row offset
   const int gc = c0 + lane;
   const bool col_in = gc >= 0 && gc < cols;
 

@@ -737,4 +737,36 @@ rt_vec* rt_recttri_min_hex(int n) {
   return make_vec(recttri_min_hex(n));
 }
 
+// Bibone statics.  Tilings are N x N with N even.
+static bool bibone_side(size_t n, int* N) {
+  if (!is_square(n)) return false;
+  *N = (int)std::lround(std::sqrt((double)n));
+  return *N >= 6 && *N % 2 == 0;
+}
+
+rt_vec* rt_bibone_ice_cream_cone(int N, int M) {
+  if (N < 1 || M < 0) return nullptr;
+  return make_vec(bibone_ice_cream_cone(N, M));
+}
+
+rt_vec* rt_bibone_test_butterfly(int type) { return make_vec(bibone_test_butterfly(type)); }
+
+#define RT_BIBONE_STATIC(name, fn)                          \
+  rt_vec* name(const int32_t* t, size_t n) {                \
+    int N;                                                  \
+    if (!t || !bibone_side(n, &N)) return nullptr;          \
+    return make_vec(fn(from_c(t, n), N));                   \
+  }
+RT_BIBONE_STATIC(rt_bibone_triangle_flips, bibone_triangle_flips)
+RT_BIBONE_STATIC(rt_bibone_butterfly_flips, bibone_butterfly_flips)
+RT_BIBONE_STATIC(rt_bibone_tiling_to_vertices, bibone_tiling_to_vertices)
+RT_BIBONE_STATIC(rt_bibone_tiling_to_domain, bibone_tiling_to_domain)
+#undef RT_BIBONE_STATIC
+
+rt_vec* rt_bibone_dimer_indicator(const int32_t* t, size_t n, int c) {
+  int N;
+  if (!t || !bibone_side(n, &N)) return nullptr;
+  return make_vec(bibone_dimer_indicator(from_c(t, n), N, c));
+}
+
 }  // extern "C"

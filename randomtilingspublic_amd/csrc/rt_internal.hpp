@@ -53,4 +53,13 @@ domain lozenge_tiling_to_domain(const tiling& t);
 tiling recttri_max_hex(int n);
 tiling recttri_min_hex(int n);
 
+// Bibone statics (rt_bibone_host.cpp); N is the tiling side.
+tiling bibone_ice_cream_cone(int N, int M);
+tiling bibone_test_butterfly(int type);
+std::vector<int> bibone_triangle_flips(const tiling& t, int N);
+std::vector<int> bibone_butterfly_flips(const tiling& t, int N);
+std::vector<int> bibone_tiling_to_vertices(const tiling& t, int N);
+domain bibone_tiling_to_domain(const tiling& t, int N);
+std::vector<int> bibone_dimer_indicator(const tiling& t, int N, int c);
+
 }  // namespace rt

@@ -82,6 +82,18 @@ def ref_host():
     for n in (1, 2, 5, 8):
         arrs[f"recttri_max{n}"] = call("ref_recttri_max_hex", n)
         arrs[f"recttri_min{n}"] = call("ref_recttri_min_hex", n)
+        arrs[f"recttri_slope{n}"] = call("ref_recttri_slope_hex", n)
+    for n, m in ((3, 0), (6, 0), (7, 2), (10, 3), (13, 5)):
+        t = call("ref_bibone_ice_cream_cone", n, m)
+        arrs[f"bibone_cone{n}_{m}"] = t
+        arrs[f"bibone_cone{n}_{m}_tri"] = call("ref_bibone_triangle_flips", t)
+        arrs[f"bibone_cone{n}_{m}_bf"] = call("ref_bibone_butterfly_flips", t)
+        arrs[f"bibone_cone{n}_{m}_vertices"] = call("ref_bibone_tiling_to_vertices", t)
+        arrs[f"bibone_cone{n}_{m}_domain"] = call("ref_bibone_tiling_to_domain", t)
+        for c in range(3):
+            arrs[f"bibone_cone{n}_{m}_dimers{c}"] = call("ref_bibone_dimer_indicator", t, c)
+    for k in range(3):
+        arrs[f"bibone_butterfly{k}"] = call("ref_bibone_test_butterfly", k)
     np.savez_compressed(os.path.join(HERE, "reference_host.npz"), **arrs)
 
 

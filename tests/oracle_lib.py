@@ -58,6 +58,12 @@ _L.orc_lozenge_walk_segments.argtypes = [_I32P, C.c_int, _LONGP, _LONGP, C.c_int
 _L.orc_lozenge_cftp.argtypes = [_I32P, _I32P, C.c_int, C.c_uint32, C.c_int, C.c_long, C.c_int,
                                 _U32P, C.c_size_t, _I32P, C.POINTER(CftpStats)]
 _L.orc_recttri_walk.argtypes = [_I32P, C.c_int, C.c_long, C.c_long, _U32P, C.c_size_t, _F32P]
+_L.orc_bibone_walk.argtypes = [_I32P, C.c_int, C.c_long, C.c_long, _U32P, C.c_size_t,
+                               C.c_void_p]
+_L.orc_bibone_triangle_flips.argtypes = [_I32P, C.c_int, _I32P]
+_L.orc_bibone_triangle_flips.restype = None
+_L.orc_bibone_butterfly_flips.argtypes = [_I32P, C.c_int, _I32P]
+_L.orc_bibone_butterfly_flips.restype = None
 _L.orc_set_threads.argtypes = [C.c_int]
 
 set_threads = _L.orc_set_threads
@@ -179,3 +185,31 @@ def recttri_walk(t, steps, seed, params3, weights16=None):
     _check(_L.orc_recttri_walk(t, _side(t), steps, seed, p, len(p) // 3,
                                np.ascontiguousarray(w)))
     return t
+
+
+def bibone_triangle_flips(t):
+    t = np.ascontiguousarray(t, dtype=np.int32)
+    N = _side(t)
+    out = np.zeros((N // 2) * N, np.int32)
+    _L.orc_bibone_triangle_flips(t, N, out)
+    return out
+
+
+def bibone_butterfly_flips(t):
+    t = np.ascontiguousarray(t, dtype=np.int32)
+    N = _side(t)
+    out = np.zeros(N * N, np.int32)
+    _L.orc_bibone_butterfly_flips(t, N, out)
+    return out
+
+
+def bibone_walk(t, steps, seed, params3, planes=False):
+    """Returns the tiling, or (tiling, 8 x M x M planes) with planes=True."""
+    t = np.ascontiguousarray(t, dtype=np.int32).copy()
+    N = _side(t)
+    M = N // 2
+    p = _p(params3)
+    pl = np.zeros(8 * M * M, np.int32) if planes else None
+    _check(_L.orc_bibone_walk(t, N, steps, seed, p, len(p) // 3,
+                              pl.ctypes.data_as(C.c_void_p) if planes else None))
+    return (t, pl.reshape(8, M, M)) if planes else t

@@ -36,6 +36,10 @@ int main(int argc, char** argv) {
   } catch (const rt::Error& e) {
     std::printf("untileable %d\n", e.code);
   }
+  tiling cone = TriangleDimerTiler::IceCreamCone(9, 2);
+  s = 0;
+  for (size_t k = 0; k < cone.size(); k++) s += (long)(k % 83 + 1) * cone[k];
+  std::printf("cone %zu %ld\n", cone.size(), s);
   if (argc > 1) {  // device part
     DominoTiler D(0, "dominokernelCFTP.cl");
     std::vector<uint32_t> p(3 * t.size() / 2);
@@ -50,6 +54,15 @@ int main(int argc, char** argv) {
     s = 0;
     for (size_t k = 0; k < c.size(); k++) s += (long)(k % 97 + 1) * c[k];
     std::printf("cftp %ld\n", s);
+    TriangleDimerTiler B(0);
+    std::vector<uint32_t> pb(3 * cone.size() / 4);
+    for (size_t k = 0; k < pb.size(); k++) pb[k] = (uint32_t)(k * 2246822519u + 777u);
+    B.LoadTinyMT(pb);
+    tiling bw = cone;
+    B.Walk(bw, 300, 5);
+    s = 0;
+    for (size_t k = 0; k < bw.size(); k++) s += (long)(k % 83 + 1) * bw[k];
+    std::printf("bibone %ld\n", s);
   }
   return 0;
 }
@@ -83,6 +96,8 @@ def test_cpp_dropin_statics(tmp_path):
     assert out[1] == f"loz {len(l)} {_checksum(l, 89)}"
     assert out[2] == "io 1"
     assert out[3] == "untileable -7"
+    cone = rt.TriangleDimerTiler.IceCreamCone(9, 2)
+    assert out[4] == f"cone {len(cone)} {_checksum(cone, 83)}"
 
 
 @pytest.mark.gpu
@@ -98,9 +113,15 @@ def test_cpp_dropin_walk_and_cftp(tmp_path):
     p = ((np.arange(3 * n, dtype=np.uint64) * np.uint64(2654435761) + np.uint64(12345))
          & np.uint64(0xFFFFFFFF)).astype(np.uint32)
     w = o.domino_walk(t, 1000, 1, p, 1)
-    assert out[4] == f"walk {_checksum(w, 97)}"
+    assert out[5] == f"walk {_checksum(w, 97)}"
     c, _ = o.domino_cftp(rt.DominoTiler.MaxTiling(d), t, 345903, p, 0, 128, 64, 1)
-    assert out[5] == f"cftp {_checksum(c, 97)}"
+    assert out[6] == f"cftp {_checksum(c, 97)}"
+    cone = rt.TriangleDimerTiler.IceCreamCone(9, 2)
+    nb = len(cone) // 4
+    pb = ((np.arange(3 * nb, dtype=np.uint64) * np.uint64(2246822519) + np.uint64(777))
+          & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+    bw = o.bibone_walk(cone, 300, 5, pb)
+    assert out[7] == f"bibone {_checksum(bw, 83)}"
 
 
 def test_savematrix_format_matches_reference_file(tmp_path):
