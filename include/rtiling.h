@@ -75,6 +75,14 @@ int rt_last_timing(const rt_ctx* ctx, rt_timing* out);
  * shape allows), 1 = one launch per colour step, 2 = fused only. */
 int rt_set_engine(rt_ctx* ctx, int engine);
 
+/* Tile shape of the sweep-block kernels (no reference counterpart; a tuning
+ * knob for walks and batched CFTP, results do not depend on it).  Tiles are
+ * 64 columns wide: 1 = 64 rows worked by 8 waves x 8 rows per lane, 2 = 64
+ * rows by 16 waves x 4 rows, 3 = 32 rows by 16 waves x 2 rows; 0 = chosen
+ * per job from its tile count against the CU count.  The serving loop always
+ * uses 64-row tiles. */
+int rt_set_tile_shape(rt_ctx* ctx, int shape);
+
 /* ---------------------------------------------------------------- domino */
 /* Flip rules: the four reference kernel files. */
 #define RT_DOMINO_TOGGLE08 0    /* dominokernel.cl:4-18 */
@@ -252,6 +260,7 @@ rt_vec* rt_lozenge_domain_to_vertices(const int32_t* d, size_t dsize);    /* 483
 /* RectTriangleTiler statics (RectTriangleTiler.h:77-82). */
 rt_vec* rt_recttri_max_hex(int n);                       /* RectTriangleTiler.cpp:480-538 */
 rt_vec* rt_recttri_min_hex(int n);                       /* 540-600 */
+rt_vec* rt_recttri_slope_hex(int n);                     /* 173-478 */
 
 /* TriangleDimerTiler statics (TriangleDimerTiler.h:194-209). */
 rt_vec* rt_bibone_ice_cream_cone(int N, int M);          /* TriangleDimerTiler.cpp:697-793 */
@@ -261,6 +270,24 @@ rt_vec* rt_bibone_butterfly_flips(const int32_t* t, size_t n);   /* 875-921 */
 rt_vec* rt_bibone_tiling_to_vertices(const int32_t* t, size_t n); /* 923-949 */
 rt_vec* rt_bibone_tiling_to_domain(const int32_t* t, size_t n);  /* 951-982 */
 rt_vec* rt_bibone_dimer_indicator(const int32_t* t, size_t n, int c); /* 35-95 */
+
+/* SVG writers (drop-in drawing helpers; files are the reference's byte for
+ * byte).  Return RT_OK or RT_EIO.  A NULL domain in a *_dimer_svg call means
+ * TilingToDomain(t), as in the reference's one-argument overloads. */
+int rt_domino_domain_svg(const int32_t* d, size_t nd, const char* path);   /* DominoTiler.cpp:640-668 */
+int rt_domino_tiling_svg(const int32_t* t, size_t nt, const char* path);   /* 745-774 */
+int rt_domino_maya_svg(const int32_t* t, size_t nt, const char* path);     /* 776-816 */
+int rt_domino_dimer_svg(const int32_t* t, size_t nt, const int32_t* d, size_t nd,
+                        const char* path);                                 /* 670-743 */
+int rt_lozenge_domain_svg(const int32_t* d, size_t nd, const char* path);  /* LozengeTiler.cpp:669-693 */
+int rt_lozenge_tiling_svg(const int32_t* t, size_t nt, const char* path);  /* 643-667 */
+int rt_lozenge_dimer_svg(const int32_t* t, size_t nt, const int32_t* d, size_t nd,
+                         const char* path);                                /* 558-641 */
+int rt_recttri_tiling_svg(const int32_t* t, size_t nt, const char* path);  /* RectTriangleTiler.cpp:646-776 */
+int rt_bibone_domain_svg(const int32_t* d, size_t nd, const char* path);   /* TriangleDimerTiler.cpp:984-1006 */
+int rt_bibone_tiling_svg(const int32_t* t, size_t nt, const char* path);   /* 1013-1044 */
+int rt_bibone_dimer_svg(const int32_t* t, size_t nt, const int32_t* d, size_t nd,
+                        const char* path);                                 /* 1008-1011, 1046-1117 */
 
 #ifdef __cplusplus
 }

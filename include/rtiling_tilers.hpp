@@ -52,6 +52,10 @@ inline std::vector<int> thurston(int (*f)(const int32_t*, size_t, rt_vec**), con
   return take(v);
 }
 
+inline void svg(int rc) {
+  if (rc) throw Error(rc, "cannot write the SVG file");
+}
+
 inline int side_of(const std::vector<int>& t) {
   int n = 0;
   while ((size_t)(n + 1) * (n + 1) <= t.size()) n++;
@@ -188,6 +192,21 @@ class DominoTiler : public rt::Device {
   static domain TilingToDomain(const tiling& t) {
     return rt::take(rt_domino_tiling_to_domain(t.data(), t.size()));
   }
+  static void DomainToSVG(const domain& d, std::string f) {
+    rt::svg(rt_domino_domain_svg(d.data(), d.size(), f.c_str()));
+  }
+  static void TilingToSVG(const tiling& t, std::string f) {
+    rt::svg(rt_domino_tiling_svg(t.data(), t.size(), f.c_str()));
+  }
+  static void MayaToSVG(const tiling& t, std::string f) {
+    rt::svg(rt_domino_maya_svg(t.data(), t.size(), f.c_str()));
+  }
+  static void DimerToSVG(const tiling& t, std::string f) {
+    rt::svg(rt_domino_dimer_svg(t.data(), t.size(), nullptr, 0, f.c_str()));
+  }
+  static void DimerToSVG(const tiling& t, const domain& d, std::string f) {
+    rt::svg(rt_domino_dimer_svg(t.data(), t.size(), d.data(), d.size(), f.c_str()));
+  }
   static tiling HeightfuncToTiling(const heightfunc& hf, const domain& d) {
     return rt::take(rt_domino_heightfunc_to_tiling(hf.data(), hf.size(), d.data(), d.size()));
   }
@@ -231,6 +250,18 @@ class LozengeTiler : public rt::Device {
   static domain TilingToDomain(const tiling& t) {
     return rt::take(rt_lozenge_tiling_to_domain(t.data(), t.size()));
   }
+  static void DomainToSVG(const domain& d, std::string f) {
+    rt::svg(rt_lozenge_domain_svg(d.data(), d.size(), f.c_str()));
+  }
+  static void TilingToSVG(const tiling& t, std::string f) {
+    rt::svg(rt_lozenge_tiling_svg(t.data(), t.size(), f.c_str()));
+  }
+  static void DimerToSVG(const tiling& t, std::string f) {
+    rt::svg(rt_lozenge_dimer_svg(t.data(), t.size(), nullptr, 0, f.c_str()));
+  }
+  static void DimerToSVG(const tiling& t, const domain& d, std::string f) {
+    rt::svg(rt_lozenge_dimer_svg(t.data(), t.size(), d.data(), d.size(), f.c_str()));
+  }
   static tiling HeightfuncToTiling(const heightfunc& hf, const domain& d) {
     return rt::take(rt_lozenge_heightfunc_to_tiling(hf.data(), hf.size(), d.data(), d.size()));
   }
@@ -254,6 +285,8 @@ class RectTriangleTiler : public rt::Device {
 
   static tiling maxHex(int N) { return rt::take(rt_recttri_max_hex(N)); }
   static tiling minHex(int N) { return rt::take(rt_recttri_min_hex(N)); }
+  static tiling slopeHex(int N) { return rt::take(rt_recttri_slope_hex(N)); }
+  static void TilingToSVG(tiling& t, std::string f) { rt::svg(rt_recttri_tiling_svg(t.data(), t.size(), f.c_str())); }
 
  private:
   std::vector<float> w_;
@@ -287,4 +320,12 @@ class TriangleDimerTiler : public rt::Device {
   static domain TilingToDomain(tiling& t) {
     return rt::take(rt_bibone_tiling_to_domain(t.data(), t.size()));
   }
+  static void TilingToSVG(tiling& t, std::string f) { rt::svg(rt_bibone_tiling_svg(t.data(), t.size(), f.c_str())); }
+  static void DimerToSVG(tiling& t, std::string f) {
+    rt::svg(rt_bibone_dimer_svg(t.data(), t.size(), nullptr, 0, f.c_str()));
+  }
+  static void DimerToSVG(tiling& t, domain& d, std::string f) {
+    rt::svg(rt_bibone_dimer_svg(t.data(), t.size(), d.data(), d.size(), f.c_str()));
+  }
+  static void DomainToSVG(domain& d, std::string f) { rt::svg(rt_bibone_domain_svg(d.data(), d.size(), f.c_str())); }
 };

@@ -5,6 +5,7 @@
 // host helpers against the reference implementation itself.
 #include <cstdint>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "Domino/DominoTiler.h"
@@ -95,6 +96,36 @@ long ref_bibone_tiling_to_vertices(const int32_t* t, long nt, int32_t* out, long
 long ref_bibone_tiling_to_domain(const int32_t* t, long nt, int32_t* out, long cap) {
   std::vector<int> v = vec(t, nt);
   return give(TriangleDimerTiler::TilingToDomain(v), out, cap);
+}
+// SVG writers: write `path`, return 0.
+int ref_svg(const char* model, const char* what, const int32_t* t, long nt, const int32_t* d,
+            long nd, const char* path) {
+  std::vector<int> a = vec(t, nt);
+  std::vector<int> b = d ? vec(d, nd) : std::vector<int>();
+  const std::string m(model), w(what), f(path);
+  if (m == "domino") {
+    if (w == "domain") DominoTiler::DomainToSVG(a, f);
+    else if (w == "tiling") DominoTiler::TilingToSVG(a, f);
+    else if (w == "maya") DominoTiler::MayaToSVG(a, f);
+    else if (w == "dimer") d ? DominoTiler::DimerToSVG(a, b, f) : DominoTiler::DimerToSVG(a, f);
+    else return -1;
+  } else if (m == "lozenge") {
+    if (w == "domain") LozengeTiler::DomainToSVG(a, f);
+    else if (w == "tiling") LozengeTiler::TilingToSVG(a, f);
+    else if (w == "dimer") d ? LozengeTiler::DimerToSVG(a, b, f) : LozengeTiler::DimerToSVG(a, f);
+    else return -1;
+  } else if (m == "recttri") {
+    if (w == "tiling") RectTriangleTiler::TilingToSVG(a, f);
+    else return -1;
+  } else if (m == "bibone") {
+    if (w == "domain") TriangleDimerTiler::DomainToSVG(a, f);
+    else if (w == "tiling") TriangleDimerTiler::TilingToSVG(a, f);
+    else if (w == "dimer") d ? TriangleDimerTiler::DimerToSVG(a, b, f) : TriangleDimerTiler::DimerToSVG(a, f);
+    else return -1;
+  } else {
+    return -1;
+  }
+  return 0;
 }
 long ref_bibone_dimer_indicator(const int32_t* t, long nt, int c, int32_t* out, long cap) {
   std::vector<int> v = vec(t, nt);

@@ -101,6 +101,7 @@ def lib():
                                   C.c_int, _I32P, C.POINTER(CftpStats)]
     L.rt_recttri_walk.argtypes = [vp, _I32P, C.c_int, C.c_long, C.c_long, C.c_void_p]
     L.rt_bibone_walk.argtypes = [vp, _I32P, C.c_int, C.c_long, C.c_long]
+    L.rt_set_tile_shape.argtypes = [vp, C.c_int]
     L.rt_lozenge_cftp_batch.argtypes = [vp, _I32P, _I32P, C.c_int, _U32P, C.c_int, C.c_int,
                                         C.c_long, C.c_int, _I32P, C.c_void_p]
     L.rt_domino_cftp_batch.argtypes = [vp, _I32P, _I32P, C.c_int, _U32P, C.c_int, C.c_int,
@@ -138,6 +139,7 @@ def lib():
         ("rt_lozenge_domain_to_vertices", [_I32P, C.c_size_t]),
         ("rt_recttri_max_hex", [C.c_int]),
         ("rt_recttri_min_hex", [C.c_int]),
+        ("rt_recttri_slope_hex", [C.c_int]),
         ("rt_bibone_ice_cream_cone", [C.c_int, C.c_int]),
         ("rt_bibone_test_butterfly", [C.c_int]),
         ("rt_bibone_triangle_flips", [_I32P, C.c_size_t]),
@@ -149,6 +151,12 @@ def lib():
         f = getattr(L, name)
         f.argtypes = args
         f.restype = vp
+    for name in ["rt_domino_domain_svg", "rt_domino_tiling_svg", "rt_domino_maya_svg",
+                 "rt_lozenge_domain_svg", "rt_lozenge_tiling_svg", "rt_recttri_tiling_svg",
+                 "rt_bibone_domain_svg", "rt_bibone_tiling_svg"]:
+        getattr(L, name).argtypes = [_I32P, C.c_size_t, C.c_char_p]
+    for name in ["rt_domino_dimer_svg", "rt_lozenge_dimer_svg", "rt_bibone_dimer_svg"]:
+        getattr(L, name).argtypes = [_I32P, C.c_size_t, C.c_void_p, C.c_size_t, C.c_char_p]
     for name in ["rt_domino_max_tiling", "rt_domino_min_tiling",
                  "rt_lozenge_max_tiling", "rt_lozenge_min_tiling"]:
         getattr(L, name).argtypes = [_I32P, C.c_size_t, C.POINTER(vp)]
@@ -164,6 +172,22 @@ def _take_vec(ptr):
     out = np.ctypeslib.as_array(L.rt_vec_data(ptr), shape=(n,)).copy() if n else np.zeros(0, np.int32)
     L.rt_vec_free(ptr)
     return out.astype(np.int32)
+
+
+def _svg(fn, a, path):
+    a = _i32(a)
+    rc = getattr(lib(), fn)(a, len(a), os.fsencode(path))
+    if rc:
+        raise RTError(rc, f"{fn}: cannot write {path}")
+
+
+def _dimer_svg(fn, t, d, path):
+    t = _i32(t)
+    dd = None if d is None else _i32(d)
+    rc = getattr(lib(), fn)(t, len(t), None if dd is None else dd.ctypes.data_as(C.c_void_p),
+                            0 if dd is None else len(dd), os.fsencode(path))
+    if rc:
+        raise RTError(rc, f"{fn}: cannot write {path}")
 
 
 def _i32(a):
@@ -210,6 +234,11 @@ class Context:
 
     def set_engine(self, engine):
         self._chk(self._L.rt_set_engine(self._h, engine))
+
+    def set_tile_shape(self, shape):
+        """Sweep-block tile shape: 0 auto, 1 = 64 rows / 8 waves, 2 = 64 rows /
+        16 waves, 3 = 32 rows / 16 waves."""
+        self._chk(self._L.rt_set_tile_shape(self._h, shape))
 
     def set_tinymt_params(self, params3):
         p = np.ascontiguousarray(np.asarray(params3, dtype=np.uint32).reshape(-1, 3).ravel())
@@ -457,6 +486,22 @@ class DominoTiler:
         d = _i32(d)
         return _take_vec(lib().rt_domino_domain_to_vertices(d, len(d)))
 
+    @staticmethod
+    def DomainToSVG(d, path):
+        _svg("rt_domino_domain_svg", d, path)
+
+    @staticmethod
+    def TilingToSVG(t, path):
+        _svg("rt_domino_tiling_svg", t, path)
+
+    @staticmethod
+    def MayaToSVG(t, path):
+        _svg("rt_domino_maya_svg", t, path)
+
+    @staticmethod
+    def DimerToSVG(t, path, d=None):
+        _dimer_svg("rt_domino_dimer_svg", t, d, path)
+
 
 class LozengeTiler:
     """Static helpers of the reference's LozengeTiler (LozengeTiler.h:77-92)."""
@@ -493,6 +538,18 @@ class LozengeTiler:
         d = _i32(d)
         return _take_vec(lib().rt_lozenge_domain_to_vertices(d, len(d)))
 
+    @staticmethod
+    def DomainToSVG(d, path):
+        _svg("rt_lozenge_domain_svg", d, path)
+
+    @staticmethod
+    def TilingToSVG(t, path):
+        _svg("rt_lozenge_tiling_svg", t, path)
+
+    @staticmethod
+    def DimerToSVG(t, path, d=None):
+        _dimer_svg("rt_lozenge_dimer_svg", t, d, path)
+
 
 class RectTriangleTiler:
     """Static helpers of the reference's RectTriangleTiler (RectTriangleTiler.h:77-82)."""
@@ -504,6 +561,14 @@ class RectTriangleTiler:
     @staticmethod
     def minHex(n):
         return _take_vec(lib().rt_recttri_min_hex(n))
+
+    @staticmethod
+    def slopeHex(n):
+        return _take_vec(lib().rt_recttri_slope_hex(n))
+
+    @staticmethod
+    def TilingToSVG(t, path):
+        _svg("rt_recttri_tiling_svg", t, path)
 
 
 class TriangleDimerTiler:
@@ -542,6 +607,18 @@ class TriangleDimerTiler:
     def DimerIndicator(t, c):
         t = _i32(t)
         return _take_vec(lib().rt_bibone_dimer_indicator(t, len(t), c))
+
+    @staticmethod
+    def DomainToSVG(d, path):
+        _svg("rt_bibone_domain_svg", d, path)
+
+    @staticmethod
+    def TilingToSVG(t, path):
+        _svg("rt_bibone_tiling_svg", t, path)
+
+    @staticmethod
+    def DimerToSVG(t, path, d=None):
+        _dimer_svg("rt_bibone_dimer_svg", t, d, path)
 
 
 def synthetic_params(count, seed=0x5eed):

@@ -13,7 +13,7 @@ ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "librtiling.so")
 
 HIP_SOURCES = ["rt_device.hip", "rt_fused.hip", "rt_batch.hip", "rt_serve.hip", "rt_bibone.hip"]
-CXX_SOURCES = ["rt_host.cpp", "rt_bibone_host.cpp"]
+CXX_SOURCES = ["rt_host.cpp", "rt_bibone_host.cpp", "rt_svg.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("RT_OFFLOAD_ARCH", "gfx950")

@@ -238,7 +238,7 @@ static int cftp_batch(rt_ctx* ctx, int model, const int32_t* tmax, const int32_t
         job.hist[u].append(S[k].colours[nseg - 1 - s], job.ops[u]);
       }
     });
-    rc = ctx->reserve_aux(fused_scratch_bytes(job));
+    rc = ctx->reserve_aux(fused_scratch_bytes(ctx, job));
     if (rc) return rc;
     ctx->last = rt_timing{};
     HIPCHK_RET(ctx, hipEventRecord(ctx->ev0, ctx->stream));
@@ -433,7 +433,7 @@ extern "C" int rt_domino_walk_batch(rt_ctx* ctx, int32_t* const* tilings, const 
     split_planes(M_DOMINO, tilings[k], Ns[k], h.data());
     HIPCHK_RET(ctx, hipMemcpy(planes[k], h.data(), h.size(), hipMemcpyHostToDevice));
   }
-  rc = ctx->reserve_aux(fused_scratch_bytes(job));
+  rc = ctx->reserve_aux(fused_scratch_bytes(ctx, job));
   if (rc) return rc;
   FusedPlan plan;
   rc = fused_upload(ctx, job, ctx->aux, &plan);

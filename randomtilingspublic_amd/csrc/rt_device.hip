@@ -296,6 +296,12 @@ extern "C" int rt_set_engine(rt_ctx* ctx, int engine) {
   return RT_OK;
 }
 
+extern "C" int rt_set_tile_shape(rt_ctx* ctx, int shape) {
+  if (!ctx || shape < 0 || shape > 3) return RT_EINVAL;
+  ctx->tile_shape = shape;
+  return RT_OK;
+}
+
 extern "C" int rt_set_tinymt_params(rt_ctx* ctx, const uint32_t* p3, size_t count) {
   if (!ctx || (!p3 && count)) return RT_EINVAL;
   HIPCHK(ctx, hipSetDevice(ctx->device));
@@ -570,7 +576,7 @@ int rt::domino_fused_walk(rt_ctx* ctx, DominoArrays* a, int nc, int N, int rule,
     job.seg_seeds[0].push_back(single ? base : (uint32_t)seeds[k]);
     job.hist[0].append(colours, job.ops[0]);
   }
-  int rc = ctx->reserve_aux(fused_scratch_bytes(job));
+  int rc = ctx->reserve_aux(fused_scratch_bytes(ctx, job));
   if (rc) return rc;
   rc = fused_run(ctx, job, ctx->aux);
   if (rc) return rc;
@@ -664,7 +670,7 @@ int rt::lozenge_device_walk(rt_ctx* ctx, TriArrays8* a, int nc, int N,
       job.seg_seeds[0].push_back((uint32_t)seeds[k]);
       job.hist[0].append(colours, job.ops[0]);
     }
-    int rc = ctx->reserve_aux(fused_scratch_bytes(job));
+    int rc = ctx->reserve_aux(fused_scratch_bytes(ctx, job));
     if (rc) return rc;
     rc = fused_run(ctx, job, ctx->aux);
     if (rc) return rc;
@@ -804,7 +810,7 @@ int rt::recttri_device_walk(rt_ctx* ctx, TriArrays32* a, int nc, int N,
       job.seg_seeds[0].push_back((uint32_t)seeds[k]);
       job.hist[0].append(colours, job.ops[0]);
     }
-    int rc = ctx->reserve_aux(fused_scratch_bytes(job));
+    int rc = ctx->reserve_aux(fused_scratch_bytes(ctx, job));
     if (rc) return rc;
     rc = fused_run(ctx, job, ctx->aux);
     if (rc) return rc;

@@ -119,7 +119,7 @@ __device__ __forceinline__ void sweep_step(Cell* lds, const Lanes& L, Tiny* st, 
 // W wave64s per workgroup; each lane owns one tile column and R rows
 // (rows wv, wv+W, ...), so the loaded tile is (W*R) x 64 cells.
 template <int MODEL, int NC, int W, int R>
-__global__ __launch_bounds__(64 * W, 2) void k_fused(const UnitDesc* __restrict__ units,
+__global__ __launch_bounds__(64 * W, W >= 16 ? 1 : 2) void k_fused(const UnitDesc* __restrict__ units,
                                                     const WorkItem* __restrict__ work, long s0,
                                                     int K, int init, int seg, int H,
                                                     ModelParams mp,
@@ -261,24 +261,23 @@ long OpBuilder::append(const std::vector<uint8_t>& colours, std::vector<uint8_t>
   return pulls;
 }
 
-// Tile geometry per (model, chains): waves W, rows per thread R, halo H.
-// 64 x 64 tiles for byte cells; int32 rect-triangle pairs use 32 x 64 to
-// keep two chains' three planes within 64 KB of LDS.
+// Tile shapes: waves W, rows per lane R (tile height W*R), halo H.  Byte
+// cells use 64-row tiles (8 x 8 or 16 x 4, same tiles, same halo) or 32-row
+// tiles (16 x 2); int32 rect-triangle pairs use 32-row tiles (8 x 4, H = 4)
+// to keep two chains' three planes within 64 KB of LDS.
 struct Geo {
   int W, R, H;
 };
+enum TileShape { SH_AUTO = 0, SH_8x8 = 1, SH_16x4 = 2, SH_16x2 = 3, SH_RT8x4 = 4 };
 
-static Geo geometry(int model, int nc) {
-  if (model == M_RECTTRI && nc == 2) return Geo{8, 4, 4};
-  return Geo{8, 8, 6};
+static Geo shape_geo(int shape) {
+  switch (shape) {
+    case SH_16x4: return Geo{16, 4, 6};
+    case SH_16x2: return Geo{16, 2, 6};
+    case SH_RT8x4: return Geo{8, 4, 4};
+    default: return Geo{8, 8, 6};
+  }
 }
-
-int fused_rows_per_thread(const FusedJob& job) {
-  const Geo g = geometry(job.model, job.nc);
-  return g.W * g.R;
-}
-
-int fused_halo(const FusedJob& job) { return geometry(job.model, job.nc).H; }
 
 static void tile_counts(const UnitDesc& u, int LH, int H, int* tr, int* tc) {
   const int TH = LH - 2 * H, TW = kLW - 2 * H;
@@ -286,12 +285,36 @@ static void tile_counts(const UnitDesc& u, int LH, int H, int* tr, int* tc) {
   *tc = (u.cols + TW - 1) / TW;
 }
 
-size_t fused_scratch_bytes(const FusedJob& job) {
-  const int LH = fused_rows_per_thread(job), H = fused_halo(job);
+// Shape of a walk / batch job.  Few tiles (a sample or chain running alone)
+// means idle CUs and a launch bound by each wave's chain of dependent LDS
+// round trips: 16 waves halve that chain (-30 % per launch for one side-300
+// hexagon CFTP sample), and 32-row tiles halve it again while putting 2.6x
+// more workgroups on the chip, at a higher halo share.  With several tiles
+// per CU the 8-wave shape's ILP per wave wins (-12 % per launch at 32
+// samples).
+static int choose_shape(const rt_ctx* ctx, const FusedJob& job) {
+  if (job.model == M_RECTTRI) return job.nc == 2 ? SH_RT8x4 : SH_8x8;
+  if (ctx->tile_shape != SH_AUTO) return ctx->tile_shape;
+  long t64 = 0;
+  for (const UnitDesc& u : job.units) {
+    int tr, tc;
+    tile_counts(u, 64, 6, &tr, &tc);
+    t64 += (long)tr * tc;
+  }
+  if (3 * t64 <= ctx->num_cu) return SH_16x2;
+  if (t64 <= ctx->num_cu) return SH_16x4;
+  return SH_8x8;
+}
+
+static Geo job_geo(const FusedJob& job) { return shape_geo(job.shape); }
+
+size_t fused_scratch_bytes(const rt_ctx* ctx, FusedJob& job) {
+  job.shape = choose_shape(ctx, job);
+  const Geo g = job_geo(job);
   size_t work = 0;
   for (const UnitDesc& u : job.units) {
     int tr, tc;
-    tile_counts(u, LH, H, &tr, &tc);
+    tile_counts(u, g.W * g.R, g.H, &tr, &tc);
     work += (size_t)tr * tc;
   }
   long T = 0;
@@ -312,15 +335,24 @@ static void launch(rt_ctx* ctx, unsigned nblocks, const UnitDesc* units, const W
 typedef void (*LaunchFn)(rt_ctx*, unsigned, const UnitDesc*, const WorkItem*, long, int, int, int,
                          int, const ModelParams&, const WindowRec*);
 
-static LaunchFn pick(int model, int nc) {
-  if (model == M_DOMINO) return nc == 1 ? launch<M_DOMINO, 1, 8, 8> : launch<M_DOMINO, 2, 8, 8>;
+template <int MODEL, int NC>
+static LaunchFn pick_byte(int shape) {
+  if (shape == SH_16x4) return launch<MODEL, NC, 16, 4>;
+  if (shape == SH_16x2) return launch<MODEL, NC, 16, 2>;
+  return launch<MODEL, NC, 8, 8>;
+}
+
+static LaunchFn pick(int model, int nc, int shape) {
+  if (model == M_DOMINO) return nc == 1 ? pick_byte<M_DOMINO, 1>(shape) : pick_byte<M_DOMINO, 2>(shape);
   if (model == M_LOZENGE)
-    return nc == 1 ? launch<M_LOZENGE, 1, 8, 8> : launch<M_LOZENGE, 2, 8, 8>;
+    return nc == 1 ? pick_byte<M_LOZENGE, 1>(shape) : pick_byte<M_LOZENGE, 2>(shape);
   return nc == 1 ? launch<M_RECTTRI, 1, 8, 8> : launch<M_RECTTRI, 2, 8, 4>;
 }
 
 int fused_upload(rt_ctx* ctx, FusedJob& job, void* scratch, FusedPlan* plan) {
-  const int LH = fused_rows_per_thread(job), H = fused_halo(job);
+  if (job.shape == SH_AUTO) job.shape = choose_shape(ctx, job);
+  const Geo geo = job_geo(job);
+  const int LH = geo.W * geo.R, H = geo.H;
   const size_t nu = job.units.size();
   *plan = FusedPlan();
   long T = 0;
@@ -397,8 +429,8 @@ int fused_upload(rt_ctx* ctx, FusedJob& job, void* scratch, FusedPlan* plan) {
 
 int fused_launch(rt_ctx* ctx, const FusedJob& job, const FusedPlan& plan) {
   if (plan.windows.empty()) return RT_OK;
-  LaunchFn fn = pick(job.model, job.nc);
-  const int H = fused_halo(job);
+  LaunchFn fn = pick(job.model, job.nc, job.shape);
+  const int H = job_geo(job).H;
   for (const FusedPlan::Window& w : plan.windows)
     fn(ctx, plan.nwork, plan.d_units, plan.d_work, w.s0, w.K, w.init, w.seg, H, job.mp, nullptr);
   HIPCHK_RET(ctx, hipGetLastError());
@@ -412,15 +444,17 @@ int fused_launch_windows(rt_ctx* ctx, int model, int nc, const ModelParams& mp,
                          const UnitDesc* d_units, const WorkItem* d_work, unsigned nwork,
                          const WindowRec* d_wins) {
   if (nwork == 0) return RT_OK;
-  const Geo g = geometry(model, nc);
-  pick(model, nc)(ctx, nwork, d_units, d_work, 0, 0, 0, 0, g.H, mp, d_wins);
+  // 64-row tiles (fused_tiles); 16 waves when there are no more tiles than CUs.
+  const int shape = model == M_RECTTRI ? (nc == 2 ? SH_RT8x4 : SH_8x8)
+                    : nwork <= (unsigned)ctx->num_cu ? SH_16x4 : SH_8x8;
+  pick(model, nc, shape)(ctx, nwork, d_units, d_work, 0, 0, 0, 0, shape_geo(shape).H, mp, d_wins);
   HIPCHK_RET(ctx, hipGetLastError());
   ctx->last.launches += 1;
   return RT_OK;
 }
 
 void fused_tiles(int model, int nc, int rows, int cols, int* tiles_r, int* tiles_c) {
-  const Geo g = geometry(model, nc);
+  const Geo g = shape_geo(model == M_RECTTRI && nc == 2 ? SH_RT8x4 : SH_8x8);
   UnitDesc u{};
   u.rows = rows;
   u.cols = cols;
@@ -428,7 +462,7 @@ void fused_tiles(int model, int nc, int rows, int cols, int* tiles_r, int* tiles
 }
 
 int fused_run(rt_ctx* ctx, FusedJob& job, void* scratch) {
-  FusedPlan plan;
+  FusedPlan plan;  // job.shape was fixed when the scratch was sized
   int rc = fused_upload(ctx, job, scratch, &plan);
   if (rc) return rc;
   return fused_launch(ctx, job, plan);

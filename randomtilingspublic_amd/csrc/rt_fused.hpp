@@ -90,11 +90,13 @@ struct FusedJob {
   std::vector<std::vector<uint8_t>> ops;          // [unit] concatenated op bytes
   std::vector<OpBuilder> hist;        // [unit]
   bool adaptive = true;               // single unit: stretch windows to H pulls
+  int shape = 0;                      // tile shape, resolved by fused_scratch_bytes
 };
 
 // Run all windows of the job on ctx->stream (no final sync).  `scratch`
-// must hold fused_scratch_bytes(job) bytes of device memory.
-size_t fused_scratch_bytes(const FusedJob& job);
+// must hold fused_scratch_bytes(ctx, job) bytes of device memory; that call
+// also fixes the job's tile shape (from the tile count and ctx->tile_shape).
+size_t fused_scratch_bytes(const rt_ctx* ctx, FusedJob& job);
 int fused_run(rt_ctx* ctx, FusedJob& job, void* scratch);
 
 // The same in two phases, so the launch sequence can be captured in a
@@ -127,8 +129,5 @@ void fused_tiles(int model, int nc, int rows, int cols, int* tiles_r, int* tiles
 // Bring every colour plane of every unit up to date (end of a walk).
 int fused_sync(rt_ctx* ctx, const FusedJob& job);
 
-// Tile geometry (256 threads, 64 columns, 4*R rows per tile incl. halo).
-int fused_rows_per_thread(const FusedJob& job);
-int fused_halo(const FusedJob& job);
 
 }  // namespace rt

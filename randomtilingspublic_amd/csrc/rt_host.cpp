@@ -576,6 +576,100 @@ tiling recttri_max_hex(int n) {
   return recttri_shear(a, n);
 }
 
+// slopeHex(N) (RectTriangleTiler.cpp:173-478): the "sloped" start tiling.
+// The hexagon's three axes through the centre and its main diagonal split
+// it into six slices; each slice has a bulk code, codes for its two edge
+// lines (next to an axis or the diagonal) and for their corner, chosen by a
+// parity; the codes are the reference's table.
+namespace {
+struct SliceCodes {
+  int edge1[2], edge2[2], corner[2], bulk[2];  // [0] = parity even, [1] = odd; corner by N
+};
+}  // namespace
+
+tiling recttri_slope_hex(int n) {
+  const int M = 2 * n + 1, N = n;
+  std::vector<int> a((size_t)M * M, 0);
+  const int Nodd = N & 1;
+  // Slices: A i<N, j<i; B i>N, j>i; C j<N, i<j; D j>N, i>j; E i<N, j>N, i>j-N;
+  // F i>N, j<N, i<j+N.  edge1 / edge2 as in the comments per slice.
+  static const SliceCodes A = {{0x58881b, 0x855be8}, {0x58388e, 0x8eb558}, {0x583812, 0x8ebbe8},
+                               {0x588885, 0x855558}};
+  static const SliceCodes B = {{0xb18885, 0x8eb558}, {0xe88385, 0x855be8}, {0x218385, 0x8ebbe8},
+                               {0x588885, 0x855558}};
+  static const SliceCodes Cs = {{0x7477d2, 0x47d47a}, {0x7473a7, 0xa74d74}, {0x747312, 0xa7dd7a},
+                                {0x747747, 0x474474}};
+  static const SliceCodes D = {{0x2d7747, 0xa74d74}, {0x7a3747, 0x47d47a}, {0x213747, 0xa7dd7a},
+                               {0x747747, 0x474474}};
+  static const SliceCodes E = {{0x996f19, 0x6699cf}, {0x296c99, 0xfc9966}, {0x296319, 0xfc99cf},
+                               {0x996699, 0x669966}};
+  static const SliceCodes F = {{0x91f699, 0xfc9966}, {0x99c692, 0x6699cf}, {0x913692, 0xfc99cf},
+                               {0x996699, 0x669966}};
+  auto slice = [&](const SliceCodes& c, bool e1, bool e2, int par) {
+    if (e1 && e2) return c.corner[Nodd];
+    if (e1) return c.edge1[par];
+    if (e2) return c.edge2[par];
+    return c.bulk[par];
+  };
+  // Codes on the axes i == N, j == N and the diagonal i == j:
+  // {far before odd, far before even, just before (N even, N odd),
+  //  far after odd, far after even, just after (N even, N odd)}.
+  struct Axis {
+    int before[2], near_before[2], after[2], near_after[2];
+  };
+  static const Axis row = {{0xe81c91, 0x1be1fc}, {0x122133, 0xe81c91}, {0x19c18e, 0xcf1eb1},
+                           {0x331221, 0x29c38e}};
+  static const Axis col = {{0x7af229, 0xd22afc}, {0x122133, 0x7af229}, {0x922fa7, 0xcfa22d},
+                           {0x331221, 0x913fa7}};
+  static const Axis dia = {{0xbd7833, 0x33aebd}, {0x331221, 0xbd7812}, {0x3387db, 0xdbea33},
+                           {0x122133, 0x3387db}};
+  auto axis = [&](const Axis& x, int k) {
+    if (k < N - 1) return x.before[k & 1];
+    if (k == N - 1) return x.near_before[Nodd];
+    if (k > N + 1) return x.after[k & 1];
+    return x.near_after[Nodd];  // k == N + 1
+  };
+  for (int i = 1; i < M - 1; i++)
+    for (int j = 1; j < M - 1; j++) {
+      int v = 0;
+      const int pe = ((i + j) % 2 != Nodd) ? 1 : 0;
+      if (i == N && j == N) v = Nodd ? 0x122133 : 0x213312;
+      else if (i == N) v = axis(row, j);
+      else if (j == N) v = axis(col, i);
+      else if (i == j) v = axis(dia, i);
+      else if (i < N && j < i) v = slice(A, i == N - 1, j == i - 1, j & 1);
+      else if (i > N && j > i) v = slice(B, i == N + 1, j == i + 1, j & 1);
+      else if (j < N && i < j) v = slice(Cs, j == N - 1, i == j - 1, i & 1);
+      else if (j > N && i > j) v = slice(D, j == N + 1, i == j + 1, i & 1);
+      else if (i < N && j > N && i > j - N) v = slice(E, i == N - 1, j == N + 1, pe);
+      else if (i > N && j < N && i < j + N) v = slice(F, i == N + 1, j == N - 1, pe);
+      a[(size_t)i * M + j] = v;
+    }
+  auto at = [&](int i, int j) -> int& { return a[(size_t)i * M + j]; };
+  for (int i = 2; i < N - 1; i++) at(i, 0) = 0x008085, at(0, i) = 0x000747;
+  at(1, 0) = 0x00308e;
+  at(0, 1) = 0x0003a7;
+  at(N - 1, 0) = 0x00801b;
+  at(0, N - 1) = 0x0007d2;
+  for (int i = N + 2; i < M - 2; i++) at(i, M - 1) = 0x580800, at(M - 1, i) = 0x747000;
+  at(N + 1, M - 1) = 0xb10800;
+  at(M - 1, N + 1) = 0x2d7000;
+  at(M - 2, M - 1) = 0xe80300;
+  at(M - 1, M - 2) = 0x7a3000;
+  for (int i = 2; i < N - 1; i++) at(i, i + N) = 0x900690, at(i + N, i) = 0x096009;
+  at(1, N + 1) = 0x200c90;
+  at(N + 1, 1) = 0x01f009;
+  at(N - 1, M - 2) = 0x900f10;
+  at(M - 2, N - 1) = 0x09c002;
+  at(0, 0) = N == 1 ? 0x000012 : 0x000033;
+  at(M - 1, N) = N == 1 ? 0x013000 : 0x022000;
+  at(N, M - 1) = N == 1 ? 0x200300 : 0x100100;
+  at(0, N) = 0x000220;
+  at(N, 0) = 0x001001;
+  at(M - 1, M - 1) = 0x330000;
+  return recttri_shear(a, n);
+}
+
 tiling recttri_min_hex(int n) {
   const int M = 2 * n + 1;
   std::vector<int> a((size_t)M * M, 0);
@@ -735,6 +829,11 @@ rt_vec* rt_recttri_max_hex(int n) {
 rt_vec* rt_recttri_min_hex(int n) {
   if (n < 1) return nullptr;
   return make_vec(recttri_min_hex(n));
+}
+
+rt_vec* rt_recttri_slope_hex(int n) {
+  if (n < 1) return nullptr;
+  return make_vec(recttri_slope_hex(n));
 }
 
 // Bibone statics.  Tilings are N x N with N even.

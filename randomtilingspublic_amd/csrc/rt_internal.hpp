@@ -52,6 +52,7 @@ domain lozenge_tiling_to_domain(const tiling& t);
 
 tiling recttri_max_hex(int n);
 tiling recttri_min_hex(int n);
+tiling recttri_slope_hex(int n);
 
 // Bibone statics (rt_bibone_host.cpp); N is the tiling side.
 tiling bibone_ice_cream_cone(int N, int M);

@@ -40,6 +40,7 @@ struct rt_ctx {
   uint32_t* h_flag = nullptr;
   rt_timing last{};
   int engine = 0;
+  int tile_shape = 0;  // sweep-block tile shape: 0 auto, else a TileShape
 
   int reserve(size_t bytes);
   int reserve_aux(size_t bytes);

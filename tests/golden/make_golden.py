@@ -97,7 +97,44 @@ def ref_host():
     np.savez_compressed(os.path.join(HERE, "reference_host.npz"), **arrs)
 
 
+def svg_digests():
+    """SHA-256 of the reference's example SVGs whose inputs are deterministic
+    (Examples/ExampleOuts), keyed by the call that made them.  One line of
+    TriangleDimer/DimerStart.svg comes from an out-of-bounds read in the
+    reference (TriangleDimerTiler.cpp:1080-1082 reads d[(i-1)*2*M+j] at
+    i = 0, a negative index, which happened to hold 1); that line is dropped
+    before hashing, and the engine's writer reads outside the domain as 0."""
+    import hashlib
+    out_dir = "/root/reference/Examples/ExampleOuts"
+    if not os.path.isdir(out_dir):
+        print("reference checkout absent; skipping svg_digests.json")
+        return
+    files = {
+        "domino_tiling_MinimalDomino_StartTiling": "MinimalDomino/StartTiling.svg",
+        "domino_tiling_aztec16_min": "WeightedAztecDiamond/TilingStart.svg",
+        "domino_tiling_aztec16_random_end": "WeightedAztecDiamond/TilingEnd.svg",
+        "lozenge_domain_hex10_2": "MinimalLozenge/domainAlmostHex.svg",
+        "lozenge_tiling_hex10_2_max": "MinimalLozenge/tilingStartAlmostHex.svg",
+        "recttri_tiling_max5": "RectTriangle/MaxTiling.svg",
+        "recttri_tiling_min5": "RectTriangle/MinTiling.svg",
+        "recttri_tiling_slope5": "RectTriangle/SlopeTiling.svg",
+        "recttri_tiling_random_end5": "RectTriangle/EndTiling.svg",
+        "bibone_domain_cone6": "TriangleDimer/Domain.svg",
+        "bibone_tiling_cone6": "TriangleDimer/TilingStart.svg",
+        "bibone_dimer_cone6": "TriangleDimer/DimerStart.svg",
+    }
+    dig = {}
+    for key, rel in files.items():
+        data = open(os.path.join(out_dir, rel), "rb").read()
+        if key == "bibone_dimer_cone6":
+            data = data.replace(b'<use xlink:href = "#eH" x = "3.5" y = "0"/>\n', b"", 1)
+        dig[key] = {"sha256": hashlib.sha256(data).hexdigest(), "bytes": len(data), "file": rel}
+    with open(os.path.join(HERE, "svg_digests.json"), "w") as f:
+        json.dump(dig, f, indent=1)
+
+
 if __name__ == "__main__":
     schedules()
     ref_host()
+    svg_digests()
     print("golden fixtures written", file=sys.stderr)

@@ -205,3 +205,37 @@ def test_engines_agree_with_oracle(gpu_ctx, engine):
         assert np.array_equal(gpu_ctx.recttri_walk(r, 555, 2), o.recttri_walk(r, 555, 2, PR))
     finally:
         gpu_ctx.set_engine(rt.ENGINE_AUTO)
+
+
+@pytest.mark.parametrize("shape", [1, 2, 3])
+def test_tile_shapes_agree_with_oracle(gpu_ctx, shape):
+    """Every sweep-block tile shape (64 rows / 8 waves, 64 rows / 16 waves,
+    32 rows / 16 waves) on domains spanning several tiles, for a walk of each
+    byte-cell model and a batched CFTP."""
+    gpu_ctx.set_tile_shape(shape)
+    try:
+        d, tmin, tmax = _domino(45)
+        N = rt.side(tmin)
+        P = _params(N * N // 2, seed=12)
+        gpu_ctx.set_tinymt_params(P)
+        _say("shape", shape, "domino-45")
+        assert np.array_equal(gpu_ctx.domino_walk(tmin, 901, 5, rt.DOMINO_HEATBATH),
+                              o.domino_walk(tmin, 901, 5, P, rt.DOMINO_HEATBATH))
+        dl, lmin, lmax = _lozenge(33, 4)
+        NL = rt.side(lmin)
+        PL = _params(NL * NL, seed=13)
+        gpu_ctx.set_tinymt_params(PL)
+        _say("shape", shape, "lozenge-33")
+        assert np.array_equal(gpu_ctx.lozenge_walk(lmin, 1201, 8), o.lozenge_walk(lmin, 1201, 8, PL))
+        dl, lmin, lmax = _lozenge(6)
+        NL = rt.side(lmin)
+        PL = _params(NL * NL, seed=14)
+        gpu_ctx.set_tinymt_params(PL)
+        _say("shape", shape, "lozenge-6 cftp batch")
+        outs, stats = gpu_ctx.lozenge_cftp_batch(lmax, lmin, [4, 5, 6], rt.SCHED_DOUBLING, 64, 40)
+        for k, master in enumerate([4, 5, 6]):
+            w, ws = o.lozenge_cftp(lmax, lmin, master, PL, rt.SCHED_DOUBLING, 64, 40)
+            assert np.array_equal(outs[k], w)
+            assert stats[k]["rounds"] == ws["rounds"]
+    finally:
+        gpu_ctx.set_tile_shape(0)

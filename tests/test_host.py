@@ -27,11 +27,17 @@ def test_aztec16_min_tiling_matches_example_output():
     assert np.array_equal(t, load_matrix("aztec16_min_tiling.txt"))
 
 
-@pytest.mark.parametrize("kind", ["max", "min"])
+@pytest.mark.parametrize("kind", ["max", "min", "slope"])
 def test_recttri_hex5_matches_example_output(kind):
     # RectTriangleEx.cpp:29-33
-    f = rt.RectTriangleTiler.maxHex if kind == "max" else rt.RectTriangleTiler.minHex
+    f = getattr(rt.RectTriangleTiler, kind + "Hex")
     assert np.array_equal(f(5), load_matrix(f"recttri5_{kind}.txt"))
+
+
+def test_recttri_slope_hex_matches_reference_vectors():
+    z = np.load(os.path.join(GOLD, "reference_host.npz"))
+    for n in (1, 2, 5, 8):
+        assert np.array_equal(rt.RectTriangleTiler.slopeHex(n), z[f"recttri_slope{n}"])
 
 
 def test_random_end_state_is_a_tiling_of_the_same_domain():
@@ -158,3 +164,4 @@ def test_against_reference_library(n):
     assert np.array_equal(rt.LozengeTiler.MinTiling(dl), call("ref_lozenge_min_tiling", dl))
     assert np.array_equal(rt.RectTriangleTiler.maxHex(n), call("ref_recttri_max_hex", n))
     assert np.array_equal(rt.RectTriangleTiler.minHex(n), call("ref_recttri_min_hex", n))
+    assert np.array_equal(rt.RectTriangleTiler.slopeHex(n), call("ref_recttri_slope_hex", n))
