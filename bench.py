@@ -29,6 +29,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# Sweep-block kernel instantiation per tile shape (rt_timing.shape_*[k]):
+# k_fused<model 1 = lozenge, 2 chains, waves, rows per lane>.
+KERNEL_BY_SHAPE = ["k_fused<1, 2, 8, 8>", "k_fused<1, 2, 16, 4>", "k_fused<1, 2, 16, 2>",
+                   "k_fused<2, 2, 8, 4>"]
 
 
 def cpu_model():
@@ -70,6 +74,37 @@ def cpu_baseline(N, tmin, params, units_attempts, nsamples, threads, budget_s=12
     }
 
 
+def plain_sweep(rt, order=1000, steps=2000):
+    """The plain Aztec-diamond sweep at HBM scale: heat-bath domino walk on
+    the Aztec diamond of order 1000 (N = 2006, 2 x 2 060 036-cell colour
+    planes, 2.0 M TinyMT streams), one launch per colour step (the
+    reference's launch shape, ENGINE_STEP).  Algorithmic bytes per flip
+    attempt: TinyMT state 16 B read + 16 B written + parameters 12 B, the
+    rotated cell 1 B read + 1 B written and its four neighbours 1 B
+    each read once per step = 48 B."""
+    d = rt.DominoTiler.AztecDiamond(order)
+    t = rt.DominoTiler.MinTiling(d)
+    N = rt.side(t)
+    ctx = rt.Context(0)
+    try:
+        ctx.set_engine(rt.ENGINE_STEP)
+        ctx.set_tinymt_params(rt.synthetic_params(N * N // 2, None))
+        ctx.domino_walk(t, 50, 1, rt.DOMINO_HEATBATH)
+        ctx.domino_walk(t, steps, 1, rt.DOMINO_HEATBATH)
+        tm = ctx.last_timing()
+    finally:
+        ctx.close()
+    att_s = tm["attempts"] / (tm["device_ms"] / 1e3)
+    gbs = 48.0 * att_s / 1e9
+    return {"workload": f"domino Aztec diamond order {order} (N={N}), heat-bath, {steps} steps, "
+                        "one launch per colour step",
+            "kernel": "k_domino_step<1>", "us_per_step": round(tm["device_ms"] * 1e3 / steps, 2),
+            "flip_attempts_per_s": round(att_s, 1),
+            "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                         "alg_bytes_per_attempt": 48}}
+
+
 def copy_probe(mib):
     """Plain device-to-device copy of `mib` MiB, five times: a kernel whose
     HBM bytes are known (mib MiB read, mib MiB written per launch), to scale
@@ -108,6 +143,29 @@ def profile_rounds(ctx, rt, tmax, tmin, masters, t0, rounds):
                       "steps": tm["steps"]}), flush=True)
 
 
+def cpu_c1(threads):
+    """The reference's CPU-runnable case (Aztec diamond of order 100, heat
+    bath, seed 1) on the oracle, 1 thread and `threads` threads; 2000 of the
+    10^4 steps, scaled."""
+    import randomtilingspublic_amd as rt
+    from tests import oracle_lib as o
+    d = rt.DominoTiler.AztecDiamond(100)
+    t = rt.DominoTiler.MinTiling(d)
+    N = rt.side(t)
+    p = rt.synthetic_params(N * N // 2, None)
+    out = {"workload": "domino Aztec diamond order 100, heat-bath, seed 1, 2000 of 10^4 steps"}
+    sites = (N - 2) * (N // 2 - 2)
+    for th in (1, threads):
+        o.set_threads(th)
+        t0 = time.time()
+        o.domino_walk(t, 2000, 1, p, rt.DOMINO_HEATBATH)
+        dt = time.time() - t0
+        out[f"threads_{th}"] = {"flip_attempts_per_s": round(2000 * sites / dt, 1),
+                                "s_per_1e4_steps": round(dt * 5, 3)}
+    o.set_threads(1)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -118,6 +176,7 @@ def main():
     ap.add_argument("--max-rounds", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--profile-rounds", type=int, default=0,
                     help="profiling run: the same batch capped at R doubling rounds (no sample "
                          "needs to finish); prints the sweep kernel's timing only")
@@ -167,25 +226,41 @@ def main():
     for k in range(len(masters)):
         assert np.array_equal(rt.LozengeTiler.TilingToDomain(outs[k]), dom)
 
-    # Plain path: the reference's shape, one sample per call on one chain pair.
-    plain = None
+    # Plain paths: one sample per call.  "plain" is the reference's launch
+    # shape (one launch per colour step, ENGINE_STEP); "plain_fused" is the
+    # same call on this engine's sweep-block kernel.  Both must return the
+    # batched run's sample bit for bit.
+    plain = plain_fused = None
     if not args.no_plain:
-        t1 = time.time()
-        o1, s1 = ctx.lozenge_cftp(tmax, tmin, masters[0], rt.SCHED_DOUBLING, args.t0,
-                                  args.max_rounds)
-        pw = time.time() - t1
-        assert np.array_equal(o1, outs[0])
-        plain = {"samples_per_s": 1.0 / pw, "seed": masters[0], "wall_s": pw,
-                 "rounds": s1["rounds"]}
-        print(f"[bench] plain CFTP seed {masters[0]} wall {pw:.2f}s", file=sys.stderr, flush=True)
+        for engine in (rt.ENGINE_STEP, 0):
+            ctx.set_engine(engine)
+            t1 = time.time()
+            o1, s1 = ctx.lozenge_cftp(tmax, tmin, masters[0], rt.SCHED_DOUBLING, args.t0,
+                                      args.max_rounds)
+            pw = time.time() - t1
+            assert np.array_equal(o1, outs[0])
+            rec = {"samples_per_s": 1.0 / pw, "seed": masters[0], "wall_s": round(pw, 3),
+                   "rounds": s1["rounds"],
+                   "engine": "one launch per colour step" if engine else "sweep-block kernel"}
+            if engine:
+                plain = rec
+            else:
+                plain_fused = rec
+            print(f"[bench] plain CFTP ({rec['engine']}) seed {masters[0]} wall {pw:.2f}s",
+                  file=sys.stderr, flush=True)
+        ctx.set_engine(0)
 
-    # Roofline of the sweep kernel: algorithmic bytes per launch = for every
-    # sample's sites, TinyMT state (16 B) + parameters (12 B) read, state
-    # written (16 B), and 3 colour planes x 2 chains read and written (12 B).
-    klaunch = max(tm["kernel_launches"], 1)
-    kms = tm["kernel_ms"]
+    # Roofline of the sweep kernel with the most kernel time (the batch picks
+    # a tile shape per round from its tile count): algorithmic bytes per
+    # launch = for every sample's sites, TinyMT state (16 B) + parameters
+    # (12 B) read, state written (16 B), and 3 colour planes x 2 chains read
+    # and written (12 B).
+    shape = int(np.argmax(tm["shape_ms"]))
+    kname = KERNEL_BY_SHAPE[shape]
+    klaunch = max(tm["shape_launches"][shape], 1)
+    kms = tm["shape_ms"][shape]
     avg_launch_s = kms / 1e3 / klaunch
-    sites_per_launch = tm["kernel_sites"] / klaunch
+    sites_per_launch = tm["shape_sites"][shape] / klaunch
     alg_bytes = 56.0 * sites_per_launch
     achieved_gbs = alg_bytes / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
     traffic = None
@@ -195,16 +270,21 @@ def main():
         # scaled by a plain copy's readings, as bytes per site: times this
         # run's sites per launch.
         with open(counters) as f:
-            traffic = int(json.load(f)["hbm_bytes_per_site"] * sites_per_launch)
+            per_kernel = json.load(f)
+        if kname in per_kernel:
+            traffic = int(per_kernel[kname]["hbm_bytes_per_site"] * sites_per_launch)
     roof = {"bound": "hbm", "achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "kernel": "k_fused<lozenge, 2 chains>", "avg_launch_us": round(avg_launch_s * 1e6, 2),
-            "launches": klaunch, "steps_per_launch": round(tm["steps"] / klaunch, 2),
+            "kernel": kname, "avg_launch_us": round(avg_launch_s * 1e6, 2),
+            "launches": klaunch, "kernel_time_share": round(kms / max(tm["kernel_ms"], 1e-9), 3),
             "sites_per_launch": int(sites_per_launch), "alg_bytes_per_launch": int(alg_bytes),
-            "flip_attempts_per_s_in_kernel": round(2 * draws / (kms / 1e3), 1) if kms else None}
+            "flip_attempts_per_s_in_kernels": round(2 * draws / (tm["kernel_ms"] / 1e3), 1)
+            if tm["kernel_ms"] else None}
+    sweep = None if args.no_sweep else plain_sweep(rt)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(N, tmin, params, attempts, len(masters), min(os.cpu_count() or 1, 16))
+        cpu["c1"] = cpu_c1(cpu["cores"])
         print(f"[bench] cpu baseline {cpu}", file=sys.stderr, flush=True)
 
     line = {
@@ -229,7 +309,11 @@ def main():
         "device_s": round(dev_s, 3),
         "rounds": [s["rounds"] for s in stats],
         "plain": plain,
+        "plain_fused": plain_fused,
         "batched_vs_plain": round(samples_s / plain["samples_per_s"], 2) if plain else None,
+        "batched_vs_plain_fused":
+            round(samples_s / plain_fused["samples_per_s"], 2) if plain_fused else None,
+        "plain_sweep": sweep,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

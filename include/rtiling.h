@@ -68,11 +68,20 @@ typedef struct {
   double kernel_ms;   /* time inside the sweep kernels alone (batched calls) */
   long kernel_launches;
   long kernel_sites;  /* sum over those launches of the TinyMT sites they own */
+  /* The same three split by sweep-block tile shape (index shape - 1:
+   * 64 rows / 8 waves, 64 rows / 16 waves, 32 rows / 16 waves, 32 rows /
+   * 8 waves for rect-triangle pairs); no reference counterpart. */
+  double shape_ms[4];
+  long shape_launches[4];
+  long shape_sites[4];
 } rt_timing;
 int rt_last_timing(const rt_ctx* ctx, rt_timing* out);
 
 /* Kernel selection for walks: 0 = auto (fused LDS sweep blocks where the
- * shape allows), 1 = one launch per colour step, 2 = fused only. */
+ * shape allows), 1 = one launch per colour step, 2 = fused only, 3 = fused
+ * with one cell per lane for every model (lozenge otherwise packs four
+ * cells per lane-word); results never depend on it. */
+enum { RT_ENGINE_AUTO = 0, RT_ENGINE_STEP = 1, RT_ENGINE_FUSED = 2, RT_ENGINE_FUSED_CELL = 3 };
 int rt_set_engine(rt_ctx* ctx, int engine);
 
 /* Tile shape of the sweep-block kernels (no reference counterpart; a tuning

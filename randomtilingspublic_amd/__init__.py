@@ -24,7 +24,7 @@ ERRORS = {
 
 DOMINO_TOGGLE08, DOMINO_HEATBATH, DOMINO_METRO2P, DOMINO_HEATBATH2P = 0, 1, 2, 3
 SCHED_REFERENCE, SCHED_DOUBLING = 0, 1
-ENGINE_AUTO, ENGINE_STEP, ENGINE_FUSED = 0, 1, 2
+ENGINE_AUTO, ENGINE_STEP, ENGINE_FUSED, ENGINE_FUSED_CELL = 0, 1, 2, 3
 
 
 class RTError(RuntimeError):
@@ -37,7 +37,8 @@ class _Timing(C.Structure):
     _fields_ = [("device_ms", C.c_double), ("launches", C.c_long),
                 ("steps", C.c_long), ("attempts", C.c_long),
                 ("kernel_ms", C.c_double), ("kernel_launches", C.c_long),
-                ("kernel_sites", C.c_long)]
+                ("kernel_sites", C.c_long), ("shape_ms", C.c_double * 4),
+                ("shape_launches", C.c_long * 4), ("shape_sites", C.c_long * 4)]
 
 
 class DominoRule(C.Structure):
@@ -252,7 +253,9 @@ class Context:
         self._chk(self._L.rt_last_timing(self._h, C.byref(t)))
         return {"device_ms": t.device_ms, "launches": t.launches, "steps": t.steps,
                 "attempts": t.attempts, "kernel_ms": t.kernel_ms,
-                "kernel_launches": t.kernel_launches, "kernel_sites": t.kernel_sites}
+                "kernel_launches": t.kernel_launches, "kernel_sites": t.kernel_sites,
+                "shape_ms": list(t.shape_ms), "shape_launches": list(t.shape_launches),
+                "shape_sites": list(t.shape_sites)}
 
     # ---- domino ----
     def domino_walk(self, t, steps, seed, rule=DOMINO_TOGGLE08, a=1.0, b=1.0):

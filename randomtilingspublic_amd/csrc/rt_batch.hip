@@ -283,6 +283,11 @@ static int cftp_batch(rt_ctx* ctx, int model, const int32_t* tmax, const int32_t
     acc.kernel_ms += kms;
     acc.kernel_launches += (long)plan.windows.size();
     acc.kernel_sites += (long)plan.windows.size() * plan.streams;
+    if (job.shape >= 1 && job.shape <= 4) {
+      acc.shape_ms[job.shape - 1] += kms;
+      acc.shape_launches[job.shape - 1] += (long)plan.windows.size();
+      acc.shape_sites[job.shape - 1] += (long)plan.windows.size() * plan.streams;
+    }
     acc.launches += ctx->last.launches;
     acc.steps += ctx->last.steps;
     acc.attempts += ctx->last.attempts;

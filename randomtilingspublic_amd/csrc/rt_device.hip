@@ -68,7 +68,7 @@ __global__ void k_domino_step(DominoArrays c0, DominoArrays c1, StreamTable S, i
   const long P = (long)(N - 2) * Wd;
   const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= P) return;
-  const int i = (int)(gid / Wd) + 1, j = (int)(gid % Wd) + 1;
+  const int i = (int)((unsigned)gid / (unsigned)Wd) + 1, j = (int)((unsigned)gid % (unsigned)Wd) + 1;
   Tiny t = tiny_load(S, gid);
   const uint32_t rd = tiny_bits(t);
   tiny_store(S, gid, t);
@@ -89,7 +89,7 @@ __global__ void k_domino_sync(DominoArrays c0, DominoArrays c1, int N, int last)
   const long P = (long)(N - 2) * Wd;
   const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= P) return;
-  const int i = (int)(gid / Wd) + 1, j = (int)(gid % Wd) + 1;
+  const int i = (int)((unsigned)gid / (unsigned)Wd) + 1, j = (int)((unsigned)gid % (unsigned)Wd) + 1;
 #pragma unroll
   for (int c = 0; c < NC; c++) {
     const DominoArrays& A = c == 0 ? c0 : c1;
@@ -107,7 +107,7 @@ __global__ void k_lozenge_step(TriArrays8 c0, TriArrays8 c1, StreamTable S, int 
   const long P = (long)N * M;
   const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= P) return;
-  const int i = (int)(gid / M), j = (int)(gid % M);
+  const int i = (int)((unsigned)gid / (unsigned)M), j = (int)((unsigned)gid % (unsigned)M);
   Tiny t = tiny_load(S, gid);
   const uint32_t rd = tiny_bits(t);
   tiny_store(S, gid, t);
@@ -133,7 +133,7 @@ __global__ void k_lozenge_sync(TriArrays8 c0, TriArrays8 c1, int N, int colour, 
   const long P = (long)N * M;
   const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= P) return;
-  const int i = (int)(gid / M), j = (int)(gid % M);
+  const int i = (int)((unsigned)gid / (unsigned)M), j = (int)((unsigned)gid % (unsigned)M);
   if (!(i >= 1 && i <= N - 2 && j >= 1 && j <= M - 2)) return;
   const int ca = (colour + 2) % 3, cb = (colour + 1) % 3;
 #pragma unroll
@@ -156,7 +156,7 @@ __global__ void k_recttri_step(TriArrays32 c0, TriArrays32 c1, StreamTable S, in
   const long P = (long)N * M;
   const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= P) return;
-  const int i = (int)(gid / M), j = (int)(gid % M);
+  const int i = (int)((unsigned)gid / (unsigned)M), j = (int)((unsigned)gid % (unsigned)M);
   Tiny t = tiny_load(S, gid);
   const uint32_t rd = tiny_bits(t);
   tiny_store(S, gid, t);
@@ -179,7 +179,7 @@ __global__ void k_recttri_sync(TriArrays32 c0, TriArrays32 c1, int N, int last) 
   const long P = (long)N * M;
   const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= P) return;
-  const int i = (int)(gid / M), j = (int)(gid % M);
+  const int i = (int)((unsigned)gid / (unsigned)M), j = (int)((unsigned)gid % (unsigned)M);
   if (!(i >= 1 && i <= N - 2 && j >= 1 && j <= M - 2)) return;
 #pragma unroll
   for (int c = 0; c < NC; c++) {
@@ -291,7 +291,7 @@ extern "C" int rt_last_timing(const rt_ctx* ctx, rt_timing* out) {
 }
 
 extern "C" int rt_set_engine(rt_ctx* ctx, int engine) {
-  if (!ctx || engine < 0 || engine > 2) return RT_EINVAL;
+  if (!ctx || engine < RT_ENGINE_AUTO || engine > RT_ENGINE_FUSED_CELL) return RT_EINVAL;
   ctx->engine = engine;
   return RT_OK;
 }

@@ -238,6 +238,12 @@ __global__ __launch_bounds__(64 * W, W >= 16 ? 1 : 2) void k_fused(const UnitDes
   }
 }
 
+}  // namespace rt
+
+#include "swar_lozenge.hpp"
+
+namespace rt {
+
 // ------------------------------------------------------------- host side
 long OpBuilder::append(const std::vector<uint8_t>& colours, std::vector<uint8_t>& ops) {
   long pulls = 0;
@@ -342,8 +348,27 @@ static LaunchFn pick_byte(int shape) {
   return launch<MODEL, NC, 8, 8>;
 }
 
-static LaunchFn pick(int model, int nc, int shape) {
+template <int NC, int W, int R>
+static void launch_swar(rt_ctx* ctx, unsigned nblocks, const UnitDesc* units, const WorkItem* work,
+                        long s0, int K, int init, int seg, int H, const ModelParams& mp,
+                        const WindowRec* wins) {
+  hipLaunchKernelGGL((k_swar_lozenge<NC, W, R>), dim3(nblocks), dim3(64 * W), 0, ctx->stream,
+                     units, work, s0, K, init, seg, H, mp, wins);
+}
+
+// Lozenge: four cells per lane-word (k_swar_lozenge), same tile per shape:
+// 64 rows = 8 waves x 2 row slots or 16 waves x 1; 32 rows = 8 waves x 1.
+template <int NC>
+static LaunchFn pick_swar(int shape) {
+  if (shape == SH_16x4) return launch_swar<NC, 16, 1>;
+  if (shape == SH_16x2) return launch_swar<NC, 8, 1>;
+  return launch_swar<NC, 8, 2>;
+}
+
+static LaunchFn pick(const rt_ctx* ctx, int model, int nc, int shape) {
   if (model == M_DOMINO) return nc == 1 ? pick_byte<M_DOMINO, 1>(shape) : pick_byte<M_DOMINO, 2>(shape);
+  if (model == M_LOZENGE && ctx->engine != RT_ENGINE_FUSED_CELL)
+    return nc == 1 ? pick_swar<1>(shape) : pick_swar<2>(shape);
   if (model == M_LOZENGE)
     return nc == 1 ? pick_byte<M_LOZENGE, 1>(shape) : pick_byte<M_LOZENGE, 2>(shape);
   return nc == 1 ? launch<M_RECTTRI, 1, 8, 8> : launch<M_RECTTRI, 2, 8, 4>;
@@ -429,7 +454,7 @@ int fused_upload(rt_ctx* ctx, FusedJob& job, void* scratch, FusedPlan* plan) {
 
 int fused_launch(rt_ctx* ctx, const FusedJob& job, const FusedPlan& plan) {
   if (plan.windows.empty()) return RT_OK;
-  LaunchFn fn = pick(job.model, job.nc, job.shape);
+  LaunchFn fn = pick(ctx, job.model, job.nc, job.shape);
   const int H = job_geo(job).H;
   for (const FusedPlan::Window& w : plan.windows)
     fn(ctx, plan.nwork, plan.d_units, plan.d_work, w.s0, w.K, w.init, w.seg, H, job.mp, nullptr);
@@ -447,7 +472,8 @@ int fused_launch_windows(rt_ctx* ctx, int model, int nc, const ModelParams& mp,
   // 64-row tiles (fused_tiles); 16 waves when there are no more tiles than CUs.
   const int shape = model == M_RECTTRI ? (nc == 2 ? SH_RT8x4 : SH_8x8)
                     : nwork <= (unsigned)ctx->num_cu ? SH_16x4 : SH_8x8;
-  pick(model, nc, shape)(ctx, nwork, d_units, d_work, 0, 0, 0, 0, shape_geo(shape).H, mp, d_wins);
+  pick(ctx, model, nc, shape)(ctx, nwork, d_units, d_work, 0, 0, 0, 0, shape_geo(shape).H, mp,
+                              d_wins);
   HIPCHK_RET(ctx, hipGetLastError());
   ctx->last.launches += 1;
   return RT_OK;

@@ -1,0 +1,236 @@
+// swar_lozenge.hpp -- lozenge sweep-block kernel with four byte cells per
+// 32-bit word (included by rt_fused.hip).
+//
+// Same tiles, halo, windows and op bytes as k_fused (one 64-column tile
+// per workgroup, H = 6), but each lane owns one word = 4 consecutive cells
+// of a row, and a wave covers 4 rows x 16 words.  The lozenge pulls
+// (lozengekernel.cl:49-52, see lozenge_pull_prev_m / _prev2_m) only move
+// single bits inside a cell, so they are whole-word mask-and-shift
+// operations; the +-1 column neighbours are byte funnel shifts of two
+// adjacent words (v_alignbyte).  The cube move (lozengekernel.cl:20-25:
+// 25 -> 38 when rd < .4, 38 -> 25 when .4 <= rd < .8) is a per-byte
+// equality test and an XOR with 25 ^ 38 = 0x3F under byte masks built from
+// the four cells' own TinyMT draws.  Every cell still draws from its own
+// stream, so results are those of the one-cell-per-lane kernel.
+#pragma once
+
+namespace rt {
+
+// 0xFF in every byte of x that is zero, 0x00 elsewhere (exact: no carries
+// cross byte lanes since (x & 0x7F) + 0x7F <= 0xFE).
+__device__ __forceinline__ uint32_t swar_zero_bytes(uint32_t x) {
+  const uint32_t t = ~(((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x | 0x7F7F7F7Fu);
+  return (t - (t >> 7)) | t;
+}
+
+__device__ __forceinline__ uint32_t swar_byte_mask(bool b0, bool b1, bool b2, bool b3) {
+  return (b0 ? 0x000000FFu : 0u) | (b1 ? 0x0000FF00u : 0u) | (b2 ? 0x00FF0000u : 0u) |
+         (b3 ? 0xFF000000u : 0u);
+}
+
+// Word of the cells one column to the right (+1) or left (-1) of word `lo`
+// (`hi` / `lo` = the next / previous word in the row).
+__device__ __forceinline__ uint32_t swar_right(uint32_t cur, uint32_t next) {
+  return __builtin_amdgcn_alignbyte(next, cur, 1);
+}
+__device__ __forceinline__ uint32_t swar_left(uint32_t cur, uint32_t prev) {
+  return __builtin_amdgcn_alignbyte(cur, prev, 3);
+}
+
+struct SwarRow {
+  int li;         // LDS word index of the lane's word in this row slot
+  int im;         // global row mod 3
+  uint32_t in;    // byte mask: cell inside the domain (has a TinyMT stream)
+  uint32_t interior;  // byte mask: cell pulls from its neighbours
+};
+
+// One colour step.  All reads precede all writes (see sweep_step).
+template <int NC, int R, int PULL, int PLW>
+__device__ __forceinline__ void swar_step(uint32_t* lds, const SwarRow* row, Tiny (*st)[4],
+                                          uint32_t op) {
+  const int c = op & OP_COLOUR;
+  const int ca = (c + 2) % 3, cb = (c + 1) % 3;
+  uint32_t up[R], down[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    uint32_t b[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) b[k] = tiny_bits(st[r][k]);
+    up[r] = swar_byte_mask(b[0] < kBelow04, b[1] < kBelow04, b[2] < kBelow04, b[3] < kBelow04);
+    const uint32_t lt8 =
+        swar_byte_mask(b[0] < kBelow08, b[1] < kBelow08, b[2] < kBelow08, b[3] < kBelow08);
+    down[r] = lt8 & ~up[r];
+  }
+  uint32_t out[NC][R];
+#pragma unroll
+  for (int ch = 0; ch < NC; ch++) {
+    const uint32_t* P = lds + ch * 3 * PLW;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int li = row[r].li, im = row[r].im;
+      uint32_t v = P[c * PLW + li];
+      if (PULL & 1) {  // bits {2,4,32} from colour c-1
+        const uint32_t* Q = P + ca * PLW;
+        const bool p2 = im == (2 * ca + 1) % 3, p4 = im == (2 * ca + 2) % 3;
+        const uint32_t a0 = Q[li - 16], a1 = Q[li - 15];
+        const uint32_t b0 = Q[li], bm = Q[li - 1];
+        const uint32_t cc = Q[li + 16];
+        const uint32_t a = p2 ? swar_right(a0, a1) : a0;
+        const uint32_t b = p4 ? swar_left(b0, bm) : b0;
+        const uint32_t pv = (v & 0xD9D9D9D9u) | ((a & 0x10101010u) >> 3) |
+                            ((b & 0x08080808u) >> 1) | ((cc & 0x01010101u) << 5);
+        v = (pv & row[r].interior) | (v & ~row[r].interior);
+      }
+      if (PULL & 2) {  // bits {1,8,16} from colour c-2
+        const uint32_t* Q = P + cb * PLW;
+        const bool p8 = im == (2 * cb) % 3, p16 = im == (2 * cb + 1) % 3;
+        const uint32_t a = Q[li - 16];
+        const uint32_t b0 = Q[li], b1 = Q[li + 1];
+        const uint32_t c0 = Q[li + 16], cm = Q[li + 15];
+        const uint32_t b = p8 ? swar_right(b0, b1) : b0;
+        const uint32_t cc = p16 ? swar_left(c0, cm) : c0;
+        const uint32_t pv = (v & 0xE6E6E6E6u) | ((a & 0x20202020u) >> 5) |
+                            ((b & 0x04040404u) << 1) | ((cc & 0x02020202u) << 3);
+        v = (pv & row[r].interior) | (v & ~row[r].interior);
+      }
+      const uint32_t e25 = swar_zero_bytes(v ^ 0x19191919u);
+      const uint32_t e38 = swar_zero_bytes(v ^ 0x26262626u);
+      const uint32_t flip = ((e25 & up[r]) | (e38 & down[r])) & row[r].in & 0x3F3F3F3Fu;
+      out[ch][r] = v ^ flip;
+    }
+  }
+#pragma unroll
+  for (int ch = 0; ch < NC; ch++)
+#pragma unroll
+    for (int r = 0; r < R; r++) lds[ch * 3 * PLW + c * PLW + row[r].li] = out[ch][r];
+}
+
+// Tile of 4*W*R rows x 64 columns; wave wv, row slot r, lane quarter q
+// (lane >> 4) own tile row 4*(wv + W*r) + q, word w = lane & 15 (cells
+// 4w .. 4w+3).  Kernel arguments as k_fused.
+template <int NC, int W, int R>
+__global__ __launch_bounds__(64 * W, W >= 16 ? 1 : 2) void k_swar_lozenge(
+    const UnitDesc* __restrict__ units, const WorkItem* __restrict__ work, long s0, int K,
+    int init, int seg, int H, ModelParams mp, const WindowRec* __restrict__ wins) {
+  constexpr int kThreads = 64 * W;
+  constexpr int LH = 4 * W * R;
+  constexpr int PLW = (LH + 2) * 16;  // words per plane, with a zero margin row each side
+  __shared__ uint32_t lds[NC * 3 * PLW];
+  __shared__ uint8_t sops[kKMax];
+
+  const WorkItem wi = work[blockIdx.x];
+  const UnitDesc U = units[wi.unit];
+  uint32_t seed = 0;
+  if (wins) {
+    const WindowRec w = wins[wi.unit];
+    s0 = w.s0;
+    K = w.K;
+    init = w.init;
+    seed = w.seed;
+    if (K <= 0 && !init) return;
+  } else if (init) {
+    seed = U.seeds[seg];
+  }
+  const int rows = U.rows, cols = U.cols;
+  const int TH = LH - 2 * H, TW = kLW - 2 * H;
+  const int tr = wi.tile / U.tiles_c, tc = wi.tile % U.tiles_c;
+  const int r0 = tr * TH - H, c0 = tc * TW - H;
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int q = lane >> 4, w = lane & 15;
+  const int gc0 = c0 + 4 * w;
+
+  for (int k = threadIdx.x; k < NC * 3 * PLW; k += kThreads) lds[k] = 0;
+  if (threadIdx.x < K) sops[threadIdx.x] = U.ops[s0 + threadIdx.x];
+  __syncthreads();
+
+  SwarRow row[R];
+  Tiny st[R][4];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int lr = 4 * (wv + W * r) + q, gr = r0 + lr;
+    row[r].li = (lr + 1) * 16 + w;
+    row[r].im = ((gr % 3) + 3) % 3;
+    const bool row_in = gr >= 0 && gr < rows;
+    const bool row_int = gr >= 1 && gr <= rows - 2;
+    uint32_t in = 0, inter = 0;
+    uint32_t cell[NC][3];
+#pragma unroll
+    for (int ch = 0; ch < NC; ch++)
+#pragma unroll
+      for (int k = 0; k < 3; k++) cell[ch][k] = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int gc = gc0 + k;
+      const bool hs = row_in && gc >= 0 && gc < cols;
+      st[r][k] = Tiny{0, 0, 0, 0, 0, 0, 0};
+      if (!hs) continue;
+      in |= 0xFFu << (8 * k);
+      if (row_int && gc >= 1 && gc <= cols - 2) inter |= 0xFFu << (8 * k);
+      const long cidx = (long)gr * cols + gc;
+#pragma unroll
+      for (int ch = 0; ch < NC; ch++)
+#pragma unroll
+        for (int pl = 0; pl < 3; pl++)
+          cell[ch][pl] |= (uint32_t)((const uint8_t*)U.plane[ch][pl])[cidx] << (8 * k);
+      const long id = cidx;  // lozenge stream id j + (N/3) i
+      st[r][k].mat1 = U.param[id];
+      st[r][k].mat2 = U.param[U.pstride + id];
+      st[r][k].tmat = U.param[2 * U.pstride + id];
+      if (init) {
+        tiny_init(st[r][k], seed + (uint32_t)id);
+      } else {
+        st[r][k].s0 = U.state[id];
+        st[r][k].s1 = U.state[U.nstream + id];
+        st[r][k].s2 = U.state[2 * U.nstream + id];
+        st[r][k].s3 = U.state[3 * U.nstream + id];
+      }
+    }
+    row[r].in = in;
+    row[r].interior = inter;
+#pragma unroll
+    for (int ch = 0; ch < NC; ch++)
+#pragma unroll
+      for (int pl = 0; pl < 3; pl++) lds[(ch * 3 + pl) * PLW + row[r].li] = cell[ch][pl];
+  }
+  __syncthreads();
+
+  for (int s = 0; s < K; s++) {
+    const uint32_t op = sops[s];
+    switch ((op >> 2) & 3) {
+      case 0: swar_step<NC, R, 0, PLW>(lds, row, st, op); break;
+      case 1: swar_step<NC, R, 1, PLW>(lds, row, st, op); break;
+      case 2: swar_step<NC, R, 2, PLW>(lds, row, st, op); break;
+      default: swar_step<NC, R, 3, PLW>(lds, row, st, op); break;
+    }
+    __syncthreads();
+  }
+
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int lr = 4 * (wv + W * r) + q, gr = r0 + lr;
+    if (lr < H || lr >= H + TH) continue;
+    uint32_t words[NC][3];
+#pragma unroll
+    for (int ch = 0; ch < NC; ch++)
+#pragma unroll
+      for (int pl = 0; pl < 3; pl++) words[ch][pl] = lds[(ch * 3 + pl) * PLW + row[r].li];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int lc = 4 * w + k, gc = gc0 + k;
+      if (lc < H || lc >= H + TW || !((row[r].in >> (8 * k)) & 1u)) continue;
+      const long cidx = (long)gr * cols + gc;
+#pragma unroll
+      for (int ch = 0; ch < NC; ch++)
+#pragma unroll
+        for (int pl = 0; pl < 3; pl++)
+          ((uint8_t*)U.plane[ch][pl])[cidx] = (uint8_t)(words[ch][pl] >> (8 * k));
+      U.state[cidx] = st[r][k].s0;
+      U.state[U.nstream + cidx] = st[r][k].s1;
+      U.state[2 * U.nstream + cidx] = st[r][k].s2;
+      U.state[3 * U.nstream + cidx] = st[r][k].s3;
+    }
+  }
+}
+
+}  // namespace rt
