@@ -78,10 +78,11 @@ typedef struct {
 int rt_last_timing(const rt_ctx* ctx, rt_timing* out);
 
 /* Kernel selection for walks: 0 = auto (fused LDS sweep blocks where the
- * shape allows), 1 = one launch per colour step, 2 = fused only, 3 = fused
- * with one cell per lane for every model (lozenge otherwise packs four
- * cells per lane-word); results never depend on it. */
-enum { RT_ENGINE_AUTO = 0, RT_ENGINE_STEP = 1, RT_ENGINE_FUSED = 2, RT_ENGINE_FUSED_CELL = 3 };
+ * shape allows), 1 = one launch per colour step, 2 = fused only, 3 = fused,
+ * with the lozenge kernel that packs four cells per lane-word (fewer VALU
+ * operations per cell, slower on the side-300 hexagon because its per-cell
+ * stream loads are strided); results never depend on it. */
+enum { RT_ENGINE_AUTO = 0, RT_ENGINE_STEP = 1, RT_ENGINE_FUSED = 2, RT_ENGINE_FUSED_SWAR = 3 };
 int rt_set_engine(rt_ctx* ctx, int engine);
 
 /* Tile shape of the sweep-block kernels (no reference counterpart; a tuning

@@ -24,7 +24,7 @@ ERRORS = {
 
 DOMINO_TOGGLE08, DOMINO_HEATBATH, DOMINO_METRO2P, DOMINO_HEATBATH2P = 0, 1, 2, 3
 SCHED_REFERENCE, SCHED_DOUBLING = 0, 1
-ENGINE_AUTO, ENGINE_STEP, ENGINE_FUSED, ENGINE_FUSED_CELL = 0, 1, 2, 3
+ENGINE_AUTO, ENGINE_STEP, ENGINE_FUSED, ENGINE_FUSED_SWAR = 0, 1, 2, 3
 
 
 class RTError(RuntimeError):

@@ -291,7 +291,7 @@ extern "C" int rt_last_timing(const rt_ctx* ctx, rt_timing* out) {
 }
 
 extern "C" int rt_set_engine(rt_ctx* ctx, int engine) {
-  if (!ctx || engine < RT_ENGINE_AUTO || engine > RT_ENGINE_FUSED_CELL) return RT_EINVAL;
+  if (!ctx || engine < RT_ENGINE_AUTO || engine > RT_ENGINE_FUSED_SWAR) return RT_EINVAL;
   ctx->engine = engine;
   return RT_OK;
 }

@@ -356,18 +356,17 @@ static void launch_swar(rt_ctx* ctx, unsigned nblocks, const UnitDesc* units, co
                      units, work, s0, K, init, seg, H, mp, wins);
 }
 
-// Lozenge: four cells per lane-word (k_swar_lozenge), same tile per shape:
-// 64 rows = 8 waves x 2 row slots or 16 waves x 1; 32 rows = 8 waves x 1.
+// Lozenge: four cells per lane-word (k_swar_lozenge), same tiles: 64-row
+// tiles by 16 waves x 1 word row, 32-row tiles by 8 waves x 1.
 template <int NC>
 static LaunchFn pick_swar(int shape) {
-  if (shape == SH_16x4) return launch_swar<NC, 16, 1>;
   if (shape == SH_16x2) return launch_swar<NC, 8, 1>;
-  return launch_swar<NC, 8, 2>;
+  return launch_swar<NC, 16, 1>;
 }
 
 static LaunchFn pick(const rt_ctx* ctx, int model, int nc, int shape) {
   if (model == M_DOMINO) return nc == 1 ? pick_byte<M_DOMINO, 1>(shape) : pick_byte<M_DOMINO, 2>(shape);
-  if (model == M_LOZENGE && ctx->engine != RT_ENGINE_FUSED_CELL)
+  if (model == M_LOZENGE && ctx->engine == RT_ENGINE_FUSED_SWAR)
     return nc == 1 ? pick_swar<1>(shape) : pick_swar<2>(shape);
   if (model == M_LOZENGE)
     return nc == 1 ? pick_byte<M_LOZENGE, 1>(shape) : pick_byte<M_LOZENGE, 2>(shape);

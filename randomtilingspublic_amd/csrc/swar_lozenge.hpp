@@ -7,7 +7,7 @@
 // (lozengekernel.cl:49-52, see lozenge_pull_prev_m / _prev2_m) only move
 // single bits inside a cell, so they are whole-word mask-and-shift
 // operations; the +-1 column neighbours are byte funnel shifts of two
-// adjacent words (v_alignbyte).  The cube move (lozengekernel.cl:20-25:
+// adjacent words by a per-lane amount (no divergent branches).  The cube move (lozengekernel.cl:20-25:
 // 25 -> 38 when rd < .4, 38 -> 25 when .4 <= rd < .8) is a per-byte
 // equality test and an XOR with 25 ^ 38 = 0x3F under byte masks built from
 // the four cells' own TinyMT draws.  Every cell still draws from its own
@@ -16,32 +16,32 @@
 
 namespace rt {
 
-// 0xFF in every byte of x that is zero, 0x00 elsewhere (exact: no carries
+// 0x80 in every byte of x that is zero, 0x00 elsewhere (exact: no carries
 // cross byte lanes since (x & 0x7F) + 0x7F <= 0xFE).
-__device__ __forceinline__ uint32_t swar_zero_bytes(uint32_t x) {
-  const uint32_t t = ~(((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x | 0x7F7F7F7Fu);
-  return (t - (t >> 7)) | t;
+__device__ __forceinline__ uint32_t swar_zero80(uint32_t x) {
+  return ~(((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x) & 0x80808080u;
 }
 
-__device__ __forceinline__ uint32_t swar_byte_mask(bool b0, bool b1, bool b2, bool b3) {
-  return (b0 ? 0x000000FFu : 0u) | (b1 ? 0x0000FF00u : 0u) | (b2 ? 0x00FF0000u : 0u) |
-         (b3 ? 0xFF000000u : 0u);
+// 0x80 in byte k when b[k] < t.
+__device__ __forceinline__ uint32_t swar_below80(const uint32_t* b, uint32_t t) {
+  return (b[0] < t ? 0x00000080u : 0u) | (b[1] < t ? 0x00008000u : 0u) |
+         (b[2] < t ? 0x00800000u : 0u) | (b[3] < t ? 0x80000000u : 0u);
 }
 
-// Word of the cells one column to the right (+1) or left (-1) of word `lo`
-// (`hi` / `lo` = the next / previous word in the row).
-__device__ __forceinline__ uint32_t swar_right(uint32_t cur, uint32_t next) {
-  return __builtin_amdgcn_alignbyte(next, cur, 1);
-}
-__device__ __forceinline__ uint32_t swar_left(uint32_t cur, uint32_t prev) {
-  return __builtin_amdgcn_alignbyte(cur, prev, 3);
+// Cells one column to the right (shift 8: byte k <- cell k+1) or unshifted
+// (shift 0), from the lane's word `cur` and the next word `next`; and one
+// column to the left (shift 24: byte k <- cell k-1) or unshifted (shift 32)
+// from `cur` and the previous word.  A 64-bit funnel shift by a per-lane
+// amount keeps both words in use, so no lane branches on its row.
+__device__ __forceinline__ uint32_t swar_funnel(uint32_t hi, uint32_t lo, uint32_t sh) {
+  return (uint32_t)((((uint64_t)hi << 32) | lo) >> sh);
 }
 
 struct SwarRow {
-  int li;         // LDS word index of the lane's word in this row slot
-  int im;         // global row mod 3
-  uint32_t in;    // byte mask: cell inside the domain (has a TinyMT stream)
-  uint32_t interior;  // byte mask: cell pulls from its neighbours
+  int li;             // LDS word index of the lane's word in this row slot
+  int im;             // global row mod 3
+  uint32_t in;        // 0x3F in bytes of cells inside the domain (they move)
+  uint32_t interior;  // 0xFF in bytes of cells that pull from neighbours
 };
 
 // One colour step.  All reads precede all writes (see sweep_step).
@@ -50,16 +50,16 @@ __device__ __forceinline__ void swar_step(uint32_t* lds, const SwarRow* row, Tin
                                           uint32_t op) {
   const int c = op & OP_COLOUR;
   const int ca = (c + 2) % 3, cb = (c + 1) % 3;
+  const int pa2 = (2 * ca + 1) % 3, pa4 = (2 * ca + 2) % 3;  // rows where prev shifts
+  const int pb8 = (2 * cb) % 3, pb16 = (2 * cb + 1) % 3;      // rows where prev2 shifts
   uint32_t up[R], down[R];
 #pragma unroll
   for (int r = 0; r < R; r++) {
     uint32_t b[4];
 #pragma unroll
     for (int k = 0; k < 4; k++) b[k] = tiny_bits(st[r][k]);
-    up[r] = swar_byte_mask(b[0] < kBelow04, b[1] < kBelow04, b[2] < kBelow04, b[3] < kBelow04);
-    const uint32_t lt8 =
-        swar_byte_mask(b[0] < kBelow08, b[1] < kBelow08, b[2] < kBelow08, b[3] < kBelow08);
-    down[r] = lt8 & ~up[r];
+    up[r] = swar_below80(b, kBelow04);
+    down[r] = swar_below80(b, kBelow08) & ~up[r];
   }
   uint32_t out[NC][R];
 #pragma unroll
@@ -69,34 +69,27 @@ __device__ __forceinline__ void swar_step(uint32_t* lds, const SwarRow* row, Tin
     for (int r = 0; r < R; r++) {
       const int li = row[r].li, im = row[r].im;
       uint32_t v = P[c * PLW + li];
-      if (PULL & 1) {  // bits {2,4,32} from colour c-1
+      if (PULL & 1) {  // bits {2,4,32} from colour c-1: (i-1, j+p2), (i, j-p4), (i+1, j)
         const uint32_t* Q = P + ca * PLW;
-        const bool p2 = im == (2 * ca + 1) % 3, p4 = im == (2 * ca + 2) % 3;
-        const uint32_t a0 = Q[li - 16], a1 = Q[li - 15];
-        const uint32_t b0 = Q[li], bm = Q[li - 1];
+        const uint32_t a = swar_funnel(Q[li - 15], Q[li - 16], im == pa2 ? 8u : 0u);
+        const uint32_t b = swar_funnel(Q[li], Q[li - 1], im == pa4 ? 24u : 32u);
         const uint32_t cc = Q[li + 16];
-        const uint32_t a = p2 ? swar_right(a0, a1) : a0;
-        const uint32_t b = p4 ? swar_left(b0, bm) : b0;
         const uint32_t pv = (v & 0xD9D9D9D9u) | ((a & 0x10101010u) >> 3) |
                             ((b & 0x08080808u) >> 1) | ((cc & 0x01010101u) << 5);
         v = (pv & row[r].interior) | (v & ~row[r].interior);
       }
-      if (PULL & 2) {  // bits {1,8,16} from colour c-2
+      if (PULL & 2) {  // bits {1,8,16} from colour c-2: (i-1, j), (i, j+p8), (i+1, j-p16)
         const uint32_t* Q = P + cb * PLW;
-        const bool p8 = im == (2 * cb) % 3, p16 = im == (2 * cb + 1) % 3;
         const uint32_t a = Q[li - 16];
-        const uint32_t b0 = Q[li], b1 = Q[li + 1];
-        const uint32_t c0 = Q[li + 16], cm = Q[li + 15];
-        const uint32_t b = p8 ? swar_right(b0, b1) : b0;
-        const uint32_t cc = p16 ? swar_left(c0, cm) : c0;
+        const uint32_t b = swar_funnel(Q[li + 1], Q[li], im == pb8 ? 8u : 0u);
+        const uint32_t cc = swar_funnel(Q[li + 16], Q[li + 15], im == pb16 ? 24u : 32u);
         const uint32_t pv = (v & 0xE6E6E6E6u) | ((a & 0x20202020u) >> 5) |
                             ((b & 0x04040404u) << 1) | ((cc & 0x02020202u) << 3);
         v = (pv & row[r].interior) | (v & ~row[r].interior);
       }
-      const uint32_t e25 = swar_zero_bytes(v ^ 0x19191919u);
-      const uint32_t e38 = swar_zero_bytes(v ^ 0x26262626u);
-      const uint32_t flip = ((e25 & up[r]) | (e38 & down[r])) & row[r].in & 0x3F3F3F3Fu;
-      out[ch][r] = v ^ flip;
+      const uint32_t x = v ^ 0x19191919u;  // zero bytes: cells equal to 25
+      const uint32_t sel = (swar_zero80(x) & up[r]) | (swar_zero80(x ^ 0x3F3F3F3Fu) & down[r]);
+      out[ch][r] = v ^ (((sel >> 1) - (sel >> 7)) & row[r].in);  // 0x3F in flagged bytes
     }
   }
 #pragma unroll
@@ -186,7 +179,7 @@ __global__ __launch_bounds__(64 * W, W >= 16 ? 1 : 2) void k_swar_lozenge(
         st[r][k].s3 = U.state[3 * U.nstream + id];
       }
     }
-    row[r].in = in;
+    row[r].in = in & 0x3F3F3F3Fu;
     row[r].interior = inter;
 #pragma unroll
     for (int ch = 0; ch < NC; ch++)
@@ -218,7 +211,8 @@ __global__ __launch_bounds__(64 * W, W >= 16 ? 1 : 2) void k_swar_lozenge(
 #pragma unroll
     for (int k = 0; k < 4; k++) {
       const int lc = 4 * w + k, gc = gc0 + k;
-      if (lc < H || lc >= H + TW || !((row[r].in >> (8 * k)) & 1u)) continue;
+      const bool hs = gr >= 0 && gr < rows && gc >= 0 && gc < cols;
+      if (lc < H || lc >= H + TW || !hs) continue;
       const long cidx = (long)gr * cols + gc;
 #pragma unroll
       for (int ch = 0; ch < NC; ch++)

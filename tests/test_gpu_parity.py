@@ -176,7 +176,7 @@ def test_params_too_small_is_rejected(gpu_ctx):
     assert e.value.code == -2
 
 
-@pytest.mark.parametrize("engine", [rt.ENGINE_STEP, rt.ENGINE_FUSED, rt.ENGINE_FUSED_CELL])
+@pytest.mark.parametrize("engine", [rt.ENGINE_STEP, rt.ENGINE_FUSED, rt.ENGINE_FUSED_SWAR])
 def test_engines_agree_with_oracle(gpu_ctx, engine):
     """Both kernel shapes (one launch per step; fused LDS sweep blocks) on
     domains wider and taller than one fused tile."""
