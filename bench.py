@@ -105,6 +105,104 @@ def plain_sweep(rt, order=1000, steps=2000):
                          "alg_bytes_per_attempt": 48}}
 
 
+def served_stream(rt, seconds=20.0, seed=2024):
+    """A Poisson stream of CFTP requests through the continuous-batching
+    server (rt_server), then the same requests one call at a time.  Request
+    mix, drawn from `seed`: domino Aztec diamonds of order 16..48 (heat
+    bath) and lozenge hexagons of side 8..24, doubling from 2^7.  The
+    arrival rate is 70 % of the capacity measured on a burst of 48
+    requests.  Reports samples/s, p50/p99 latency (submit to result) and
+    the plain path's samples/s on the same requests."""
+    rng = np.random.default_rng(seed)
+    cache = {}
+
+    def domain(model, n):
+        if (model, n) not in cache:
+            if model == rt.MODEL_DOMINO:
+                d = rt.DominoTiler.AztecDiamond(n)
+                cache[(model, n)] = (rt.DominoTiler.MaxTiling(d), rt.DominoTiler.MinTiling(d))
+            else:
+                d = rt.LozengeTiler.AlmostHexDomain(n, 0)
+                cache[(model, n)] = (rt.LozengeTiler.MaxTiling(d), rt.LozengeTiler.MinTiling(d))
+        return cache[(model, n)]
+
+    def draw(k):
+        model = int(rng.integers(0, 2))
+        n = int(rng.integers(16, 49)) if model == rt.MODEL_DOMINO else int(rng.integers(8, 25))
+        tmax, tmin = domain(model, n)
+        return dict(model=model, n=n, tmax=tmax, tmin=tmin, master_seed=10000 + k)
+
+    big = max(rt.side(domain(rt.MODEL_DOMINO, 48)[0]), rt.side(domain(rt.MODEL_LOZENGE, 24)[0]))
+    ctx = rt.Context(0)
+    ctx.set_tinymt_params(rt.synthetic_params(big * big, None))
+    srv = rt.Server(ctx, 8 << 30, max_active=256)
+    try:
+        def submit(q):
+            return srv.submit(q["model"], q["tmax"], q["tmin"], q["master_seed"],
+                              rt.SCHED_DOUBLING, 128, 32, rt.DOMINO_HEATBATH)
+
+        # capacity on a burst
+        burst = [draw(k) for k in range(48)]
+        t0 = time.time()
+        ids = [submit(q) for q in burst]
+        for i in ids:
+            srv.wait(i, 120.0)
+            srv.poll(i)
+        cap = len(burst) / (time.time() - t0)
+        rate = 0.7 * cap
+        print(f"[bench] served: burst capacity {cap:.1f} samples/s, stream at {rate:.1f}/s",
+              file=sys.stderr, flush=True)
+        # Poisson stream
+        stream, lat, pending = [], [], {}
+        start = time.time()
+        nxt = start
+        k = 1000
+        while time.time() - start < seconds:
+            now = time.time()
+            while nxt <= now:
+                q = draw(k)
+                k += 1
+                pending[submit(q)] = q
+                stream.append(q)
+                nxt += rng.exponential(1.0 / rate)
+            for i in list(pending):
+                r = srv.poll(i)
+                if r is not None:
+                    assert r[0] == 0
+                    lat.append(r[3])
+                    del pending[i]
+            time.sleep(0.002)
+        for i in list(pending):
+            srv.wait(i, 120.0)
+            r = srv.poll(i)
+            assert r[0] == 0
+            lat.append(r[3])
+        span = time.time() - start
+    finally:
+        srv.close()
+    served = len(stream) / span
+    # the same requests, one call each (the reference's shape: one sample per call)
+    t1 = time.time()
+    for q in stream:
+        if q["model"] == rt.MODEL_DOMINO:
+            ctx.domino_cftp(q["tmax"], q["tmin"], q["master_seed"], rt.SCHED_DOUBLING, 128, 32,
+                            rt.DOMINO_HEATBATH)
+        else:
+            ctx.lozenge_cftp(q["tmax"], q["tmin"], q["master_seed"], rt.SCHED_DOUBLING, 128, 32)
+    plain = len(stream) / (time.time() - t1)
+    ctx.close()
+    lat = np.array(lat)
+    return {"workload": "Poisson stream of CFTP requests: domino Aztec order 16-48 (heat bath) "
+                        "and lozenge hexagon side 8-24, doubling from 2^7, rate 70 % of "
+                        "burst capacity, continuous batching",
+            "requests": len(stream), "span_s": round(span, 2),
+            "samples_per_s": round(served, 2), "burst_capacity_per_s": round(cap, 2),
+            "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
+            "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
+            "plain_samples_per_s": round(plain, 2),
+            "served_vs_plain_capacity": round(cap / plain, 2)}
+
+
 def copy_probe(mib):
     """Plain device-to-device copy of `mib` MiB, five times: a kernel whose
     HBM bytes are known (mib MiB read, mib MiB written per launch), to scale
@@ -177,6 +275,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--profile-rounds", type=int, default=0,
                     help="profiling run: the same batch capped at R doubling rounds (no sample "
                          "needs to finish); prints the sweep kernel's timing only")
@@ -281,6 +380,10 @@ def main():
             "flip_attempts_per_s_in_kernels": round(2 * draws / (tm["kernel_ms"] / 1e3), 1)
             if tm["kernel_ms"] else None}
     sweep = None if args.no_sweep else plain_sweep(rt)
+    served = None
+    if not args.no_serve:
+        served = served_stream(rt)
+        print(f"[bench] served {served}", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(N, tmin, params, attempts, len(masters), min(os.cpu_count() or 1, 16))
@@ -314,6 +417,7 @@ def main():
         "batched_vs_plain_fused":
             round(samples_s / plain_fused["samples_per_s"], 2) if plain_fused else None,
         "plain_sweep": sweep,
+        "served": served,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -813,3 +813,38 @@ int orc_recttri_walk(int32_t* t, int N, long steps, long seed,
   for (int c = 0; c < 3; c++) free(C[c]);
   return ORC_OK;
 }
+
+/* Segmented walk: one RectTriangleTiler::Walk per segment, seeded with
+ * seeds[k] (TinyMT and the colour engine), as the lozenge segments are. */
+int orc_recttri_walk_segments(int32_t* t, int N, const long* steps, const long* seeds, int nseg,
+                              const uint32_t* params3, size_t nparams, const float* weights16) {
+  for (int k = 0; k < nseg; k++) {
+    int rc = orc_recttri_walk(t, N, steps[k], seeds[k], params3, nparams, weights16);
+    if (rc) return rc;
+  }
+  return ORC_OK;
+}
+
+typedef struct {
+  const uint32_t* params3;
+  size_t nparams;
+  const float* weights16;
+} recttri_ctx;
+
+static int recttri_walkseg_cb(int32_t* t, int N, const long* steps, const long* seeds, int n,
+                              const void* c) {
+  const recttri_ctx* d = (const recttri_ctx*)c;
+  return orc_recttri_walk_segments(t, N, steps, seeds, n, d->params3, d->nparams, d->weights16);
+}
+
+/* CFTP for the rectangle-triangle model from maxHex / minHex (no reference
+ * counterpart: the same driver as the other models). */
+int orc_recttri_cftp(const int32_t* tmax, const int32_t* tmin, int N, uint32_t master_seed,
+                     int sched, long t0, int max_rounds, const uint32_t* params3,
+                     size_t nparams, const float* weights16, int32_t* out, orc_cftp_stats* st) {
+  if (N < 9 || N % 3 != 0) return ORC_EINVAL;
+  if (nparams < (size_t)N * (size_t)(N / 3)) return ORC_EPARAMS;
+  recttri_ctx c = {params3, nparams, weights16};
+  return cftp_driver(tmax, tmin, N, master_seed, sched, t0, max_rounds, recttri_walkseg_cb, &c,
+                     out, st);
+}

@@ -131,6 +131,12 @@ void orc_bibone_schedule(long seed, long steps, int32_t* out5);
 int orc_bibone_walk(int32_t* t, int N, long steps, long seed,
                     const uint32_t* params3, size_t nparams, int32_t* planes);
 
+int orc_recttri_walk_segments(int32_t* t, int N, const long* steps, const long* seeds, int nseg,
+                              const uint32_t* params3, size_t nparams, const float* weights16);
+int orc_recttri_cftp(const int32_t* tmax, const int32_t* tmin, int N, uint32_t master_seed,
+                     int sched, long t0, int max_rounds, const uint32_t* params3,
+                     size_t nparams, const float* weights16, int32_t* out, orc_cftp_stats* st);
+
 /* Number of OpenMP threads the oracle uses (1 = serial). */
 void orc_set_threads(int n);
 int orc_get_threads(void);

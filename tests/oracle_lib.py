@@ -64,6 +64,10 @@ _L.orc_bibone_triangle_flips.argtypes = [_I32P, C.c_int, _I32P]
 _L.orc_bibone_triangle_flips.restype = None
 _L.orc_bibone_butterfly_flips.argtypes = [_I32P, C.c_int, _I32P]
 _L.orc_bibone_butterfly_flips.restype = None
+_L.orc_recttri_cftp.argtypes = [_I32P, _I32P, C.c_int, C.c_uint32, C.c_int, C.c_long, C.c_int,
+                                _U32P, C.c_size_t, _F32P, _I32P, C.c_void_p]
+_L.orc_recttri_walk_segments.argtypes = [_I32P, C.c_int, _LONGP, _LONGP, C.c_int, _U32P,
+                                         C.c_size_t, _F32P]
 _L.orc_set_threads.argtypes = [C.c_int]
 
 set_threads = _L.orc_set_threads
@@ -213,3 +217,31 @@ def bibone_walk(t, steps, seed, params3, planes=False):
     _check(_L.orc_bibone_walk(t, N, steps, seed, p, len(p) // 3,
                               pl.ctypes.data_as(C.c_void_p) if planes else None))
     return (t, pl.reshape(8, M, M)) if planes else t
+
+
+def recttri_cftp(tmax, tmin, master, params3, sched=0, t0=128, max_rounds=40, weights16=None):
+    tmax = np.ascontiguousarray(tmax, dtype=np.int32)
+    tmin = np.ascontiguousarray(tmin, dtype=np.int32)
+    out = np.zeros_like(tmax)
+    st = CftpStats()
+    p = _p(params3)
+    w = np.ones(16, dtype=np.float32) if weights16 is None else np.asarray(weights16, np.float32)
+    if weights16 is None:
+        w[0] = 0.0
+    rc = _L.orc_recttri_cftp(tmax, tmin, _side(tmax), master, sched, t0, max_rounds, p,
+                             len(p) // 3, np.ascontiguousarray(w), out, C.addressof(st))
+    _check(rc)
+    return out, {"rounds": st.rounds, "total_steps": st.total_steps, "steps_run": st.steps_run}
+
+
+def recttri_walk_segments(t, steps, seeds, params3, weights16=None):
+    t = np.ascontiguousarray(t, dtype=np.int32).copy()
+    p = _p(params3)
+    w = np.ones(16, dtype=np.float32) if weights16 is None else np.asarray(weights16, np.float32)
+    if weights16 is None:
+        w[0] = 0.0
+    st = np.ascontiguousarray(steps, dtype=np.int64)
+    se = np.ascontiguousarray(seeds, dtype=np.int64)
+    _check(_L.orc_recttri_walk_segments(t, _side(t), st, se, len(st), p, len(p) // 3,
+                                        np.ascontiguousarray(w)))
+    return t
