@@ -181,15 +181,25 @@ def served_stream(rt, seconds=20.0, seed=2024):
     finally:
         srv.close()
     served = len(stream) / span
-    # the same requests, one call each (the reference's shape: one sample per call)
-    t1 = time.time()
-    for q in stream:
-        if q["model"] == rt.MODEL_DOMINO:
-            ctx.domino_cftp(q["tmax"], q["tmin"], q["master_seed"], rt.SCHED_DOUBLING, 128, 32,
-                            rt.DOMINO_HEATBATH)
-        else:
-            ctx.lozenge_cftp(q["tmax"], q["tmin"], q["master_seed"], rt.SCHED_DOUBLING, 128, 32)
-    plain = len(stream) / (time.time() - t1)
+    # The same requests one call at a time (the first 120 of them, bounded):
+    # with the reference's launch shape (one launch per colour step) and with
+    # the sweep-block kernel.
+    def plain_rate(engine, reqs):
+        ctx.set_engine(engine)
+        t1 = time.time()
+        for q in reqs:
+            if q["model"] == rt.MODEL_DOMINO:
+                ctx.domino_cftp(q["tmax"], q["tmin"], q["master_seed"], rt.SCHED_DOUBLING, 128,
+                                32, rt.DOMINO_HEATBATH)
+            else:
+                ctx.lozenge_cftp(q["tmax"], q["tmin"], q["master_seed"], rt.SCHED_DOUBLING, 128,
+                                 32)
+        return len(reqs) / (time.time() - t1)
+
+    sub = stream[:120]
+    plain = plain_rate(rt.ENGINE_STEP, sub)
+    plain_fused = plain_rate(0, sub)
+    ctx.set_engine(0)
     ctx.close()
     lat = np.array(lat)
     return {"workload": "Poisson stream of CFTP requests: domino Aztec order 16-48 (heat bath) "
@@ -200,7 +210,11 @@ def served_stream(rt, seconds=20.0, seed=2024):
             "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
             "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
             "plain_samples_per_s": round(plain, 2),
-            "served_vs_plain_capacity": round(cap / plain, 2)}
+            "plain_fused_samples_per_s": round(plain_fused, 2),
+            "plain_requests": len(sub),
+            "served_vs_plain": round(served / plain, 2),
+            "capacity_vs_plain": round(cap / plain, 2),
+            "capacity_vs_plain_fused": round(cap / plain_fused, 2)}
 
 
 def copy_probe(mib):

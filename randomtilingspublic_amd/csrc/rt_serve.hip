@@ -121,6 +121,18 @@ struct Arena {
 
 constexpr int kMaxSlots = 256;
 constexpr int kOpsPerUnit = 64;  // = the fused kernel's window limit
+// Windows per tick: each tick launches kSub sweep kernels back to back from
+// one upload of window tables, so the host work per tick (tables, event,
+// flag reads) is spread over kSub windows of every live request.
+constexpr int kSub = 4;
+
+// Offset of unit u's op bytes for (buffer b, sub-window k, model m).
+static inline size_t ops_at(int b, int k, int m, size_t u) {
+  return ((((size_t)b * kSub + k) * 2 + m) * kMaxSlots + u) * kOpsPerUnit;
+}
+static inline size_t wins_at(int b, int k, int m) {
+  return (((size_t)b * kSub + k) * 2 + m) * kMaxSlots;
+}
 
 size_t footprint(int model, int N, int* ncol, long* plane, long* nstream, long* cols) {
   if (model == M_DOMINO) {
@@ -163,8 +175,8 @@ struct Server {
   size_t work_cap[2] = {0, 0};
   unsigned nwork[2] = {0, 0};
   std::vector<Slot*> unit_slot[2];  // unit index -> slot, per model
-  WindowRec* d_wins = nullptr;      // [buf][model][kMaxSlots]
-  uint8_t* d_ops = nullptr;         // [buf][model][kMaxSlots][kOpsPerUnit]
+  WindowRec* d_wins = nullptr;      // [buf][sub][model][kMaxSlots]
+  uint8_t* d_ops = nullptr;         // [buf][sub][model][kMaxSlots][kOpsPerUnit]
   uint32_t* d_flags = nullptr;      // [buf][kMaxSlots]
   WindowRec* h_wins = nullptr;
   uint8_t* h_ops = nullptr;
@@ -312,7 +324,7 @@ static int upload_tables(Server* S) {
       d.rule = s->req->mp.rule;
       d.dw = s->req->mp.dw;
       const int u = (int)units.size();
-      d.ops = S->d_ops + (size_t)m * kMaxSlots * kOpsPerUnit;  // + buffer offset via s0
+      d.ops = S->d_ops;  // window s0 = ops_at(buffer, sub, model, unit)
       int tr, tc;
       fused_tiles(model, 2, d.rows, d.cols, &tr, &tc);
       d.tiles_c = tc;
@@ -417,53 +429,53 @@ static int run_tick(Server* S) {
         return fail(S, "plane reset", RT_EDEVICE);
       s->fresh = false;
     }
-  // windows
+  // windows: kSub per tick for every live unit
   std::vector<Slot*> ended;
-  for (int m = 0; m < 2; m++) {
-    const int model = m == 0 ? M_DOMINO : M_LOZENGE;
-    const int H = 6;  // fused halo of byte-cell models (rt_fused.hip geometry)
-    (void)model;
-    for (size_t u = 0; u < S->unit_slot[m].size(); u++) {
-      Slot* s = S->unit_slot[m][u];
-      WindowRec& w = S->h_wins[((size_t)b * 2 + m) * kMaxSlots + u];
-      w = WindowRec{0, 0, 0, 0, 0};
-      if (s->phase != 0) continue;
-      // segment containing the cursor
-      int seg = 0;
-      while (seg + 1 < (int)s->seg_start.size() && s->seg_start[seg + 1] <= s->cursor) seg++;
-      const long seg_end = seg + 1 < (int)s->seg_start.size() ? s->seg_start[seg + 1] : s->total;
-      int K = 0, pulls = 0;
-      while (s->cursor + K < seg_end && K < kOpsPerUnit) {
-        const int p = (s->ops[s->cursor + K] & (OP_PULL_A | OP_PULL_B)) ? 1 : 0;
-        if (pulls + p > H) break;
-        pulls += p;
-        K++;
+  for (int k = 0; k < kSub; k++) {
+    for (int m = 0; m < 2; m++) {
+      const int H = 6;  // fused halo of byte-cell models (rt_fused.hip tile shapes)
+      for (size_t u = 0; u < S->unit_slot[m].size(); u++) {
+        Slot* s = S->unit_slot[m][u];
+        WindowRec& w = S->h_wins[wins_at(b, k, m) + u];
+        w = WindowRec{0, 0, 0, 0, 0};
+        if (s->phase != 0 || s->cursor >= s->total) continue;
+        // segment containing the cursor
+        int seg = 0;
+        while (seg + 1 < (int)s->seg_start.size() && s->seg_start[seg + 1] <= s->cursor) seg++;
+        const long seg_end = seg + 1 < (int)s->seg_start.size() ? s->seg_start[seg + 1] : s->total;
+        int K = 0, pulls = 0;
+        while (s->cursor + K < seg_end && K < kOpsPerUnit) {
+          const int p = (s->ops[s->cursor + K] & (OP_PULL_A | OP_PULL_B)) ? 1 : 0;
+          if (pulls + p > H) break;
+          pulls += p;
+          K++;
+        }
+        std::memcpy(S->h_ops + ops_at(b, k, m, u), s->ops.data() + s->cursor, (size_t)K);
+        w.s0 = (long)ops_at(b, k, m, u);
+        w.K = K;
+        w.init = s->cursor == s->seg_start[seg] ? 1 : 0;
+        w.seed = (uint32_t)s->seeds[seg];
+        s->cursor += K;
+        if (s->cursor >= s->total) ended.push_back(s);
       }
-      uint8_t* o = S->h_ops + (((size_t)b * 2 + m) * kMaxSlots + u) * kOpsPerUnit;
-      std::memcpy(o, s->ops.data() + s->cursor, (size_t)K);
-      w.s0 = (long)((size_t)b * 2 * kMaxSlots * kOpsPerUnit + u * kOpsPerUnit);
-      w.K = K;
-      w.init = s->cursor == s->seg_start[seg] ? 1 : 0;
-      w.seed = (uint32_t)s->seeds[seg];
-      s->cursor += K;
-      if (s->cursor >= s->total) ended.push_back(s);
     }
   }
-  const size_t wbytes = 2 * kMaxSlots * sizeof(WindowRec);
-  const size_t obytes = 2 * (size_t)kMaxSlots * kOpsPerUnit;
-  if (hipMemcpyAsync(S->d_wins + (size_t)b * 2 * kMaxSlots, S->h_wins + (size_t)b * 2 * kMaxSlots,
-                     wbytes, hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
-      hipMemcpyAsync(S->d_ops + (size_t)b * obytes, S->h_ops + (size_t)b * obytes, obytes,
+  const size_t wcount = (size_t)kSub * 2 * kMaxSlots;
+  const size_t obytes = wcount * kOpsPerUnit;
+  if (hipMemcpyAsync(S->d_wins + wins_at(b, 0, 0), S->h_wins + wins_at(b, 0, 0),
+                     wcount * sizeof(WindowRec), hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+      hipMemcpyAsync(S->d_ops + ops_at(b, 0, 0, 0), S->h_ops + ops_at(b, 0, 0, 0), obytes,
                      hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
     return fail(S, "tick upload", RT_EDEVICE);
-  for (int m = 0; m < 2; m++) {
-    if (S->unit_slot[m].empty()) continue;
-    const int model = m == 0 ? M_DOMINO : M_LOZENGE;
-    ModelParams mp = S->unit_slot[m][0]->req->mp;
-    int rc = fused_launch_windows(ctx, model, 2, mp, S->d_units[m], S->d_work[m], S->nwork[m],
-                                  S->d_wins + ((size_t)b * 2 + m) * kMaxSlots);
-    if (rc) return fail(S, ctx->err, rc);
-  }
+  for (int k = 0; k < kSub; k++)
+    for (int m = 0; m < 2; m++) {
+      if (S->unit_slot[m].empty()) continue;
+      const int model = m == 0 ? M_DOMINO : M_LOZENGE;
+      ModelParams mp = S->unit_slot[m][0]->req->mp;
+      int rc = fused_launch_windows(ctx, model, 2, mp, S->d_units[m], S->d_work[m], S->nwork[m],
+                                    S->d_wins + wins_at(b, k, m));
+      if (rc) return fail(S, ctx->err, rc);
+    }
   // slots whose round ended: lazy sync of their planes, then coalescence flag
   for (Slot* s : ended) {
     FusedJob one;
@@ -566,11 +578,11 @@ extern "C" int rt_server_create(rt_ctx* ctx, size_t hbm_budget, int max_active, 
   bool ok = hipMalloc(&S->d_arena, hbm_budget) == hipSuccess;
   for (int m = 0; m < 2 && ok; m++)
     ok = hipMalloc(&S->d_units[m], kMaxSlots * sizeof(UnitDesc)) == hipSuccess;
-  ok = ok && hipMalloc(&S->d_wins, 2 * 2 * kMaxSlots * sizeof(WindowRec)) == hipSuccess;
-  ok = ok && hipMalloc(&S->d_ops, 2 * 2 * (size_t)kMaxSlots * kOpsPerUnit) == hipSuccess;
+  ok = ok && hipMalloc(&S->d_wins, 2 * kSub * 2 * kMaxSlots * sizeof(WindowRec)) == hipSuccess;
+  ok = ok && hipMalloc(&S->d_ops, 2 * kSub * 2 * (size_t)kMaxSlots * kOpsPerUnit) == hipSuccess;
   ok = ok && hipMalloc(&S->d_flags, 2 * kMaxSlots * sizeof(uint32_t)) == hipSuccess;
-  ok = ok && hipHostMalloc(&S->h_wins, 2 * 2 * kMaxSlots * sizeof(WindowRec)) == hipSuccess;
-  ok = ok && hipHostMalloc(&S->h_ops, 2 * 2 * (size_t)kMaxSlots * kOpsPerUnit) == hipSuccess;
+  ok = ok && hipHostMalloc(&S->h_wins, 2 * kSub * 2 * kMaxSlots * sizeof(WindowRec)) == hipSuccess;
+  ok = ok && hipHostMalloc(&S->h_ops, 2 * kSub * 2 * (size_t)kMaxSlots * kOpsPerUnit) == hipSuccess;
   ok = ok && hipHostMalloc(&S->h_flags, 2 * kMaxSlots * sizeof(uint32_t)) == hipSuccess;
   ok = ok && hipEventCreate(&S->ev[0]) == hipSuccess && hipEventCreate(&S->ev[1]) == hipSuccess;
   if (!ok) {
