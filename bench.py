@@ -74,14 +74,14 @@ def cpu_baseline(N, tmin, params, units_attempts, nsamples, threads, budget_s=12
     }
 
 
-def plain_sweep(rt, order=1000, steps=2000):
+def plain_sweep(rt, order=3000, steps=1000):
     """The plain Aztec-diamond sweep at HBM scale: heat-bath domino walk on
-    the Aztec diamond of order 1000 (N = 2006, 2 x 2 060 036-cell colour
-    planes, 2.0 M TinyMT streams), one launch per colour step (the
-    reference's launch shape, ENGINE_STEP).  Algorithmic bytes per flip
-    attempt: TinyMT state 16 B read + 16 B written + parameters 12 B, the
-    rotated cell 1 B read + 1 B written and its four neighbours 1 B
-    each read once per step = 48 B."""
+    the Aztec diamond of order 3000 (N = 6006, 18.0 M TinyMT streams: 505 MB
+    of state and parameters plus 36 MB of colour planes, twice the 256 MB
+    last-level cache), one launch per colour step (the reference's launch
+    shape, ENGINE_STEP).  Algorithmic bytes per flip attempt: TinyMT state
+    16 B read + 16 B written + parameters 12 B, the rotated cell 1 B read +
+    1 B written and its four neighbours 1 B each read once per step = 48 B."""
     d = rt.DominoTiler.AztecDiamond(order)
     t = rt.DominoTiler.MinTiling(d)
     N = rt.side(t)
@@ -103,6 +103,58 @@ def plain_sweep(rt, order=1000, steps=2000):
             "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
                          "alg_bytes_per_attempt": 48}}
+
+
+def batched_walks(rt, seed=4, nchains=256, steps=10 ** 4):
+    """Batched configuration: 256 independent heat-bath domino chains on
+    uneven domains drawn from `seed` (Aztec diamonds of order U[32, 512]
+    and tileable unions of 1-8 rectangles with sides U[16, 256],
+    randomtilingspublic_amd.workloads), each started from its minimal
+    tiling and walked 10^4 steps, all chains advanced by the same sweep-block
+    launches, the launch sequence captured in one hipGraph and replayed; the
+    tile shape comes from the autotuner (tuned on a 400-step warm-up batch).
+    Algorithmic bytes per site per launch of k_fused<0, 1, *, *>: TinyMT
+    state 16 B read + 16 B written + parameters 12 B + the two colour planes
+    2 B read + 2 B written = 48 B."""
+    from randomtilingspublic_amd import workloads as W
+    t0 = time.time()
+    batch = W.uneven_domino_batch(seed, nchains)
+    gen_s = time.time() - t0
+    tilings = [b["tiling"] for b in batch]
+    big = max(b["N"] for b in batch)
+    sites = sum((b["N"] - 2) * (b["N"] // 2 - 2) for b in batch)
+    ctx = rt.Context(0)
+    try:
+        ctx.set_tinymt_params(rt.synthetic_params(big * big // 2, None))
+        seeds = [1 + k for k in range(nchains)]
+        ctx.domino_walk_batch(tilings, 400, seeds, rt.DOMINO_HEATBATH, use_graph=False)
+        warm = ctx.last_timing()
+        w = time.time()
+        ctx.domino_walk_batch(tilings, steps, seeds, rt.DOMINO_HEATBATH, use_graph=True)
+        wall = time.time() - w
+        tm = ctx.last_timing()
+    finally:
+        ctx.close()
+    dev_s = tm["device_ms"] / 1e3
+    att_s = tm["attempts"] / dev_s
+    launches = tm["launches"]
+    alg = 48.0 * sites * launches
+    gbs = alg / dev_s / 1e9
+    foot = sum(b["N"] * b["N"] // 2 * (16 + 2) for b in batch)
+    return {"workload": f"{nchains} heat-bath domino chains on uneven domains from seed {seed} "
+                        f"(Aztec order U[32,512] and rectangle unions), {steps} steps each, "
+                        "one hipGraph, autotuned tile shape",
+            "chains": nchains, "aztec": sum(b["kind"] == "aztec" for b in batch),
+            "N_min": min(b["N"] for b in batch), "N_max": big, "sites_per_step": sites,
+            "state_footprint_MB": round(foot / 1e6, 1), "generate_s": round(gen_s, 2),
+            "wall_s": round(wall, 3), "device_s": round(dev_s, 3),
+            "flip_attempts_per_s": round(att_s, 1),
+            "tile_shape": tm["shape"], "tuned_on_warmup": warm["shape_source"] == 1,
+            "launches": launches,
+            "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                         "alg_bytes_per_site_per_launch": 48,
+                         "note": "device time includes the final colour-plane sync"}}
 
 
 def served_stream(rt, seconds=20.0, seed=2024):
@@ -290,6 +342,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-batch-walks", action="store_true")
     ap.add_argument("--profile-rounds", type=int, default=0,
                     help="profiling run: the same batch capped at R doubling rounds (no sample "
                          "needs to finish); prints the sweep kernel's timing only")
@@ -394,6 +447,11 @@ def main():
             "flip_attempts_per_s_in_kernels": round(2 * draws / (tm["kernel_ms"] / 1e3), 1)
             if tm["kernel_ms"] else None}
     sweep = None if args.no_sweep else plain_sweep(rt)
+    if sweep:
+        print(f"[bench] plain sweep {sweep}", file=sys.stderr, flush=True)
+    walks = None if args.no_batch_walks else batched_walks(rt)
+    if walks:
+        print(f"[bench] batched walks {walks}", file=sys.stderr, flush=True)
     served = None
     if not args.no_serve:
         served = served_stream(rt)
@@ -431,6 +489,7 @@ def main():
         "batched_vs_plain_fused":
             round(samples_s / plain_fused["samples_per_s"], 2) if plain_fused else None,
         "plain_sweep": sweep,
+        "batched_walks": walks,
         "served": served,
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -74,6 +74,11 @@ typedef struct {
   double shape_ms[4];
   long shape_launches[4];
   long shape_sites[4];
+  /* Tile shape of the bulk of the last sweep-block job and how it was
+   * picked: 0 forced or rule of thumb, 1 tuned in this call, 2 tuned by an
+   * earlier call on a job like it. */
+  int shape;
+  int shape_source;
 } rt_timing;
 int rt_last_timing(const rt_ctx* ctx, rt_timing* out);
 
@@ -92,6 +97,16 @@ int rt_set_engine(rt_ctx* ctx, int engine);
  * per job from its tile count against the CU count.  The serving loop always
  * uses 64-row tiles. */
 int rt_set_tile_shape(rt_ctx* ctx, int shape);
+
+/* Tile-shape autotuner (no reference counterpart; on by default).  With
+ * shape 0 above, a sweep-block job of at least 24 launch windows whose
+ * kind (model, chains, tile count to a factor of two) has not been tuned
+ * on this context runs its first windows split among the candidate
+ * shapes, timed with events, and keeps the fastest for the rest of the
+ * job and for later jobs of that kind.  Results never depend on the shape.
+ * mode 0 = off (rule of thumb on the tile count), 1 = on, 2 = on and forget
+ * earlier choices. */
+int rt_set_autotune(rt_ctx* ctx, int mode);
 
 /* ---------------------------------------------------------------- domino */
 /* Flip rules: the four reference kernel files. */
@@ -247,6 +262,11 @@ void rt_vec_free(rt_vec* v);
 rt_vec* rt_domino_rectangle(int M, int N);               /* DominoTiler.cpp:249-260 */
 rt_vec* rt_domino_aztec_diamond(int n);                  /* DominoTiler.cpp:262-265 */
 rt_vec* rt_domino_aztec_rectangle(int n, const int* remove, int k); /* 267-324 */
+/* Union of n rectangles {i0, j0, h, w} of unit squares, framed like
+ * Rectangle (DominoTiler.cpp:249-260), holes filled.  No reference
+ * counterpart: a domain family for batches of uneven chains; tileability is
+ * what rt_domino_min_tiling reports. */
+rt_vec* rt_domino_rect_union(const int* rects, int n);
 int rt_domino_max_tiling(const int32_t* d, size_t dsize, rt_vec** out);   /* 345-438 */
 int rt_domino_min_tiling(const int32_t* d, size_t dsize, rt_vec** out);   /* 440-531 */
 rt_vec* rt_domino_heightfunc_to_tiling(const int32_t* hf, size_t n,

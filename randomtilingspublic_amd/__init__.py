@@ -38,7 +38,8 @@ class _Timing(C.Structure):
                 ("steps", C.c_long), ("attempts", C.c_long),
                 ("kernel_ms", C.c_double), ("kernel_launches", C.c_long),
                 ("kernel_sites", C.c_long), ("shape_ms", C.c_double * 4),
-                ("shape_launches", C.c_long * 4), ("shape_sites", C.c_long * 4)]
+                ("shape_launches", C.c_long * 4), ("shape_sites", C.c_long * 4),
+                ("shape", C.c_int), ("shape_source", C.c_int)]
 
 
 class DominoRule(C.Structure):
@@ -103,6 +104,7 @@ def lib():
     L.rt_recttri_walk.argtypes = [vp, _I32P, C.c_int, C.c_long, C.c_long, C.c_void_p]
     L.rt_bibone_walk.argtypes = [vp, _I32P, C.c_int, C.c_long, C.c_long]
     L.rt_set_tile_shape.argtypes = [vp, C.c_int]
+    L.rt_set_autotune.argtypes = [vp, C.c_int]
     L.rt_lozenge_cftp_batch.argtypes = [vp, _I32P, _I32P, C.c_int, _U32P, C.c_int, C.c_int,
                                         C.c_long, C.c_int, _I32P, C.c_void_p]
     L.rt_domino_cftp_batch.argtypes = [vp, _I32P, _I32P, C.c_int, _U32P, C.c_int, C.c_int,
@@ -129,6 +131,7 @@ def lib():
         ("rt_domino_rectangle", [C.c_int, C.c_int]),
         ("rt_domino_aztec_diamond", [C.c_int]),
         ("rt_domino_aztec_rectangle", [C.c_int, C.c_void_p, C.c_int]),
+        ("rt_domino_rect_union", [C.c_void_p, C.c_int]),
         ("rt_domino_heightfunc_to_tiling", [_I32P, C.c_size_t, _I32P, C.c_size_t]),
         ("rt_domino_tiling_to_heightfunc", [_I32P, C.c_size_t, _I32P, C.c_size_t]),
         ("rt_domino_tiling_to_domain", [_I32P, C.c_size_t]),
@@ -241,6 +244,11 @@ class Context:
         16 waves, 3 = 32 rows / 16 waves."""
         self._chk(self._L.rt_set_tile_shape(self._h, shape))
 
+    def set_autotune(self, mode):
+        """Tile-shape autotuner: 0 off, 1 on (default), 2 on and forget
+        earlier choices."""
+        self._chk(self._L.rt_set_autotune(self._h, mode))
+
     def set_tinymt_params(self, params3):
         p = np.ascontiguousarray(np.asarray(params3, dtype=np.uint32).reshape(-1, 3).ravel())
         self._chk(self._L.rt_set_tinymt_params(self._h, p, len(p) // 3))
@@ -255,7 +263,8 @@ class Context:
                 "attempts": t.attempts, "kernel_ms": t.kernel_ms,
                 "kernel_launches": t.kernel_launches, "kernel_sites": t.kernel_sites,
                 "shape_ms": list(t.shape_ms), "shape_launches": list(t.shape_launches),
-                "shape_sites": list(t.shape_sites)}
+                "shape_sites": list(t.shape_sites), "shape": t.shape,
+                "shape_source": t.shape_source}
 
     # ---- domino ----
     def domino_walk(self, t, steps, seed, rule=DOMINO_TOGGLE08, a=1.0, b=1.0):
@@ -460,6 +469,13 @@ class DominoTiler:
         r = np.ascontiguousarray(remove, dtype=np.int32)
         ptr = r.ctypes.data_as(C.c_void_p) if len(r) else None
         return _take_vec(lib().rt_domino_aztec_rectangle(n, ptr, len(r)))
+
+    @staticmethod
+    def RectangleUnion(rects):
+        """Union of rectangles (i0, j0, h, w), framed like Rectangle, holes
+        filled (no reference counterpart)."""
+        r = np.ascontiguousarray(rects, dtype=np.int32).reshape(-1, 4)
+        return _take_vec(lib().rt_domino_rect_union(r.ctypes.data_as(C.c_void_p), len(r)))
 
     @staticmethod
     def MaxTiling(d):

@@ -248,6 +248,7 @@ static int cftp_batch(rt_ctx* ctx, int model, const int32_t* tmax, const int32_t
     }
     FusedPlan plan;
     rc = fused_upload(ctx, job, ctx->aux, &plan);
+    if (!rc) rc = fused_tune(ctx, job, &plan);
     if (rc) return rc;
     HIPCHK_RET(ctx, hipEventRecord(ctx->evk0, ctx->stream));
     rc = fused_launch(ctx, job, plan);
@@ -280,14 +281,18 @@ static int cftp_batch(rt_ctx* ctx, int model, const int32_t* tmax, const int32_t
     ms_total += ms;
     float kms = 0;
     if (!plan.windows.empty()) HIPCHK_RET(ctx, hipEventElapsedTime(&kms, ctx->evk0, ctx->evk1));
+    const long nlaunch = (long)(plan.windows.size() - std::min(plan.first, plan.windows.size()));
     acc.kernel_ms += kms;
-    acc.kernel_launches += (long)plan.windows.size();
-    acc.kernel_sites += (long)plan.windows.size() * plan.streams;
+    acc.kernel_launches += nlaunch;
+    acc.kernel_sites += nlaunch * plan.streams;
     if (job.shape >= 1 && job.shape <= 4) {
       acc.shape_ms[job.shape - 1] += kms;
-      acc.shape_launches[job.shape - 1] += (long)plan.windows.size();
-      acc.shape_sites[job.shape - 1] += (long)plan.windows.size() * plan.streams;
+      acc.shape_launches[job.shape - 1] += nlaunch;
+      acc.shape_sites[job.shape - 1] += nlaunch * plan.streams;
     }
+    fused_add_trials(plan, &acc);
+    acc.shape = job.shape;
+    if (job.shape_source == 1 || acc.shape_source != 1) acc.shape_source = job.shape_source;
     acc.launches += ctx->last.launches;
     acc.steps += ctx->last.steps;
     acc.attempts += ctx->last.attempts;
@@ -441,9 +446,10 @@ extern "C" int rt_domino_walk_batch(rt_ctx* ctx, int32_t* const* tilings, const 
   rc = ctx->reserve_aux(fused_scratch_bytes(ctx, job));
   if (rc) return rc;
   FusedPlan plan;
-  rc = fused_upload(ctx, job, ctx->aux, &plan);
-  if (rc) return rc;
   ctx->last = rt_timing{};
+  rc = fused_upload(ctx, job, ctx->aux, &plan);
+  if (!rc) rc = fused_tune(ctx, job, &plan);  // eager: its timing cannot be captured
+  if (rc) return rc;
   if (use_graph) {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -474,13 +480,21 @@ extern "C" int rt_domino_walk_batch(rt_ctx* ctx, int32_t* const* tilings, const 
     HIPCHK_RET(ctx, hipEventSynchronize(ctx->ev1));
     float kms = 0;
     HIPCHK_RET(ctx, hipEventElapsedTime(&kms, ctx->ev0, ctx->evk1));
+    const long nlaunch = (long)(plan.windows.size() - std::min(plan.first, plan.windows.size()));
     ctx->last.kernel_ms = kms;
-    ctx->last.kernel_launches = (long)plan.windows.size();
-    ctx->last.kernel_sites = (long)plan.windows.size() * plan.streams;
+    ctx->last.kernel_launches = nlaunch;
+    ctx->last.kernel_sites = nlaunch * plan.streams;
+    if (job.shape >= 1 && job.shape <= 4) {
+      ctx->last.shape_ms[job.shape - 1] = kms;
+      ctx->last.shape_launches[job.shape - 1] = nlaunch;
+      ctx->last.shape_sites[job.shape - 1] = nlaunch * plan.streams;
+    }
   }
+  fused_add_trials(plan, &ctx->last);
   float ms = 0;
   HIPCHK_RET(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
   ctx->last.device_ms = ms;
+  for (double t : plan.trial_ms) ctx->last.device_ms += t;  // tuning windows ran before ev0
   for (int k = 0; k < nchains; k++) {
     const Geometry g = geometry_of(M_DOMINO, Ns[k]);
     std::vector<uint8_t> h(2 * (size_t)g.plane);

@@ -246,6 +246,11 @@ extern "C" int rt_create(rt_ctx** out, int device_ordinal) {
   if (hipSetDevice(device_ordinal) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
       hipEventCreate(&c->evk0) != hipSuccess || hipEventCreate(&c->evk1) != hipSuccess ||
+      [&] {
+        for (hipEvent_t& e : c->evt)
+          if (hipEventCreate(&e) != hipSuccess) return true;
+        return false;
+      }() ||
       hipMalloc(&c->d_flag, 64 * sizeof(uint32_t)) != hipSuccess ||
       hipHostMalloc(&c->h_flag, 64 * sizeof(uint32_t)) != hipSuccess) {
     delete c;
@@ -271,6 +276,8 @@ rt_ctx::~rt_ctx() {
   if (ev1) hipEventDestroy(ev1);
   if (evk0) hipEventDestroy(evk0);
   if (evk1) hipEventDestroy(evk1);
+  for (hipEvent_t e : evt)
+    if (e) hipEventDestroy(e);
   if (stream) hipStreamDestroy(stream);
 }
 
@@ -299,6 +306,16 @@ extern "C" int rt_set_engine(rt_ctx* ctx, int engine) {
 extern "C" int rt_set_tile_shape(rt_ctx* ctx, int shape) {
   if (!ctx || shape < 0 || shape > 3) return RT_EINVAL;
   ctx->tile_shape = shape;
+  return RT_OK;
+}
+
+extern "C" int rt_set_autotune(rt_ctx* ctx, int mode) {
+  if (!ctx || mode < 0 || mode > 2) return RT_EINVAL;
+  if (mode == 2) {
+    ctx->tuned.clear();
+    mode = 1;
+  }
+  ctx->autotune = mode;
   return RT_OK;
 }
 

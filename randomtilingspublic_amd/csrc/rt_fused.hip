@@ -116,6 +116,29 @@ __device__ __forceinline__ void sweep_step(Cell* lds, const Lanes& L, Tiny* st, 
       lds[ch * NCOL * PL + L.li0 + c * PL + r * W * RS] = (Cell)out[ch][r];
 }
 
+// The K colour steps of a window on the LDS tile, NCR chains.
+template <int MODEL, int NCR, int W, int R, class Cell, int RS, int PL>
+__device__ __forceinline__ void run_window(Cell* lds, const Lanes& L, Tiny* st,
+                                           const uint8_t* sops, int K, const UnitDesc& U,
+                                           const ModelParams& mp) {
+  for (int s = 0; s < K; s++) {
+    const uint32_t op = sops[s];
+    const int pull = (op >> 2) & 3;
+    if (MODEL == M_LOZENGE) {
+      switch (pull) {
+        case 0: sweep_step<MODEL, NCR, W, R, 0, Cell, RS, PL>(lds, L, st, op, U, mp); break;
+        case 1: sweep_step<MODEL, NCR, W, R, 1, Cell, RS, PL>(lds, L, st, op, U, mp); break;
+        case 2: sweep_step<MODEL, NCR, W, R, 2, Cell, RS, PL>(lds, L, st, op, U, mp); break;
+        default: sweep_step<MODEL, NCR, W, R, 3, Cell, RS, PL>(lds, L, st, op, U, mp); break;
+      }
+    } else {
+      if (pull & 1) sweep_step<MODEL, NCR, W, R, 1, Cell, RS, PL>(lds, L, st, op, U, mp);
+      else sweep_step<MODEL, NCR, W, R, 0, Cell, RS, PL>(lds, L, st, op, U, mp);
+    }
+    __syncthreads();
+  }
+}
+
 // W wave64s per workgroup; each lane owns one tile column and R rows
 // (rows wv, wv+W, ...), so the loaded tile is (W*R) x 64 cells.
 template <int MODEL, int NC, int W, int R>
@@ -199,22 +222,7 @@ __global__ __launch_bounds__(64 * W, W >= 16 ? 1 : 2) void k_fused(const UnitDes
   }
   __syncthreads();
 
-  for (int s = 0; s < K; s++) {
-    const uint32_t op = sops[s];
-    const int pull = (op >> 2) & 3;
-    if (MODEL == M_LOZENGE) {
-      switch (pull) {
-        case 0: sweep_step<MODEL, NC, W, R, 0, Cell, RS, PL>(lds, L, st, op, U, mp); break;
-        case 1: sweep_step<MODEL, NC, W, R, 1, Cell, RS, PL>(lds, L, st, op, U, mp); break;
-        case 2: sweep_step<MODEL, NC, W, R, 2, Cell, RS, PL>(lds, L, st, op, U, mp); break;
-        default: sweep_step<MODEL, NC, W, R, 3, Cell, RS, PL>(lds, L, st, op, U, mp); break;
-      }
-    } else {
-      if (pull & 1) sweep_step<MODEL, NC, W, R, 1, Cell, RS, PL>(lds, L, st, op, U, mp);
-      else sweep_step<MODEL, NC, W, R, 0, Cell, RS, PL>(lds, L, st, op, U, mp);
-    }
-    __syncthreads();
-  }
+  run_window<MODEL, NC, W, R, Cell, RS, PL>(lds, L, st, sops, K, U, mp);
 
 #pragma unroll
   for (int r = 0; r < R; r++) {
@@ -291,43 +299,127 @@ static void tile_counts(const UnitDesc& u, int LH, int H, int* tr, int* tc) {
   *tc = (u.cols + TW - 1) / TW;
 }
 
-// Shape of a walk / batch job.  Few tiles (a sample or chain running alone)
-// means idle CUs and a launch bound by each wave's chain of dependent LDS
-// round trips: 16 waves halve that chain (-30 % per launch for one side-300
-// hexagon CFTP sample), and 32-row tiles halve it again while putting 2.6x
-// more workgroups on the chip, at a higher halo share.  With several tiles
-// per CU the 8-wave shape's ILP per wave wins (-12 % per launch at 32
-// samples).
-static int choose_shape(const rt_ctx* ctx, const FusedJob& job) {
-  if (job.model == M_RECTTRI) return job.nc == 2 ? SH_RT8x4 : SH_8x8;
-  if (ctx->tile_shape != SH_AUTO) return ctx->tile_shape;
+static long tiles64(const FusedJob& job) {
   long t64 = 0;
   for (const UnitDesc& u : job.units) {
     int tr, tc;
     tile_counts(u, 64, 6, &tr, &tc);
     t64 += (long)tr * tc;
   }
+  return t64;
+}
+
+// Rule of thumb for a walk / batch job's shape.  Few tiles (a sample or
+// chain running alone) means idle CUs and a launch bound by each wave's
+// chain of dependent LDS round trips: 16 waves halve that chain (-30 % per
+// launch for one side-300 hexagon CFTP sample), and 32-row tiles halve it
+// again while putting 2.6x more workgroups on the chip, at a higher halo
+// share.  With several tiles per CU the 8-wave shape's ILP per wave wins
+// (-12 % per launch at 32 samples).
+static int rule_of_thumb(const rt_ctx* ctx, long t64) {
   if (3 * t64 <= ctx->num_cu) return SH_16x2;
   if (t64 <= ctx->num_cu) return SH_16x4;
   return SH_8x8;
 }
 
-static Geo job_geo(const FusedJob& job) { return shape_geo(job.shape); }
+// Jobs of one kind share a tuned shape: model, chains per unit and the
+// 64-row tile count to a factor of two.
+static uint32_t tune_key(const FusedJob& job, long t64) {
+  int lg = 0;
+  while (lg < 40 && (2L << lg) <= t64) lg++;
+  return (uint32_t)job.model | ((uint32_t)job.nc << 4) | ((uint32_t)lg << 8);
+}
 
-size_t fused_scratch_bytes(const rt_ctx* ctx, FusedJob& job) {
-  job.shape = choose_shape(ctx, job);
-  const Geo g = job_geo(job);
+constexpr int kTuneMinWindows = 24;  // shorter jobs keep the rule of thumb
+constexpr int kTrialWindows = 2;     // windows per timed trial
+constexpr int kTrialReps = 2;        // trials per candidate shape
+static const int kTuneShapes[3] = {SH_8x8, SH_16x4, SH_16x2};
+
+static bool tunable(const rt_ctx* ctx, const FusedJob& job) {
+  return ctx->autotune && ctx->tile_shape == SH_AUTO && job.model != M_RECTTRI &&
+         !(job.model == M_LOZENGE && ctx->engine == RT_ENGINE_FUSED_SWAR);
+}
+
+static void choose_shape(const rt_ctx* ctx, FusedJob& job) {
+  job.tune = false;
+  job.shape_source = 0;
+  if (job.model == M_RECTTRI) {
+    job.shape = job.nc == 2 ? SH_RT8x4 : SH_8x8;
+    return;
+  }
+  if (ctx->tile_shape != SH_AUTO) {
+    job.shape = ctx->tile_shape;
+    return;
+  }
+  const long t64 = tiles64(job);
+  job.shape = rule_of_thumb(ctx, t64);
+  if (!tunable(ctx, job)) return;
+  auto it = ctx->tuned.find(tune_key(job, t64));
+  if (it != ctx->tuned.end()) {
+    job.shape = it->second;
+    job.shape_source = 2;
+    return;
+  }
+  job.tune = job.windows.size() >= (size_t)kTuneMinWindows;
+}
+
+// Launch windows: at most kKMax steps and at most H pull steps (over all
+// units) each, a new window at every segment start (TinyMT re-seeding).
+static void cut_windows(FusedJob& job, int H) {
+  long T = 0;
+  for (long s : job.seg_steps) T += s;
+  job.windows.clear();
+  if (job.units.empty() || T == 0) return;
+  std::vector<uint8_t> anypull(T, 0);
+  for (const std::vector<uint8_t>& o : job.ops) {
+    const long n = std::min<long>(T, (long)o.size());
+    for (long s = 0; s < n; s++) anypull[s] |= (o[s] & (OP_PULL_A | OP_PULL_B)) ? 1 : 0;
+  }
+  long s0 = 0;
+  for (size_t k = 0; k < job.seg_steps.size(); k++) {
+    const long end = s0 + job.seg_steps[k];
+    bool first = true;
+    while (s0 < end) {
+      int K = 0, pulls = 0;
+      while (s0 + K < end && K < kKMax) {
+        const int p1 = anypull[s0 + K];
+        if (pulls + p1 > H) break;
+        pulls += p1;
+        K++;
+      }
+      job.windows.push_back(FusedWindow{s0, K, first ? 1 : 0, (int)k});
+      first = false;
+      s0 += K;
+    }
+  }
+}
+
+static size_t tiles_bytes(const FusedJob& job, int shape) {
+  const Geo g = shape_geo(shape);
   size_t work = 0;
   for (const UnitDesc& u : job.units) {
     int tr, tc;
     tile_counts(u, g.W * g.R, g.H, &tr, &tc);
     work += (size_t)tr * tc;
   }
+  return ((job.units.size() * sizeof(UnitDesc) + 255) & ~(size_t)255) +
+         ((work * sizeof(WorkItem) + 255) & ~(size_t)255);
+}
+
+size_t fused_scratch_bytes(rt_ctx* ctx, FusedJob& job) {
+  // Every shape of a model has the same halo, so one window list serves all.
+  cut_windows(job, shape_geo(job.model == M_RECTTRI && job.nc == 2 ? SH_RT8x4 : SH_8x8).H);
+  choose_shape(ctx, job);
   long T = 0;
   for (long s : job.seg_steps) T += s;
   const size_t nu = job.units.size();
-  return nu * sizeof(UnitDesc) + work * sizeof(WorkItem) + nu * (size_t)T +
-         nu * job.seg_steps.size() * sizeof(uint32_t) + 1024;
+  size_t bytes = ((nu * (size_t)T + 255) & ~(size_t)255) +
+                 ((nu * job.seg_steps.size() * sizeof(uint32_t) + 255) & ~(size_t)255) + 1024;
+  if (job.tune)
+    for (int sh : kTuneShapes) bytes += tiles_bytes(job, sh);
+  else
+    bytes += tiles_bytes(job, job.shape);
+  return bytes;
 }
 
 template <int MODEL, int NC, int W, int R>
@@ -373,10 +465,38 @@ static LaunchFn pick(const rt_ctx* ctx, int model, int nc, int shape) {
   return nc == 1 ? launch<M_RECTTRI, 1, 8, 8> : launch<M_RECTTRI, 2, 8, 4>;
 }
 
+// Unit table and work list of one tile shape, at p (advanced past them).
+static int upload_tiles(rt_ctx* ctx, const FusedJob& job, int shape, uint8_t*& p,
+                        FusedPlan::Tiles* out) {
+  const Geo geo = shape_geo(shape);
+  std::vector<UnitDesc> units = job.units;
+  std::vector<WorkItem> work;
+  for (size_t u = 0; u < units.size(); u++) {
+    int tr, tc;
+    tile_counts(units[u], geo.W * geo.R, geo.H, &tr, &tc);
+    units[u].tiles_c = tc;
+    for (int t = 0; t < tr * tc; t++) work.push_back(WorkItem{(int)u, t});
+  }
+  UnitDesc* d_units = (UnitDesc*)p;
+  p += (units.size() * sizeof(UnitDesc) + 255) & ~(size_t)255;
+  WorkItem* d_work = (WorkItem*)p;
+  p += (work.size() * sizeof(WorkItem) + 255) & ~(size_t)255;
+  HIPCHK_RET(ctx, hipMemcpyAsync(d_units, units.data(), units.size() * sizeof(UnitDesc),
+                                 hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK_RET(ctx, hipMemcpyAsync(d_work, work.data(), work.size() * sizeof(WorkItem),
+                                 hipMemcpyHostToDevice, ctx->stream));
+  // Pageable host sources: the copies may still read them after the calls
+  // return, so wait before the vectors go out of scope.
+  HIPCHK_RET(ctx, hipStreamSynchronize(ctx->stream));
+  *out = FusedPlan::Tiles{d_units, d_work, (unsigned)work.size(), shape};
+  return RT_OK;
+}
+
 int fused_upload(rt_ctx* ctx, FusedJob& job, void* scratch, FusedPlan* plan) {
-  if (job.shape == SH_AUTO) job.shape = choose_shape(ctx, job);
-  const Geo geo = job_geo(job);
-  const int LH = geo.W * geo.R, H = geo.H;
+  if (job.shape == SH_AUTO) {
+    cut_windows(job, shape_geo(job.model == M_RECTTRI && job.nc == 2 ? SH_RT8x4 : SH_8x8).H);
+    choose_shape(ctx, job);
+  }
   const size_t nu = job.units.size();
   *plan = FusedPlan();
   long T = 0;
@@ -384,83 +504,142 @@ int fused_upload(rt_ctx* ctx, FusedJob& job, void* scratch, FusedPlan* plan) {
   plan->total_steps = T;
   for (const UnitDesc& u : job.units) plan->streams += u.nstream;
   if (nu == 0 || T == 0) return RT_OK;
-  // Device layout of the scratch block: units | work | ops | seeds.
+  // Device layout of the scratch block: ops | seeds | (units | work) per shape.
   uint8_t* p = (uint8_t*)scratch;
-  UnitDesc* d_units = (UnitDesc*)p;
-  p += nu * sizeof(UnitDesc);
-  std::vector<WorkItem> work;
-  for (size_t u = 0; u < nu; u++) {
-    int tr, tc;
-    tile_counts(job.units[u], LH, H, &tr, &tc);
-    job.units[u].tiles_c = tc;
-    job.units[u].rule = job.mp.rule;
-    job.units[u].dw = job.mp.dw;
-    for (int t = 0; t < tr * tc; t++) work.push_back(WorkItem{(int)u, t});
-  }
-  WorkItem* d_work = (WorkItem*)p;
-  p += work.size() * sizeof(WorkItem);
   uint8_t* d_ops = p;
-  p += nu * (size_t)T;
-  p = (uint8_t*)(((uintptr_t)p + 15) & ~(uintptr_t)15);
+  p += (nu * (size_t)T + 255) & ~(size_t)255;
   uint32_t* d_seeds = (uint32_t*)p;
   const size_t nseg = job.seg_steps.size();
+  p += (nu * nseg * sizeof(uint32_t) + 255) & ~(size_t)255;
   std::vector<uint32_t> seeds(nu * nseg);
-  std::vector<uint8_t> anypull(T, 0);
   for (size_t u = 0; u < nu; u++) {
     for (size_t k = 0; k < nseg; k++) seeds[u * nseg + k] = job.seg_seeds[u][k];
     job.units[u].ops = d_ops + u * (size_t)T;
     job.units[u].seeds = d_seeds + u * nseg;
+    job.units[u].rule = job.mp.rule;
+    job.units[u].dw = job.mp.dw;
     const std::vector<uint8_t>& o = job.ops[u];
     if ((long)o.size() != T) {
       ctx->err = "fused job: op stream length differs from the segment total";
       return RT_EINVAL;
     }
-    for (long s = 0; s < T; s++) anypull[s] |= (o[s] & (OP_PULL_A | OP_PULL_B)) ? 1 : 0;
     HIPCHK_RET(ctx, hipMemcpyAsync(d_ops + u * (size_t)T, o.data(), T, hipMemcpyHostToDevice,
                                    ctx->stream));
   }
   HIPCHK_RET(ctx, hipMemcpyAsync(d_seeds, seeds.data(), seeds.size() * 4, hipMemcpyHostToDevice,
                                  ctx->stream));
-  HIPCHK_RET(ctx, hipMemcpyAsync(d_units, job.units.data(), nu * sizeof(UnitDesc),
-                                 hipMemcpyHostToDevice, ctx->stream));
-  HIPCHK_RET(ctx, hipMemcpyAsync(d_work, work.data(), work.size() * sizeof(WorkItem),
-                                 hipMemcpyHostToDevice, ctx->stream));
-  // Pageable host sources: the copies above may still read them after the
-  // calls return, so wait before the vectors go out of scope.
-  HIPCHK_RET(ctx, hipStreamSynchronize(ctx->stream));
-  plan->d_units = d_units;
-  plan->d_work = d_work;
-  plan->nwork = (unsigned)work.size();
-  long s0 = 0;
-  for (size_t k = 0; k < nseg; k++) {
-    const long end = s0 + job.seg_steps[k];
-    bool first = true;
-    while (s0 < end) {
-      int K = 0, pulls = 0;
-      while (s0 + K < end && K < kKMax) {
-        const int p1 = anypull[s0 + K];
-        if (pulls + p1 > H) break;
-        pulls += p1;
-        K++;
-      }
-      plan->windows.push_back(FusedPlan::Window{s0, K, first ? 1 : 0, (int)k});
-      first = false;
-      s0 += K;
+  FusedPlan::Tiles t;
+  int rc = upload_tiles(ctx, job, job.shape, p, &t);
+  if (rc) return rc;
+  plan->cand.push_back(t);
+  if (job.tune)
+    for (int sh : kTuneShapes) {
+      if (sh == job.shape) continue;
+      rc = upload_tiles(ctx, job, sh, p, &t);
+      if (rc) return rc;
+      plan->cand.push_back(t);
     }
-  }
+  plan->d_units = plan->cand[0].d_units;
+  plan->d_work = plan->cand[0].d_work;
+  plan->nwork = plan->cand[0].nwork;
+  plan->windows = job.windows;
   return RT_OK;
 }
 
+static void count_windows(rt_ctx* ctx, const FusedPlan& plan, size_t w0, size_t w1) {
+  long steps = 0;
+  for (size_t w = w0; w < w1; w++) steps += plan.windows[w].K;
+  ctx->last.launches += (long)(w1 - w0);
+  ctx->last.steps += steps;
+  ctx->last.attempts += steps * plan.streams;
+}
+
+int fused_tune(rt_ctx* ctx, FusedJob& job, FusedPlan* plan) {
+  const size_t nc = plan->cand.size();
+  const size_t need = 1 + (size_t)kTrialReps * nc * kTrialWindows;
+  if (!job.tune || nc < 2 || plan->windows.size() < need) return RT_OK;
+  const int H = shape_geo(job.shape).H;
+  // Window 0 (TinyMT seeding) under the rule-of-thumb shape, then
+  // kTrialReps rounds of kTrialWindows windows under each candidate; event
+  // k + 1 closes launch group k.
+  struct Trial {
+    int cand;
+    size_t w0, w1;
+  };
+  std::vector<Trial> trials{{0, 0, 1}};
+  size_t w = 1;
+  for (int r = 0; r < kTrialReps; r++)
+    for (size_t c = 0; c < nc; c++, w += kTrialWindows)
+      trials.push_back(Trial{(int)c, w, w + kTrialWindows});
+  if (trials.size() + 1 > sizeof(ctx->evt) / sizeof(ctx->evt[0])) return RT_OK;
+  HIPCHK_RET(ctx, hipEventRecord(ctx->evt[0], ctx->stream));
+  for (size_t k = 0; k < trials.size(); k++) {
+    const FusedPlan::Tiles& t = plan->cand[trials[k].cand];
+    LaunchFn fn = pick(ctx, job.model, job.nc, t.shape);
+    for (size_t x = trials[k].w0; x < trials[k].w1; x++) {
+      const FusedWindow& win = plan->windows[x];
+      fn(ctx, t.nwork, t.d_units, t.d_work, win.s0, win.K, win.init, win.seg, H, job.mp, nullptr);
+    }
+    HIPCHK_RET(ctx, hipGetLastError());
+    HIPCHK_RET(ctx, hipEventRecord(ctx->evt[k + 1], ctx->stream));
+  }
+  HIPCHK_RET(ctx, hipEventSynchronize(ctx->evt[trials.size()]));
+  std::vector<double> ms_per_step(nc, 0.0), ms(nc, 0.0);
+  std::vector<long> steps(nc, 0);
+  for (size_t k = 0; k < trials.size(); k++) {
+    float e = 0;
+    HIPCHK_RET(ctx, hipEventElapsedTime(&e, ctx->evt[k], ctx->evt[k + 1]));
+    const int c = trials[k].cand, sh = plan->cand[c].shape;
+    long K = 0;
+    for (size_t x = trials[k].w0; x < trials[k].w1; x++) K += plan->windows[x].K;
+    plan->trial_ms[sh - 1] += e;
+    plan->trial_launches[sh - 1] += (long)(trials[k].w1 - trials[k].w0);
+    plan->trial_sites[sh - 1] += (long)(trials[k].w1 - trials[k].w0) * plan->streams;
+    if (k == 0) continue;  // the seeding window is not a fair trial
+    ms[c] += e;
+    steps[c] += K;
+  }
+  size_t best = 0;
+  for (size_t c = 0; c < nc; c++) {
+    ms_per_step[c] = steps[c] ? ms[c] / (double)steps[c] : 1e30;
+    if (ms_per_step[c] < ms_per_step[best]) best = c;
+  }
+  count_windows(ctx, *plan, 0, w);
+  plan->first = w;
+  job.shape = plan->cand[best].shape;
+  job.shape_source = 1;
+  std::swap(plan->cand[0], plan->cand[best]);
+  plan->d_units = plan->cand[0].d_units;
+  plan->d_work = plan->cand[0].d_work;
+  plan->nwork = plan->cand[0].nwork;
+  ctx->tuned[tune_key(job, tiles64(job))] = job.shape;
+  return RT_OK;
+}
+
+void fused_add_trials(const FusedPlan& plan, rt_timing* acc) {
+  for (int s = 0; s < 4; s++) {
+    acc->kernel_ms += plan.trial_ms[s];
+    acc->kernel_launches += plan.trial_launches[s];
+    acc->kernel_sites += plan.trial_sites[s];
+    acc->shape_ms[s] += plan.trial_ms[s];
+    acc->shape_launches[s] += plan.trial_launches[s];
+    acc->shape_sites[s] += plan.trial_sites[s];
+  }
+}
+
 int fused_launch(rt_ctx* ctx, const FusedJob& job, const FusedPlan& plan) {
-  if (plan.windows.empty()) return RT_OK;
+  ctx->last.shape = job.shape;
+  ctx->last.shape_source = job.shape_source;
+  if (plan.first >= plan.windows.size()) return RT_OK;
   LaunchFn fn = pick(ctx, job.model, job.nc, job.shape);
-  const int H = job_geo(job).H;
-  for (const FusedPlan::Window& w : plan.windows)
-    fn(ctx, plan.nwork, plan.d_units, plan.d_work, w.s0, w.K, w.init, w.seg, H, job.mp, nullptr);
+  const int H = shape_geo(job.shape).H;
+  for (size_t w = plan.first; w < plan.windows.size(); w++) {
+    const FusedWindow& win = plan.windows[w];
+    fn(ctx, plan.nwork, plan.d_units, plan.d_work, win.s0, win.K, win.init, win.seg, H, job.mp,
+       nullptr);
+  }
   HIPCHK_RET(ctx, hipGetLastError());
-  ctx->last.launches += (long)plan.windows.size();
-  ctx->last.steps += plan.total_steps;
-  ctx->last.attempts += plan.total_steps * plan.streams;
+  count_windows(ctx, plan, plan.first, plan.windows.size());
   return RT_OK;
 }
 
@@ -489,6 +668,7 @@ void fused_tiles(int model, int nc, int rows, int cols, int* tiles_r, int* tiles
 int fused_run(rt_ctx* ctx, FusedJob& job, void* scratch) {
   FusedPlan plan;  // job.shape was fixed when the scratch was sized
   int rc = fused_upload(ctx, job, scratch, &plan);
+  if (!rc) rc = fused_tune(ctx, job, &plan);
   if (rc) return rc;
   return fused_launch(ctx, job, plan);
 }

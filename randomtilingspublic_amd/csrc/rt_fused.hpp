@@ -79,6 +79,15 @@ struct OpBuilder {
   long append(const std::vector<uint8_t>& colours, std::vector<uint8_t>& ops);
 };
 
+// One launch's share of a job: K steps from global step s0; init re-seeds
+// the TinyMT streams with segment seg's seed before the first step.
+struct FusedWindow {
+  long s0;
+  int K;
+  int init;
+  int seg;
+};
+
 // Host job: units on one device advancing through a shared segment list.
 struct FusedJob {
   int model = M_DOMINO;
@@ -91,33 +100,52 @@ struct FusedJob {
   std::vector<OpBuilder> hist;        // [unit]
   bool adaptive = true;               // single unit: stretch windows to H pulls
   int shape = 0;                      // tile shape, resolved by fused_scratch_bytes
+  int shape_source = 0;               // 0 rule of thumb / forced, 1 tuned now, 2 tuned before
+  bool tune = false;                  // time the candidate shapes on the first windows
+  std::vector<FusedWindow> windows;   // launch windows, cut by fused_scratch_bytes
 };
 
 // Run all windows of the job on ctx->stream (no final sync).  `scratch`
 // must hold fused_scratch_bytes(ctx, job) bytes of device memory; that call
-// also fixes the job's tile shape (from the tile count and ctx->tile_shape).
-size_t fused_scratch_bytes(const rt_ctx* ctx, FusedJob& job);
+// also cuts the launch windows and fixes the job's tile shape (forced by
+// ctx->tile_shape, remembered from an earlier tuning of jobs like it, or a
+// rule of thumb on the tile count, with job.tune set when the first windows
+// should time the candidates).
+size_t fused_scratch_bytes(rt_ctx* ctx, FusedJob& job);
 int fused_run(rt_ctx* ctx, FusedJob& job, void* scratch);
 
-// The same in two phases, so the launch sequence can be captured in a
-// hipGraph: fused_upload copies unit tables, work list, ops and seeds to
-// the device; fused_launch enqueues only kernels.
+// The same in phases, so the launch sequence can be captured in a
+// hipGraph: fused_upload copies unit tables, work lists, ops and seeds to
+// the device; fused_tune (a no-op unless job.tune) runs the first windows
+// split among the candidate tile shapes, timed, keeps the fastest for the
+// rest of the job and for later jobs like it; fused_launch enqueues only
+// kernels, from window plan.first on.  Results never depend on the shape.
 struct FusedPlan {
-  const UnitDesc* d_units = nullptr;
+  struct Tiles {
+    const UnitDesc* d_units;
+    const WorkItem* d_work;
+    unsigned nwork;
+    int shape;
+  };
+  const UnitDesc* d_units = nullptr;  // the tiles of job.shape
   const WorkItem* d_work = nullptr;
   unsigned nwork = 0;
-  struct Window {
-    long s0;
-    int K;
-    int init;
-    int seg;
-  };
+  std::vector<Tiles> cand;            // every uploaded shape, job.shape first
+  typedef FusedWindow Window;
   std::vector<Window> windows;
+  size_t first = 0;                   // windows before this one ran in fused_tune
   long total_steps = 0;
   long streams = 0;
+  // windows run by fused_tune, by tile shape (index shape - 1)
+  double trial_ms[4] = {0, 0, 0, 0};
+  long trial_launches[4] = {0, 0, 0, 0};
+  long trial_sites[4] = {0, 0, 0, 0};
 };
 int fused_upload(rt_ctx* ctx, FusedJob& job, void* scratch, FusedPlan* plan);
+int fused_tune(rt_ctx* ctx, FusedJob& job, FusedPlan* plan);
 int fused_launch(rt_ctx* ctx, const FusedJob& job, const FusedPlan& plan);
+// Add the plan's tuning windows to per-shape kernel timing.
+void fused_add_trials(const FusedPlan& plan, rt_timing* acc);
 
 // One launch over `nwork` work items where each unit follows its own
 // window (wins[unit]); model / chains / params as in `job` (units unused).

@@ -230,6 +230,56 @@ domain domino_rectangle(int M, int N) {
   return d;
 }
 
+// Union of axis-aligned rectangles (i0, j0, h, w) of unit squares, shifted
+// so the union's bounding box starts at (0, 0) and framed like
+// domino_rectangle (two empty rows/columns on each side, odd side).  Holes
+// of the union (empty cells not connected to the frame) are filled, so the
+// region is simply connected whenever the union is connected.  No reference
+// counterpart: a domain family for batches of uneven chains.
+domain domino_rect_union(const std::vector<int>& rects) {
+  const size_t n = rects.size() / 4;
+  if (n == 0) return domain();
+  int i_lo = INT_MAX, j_lo = INT_MAX, i_hi = INT_MIN, j_hi = INT_MIN;
+  for (size_t r = 0; r < n; r++) {
+    const int* q = &rects[4 * r];
+    if (q[2] <= 0 || q[3] <= 0) continue;
+    i_lo = std::min(i_lo, q[0]);
+    j_lo = std::min(j_lo, q[1]);
+    i_hi = std::max(i_hi, q[0] + q[2]);
+    j_hi = std::max(j_hi, q[1] + q[3]);
+  }
+  if (i_lo == INT_MAX) return domain();
+  int dim = std::max(i_hi - i_lo, j_hi - j_lo) + 4;
+  if (dim % 2 == 0) dim++;
+  domain d((size_t)dim * dim, 0);
+  for (size_t r = 0; r < n; r++) {
+    const int* q = &rects[4 * r];
+    for (int i = 0; i < q[2]; i++)
+      for (int j = 0; j < q[3]; j++)
+        d[(size_t)(q[0] - i_lo + i + 2) * dim + (q[1] - j_lo + j + 2)] = 1;
+  }
+  // Flood the empty cells from the frame; whatever stays unreached is a hole.
+  std::vector<uint8_t> seen((size_t)dim * dim, 0);
+  std::vector<int> stack{0};
+  seen[0] = 1;
+  while (!stack.empty()) {
+    const int v = stack.back();
+    stack.pop_back();
+    const int i = v / dim, j = v % dim;
+    const int nb[4][2] = {{i + 1, j}, {i - 1, j}, {i, j + 1}, {i, j - 1}};
+    for (const auto& u : nb) {
+      if (u[0] < 0 || u[1] < 0 || u[0] >= dim || u[1] >= dim) continue;
+      const int w = u[0] * dim + u[1];
+      if (seen[w] || d[w]) continue;
+      seen[w] = 1;
+      stack.push_back(w);
+    }
+  }
+  for (size_t v = 0; v < d.size(); v++)
+    if (!d[v] && !seen[v]) d[v] = 1;
+  return d;
+}
+
 domain domino_aztec_rectangle(int n, const std::vector<int>& remove) {
   const int k = (int)remove.size();
   const int M = 2 * n - k;
@@ -735,6 +785,15 @@ static bool is_square(size_t n) {
 rt_vec* rt_domino_rectangle(int M, int N) {
   if (M < 0 || N < 0) return nullptr;
   return make_vec(domino_rectangle(M, N));
+}
+
+rt_vec* rt_domino_rect_union(const int* rects, int n) {
+  if (n < 1 || !rects) return nullptr;
+  for (int r = 0; r < n; r++)
+    if (rects[4 * r + 2] < 1 || rects[4 * r + 3] < 1 || rects[4 * r + 2] > 8192 ||
+        rects[4 * r + 3] > 8192 || std::abs(rects[4 * r]) > 8192 || std::abs(rects[4 * r + 1]) > 8192)
+      return nullptr;
+  return make_vec(domino_rect_union(std::vector<int>(rects, rects + 4 * (size_t)n)));
 }
 
 rt_vec* rt_domino_aztec_diamond(int n) {

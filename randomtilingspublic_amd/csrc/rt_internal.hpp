@@ -37,6 +37,7 @@ rt_vec* make_vec(std::vector<int32_t>&& v);
 // Host statics (rt_host.cpp)
 domain domino_aztec_rectangle(int n, const std::vector<int>& remove);
 domain domino_rectangle(int M, int N);
+domain domino_rect_union(const std::vector<int>& rects);
 std::vector<int> domino_domain_to_vertices(const domain& d);
 int domino_thurston(const domain& d, bool max, tiling& out);
 tiling domino_heightfunc_to_tiling(const heightfunc& hf, const domain& d);

@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <map>
 #include <random>
 #include <string>
 #include <vector>
@@ -41,6 +42,9 @@ struct rt_ctx {
   rt_timing last{};
   int engine = 0;
   int tile_shape = 0;  // sweep-block tile shape: 0 auto, else a TileShape
+  int autotune = 1;    // time candidate tile shapes on a job's first windows
+  std::map<uint32_t, int> tuned;  // tuning key (model, chains, tile count) -> shape
+  hipEvent_t evt[8] = {};         // tuner trial boundaries
 
   int reserve(size_t bytes);
   int reserve_aux(size_t bytes);

@@ -64,3 +64,61 @@ def test_domino_walk_batch_uneven(gpu_ctx, use_graph):
     for t, s, got in zip(tilings, seeds, outs):
         want = o.domino_walk(t, 1500, s, P, rt.DOMINO_TOGGLE08)
         assert np.array_equal(got, want)
+
+
+def test_domino_walk_batch_rect_unions(gpu_ctx):
+    from randomtilingspublic_amd import workloads as W
+    batch = W.uneven_domino_batch(5, 10, order=(3, 40), side=(4, 40), max_extent=64)
+    tilings = [b["tiling"] for b in batch]
+    big = max(b["N"] for b in batch)
+    P = rt.synthetic_params(big * big // 2, 24)
+    gpu_ctx.set_tinymt_params(P)
+    seeds = [7 * k + 3 for k in range(len(tilings))]
+    want = [o.domino_walk(t, 900, s, P, rt.DOMINO_HEATBATH) for t, s in zip(tilings, seeds)]
+    gpu_ctx.set_autotune(2)
+    try:
+        for use_graph in (False, True):
+            _say("domino walk batch rect unions graph=", use_graph)
+            outs = gpu_ctx.domino_walk_batch(tilings, 900, seeds, rt.DOMINO_HEATBATH,
+                                             use_graph=use_graph)
+            tm = gpu_ctx.last_timing()
+            # first call tunes (900 steps cut into >= 24 windows), the second reuses it
+            assert tm["shape_source"] == (2 if use_graph else 1), tm
+            assert tm["shape"] in (1, 2, 3)
+            for got, w in zip(outs, want):
+                assert np.array_equal(got, w)
+    finally:
+        gpu_ctx.set_autotune(1)
+
+
+def test_tile_shape_choice_never_changes_results(gpu_ctx):
+    d = rt.LozengeTiler.AlmostHexDomain(9, 0)
+    tmin, tmax = rt.LozengeTiler.MinTiling(d), rt.LozengeTiler.MaxTiling(d)
+    N = rt.side(tmin)
+    P = rt.synthetic_params(N * N, 25)
+    gpu_ctx.set_tinymt_params(P)
+    masters = [3, 8, 13]
+    ref = None
+    try:
+        for mode, shape in [(0, 0), (2, 0), (1, 0), (0, 1), (0, 2), (0, 3)]:
+            _say("lozenge cftp batch autotune", mode, "shape", shape)
+            gpu_ctx.set_autotune(mode)
+            gpu_ctx.set_tile_shape(shape)
+            outs, stats = gpu_ctx.lozenge_cftp_batch(tmax, tmin, masters, rt.SCHED_DOUBLING, 256,
+                                                     30)
+            tm = gpu_ctx.last_timing()
+            if shape:
+                assert tm["shape"] == shape and tm["shape_source"] == 0
+            if mode == 2:
+                assert tm["shape_source"] == 1, tm
+            if ref is None:
+                ref = outs
+                for k, m in enumerate(masters):
+                    want, _ = o.lozenge_cftp(tmax, tmin, m, P, rt.SCHED_DOUBLING, 256, 30)
+                    assert np.array_equal(outs[k], want)
+            else:
+                for a, b in zip(outs, ref):
+                    assert np.array_equal(a, b)
+    finally:
+        gpu_ctx.set_autotune(1)
+        gpu_ctx.set_tile_shape(0)
