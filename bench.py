@@ -138,7 +138,7 @@ def batched_walks(rt, seed=4, nchains=256, steps=10 ** 4):
     dev_s = tm["device_ms"] / 1e3
     att_s = tm["attempts"] / dev_s
     launches = tm["launches"]
-    alg = 48.0 * sites * launches
+    alg = 48.0 * tm["kernel_sites"]  # sites of the chains each launch advances
     gbs = alg / dev_s / 1e9
     foot = sum(b["N"] * b["N"] // 2 * (16 + 2) for b in batch)
     return {"workload": f"{nchains} heat-bath domino chains on uneven domains from seed {seed} "
@@ -155,6 +155,65 @@ def batched_walks(rt, seed=4, nchains=256, steps=10 ** 4):
                          "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
                          "alg_bytes_per_site_per_launch": 48,
                          "note": "device time includes the final colour-plane sync"}}
+
+
+def weighted_and_bibone(rt, cftp_seeds=0, steps=10 ** 5):
+    """Weighted 2-periodic domino walk on the Aztec diamond of order 512
+    (a = 1.0, b = 0.5, the reference's Metropolis rule of
+    dominokerneltwoperiodic.cl, 10^5 steps from the minimal tiling), the
+    bibone walk on the hexagon with sides 200, 201, 200 (10^5 steps), and
+    optionally the two-periodic heat-bath CFTP of the same diamond for
+    master seeds 1..cftp_seeds, batched."""
+    D, B = rt.DominoTiler, rt.TriangleDimerTiler
+    out = {}
+    d = D.AztecDiamond(512)
+    tmin = D.MinTiling(d)
+    N = rt.side(tmin)
+    ctx = rt.Context(0)
+    try:
+        ctx.set_tinymt_params(rt.synthetic_params(N * N // 2, None))
+        ctx.domino_walk(tmin, 200, 1, rt.DOMINO_METRO2P, 1.0, 0.5)
+        w = time.time()
+        ctx.domino_walk(tmin, steps, 1, rt.DOMINO_METRO2P, 1.0, 0.5)
+        wall = time.time() - w
+        tm = ctx.last_timing()
+        out["weighted_walk"] = {
+            "workload": f"domino Aztec diamond order 512 (N={N}), two-periodic Metropolis "
+                        f"a=1.0 b=0.5, {steps} steps from the minimal tiling",
+            "wall_s": round(wall, 3), "device_s": round(tm["device_ms"] / 1e3, 3),
+            "us_per_step": round(tm["device_ms"] * 1e3 / steps, 3),
+            "flip_attempts_per_s": round(tm["attempts"] / (tm["device_ms"] / 1e3), 1),
+            "launches": tm["launches"], "tile_shape": tm["shape"]}
+        if cftp_seeds:
+            tmax = D.MaxTiling(d)
+            w = time.time()
+            outs, stats = ctx.domino_cftp_batch(tmax, tmin, list(range(1, cftp_seeds + 1)),
+                                                rt.SCHED_DOUBLING, 1024, 30,
+                                                rt.DOMINO_HEATBATH2P, 1.0, 0.5)
+            wall = time.time() - w
+            out["weighted_cftp"] = {
+                "workload": f"two-periodic heat-bath CFTP (a=1.0, b=0.5) of the Aztec diamond "
+                            f"of order 512, doubling from 2^10, master seeds 1..{cftp_seeds}, "
+                            "batched",
+                "wall_s": round(wall, 2), "samples_per_s": round(cftp_seeds / wall, 5),
+                "rounds": [s["rounds"] for s in stats]}
+        t = B.Hexagon(200, 200, 201)
+        Nb = rt.side(t)
+        ctx.set_tinymt_params(rt.synthetic_params((Nb // 2) ** 2, None))
+        ctx.bibone_walk(t, 200, 1)
+        w = time.time()
+        ctx.bibone_walk(t, steps, 1)
+        wall = time.time() - w
+        tm = ctx.last_timing()
+        out["bibone_walk"] = {
+            "workload": f"bibone hexagon sides 200, 201, 200 (tiling side {Nb}), {steps} steps",
+            "wall_s": round(wall, 3), "device_s": round(tm["device_ms"] / 1e3, 3),
+            "us_per_step": round(tm["device_ms"] * 1e3 / steps, 3),
+            "flip_attempts_per_s": round(tm["attempts"] / (tm["device_ms"] / 1e3), 1),
+            "launches": tm["launches"]}
+    finally:
+        ctx.close()
+    return out
 
 
 def served_stream(rt, seconds=20.0, seed=2024):
@@ -343,6 +402,9 @@ def main():
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-batch-walks", action="store_true")
+    ap.add_argument("--no-c3", action="store_true")
+    ap.add_argument("--c3-cftp", type=int, default=0,
+                    help="also run the two-periodic Aztec-512 CFTP for master seeds 1..K")
     ap.add_argument("--profile-rounds", type=int, default=0,
                     help="profiling run: the same batch capped at R doubling rounds (no sample "
                          "needs to finish); prints the sweep kernel's timing only")
@@ -452,6 +514,9 @@ def main():
     walks = None if args.no_batch_walks else batched_walks(rt)
     if walks:
         print(f"[bench] batched walks {walks}", file=sys.stderr, flush=True)
+    c3 = None if args.no_c3 else weighted_and_bibone(rt, args.c3_cftp)
+    if c3:
+        print(f"[bench] weighted / bibone {c3}", file=sys.stderr, flush=True)
     served = None
     if not args.no_serve:
         served = served_stream(rt)
@@ -490,6 +555,7 @@ def main():
             round(samples_s / plain_fused["samples_per_s"], 2) if plain_fused else None,
         "plain_sweep": sweep,
         "batched_walks": walks,
+        "weighted_and_bibone": c3,
         "served": served,
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -79,6 +79,9 @@ typedef struct {
    * earlier call on a job like it. */
   int shape;
   int shape_source;
+  /* 1 when the chain pairs of that job ran one chain per workgroup (two
+   * workgroups per tile, each drawing the pair's shared numbers). */
+  int split;
 } rt_timing;
 int rt_last_timing(const rt_ctx* ctx, rt_timing* out);
 
@@ -93,9 +96,11 @@ int rt_set_engine(rt_ctx* ctx, int engine);
 /* Tile shape of the sweep-block kernels (no reference counterpart; a tuning
  * knob for walks and batched CFTP, results do not depend on it).  Tiles are
  * 64 columns wide: 1 = 64 rows worked by 8 waves x 8 rows per lane, 2 = 64
- * rows by 16 waves x 4 rows, 3 = 32 rows by 16 waves x 2 rows; 0 = chosen
- * per job from its tile count against the CU count.  The serving loop always
- * uses 64-row tiles. */
+ * rows by 16 waves x 4 rows, 3 = 32 rows by 16 waves x 2 rows; 4, 5, 6 =
+ * shapes 1, 2, 3 with CFTP chain pairs split over two workgroups per tile
+ * (each runs one chain, both draw the pair's shared numbers); 0 = chosen
+ * per job (autotuner, else a rule of thumb on the tile count).  The serving
+ * loop always uses 64-row tiles. */
 int rt_set_tile_shape(rt_ctx* ctx, int shape);
 
 /* Tile-shape autotuner (no reference counterpart; on by default).  With
@@ -295,6 +300,10 @@ rt_vec* rt_recttri_slope_hex(int n);                     /* 173-478 */
 /* TriangleDimerTiler statics (TriangleDimerTiler.h:194-209). */
 rt_vec* rt_bibone_ice_cream_cone(int N, int M);          /* TriangleDimerTiler.cpp:697-793 */
 rt_vec* rt_bibone_test_butterfly(int type);              /* 795-844 */
+/* Hexagon with sides a, b, c, a, b, c and a dimer cover, in the IceCreamCone
+ * tiling format (TriangleDimerTiler.cpp:697-793).  No reference counterpart.
+ * NULL when the hexagon has an odd number of vertices (ab+bc+ca+a+b+c+1). */
+rt_vec* rt_bibone_hexagon(int a, int b, int c);
 rt_vec* rt_bibone_triangle_flips(const int32_t* t, size_t n);    /* 846-873 */
 rt_vec* rt_bibone_butterfly_flips(const int32_t* t, size_t n);   /* 875-921 */
 rt_vec* rt_bibone_tiling_to_vertices(const int32_t* t, size_t n); /* 923-949 */

@@ -305,6 +305,8 @@ class TriangleDimerTiler : public rt::Device {
 
   static tiling IceCreamCone(int N, int M) { return rt::take(rt_bibone_ice_cream_cone(N, M)); }
   static tiling TestButterfly(int t) { return rt::take(rt_bibone_test_butterfly(t)); }
+  // Hexagon with sides a, b, c, a, b, c (no reference counterpart).
+  static tiling Hexagon(int a, int b, int c) { return rt::take(rt_bibone_hexagon(a, b, c)); }
   static std::vector<int> TilingToTriangleFlips(tiling& t) {
     return rt::take(rt_bibone_triangle_flips(t.data(), t.size()));
   }

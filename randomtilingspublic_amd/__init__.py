@@ -39,7 +39,7 @@ class _Timing(C.Structure):
                 ("kernel_ms", C.c_double), ("kernel_launches", C.c_long),
                 ("kernel_sites", C.c_long), ("shape_ms", C.c_double * 4),
                 ("shape_launches", C.c_long * 4), ("shape_sites", C.c_long * 4),
-                ("shape", C.c_int), ("shape_source", C.c_int)]
+                ("shape", C.c_int), ("shape_source", C.c_int), ("split", C.c_int)]
 
 
 class DominoRule(C.Structure):
@@ -146,6 +146,7 @@ def lib():
         ("rt_recttri_slope_hex", [C.c_int]),
         ("rt_bibone_ice_cream_cone", [C.c_int, C.c_int]),
         ("rt_bibone_test_butterfly", [C.c_int]),
+        ("rt_bibone_hexagon", [C.c_int, C.c_int, C.c_int]),
         ("rt_bibone_triangle_flips", [_I32P, C.c_size_t]),
         ("rt_bibone_butterfly_flips", [_I32P, C.c_size_t]),
         ("rt_bibone_tiling_to_vertices", [_I32P, C.c_size_t]),
@@ -241,7 +242,8 @@ class Context:
 
     def set_tile_shape(self, shape):
         """Sweep-block tile shape: 0 auto, 1 = 64 rows / 8 waves, 2 = 64 rows /
-        16 waves, 3 = 32 rows / 16 waves."""
+        16 waves, 3 = 32 rows / 16 waves; 4..6 = 1..3 with CFTP chain pairs
+        split over two workgroups per tile."""
         self._chk(self._L.rt_set_tile_shape(self._h, shape))
 
     def set_autotune(self, mode):
@@ -264,7 +266,7 @@ class Context:
                 "kernel_launches": t.kernel_launches, "kernel_sites": t.kernel_sites,
                 "shape_ms": list(t.shape_ms), "shape_launches": list(t.shape_launches),
                 "shape_sites": list(t.shape_sites), "shape": t.shape,
-                "shape_source": t.shape_source}
+                "shape_source": t.shape_source, "split": t.split}
 
     # ---- domino ----
     def domino_walk(self, t, steps, seed, rule=DOMINO_TOGGLE08, a=1.0, b=1.0):
@@ -601,6 +603,12 @@ class TriangleDimerTiler:
     @staticmethod
     def TestButterfly(kind):
         return _take_vec(lib().rt_bibone_test_butterfly(kind))
+
+    @staticmethod
+    def Hexagon(a, b, c):
+        """Hexagon with sides a, b, c, a, b, c and a dimer cover (no
+        reference counterpart); RTError when its vertex count is odd."""
+        return _take_vec(lib().rt_bibone_hexagon(a, b, c))
 
     @staticmethod
     def TilingToTriangleFlips(t):

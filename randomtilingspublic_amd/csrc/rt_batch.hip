@@ -280,18 +280,20 @@ static int cftp_batch(rt_ctx* ctx, int model, const int32_t* tmax, const int32_t
     HIPCHK_RET(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     ms_total += ms;
     float kms = 0;
-    if (!plan.windows.empty()) HIPCHK_RET(ctx, hipEventElapsedTime(&kms, ctx->evk0, ctx->evk1));
-    const long nlaunch = (long)(plan.windows.size() - std::min(plan.first, plan.windows.size()));
+    if (plan.nlaunch) HIPCHK_RET(ctx, hipEventElapsedTime(&kms, ctx->evk0, ctx->evk1));
+    const long nlaunch = (long)(plan.nlaunch - std::min(plan.first, plan.nlaunch));
+    const long nsites = plan.sites(plan.first, plan.nlaunch);
     acc.kernel_ms += kms;
     acc.kernel_launches += nlaunch;
-    acc.kernel_sites += nlaunch * plan.streams;
+    acc.kernel_sites += nsites;
     if (job.shape >= 1 && job.shape <= 4) {
       acc.shape_ms[job.shape - 1] += kms;
       acc.shape_launches[job.shape - 1] += nlaunch;
-      acc.shape_sites[job.shape - 1] += nlaunch * plan.streams;
+      acc.shape_sites[job.shape - 1] += nsites;
     }
     fused_add_trials(plan, &acc);
     acc.shape = job.shape;
+    acc.split = job.split ? 1 : 0;
     if (job.shape_source == 1 || acc.shape_source != 1) acc.shape_source = job.shape_source;
     acc.launches += ctx->last.launches;
     acc.steps += ctx->last.steps;
@@ -469,6 +471,8 @@ extern "C" int rt_domino_walk_batch(rt_ctx* ctx, int32_t* const* tilings, const 
     HIPCHK_RET(ctx, hipEventSynchronize(ctx->ev1));
     hipGraphExecDestroy(exec);
     hipGraphDestroy(graph);
+    ctx->last.kernel_launches = (long)(plan.nlaunch - std::min(plan.first, plan.nlaunch));
+    ctx->last.kernel_sites = plan.sites(plan.first, plan.nlaunch);
   } else {
     HIPCHK_RET(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     rc = fused_launch(ctx, job, plan);
@@ -480,14 +484,15 @@ extern "C" int rt_domino_walk_batch(rt_ctx* ctx, int32_t* const* tilings, const 
     HIPCHK_RET(ctx, hipEventSynchronize(ctx->ev1));
     float kms = 0;
     HIPCHK_RET(ctx, hipEventElapsedTime(&kms, ctx->ev0, ctx->evk1));
-    const long nlaunch = (long)(plan.windows.size() - std::min(plan.first, plan.windows.size()));
+    const long nlaunch = (long)(plan.nlaunch - std::min(plan.first, plan.nlaunch));
+    const long nsites = plan.sites(plan.first, plan.nlaunch);
     ctx->last.kernel_ms = kms;
     ctx->last.kernel_launches = nlaunch;
-    ctx->last.kernel_sites = nlaunch * plan.streams;
+    ctx->last.kernel_sites = nsites;
     if (job.shape >= 1 && job.shape <= 4) {
       ctx->last.shape_ms[job.shape - 1] = kms;
       ctx->last.shape_launches[job.shape - 1] = nlaunch;
-      ctx->last.shape_sites[job.shape - 1] = nlaunch * plan.streams;
+      ctx->last.shape_sites[job.shape - 1] = nsites;
     }
   }
   fused_add_trials(plan, &ctx->last);

@@ -8,6 +8,7 @@
 // cell's four dimer slots holds a dimer.  The derived arrays are
 //   triangle flips: (N/2) x N, 6-bit states (three 2-bit fields);
 //   butterfly flips: N x N, 9-bit states (four 2-bit fields + a flag).
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -79,6 +80,57 @@ tiling bibone_ice_cream_cone(int N, int M) {
         if (right_edge && i % 4 == (2 + N % 2 + 2 * k) % 4) e.r[at] = 1;
       }
     }
+  }
+  return edges_to_tiling(e);
+}
+
+// Hexagon with sides a, c, b, a, c, b (clockwise from the top edge), with a
+// dimer cover.  No reference counterpart (the reference builds only
+// IceCreamCone).  In the vertex coordinates of TilingToVertices a vertex
+// (i, j) neighbours (i, j+-1), (i+-1, j), (i-1, j+1) and (i+1, j-1); edge
+// h(i, j) joins (i, j)-(i, j+1), l(i, j) joins (i, j)-(i+1, j) and r(i, j)
+// joins (i, j+1)-(i+1, j).  Vertex rows i = 0..b+c hold the columns
+// L(i)..R(i), L(i) = -min(i, b), R(i) = a - max(0, i - c), placed at offset
+// (2, 2 + b) of the edge grid.  The row ends of consecutive rows are always
+// adjacent (L and R move by 0 or -1 per row), so the cover is built
+// top-down: a row with an odd number of unmatched vertices sends one free
+// end down to the end of the next row, and the rest pairs horizontally.
+// Returns an empty tiling when the vertex count ab + bc + ca + a + b + c + 1
+// is odd (no cover exists) or a side is negative.
+tiling bibone_hexagon(int a, int b, int c) {
+  if (a < 0 || b < 0 || c < 0) return tiling();
+  const long nv = (long)a * b + (long)b * c + (long)c * a + a + b + c + 1;
+  if (nv % 2) return tiling();
+  const int rows = b + c + 1;
+  const int E = std::max(rows, a + b + 1) + 4;
+  Edges e(E);
+  auto L = [&](int i) { return -std::min(i, b); };
+  auto R = [&](int i) { return a - std::max(0, i - c); };
+  auto at = [&](int i, int j) { return (i + 2) * E + (j + 2 + b); };
+  // Join (i, j) to (i+1, j2), j2 = j or j - 1.
+  auto down = [&](int i, int j, int j2) {
+    if (j2 == j) e.l[at(i, j)] = 1;
+    else e.r[at(i, j2)] = 1;
+  };
+  int lf = 0, rf = 0;  // row ends already matched from the row above
+  for (int i = 0; i < rows; i++) {
+    int lo = L(i) + lf, hi = R(i) - rf;
+    int nlf = 0, nrf = 0;
+    if ((hi - lo + 1) % 2) {
+      if (i + 1 >= rows) return tiling();  // cannot happen when nv is even
+      if (!lf) {
+        down(i, lo, L(i + 1));
+        lo++;
+        nlf = 1;
+      } else {
+        down(i, hi, R(i + 1));
+        hi--;
+        nrf = 1;
+      }
+    }
+    for (int j = lo; j < hi; j += 2) e.h[at(i, j)] = 1;
+    lf = nlf;
+    rf = nrf;
   }
   return edges_to_tiling(e);
 }

@@ -142,9 +142,8 @@ __device__ __forceinline__ void run_window(Cell* lds, const Lanes& L, Tiny* st,
 // W wave64s per workgroup; each lane owns one tile column and R rows
 // (rows wv, wv+W, ...), so the loaded tile is (W*R) x 64 cells.
 template <int MODEL, int NC, int W, int R>
-__global__ __launch_bounds__(64 * W, W >= 16 ? 1 : 2) void k_fused(const UnitDesc* __restrict__ units,
-                                                    const WorkItem* __restrict__ work, long s0,
-                                                    int K, int init, int seg, int H,
+__global__ __launch_bounds__(64 * W, 4) void k_fused(const UnitDesc* __restrict__ units,
+                                                    const WorkItem* __restrict__ work, int H,
                                                     ModelParams mp,
                                                     const WindowRec* __restrict__ wins) {
   using Cell = typename CellOf<MODEL>::type;
@@ -153,22 +152,18 @@ __global__ __launch_bounds__(64 * W, W >= 16 ? 1 : 2) void k_fused(const UnitDes
   constexpr int LH = W * R;
   constexpr int RS = kLW + 2;
   constexpr int PL = (LH + 2) * RS;
-  __shared__ Cell lds[NC * NCOL * PL];
+  __shared__ __align__(16) Cell lds[NC * NCOL * PL];
+  static_assert((NCOL * PL * sizeof(Cell)) % 4 == 0, "chain planes must be whole words");
   __shared__ uint8_t sops[kKMax];
 
   const WorkItem wi = work[blockIdx.x];
-  const UnitDesc U = units[wi.unit];
-  uint32_t seed = 0;
-  if (wins) {
-    const WindowRec w = wins[wi.unit];
-    s0 = w.s0;
-    K = w.K;
-    init = w.init;
-    seed = w.seed;
-    if (K <= 0 && !init) return;  // unit idle in this launch
-  } else if (init) {
-    seed = U.seeds[seg];
-  }
+  const WindowRec w = wins[wi.unit];
+  const long s0 = w.s0;
+  const int K = w.K, init = w.init;
+  const uint32_t seed = w.seed;
+  if (K <= 0 && !init) return;  // unit idle in this launch
+  UnitDesc U = units[wi.unit];
+  window_buffers(U, w.flags);
   const int rows = U.rows, cols = U.cols;
   const int TH = LH - 2 * H, TW = kLW - 2 * H;
   const int tr = wi.tile / U.tiles_c, tc = wi.tile % U.tiles_c;
@@ -222,7 +217,23 @@ __global__ __launch_bounds__(64 * W, W >= 16 ? 1 : 2) void k_fused(const UnitDes
   }
   __syncthreads();
 
-  run_window<MODEL, NC, W, R, Cell, RS, PL>(lds, L, st, sops, K, U, mp);
+  // A CFTP pair whose two chains hold the same cells over the whole loaded
+  // region (coalesced there, or outside the domain) ends the window with
+  // the same cells too: every step is a function of the loaded cells and
+  // the shared draws.  Such a tile runs one chain and writes it to both.
+  // (Not in the 8-row shape: the second loop body would push it past 128
+  // VGPRs, i.e. below two workgroups per CU.)
+  constexpr bool kOne = NC == 2 && R <= 4;
+  bool one = false;
+  if (kOne) {
+    constexpr int kWords = NCOL * PL * (int)sizeof(Cell) / 4;
+    const uint32_t* w32 = (const uint32_t*)lds;
+    int diff = 0;
+    for (int k = threadIdx.x; k < kWords; k += kThreads) diff |= w32[k] != w32[kWords + k];
+    one = !__syncthreads_or(diff);
+  }
+  if (kOne && one) run_window<MODEL, 1, W, R, Cell, RS, PL>(lds, L, st, sops, K, U, mp);
+  else run_window<MODEL, NC, W, R, Cell, RS, PL>(lds, L, st, sops, K, U, mp);
 
 #pragma unroll
   for (int r = 0; r < R; r++) {
@@ -234,14 +245,14 @@ __global__ __launch_bounds__(64 * W, W >= 16 ? 1 : 2) void k_fused(const UnitDes
     for (int ch = 0; ch < NC; ch++)
 #pragma unroll
       for (int k = 0; k < NCOL; k++)
-        ((Cell*)U.plane[ch][k])[(long)gr * cols + gc] =
-            lds[(ch * NCOL + k) * PL + L.li0 + r * W * RS];
-    if ((L.hsm >> r) & 1u) {
+        ((Cell*)U.plane_b[ch][k])[(long)gr * cols + gc] =
+            lds[((one ? 0 : ch) * NCOL + k) * PL + L.li0 + r * W * RS];
+    if (((L.hsm >> r) & 1u) && !U.no_state_write) {
       const long id = stream_id<MODEL>(gr, gc, rows, cols);
-      U.state[id] = st[r].s0;
-      U.state[U.nstream + id] = st[r].s1;
-      U.state[2 * U.nstream + id] = st[r].s2;
-      U.state[3 * U.nstream + id] = st[r].s3;
+      U.state_b[id] = st[r].s0;
+      U.state_b[U.nstream + id] = st[r].s1;
+      U.state_b[2 * U.nstream + id] = st[r].s2;
+      U.state_b[3 * U.nstream + id] = st[r].s3;
     }
   }
 }
@@ -330,10 +341,16 @@ static uint32_t tune_key(const FusedJob& job, long t64) {
   return (uint32_t)job.model | ((uint32_t)job.nc << 4) | ((uint32_t)lg << 8);
 }
 
-constexpr int kTuneMinWindows = 24;  // shorter jobs keep the rule of thumb
+constexpr int kTuneMinWindows = 32;  // shorter jobs keep the rule of thumb
 constexpr int kTrialWindows = 2;     // windows per timed trial
 constexpr int kTrialReps = 2;        // trials per candidate shape
-static const int kTuneShapes[3] = {SH_8x8, SH_16x4, SH_16x2};
+struct LaunchShape {
+  int shape;
+  bool split;
+};
+static const LaunchShape kTuneShapes[5] = {
+    {SH_8x8, false}, {SH_16x4, false}, {SH_16x2, false}, {SH_16x4, true}, {SH_16x2, true}};
+static int tune_count(const FusedJob& job) { return job.nc == 2 ? 5 : 3; }
 
 static bool tunable(const rt_ctx* ctx, const FusedJob& job) {
   return ctx->autotune && ctx->tile_shape == SH_AUTO && job.model != M_RECTTRI &&
@@ -342,13 +359,15 @@ static bool tunable(const rt_ctx* ctx, const FusedJob& job) {
 
 static void choose_shape(const rt_ctx* ctx, FusedJob& job) {
   job.tune = false;
+  job.split = false;
   job.shape_source = 0;
   if (job.model == M_RECTTRI) {
     job.shape = job.nc == 2 ? SH_RT8x4 : SH_8x8;
     return;
   }
-  if (ctx->tile_shape != SH_AUTO) {
-    job.shape = ctx->tile_shape;
+  if (ctx->tile_shape != SH_AUTO) {  // 1..3, or 4..6 = the same with pairs split
+    job.shape = ctx->tile_shape > 3 ? ctx->tile_shape - 3 : ctx->tile_shape;
+    job.split = ctx->tile_shape > 3 && job.nc == 2;
     return;
   }
   const long t64 = tiles64(job);
@@ -356,54 +375,93 @@ static void choose_shape(const rt_ctx* ctx, FusedJob& job) {
   if (!tunable(ctx, job)) return;
   auto it = ctx->tuned.find(tune_key(job, t64));
   if (it != ctx->tuned.end()) {
-    job.shape = it->second;
+    job.shape = it->second & 15;
+    job.split = job.nc == 2 && (it->second >> 4) != 0;
     job.shape_source = 2;
     return;
   }
-  job.tune = job.windows.size() >= (size_t)kTuneMinWindows;
+  job.tune = job.nlaunch >= (size_t)kTuneMinWindows;
 }
 
-// Launch windows: at most kKMax steps and at most H pull steps (over all
-// units) each, a new window at every segment start (TinyMT re-seeding).
+// Launch windows of each unit: at most kKMax steps and at most H of the
+// unit's pull steps each, a new window at every segment start (TinyMT
+// re-seeding).
 static void cut_windows(FusedJob& job, int H) {
   long T = 0;
   for (long s : job.seg_steps) T += s;
-  job.windows.clear();
+  job.windows.assign(job.units.size(), {});
+  job.nlaunch = 0;
   if (job.units.empty() || T == 0) return;
-  std::vector<uint8_t> anypull(T, 0);
-  for (const std::vector<uint8_t>& o : job.ops) {
-    const long n = std::min<long>(T, (long)o.size());
-    for (long s = 0; s < n; s++) anypull[s] |= (o[s] & (OP_PULL_A | OP_PULL_B)) ? 1 : 0;
-  }
-  long s0 = 0;
-  for (size_t k = 0; k < job.seg_steps.size(); k++) {
-    const long end = s0 + job.seg_steps[k];
-    bool first = true;
-    while (s0 < end) {
-      int K = 0, pulls = 0;
-      while (s0 + K < end && K < kKMax) {
-        const int p1 = anypull[s0 + K];
-        if (pulls + p1 > H) break;
-        pulls += p1;
-        K++;
+  for (size_t u = 0; u < job.units.size(); u++) {
+    const std::vector<uint8_t>& o = job.ops[u];
+    std::vector<FusedWindow>& win = job.windows[u];
+    long s0 = 0;
+    for (size_t k = 0; k < job.seg_steps.size(); k++) {
+      const long end = std::min<long>(s0 + job.seg_steps[k], (long)o.size());
+      bool first = true;
+      while (s0 < end) {
+        int K = 0, pulls = 0;
+        while (s0 + K < end && K < kKMax) {
+          const int p1 = (o[s0 + K] & (OP_PULL_A | OP_PULL_B)) ? 1 : 0;
+          if (pulls + p1 > H) break;
+          pulls += p1;
+          K++;
+        }
+        win.push_back(FusedWindow{s0, K, first ? 1 : 0, (int)k});
+        first = false;
+        s0 += K;
       }
-      job.windows.push_back(FusedWindow{s0, K, first ? 1 : 0, (int)k});
-      first = false;
-      s0 += K;
     }
+    job.nlaunch = std::max(job.nlaunch, win.size());
   }
 }
 
-static size_t tiles_bytes(const FusedJob& job, int shape) {
+static size_t tiles_bytes(const FusedJob& job, int shape, bool split) {
   const Geo g = shape_geo(shape);
+  const size_t per = split ? 2 : 1;
   size_t work = 0;
   for (const UnitDesc& u : job.units) {
     int tr, tc;
     tile_counts(u, g.W * g.R, g.H, &tr, &tc);
-    work += (size_t)tr * tc;
+    work += (size_t)tr * tc * per;
   }
-  return ((job.units.size() * sizeof(UnitDesc) + 255) & ~(size_t)255) +
-         ((work * sizeof(WorkItem) + 255) & ~(size_t)255);
+  size_t b = ((job.units.size() * per * sizeof(UnitDesc) + 255) & ~(size_t)255) +
+             ((work * sizeof(WorkItem) + 255) & ~(size_t)255);
+  if (split) b += (job.nlaunch * job.units.size() * 2 * sizeof(WindowRec) + 255) & ~(size_t)255;
+  return b;
+}
+
+// Second buffer set of a unit: every colour plane of every chain, and the
+// TinyMT states.
+static size_t plane_bytes(const FusedJob& job, const UnitDesc& u) {
+  return (size_t)u.rows * u.cols * (job.model == M_RECTTRI ? 4 : 1);
+}
+static size_t second_set_bytes(const FusedJob& job, const UnitDesc& u) {
+  const int ncol = job.model == M_DOMINO ? 2 : 3;
+  return (size_t)job.nc * ncol * ((plane_bytes(job, u) + 255) & ~(size_t)255) +
+         ((16 * (size_t)u.nstream + 255) & ~(size_t)255);
+}
+
+// Copy-back of a unit whose last window wrote the second set.
+struct CopyDesc {
+  const void* src;
+  void* dst;
+  long bytes;
+};
+
+__global__ void k_copy_list(const CopyDesc* __restrict__ list) {
+  const CopyDesc d = list[blockIdx.y];
+  const long stride = (long)gridDim.x * blockDim.x;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if ((((uintptr_t)d.src | (uintptr_t)d.dst | (uintptr_t)d.bytes) & 15) == 0) {
+    const uint4* s = (const uint4*)d.src;
+    uint4* o = (uint4*)d.dst;
+    for (long i = t; i < d.bytes / 16; i += stride) o[i] = s[i];
+  } else {
+    const uint8_t* s = (const uint8_t*)d.src;
+    uint8_t* o = (uint8_t*)d.dst;
+    for (long i = t; i < d.bytes; i += stride) o[i] = s[i];
+  }
 }
 
 size_t fused_scratch_bytes(rt_ctx* ctx, FusedJob& job) {
@@ -414,24 +472,26 @@ size_t fused_scratch_bytes(rt_ctx* ctx, FusedJob& job) {
   for (long s : job.seg_steps) T += s;
   const size_t nu = job.units.size();
   size_t bytes = ((nu * (size_t)T + 255) & ~(size_t)255) +
-                 ((nu * job.seg_steps.size() * sizeof(uint32_t) + 255) & ~(size_t)255) + 1024;
+                 ((job.nlaunch * nu * sizeof(WindowRec) + 255) & ~(size_t)255) + 1024;
   if (job.tune)
-    for (int sh : kTuneShapes) bytes += tiles_bytes(job, sh);
+    for (int c = 0; c < tune_count(job); c++)
+      bytes += tiles_bytes(job, kTuneShapes[c].shape, kTuneShapes[c].split);
   else
-    bytes += tiles_bytes(job, job.shape);
+    bytes += tiles_bytes(job, job.shape, job.split);
+  for (const UnitDesc& u : job.units) bytes += second_set_bytes(job, u);
+  bytes += ((nu * 7 * sizeof(CopyDesc)) + 255) & ~(size_t)255;
   return bytes;
 }
 
 template <int MODEL, int NC, int W, int R>
 static void launch(rt_ctx* ctx, unsigned nblocks, const UnitDesc* units, const WorkItem* work,
-                   long s0, int K, int init, int seg, int H, const ModelParams& mp,
-                   const WindowRec* wins) {
+                   int H, const ModelParams& mp, const WindowRec* wins) {
   hipLaunchKernelGGL((k_fused<MODEL, NC, W, R>), dim3(nblocks), dim3(64 * W), 0, ctx->stream,
-                     units, work, s0, K, init, seg, H, mp, wins);
+                     units, work, H, mp, wins);
 }
 
-typedef void (*LaunchFn)(rt_ctx*, unsigned, const UnitDesc*, const WorkItem*, long, int, int, int,
-                         int, const ModelParams&, const WindowRec*);
+typedef void (*LaunchFn)(rt_ctx*, unsigned, const UnitDesc*, const WorkItem*, int,
+                         const ModelParams&, const WindowRec*);
 
 template <int MODEL, int NC>
 static LaunchFn pick_byte(int shape) {
@@ -442,10 +502,9 @@ static LaunchFn pick_byte(int shape) {
 
 template <int NC, int W, int R>
 static void launch_swar(rt_ctx* ctx, unsigned nblocks, const UnitDesc* units, const WorkItem* work,
-                        long s0, int K, int init, int seg, int H, const ModelParams& mp,
-                        const WindowRec* wins) {
+                        int H, const ModelParams& mp, const WindowRec* wins) {
   hipLaunchKernelGGL((k_swar_lozenge<NC, W, R>), dim3(nblocks), dim3(64 * W), 0, ctx->stream,
-                     units, work, s0, K, init, seg, H, mp, wins);
+                     units, work, H, mp, wins);
 }
 
 // Lozenge: four cells per lane-word (k_swar_lozenge), same tiles: 64-row
@@ -465,11 +524,30 @@ static LaunchFn pick(const rt_ctx* ctx, int model, int nc, int shape) {
   return nc == 1 ? launch<M_RECTTRI, 1, 8, 8> : launch<M_RECTTRI, 2, 8, 4>;
 }
 
-// Unit table and work list of one tile shape, at p (advanced past them).
-static int upload_tiles(rt_ctx* ctx, const FusedJob& job, int shape, uint8_t*& p,
+// Unit table, work list (and, split, window records) of one launch shape,
+// at p (advanced past them).  `wins` are the job's [launch][unit] records.
+static int upload_tiles(rt_ctx* ctx, const FusedJob& job, LaunchShape ls, uint8_t*& p,
+                        const WindowRec* d_wins, const std::vector<WindowRec>& wins,
                         FusedPlan::Tiles* out) {
-  const Geo geo = shape_geo(shape);
-  std::vector<UnitDesc> units = job.units;
+  const Geo geo = shape_geo(ls.shape);
+  const size_t nu = job.units.size();
+  std::vector<UnitDesc> units;
+  if (ls.split) {
+    // unit 2u = chain 0 of pair u (writes the streams), 2u+1 = chain 1
+    for (size_t u = 0; u < nu; u++)
+      for (int ch = 0; ch < 2; ch++) {
+        UnitDesc v = job.units[u];
+        for (int k = 0; k < 3; k++) {
+          v.plane[0][k] = job.units[u].plane[ch][k];
+          v.plane_b[0][k] = job.units[u].plane_b[ch][k];
+          v.plane[1][k] = v.plane_b[1][k] = nullptr;
+        }
+        v.no_state_write = ch;
+        units.push_back(v);
+      }
+  } else {
+    units = job.units;
+  }
   std::vector<WorkItem> work;
   for (size_t u = 0; u < units.size(); u++) {
     int tr, tc;
@@ -485,10 +563,23 @@ static int upload_tiles(rt_ctx* ctx, const FusedJob& job, int shape, uint8_t*& p
                                  hipMemcpyHostToDevice, ctx->stream));
   HIPCHK_RET(ctx, hipMemcpyAsync(d_work, work.data(), work.size() * sizeof(WorkItem),
                                  hipMemcpyHostToDevice, ctx->stream));
+  std::vector<WindowRec> w2;
+  if (ls.split) {
+    const size_t L = job.nlaunch;
+    w2.resize(L * 2 * nu);
+    for (size_t l = 0; l < L; l++)
+      for (size_t u = 0; u < nu; u++) w2[(l * nu + u) * 2] = w2[(l * nu + u) * 2 + 1] = wins[l * nu + u];
+    WindowRec* d_w2 = (WindowRec*)p;
+    p += (w2.size() * sizeof(WindowRec) + 255) & ~(size_t)255;
+    HIPCHK_RET(ctx, hipMemcpyAsync(d_w2, w2.data(), w2.size() * sizeof(WindowRec),
+                                   hipMemcpyHostToDevice, ctx->stream));
+    d_wins = d_w2;
+  }
   // Pageable host sources: the copies may still read them after the calls
   // return, so wait before the vectors go out of scope.
   HIPCHK_RET(ctx, hipStreamSynchronize(ctx->stream));
-  *out = FusedPlan::Tiles{d_units, d_work, (unsigned)work.size(), shape};
+  *out = FusedPlan::Tiles{d_units, d_work, (unsigned)work.size(), ls.shape, ls.split, d_wins,
+                          units.size()};
   return RT_OK;
 }
 
@@ -504,18 +595,40 @@ int fused_upload(rt_ctx* ctx, FusedJob& job, void* scratch, FusedPlan* plan) {
   plan->total_steps = T;
   for (const UnitDesc& u : job.units) plan->streams += u.nstream;
   if (nu == 0 || T == 0) return RT_OK;
-  // Device layout of the scratch block: ops | seeds | (units | work) per shape.
+  // Device layout of the scratch block: ops | window records | second
+  // buffer sets | copy-back list | (units | work) per tile shape.
   uint8_t* p = (uint8_t*)scratch;
   uint8_t* d_ops = p;
   p += (nu * (size_t)T + 255) & ~(size_t)255;
-  uint32_t* d_seeds = (uint32_t*)p;
-  const size_t nseg = job.seg_steps.size();
-  p += (nu * nseg * sizeof(uint32_t) + 255) & ~(size_t)255;
-  std::vector<uint32_t> seeds(nu * nseg);
+  WindowRec* d_wins = (WindowRec*)p;
+  const size_t L = job.nlaunch;
+  p += (L * nu * sizeof(WindowRec) + 255) & ~(size_t)255;
+  const int ncol = job.model == M_DOMINO ? 2 : 3;
+  std::vector<CopyDesc> copies;
   for (size_t u = 0; u < nu; u++) {
-    for (size_t k = 0; k < nseg; k++) seeds[u * nseg + k] = job.seg_seeds[u][k];
+    UnitDesc& U = job.units[u];
+    const size_t pb = plane_bytes(job, U);
+    for (int ch = 0; ch < job.nc; ch++)
+      for (int k = 0; k < ncol; k++) {
+        U.plane_b[ch][k] = p;
+        p += (pb + 255) & ~(size_t)255;
+      }
+    U.state_b = (uint32_t*)p;
+    p += (16 * (size_t)U.nstream + 255) & ~(size_t)255;
+    if (job.windows[u].size() % 2) {  // the last window writes the second set
+      for (int ch = 0; ch < job.nc; ch++)
+        for (int k = 0; k < ncol; k++)
+          copies.push_back(CopyDesc{U.plane_b[ch][k], U.plane[ch][k], (long)pb});
+      copies.push_back(CopyDesc{U.state_b, U.state, 16 * U.nstream});
+    }
+  }
+  CopyDesc* d_copies = (CopyDesc*)p;
+  p += ((nu * 7 * sizeof(CopyDesc)) + 255) & ~(size_t)255;
+  std::vector<WindowRec> wins(L * nu, WindowRec{0, 0, 0, 0, 0});
+  plan->launch_sites.assign(L, 0);
+  plan->launch_work.assign(L, 0);
+  for (size_t u = 0; u < nu; u++) {
     job.units[u].ops = d_ops + u * (size_t)T;
-    job.units[u].seeds = d_seeds + u * nseg;
     job.units[u].rule = job.mp.rule;
     job.units[u].dw = job.mp.dw;
     const std::vector<uint8_t>& o = job.ops[u];
@@ -525,94 +638,107 @@ int fused_upload(rt_ctx* ctx, FusedJob& job, void* scratch, FusedPlan* plan) {
     }
     HIPCHK_RET(ctx, hipMemcpyAsync(d_ops + u * (size_t)T, o.data(), T, hipMemcpyHostToDevice,
                                    ctx->stream));
+    const std::vector<FusedWindow>& win = job.windows[u];
+    for (size_t l = 0; l < win.size(); l++) {
+      wins[l * nu + u] = WindowRec{win[l].s0, win[l].K, win[l].init,
+                                   job.seg_seeds[u][win[l].seg], (int)(l & 1)};
+      plan->launch_sites[l] += job.units[u].nstream;
+      plan->launch_work[l] += (long)win[l].K * job.units[u].nstream;
+    }
   }
-  HIPCHK_RET(ctx, hipMemcpyAsync(d_seeds, seeds.data(), seeds.size() * 4, hipMemcpyHostToDevice,
-                                 ctx->stream));
+  HIPCHK_RET(ctx, hipMemcpyAsync(d_wins, wins.data(), wins.size() * sizeof(WindowRec),
+                                 hipMemcpyHostToDevice, ctx->stream));
+  if (!copies.empty())
+    HIPCHK_RET(ctx, hipMemcpyAsync(d_copies, copies.data(), copies.size() * sizeof(CopyDesc),
+                                   hipMemcpyHostToDevice, ctx->stream));
+  plan->d_copies = d_copies;
+  plan->ncopies = (unsigned)copies.size();
   FusedPlan::Tiles t;
-  int rc = upload_tiles(ctx, job, job.shape, p, &t);
+  const LaunchShape first{job.shape, job.split};
+  int rc = upload_tiles(ctx, job, first, p, d_wins, wins, &t);  // waits for the copies above
   if (rc) return rc;
   plan->cand.push_back(t);
   if (job.tune)
-    for (int sh : kTuneShapes) {
-      if (sh == job.shape) continue;
-      rc = upload_tiles(ctx, job, sh, p, &t);
+    for (int c = 0; c < tune_count(job); c++) {
+      const LaunchShape ls = kTuneShapes[c];
+      if (ls.shape == first.shape && ls.split == first.split) continue;
+      rc = upload_tiles(ctx, job, ls, p, d_wins, wins, &t);
       if (rc) return rc;
       plan->cand.push_back(t);
     }
   plan->d_units = plan->cand[0].d_units;
   plan->d_work = plan->cand[0].d_work;
   plan->nwork = plan->cand[0].nwork;
-  plan->windows = job.windows;
+  plan->d_wins = plan->cand[0].d_wins;
+  plan->nlaunch = L;
   return RT_OK;
 }
 
-static void count_windows(rt_ctx* ctx, const FusedPlan& plan, size_t w0, size_t w1) {
-  long steps = 0;
-  for (size_t w = w0; w < w1; w++) steps += plan.windows[w].K;
-  ctx->last.launches += (long)(w1 - w0);
-  ctx->last.steps += steps;
-  ctx->last.attempts += steps * plan.streams;
+static void count_launches(rt_ctx* ctx, const FusedPlan& plan, size_t l0, size_t l1) {
+  long work = 0;
+  for (size_t l = l0; l < l1; l++) work += plan.launch_work[l];
+  ctx->last.launches += (long)(l1 - l0);
+  ctx->last.attempts += work;
+  if (l1 == plan.nlaunch) ctx->last.steps += plan.total_steps;
 }
 
 int fused_tune(rt_ctx* ctx, FusedJob& job, FusedPlan* plan) {
   const size_t nc = plan->cand.size();
   const size_t need = 1 + (size_t)kTrialReps * nc * kTrialWindows;
-  if (!job.tune || nc < 2 || plan->windows.size() < need) return RT_OK;
+  if (!job.tune || nc < 2 || plan->nlaunch < need) return RT_OK;
   const int H = shape_geo(job.shape).H;
-  // Window 0 (TinyMT seeding) under the rule-of-thumb shape, then
-  // kTrialReps rounds of kTrialWindows windows under each candidate; event
-  // k + 1 closes launch group k.
+  // Launch 0 (TinyMT seeding) under the rule-of-thumb shape, then
+  // kTrialReps rounds of kTrialWindows launches under each candidate;
+  // event k + 1 closes launch group k.
   struct Trial {
     int cand;
-    size_t w0, w1;
+    size_t l0, l1;
   };
   std::vector<Trial> trials{{0, 0, 1}};
-  size_t w = 1;
+  size_t l = 1;
   for (int r = 0; r < kTrialReps; r++)
-    for (size_t c = 0; c < nc; c++, w += kTrialWindows)
-      trials.push_back(Trial{(int)c, w, w + kTrialWindows});
+    for (size_t c = 0; c < nc; c++, l += kTrialWindows)
+      trials.push_back(Trial{(int)c, l, l + kTrialWindows});
   if (trials.size() + 1 > sizeof(ctx->evt) / sizeof(ctx->evt[0])) return RT_OK;
   HIPCHK_RET(ctx, hipEventRecord(ctx->evt[0], ctx->stream));
   for (size_t k = 0; k < trials.size(); k++) {
     const FusedPlan::Tiles& t = plan->cand[trials[k].cand];
-    LaunchFn fn = pick(ctx, job.model, job.nc, t.shape);
-    for (size_t x = trials[k].w0; x < trials[k].w1; x++) {
-      const FusedWindow& win = plan->windows[x];
-      fn(ctx, t.nwork, t.d_units, t.d_work, win.s0, win.K, win.init, win.seg, H, job.mp, nullptr);
-    }
+    LaunchFn fn = pick(ctx, job.model, t.split ? 1 : job.nc, t.shape);
+    for (size_t x = trials[k].l0; x < trials[k].l1; x++)
+      fn(ctx, t.nwork, t.d_units, t.d_work, H, job.mp, t.d_wins + x * t.nunits);
     HIPCHK_RET(ctx, hipGetLastError());
     HIPCHK_RET(ctx, hipEventRecord(ctx->evt[k + 1], ctx->stream));
   }
   HIPCHK_RET(ctx, hipEventSynchronize(ctx->evt[trials.size()]));
-  std::vector<double> ms_per_step(nc, 0.0), ms(nc, 0.0);
-  std::vector<long> steps(nc, 0);
+  std::vector<double> ms(nc, 0.0), work(nc, 0.0);
   for (size_t k = 0; k < trials.size(); k++) {
     float e = 0;
     HIPCHK_RET(ctx, hipEventElapsedTime(&e, ctx->evt[k], ctx->evt[k + 1]));
     const int c = trials[k].cand, sh = plan->cand[c].shape;
-    long K = 0;
-    for (size_t x = trials[k].w0; x < trials[k].w1; x++) K += plan->windows[x].K;
     plan->trial_ms[sh - 1] += e;
-    plan->trial_launches[sh - 1] += (long)(trials[k].w1 - trials[k].w0);
-    plan->trial_sites[sh - 1] += (long)(trials[k].w1 - trials[k].w0) * plan->streams;
-    if (k == 0) continue;  // the seeding window is not a fair trial
+    plan->trial_launches[sh - 1] += (long)(trials[k].l1 - trials[k].l0);
+    plan->trial_sites[sh - 1] += plan->sites(trials[k].l0, trials[k].l1);
+    if (k == 0) continue;  // the seeding launch is not a fair trial
     ms[c] += e;
-    steps[c] += K;
+    for (size_t x = trials[k].l0; x < trials[k].l1; x++) work[c] += (double)plan->launch_work[x];
   }
   size_t best = 0;
+  std::vector<double> cost(nc);
   for (size_t c = 0; c < nc; c++) {
-    ms_per_step[c] = steps[c] ? ms[c] / (double)steps[c] : 1e30;
-    if (ms_per_step[c] < ms_per_step[best]) best = c;
+    cost[c] = work[c] > 0 ? ms[c] / work[c] : 1e30;
+    if (cost[c] < cost[best]) best = c;
   }
-  count_windows(ctx, *plan, 0, w);
-  plan->first = w;
+  count_launches(ctx, *plan, 0, l);
+  plan->first = l;
   job.shape = plan->cand[best].shape;
+  job.split = plan->cand[best].split;
   job.shape_source = 1;
   std::swap(plan->cand[0], plan->cand[best]);
   plan->d_units = plan->cand[0].d_units;
   plan->d_work = plan->cand[0].d_work;
   plan->nwork = plan->cand[0].nwork;
-  ctx->tuned[tune_key(job, tiles64(job))] = job.shape;
+  plan->d_wins = plan->cand[0].d_wins;
+  ctx->tuned[tune_key(job, tiles64(job))] = job.shape | (job.split ? 16 : 0);
   return RT_OK;
 }
 
@@ -630,16 +756,18 @@ void fused_add_trials(const FusedPlan& plan, rt_timing* acc) {
 int fused_launch(rt_ctx* ctx, const FusedJob& job, const FusedPlan& plan) {
   ctx->last.shape = job.shape;
   ctx->last.shape_source = job.shape_source;
-  if (plan.first >= plan.windows.size()) return RT_OK;
-  LaunchFn fn = pick(ctx, job.model, job.nc, job.shape);
-  const int H = shape_geo(job.shape).H;
-  for (size_t w = plan.first; w < plan.windows.size(); w++) {
-    const FusedWindow& win = plan.windows[w];
-    fn(ctx, plan.nwork, plan.d_units, plan.d_work, win.s0, win.K, win.init, win.seg, H, job.mp,
-       nullptr);
-  }
+  ctx->last.split = job.split ? 1 : 0;
+  if (plan.cand.empty()) return RT_OK;  // nothing to run
+  const FusedPlan::Tiles& t = plan.cand[0];
+  LaunchFn fn = pick(ctx, job.model, t.split ? 1 : job.nc, t.shape);
+  const int H = shape_geo(t.shape).H;
+  for (size_t l = plan.first; l < plan.nlaunch; l++)
+    fn(ctx, t.nwork, t.d_units, t.d_work, H, job.mp, t.d_wins + l * t.nunits);
+  if (plan.ncopies)
+    hipLaunchKernelGGL(k_copy_list, dim3(64, plan.ncopies), dim3(256), 0, ctx->stream,
+                       plan.d_copies);
   HIPCHK_RET(ctx, hipGetLastError());
-  count_windows(ctx, plan, plan.first, plan.windows.size());
+  count_launches(ctx, plan, plan.first, plan.nlaunch);
   return RT_OK;
 }
 
@@ -650,8 +778,7 @@ int fused_launch_windows(rt_ctx* ctx, int model, int nc, const ModelParams& mp,
   // 64-row tiles (fused_tiles); 16 waves when there are no more tiles than CUs.
   const int shape = model == M_RECTTRI ? (nc == 2 ? SH_RT8x4 : SH_8x8)
                     : nwork <= (unsigned)ctx->num_cu ? SH_16x4 : SH_8x8;
-  pick(ctx, model, nc, shape)(ctx, nwork, d_units, d_work, 0, 0, 0, 0, shape_geo(shape).H, mp,
-                              d_wins);
+  pick(ctx, model, nc, shape)(ctx, nwork, d_units, d_work, shape_geo(shape).H, mp, d_wins);
   HIPCHK_RET(ctx, hipGetLastError());
   ctx->last.launches += 1;
   return RT_OK;

@@ -33,30 +33,57 @@ enum ModelKind { M_DOMINO = 0, M_LOZENGE = 1, M_RECTTRI = 2 };
 //   bits 4-5 rect-tri: previous step's colour.
 enum : uint8_t { OP_COLOUR = 3u, OP_PULL_A = 4u, OP_PULL_B = 8u };
 
+// Cells and TinyMT states are double-buffered: a window reads set `plane`
+// / `state` and writes set `plane_b` / `state_b`, or the reverse when its
+// parity bit is set.  A workgroup reads its halo from neighbouring tiles,
+// and with more tiles than fit on the chip at once some workgroups start
+// after a neighbour has finished its window, so one buffer updated in place
+// would hand them that neighbour's new cells.
 struct UnitDesc {
   void* plane[2][3];       // [chain][colour] -> rows x cols cells (uint8 or int32)
+  void* plane_b[2][3];     // second buffer set
   uint32_t* state;         // SoA s0..s3, stride = nstream
+  uint32_t* state_b;       // second buffer set
+  int no_state_write;      // the unit's streams are written by another unit
   const uint32_t* param;   // SoA mat1, mat2, tmat, stride = pstride
   long nstream;
   long pstride;
   const uint8_t* ops;      // op byte per global step of this unit
-  const uint32_t* seeds;   // TinyMT seed per segment
   int rows, cols;          // compressed grid
   int tiles_c;             // tile columns
   int rule;                // domino flip rule of this unit
   DominoWeights dw;        // domino two-periodic weights of this unit
 };
 
-// Per-unit window for launches where every unit is at its own step (the
-// serving loop): ops are read from unit.ops[s0 .. s0+K); init re-seeds the
-// unit's TinyMT streams with `seed` before the first step.
+// Per-unit window of one launch: ops are read from unit.ops[s0 .. s0+K);
+// init re-seeds the unit's TinyMT streams with `seed` before the first
+// step; flags bit 0 is the window's buffer parity (set: read the _b
+// buffers, write the others).  K = 0 without init leaves the unit idle.
 struct WindowRec {
   long s0;
   int K;
   int init;
   uint32_t seed;
-  int pad;
+  int flags;
 };
+
+// A window's source buffers in plane / state, its destination in the _b
+// fields (device side, on a private copy of the unit).
+__device__ __forceinline__ void window_buffers(UnitDesc& U, int parity) {
+  if (parity & 1) {
+#pragma unroll
+    for (int ch = 0; ch < 2; ch++)
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        void* t = U.plane[ch][k];
+        U.plane[ch][k] = U.plane_b[ch][k];
+        U.plane_b[ch][k] = t;
+      }
+    uint32_t* t = U.state;
+    U.state = U.state_b;
+    U.state_b = t;
+  }
+}
 
 struct WorkItem {
   int unit;
@@ -100,9 +127,15 @@ struct FusedJob {
   std::vector<OpBuilder> hist;        // [unit]
   bool adaptive = true;               // single unit: stretch windows to H pulls
   int shape = 0;                      // tile shape, resolved by fused_scratch_bytes
+  bool split = false;                 // pairs run as two one-chain units (see below)
   int shape_source = 0;               // 0 rule of thumb / forced, 1 tuned now, 2 tuned before
-  bool tune = false;                  // time the candidate shapes on the first windows
-  std::vector<FusedWindow> windows;   // launch windows, cut by fused_scratch_bytes
+  bool tune = false;                  // time the candidate shapes on the first launches
+  // Launch windows per unit, cut by fused_scratch_bytes from the unit's own
+  // pull steps: launch l advances unit u by windows[u][l] (nothing once
+  // the unit's windows run out), so units with independent schedules do
+  // not shorten each other's windows.
+  std::vector<std::vector<FusedWindow>> windows;
+  size_t nlaunch = 0;                 // longest window list
 };
 
 // Run all windows of the job on ctx->stream (no final sync).  `scratch`
@@ -116,27 +149,44 @@ int fused_run(rt_ctx* ctx, FusedJob& job, void* scratch);
 
 // The same in phases, so the launch sequence can be captured in a
 // hipGraph: fused_upload copies unit tables, work lists, ops and seeds to
-// the device; fused_tune (a no-op unless job.tune) runs the first windows
+// the device; fused_tune (a no-op unless job.tune) runs the first launches
 // split among the candidate tile shapes, timed, keeps the fastest for the
 // rest of the job and for later jobs like it; fused_launch enqueues only
-// kernels, from window plan.first on.  Results never depend on the shape.
+// kernels, from launch plan.first on.  Results never depend on the shape.
 struct FusedPlan {
+  // One way to launch the job: a tile shape, and for chain pairs whether
+  // the two chains run in one workgroup per tile (sharing each draw) or in
+  // two ("split": two one-chain units per pair reading the same TinyMT
+  // states, the first of them writing them back), which doubles the
+  // workgroups when the chip has idle CUs.
   struct Tiles {
     const UnitDesc* d_units;
     const WorkItem* d_work;
     unsigned nwork;
     int shape;
+    bool split;
+    const WindowRec* d_wins;  // [launch][unit of this launch shape]
+    size_t nunits;
   };
   const UnitDesc* d_units = nullptr;  // the tiles of job.shape
   const WorkItem* d_work = nullptr;
   unsigned nwork = 0;
   std::vector<Tiles> cand;            // every uploaded shape, job.shape first
-  typedef FusedWindow Window;
-  std::vector<Window> windows;
-  size_t first = 0;                   // windows before this one ran in fused_tune
-  long total_steps = 0;
-  long streams = 0;
-  // windows run by fused_tune, by tile shape (index shape - 1)
+  const WindowRec* d_wins = nullptr;  // [launch][unit]
+  const struct CopyDesc* d_copies = nullptr;  // second set -> first, after the last launch
+  unsigned ncopies = 0;
+  size_t nlaunch = 0;
+  size_t first = 0;                   // launches before this one ran in fused_tune
+  long total_steps = 0;               // steps per chain
+  long streams = 0;                   // TinyMT sites over all units
+  std::vector<long> launch_sites;     // per launch: sites of the units it advances
+  std::vector<long> launch_work;      // per launch: sum over units of K x sites
+  long sites(size_t l0, size_t l1) const {
+    long n = 0;
+    for (size_t l = l0; l < l1 && l < launch_sites.size(); l++) n += launch_sites[l];
+    return n;
+  }
+  // launches run by fused_tune, by tile shape (index shape - 1)
   double trial_ms[4] = {0, 0, 0, 0};
   long trial_launches[4] = {0, 0, 0, 0};
   long trial_sites[4] = {0, 0, 0, 0};
@@ -144,7 +194,7 @@ struct FusedPlan {
 int fused_upload(rt_ctx* ctx, FusedJob& job, void* scratch, FusedPlan* plan);
 int fused_tune(rt_ctx* ctx, FusedJob& job, FusedPlan* plan);
 int fused_launch(rt_ctx* ctx, const FusedJob& job, const FusedPlan& plan);
-// Add the plan's tuning windows to per-shape kernel timing.
+// Add the plan's tuning launches to per-shape kernel timing.
 void fused_add_trials(const FusedPlan& plan, rt_timing* acc);
 
 // One launch over `nwork` work items where each unit follows its own

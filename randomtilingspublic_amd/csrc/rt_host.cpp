@@ -909,6 +909,13 @@ rt_vec* rt_bibone_ice_cream_cone(int N, int M) {
 
 rt_vec* rt_bibone_test_butterfly(int type) { return make_vec(bibone_test_butterfly(type)); }
 
+rt_vec* rt_bibone_hexagon(int a, int b, int c) {
+  if (a < 0 || b < 0 || c < 0 || (long)a + b + c > 20000) return nullptr;
+  tiling t = bibone_hexagon(a, b, c);
+  if (t.empty()) return nullptr;
+  return make_vec(std::move(t));
+}
+
 #define RT_BIBONE_STATIC(name, fn)                          \
   rt_vec* name(const int32_t* t, size_t n) {                \
     int N;                                                  \

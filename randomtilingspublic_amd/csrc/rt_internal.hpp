@@ -58,6 +58,7 @@ tiling recttri_slope_hex(int n);
 // Bibone statics (rt_bibone_host.cpp); N is the tiling side.
 tiling bibone_ice_cream_cone(int N, int M);
 tiling bibone_test_butterfly(int type);
+tiling bibone_hexagon(int a, int b, int c);
 std::vector<int> bibone_triangle_flips(const tiling& t, int N);
 std::vector<int> bibone_butterfly_flips(const tiling& t, int N);
 std::vector<int> bibone_tiling_to_vertices(const tiling& t, int N);

@@ -68,7 +68,10 @@ struct Slot {
   int id = -1;  // flag index
   size_t off = 0, bytes = 0;
   uint8_t *init_top = nullptr, *init_bot = nullptr, *top = nullptr, *bot = nullptr;
+  uint8_t *top_b = nullptr, *bot_b = nullptr;  // second buffer set of the sweep
   uint32_t* state = nullptr;
+  uint32_t* state_b = nullptr;
+  int par = 0;         // buffer set holding the current cells and states
   int ncol = 0;
   long plane = 0, cols = 0, nstream = 0;
   std::mt19937 master;
@@ -146,7 +149,7 @@ size_t footprint(int model, int N, int* ncol, long* plane, long* nstream, long* 
   }
   *plane = (long)N * *cols;
   const size_t pb = round_up((size_t)*ncol * (size_t)*plane, 256);
-  return round_up(16 * (size_t)*nstream, 256) + 4 * pb;
+  return 2 * round_up(16 * (size_t)*nstream, 256) + 6 * pb;
 }
 
 }  // namespace
@@ -243,11 +246,15 @@ static int admit(Server* S, Req* r) {
   uint8_t* base = S->d_arena + off;
   s->state = (uint32_t*)base;
   base += round_up(16 * (size_t)nstream, 256);
+  s->state_b = (uint32_t*)base;
+  base += round_up(16 * (size_t)nstream, 256);
   const size_t pb = round_up((size_t)ncol * (size_t)plane, 256);
   s->init_top = base;
   s->init_bot = base + pb;
   s->top = base + 2 * pb;
   s->bot = base + 3 * pb;
+  s->top_b = base + 4 * pb;
+  s->bot_b = base + 5 * pb;
   std::vector<uint8_t> h((size_t)ncol * (size_t)plane);
   const int N = r->N;
   for (int which = 0; which < 2; which++) {
@@ -314,8 +321,11 @@ static int upload_tables(Server* S) {
       for (int c = 0; c < s->ncol; c++) {
         d.plane[0][c] = s->top + (size_t)c * s->plane;
         d.plane[1][c] = s->bot + (size_t)c * s->plane;
+        d.plane_b[0][c] = s->top_b + (size_t)c * s->plane;
+        d.plane_b[1][c] = s->bot_b + (size_t)c * s->plane;
       }
       d.state = s->state;
+      d.state_b = s->state_b;
       d.param = S->ctx->d_param;
       d.nstream = s->nstream;
       d.pstride = (long)S->ctx->nparam;
@@ -428,6 +438,7 @@ static int run_tick(Server* S) {
               hipSuccess)
         return fail(S, "plane reset", RT_EDEVICE);
       s->fresh = false;
+      s->par = 0;
     }
   // windows: kSub per tick for every live unit
   std::vector<Slot*> ended;
@@ -455,6 +466,8 @@ static int run_tick(Server* S) {
         w.K = K;
         w.init = s->cursor == s->seg_start[seg] ? 1 : 0;
         w.seed = (uint32_t)s->seeds[seg];
+        w.flags = s->par;
+        s->par ^= 1;
         s->cursor += K;
         if (s->cursor >= s->total) ended.push_back(s);
       }
@@ -476,8 +489,17 @@ static int run_tick(Server* S) {
                                     S->d_wins + wins_at(b, k, m));
       if (rc) return fail(S, ctx->err, rc);
     }
-  // slots whose round ended: lazy sync of their planes, then coalescence flag
+  // slots whose round ended: cells back into the first buffer set, lazy
+  // sync of their planes, then coalescence flag
   for (Slot* s : ended) {
+    if (s->par) {
+      const size_t pb = (size_t)s->ncol * (size_t)s->plane;
+      if (hipMemcpyAsync(s->top, s->top_b, pb, hipMemcpyDeviceToDevice, ctx->stream) !=
+              hipSuccess ||
+          hipMemcpyAsync(s->bot, s->bot_b, pb, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess)
+        return fail(S, "plane copy-back", RT_EDEVICE);
+      s->par = 0;
+    }
     FusedJob one;
     one.model = s->req->model;
     one.nc = 2;

@@ -100,11 +100,11 @@ __device__ __forceinline__ void swar_step(uint32_t* lds, const SwarRow* row, Tin
 
 // Tile of 4*W*R rows x 64 columns; wave wv, row slot r, lane quarter q
 // (lane >> 4) own tile row 4*(wv + W*r) + q, word w = lane & 15 (cells
-// 4w .. 4w+3).  Kernel arguments as k_fused.
+// 4w .. 4w+3).  Kernel arguments and buffer sets as k_fused.
 template <int NC, int W, int R>
 __global__ __launch_bounds__(64 * W, W >= 16 ? 1 : 2) void k_swar_lozenge(
-    const UnitDesc* __restrict__ units, const WorkItem* __restrict__ work, long s0, int K,
-    int init, int seg, int H, ModelParams mp, const WindowRec* __restrict__ wins) {
+    const UnitDesc* __restrict__ units, const WorkItem* __restrict__ work, int H, ModelParams mp,
+    const WindowRec* __restrict__ wins) {
   constexpr int kThreads = 64 * W;
   constexpr int LH = 4 * W * R;
   constexpr int PLW = (LH + 2) * 16;  // words per plane, with a zero margin row each side
@@ -112,18 +112,13 @@ __global__ __launch_bounds__(64 * W, W >= 16 ? 1 : 2) void k_swar_lozenge(
   __shared__ uint8_t sops[kKMax];
 
   const WorkItem wi = work[blockIdx.x];
-  const UnitDesc U = units[wi.unit];
-  uint32_t seed = 0;
-  if (wins) {
-    const WindowRec w = wins[wi.unit];
-    s0 = w.s0;
-    K = w.K;
-    init = w.init;
-    seed = w.seed;
-    if (K <= 0 && !init) return;
-  } else if (init) {
-    seed = U.seeds[seg];
-  }
+  const WindowRec wr = wins[wi.unit];
+  const long s0 = wr.s0;
+  const int K = wr.K, init = wr.init;
+  const uint32_t seed = wr.seed;
+  if (K <= 0 && !init) return;  // unit idle in this launch
+  UnitDesc U = units[wi.unit];
+  window_buffers(U, wr.flags);
   const int rows = U.rows, cols = U.cols;
   const int TH = LH - 2 * H, TW = kLW - 2 * H;
   const int tr = wi.tile / U.tiles_c, tc = wi.tile % U.tiles_c;
@@ -218,11 +213,11 @@ __global__ __launch_bounds__(64 * W, W >= 16 ? 1 : 2) void k_swar_lozenge(
       for (int ch = 0; ch < NC; ch++)
 #pragma unroll
         for (int pl = 0; pl < 3; pl++)
-          ((uint8_t*)U.plane[ch][pl])[cidx] = (uint8_t)(words[ch][pl] >> (8 * k));
-      U.state[cidx] = st[r][k].s0;
-      U.state[U.nstream + cidx] = st[r][k].s1;
-      U.state[2 * U.nstream + cidx] = st[r][k].s2;
-      U.state[3 * U.nstream + cidx] = st[r][k].s3;
+          ((uint8_t*)U.plane_b[ch][pl])[cidx] = (uint8_t)(words[ch][pl] >> (8 * k));
+      U.state_b[cidx] = st[r][k].s0;
+      U.state_b[U.nstream + cidx] = st[r][k].s1;
+      U.state_b[2 * U.nstream + cidx] = st[r][k].s2;
+      U.state_b[3 * U.nstream + cidx] = st[r][k].s3;
     }
   }
 }

@@ -100,7 +100,7 @@ def test_tile_shape_choice_never_changes_results(gpu_ctx):
     masters = [3, 8, 13]
     ref = None
     try:
-        for mode, shape in [(0, 0), (2, 0), (1, 0), (0, 1), (0, 2), (0, 3)]:
+        for mode, shape in [(0, 0), (2, 0), (1, 0), (0, 1), (0, 2), (0, 3), (0, 4), (0, 5), (0, 6)]:
             _say("lozenge cftp batch autotune", mode, "shape", shape)
             gpu_ctx.set_autotune(mode)
             gpu_ctx.set_tile_shape(shape)
@@ -108,7 +108,8 @@ def test_tile_shape_choice_never_changes_results(gpu_ctx):
                                                      30)
             tm = gpu_ctx.last_timing()
             if shape:
-                assert tm["shape"] == shape and tm["shape_source"] == 0
+                assert tm["shape"] == (shape - 3 if shape > 3 else shape)
+                assert tm["shape_source"] == 0 and tm["split"] == (shape > 3)
             if mode == 2:
                 assert tm["shape_source"] == 1, tm
             if ref is None:
@@ -121,4 +122,53 @@ def test_tile_shape_choice_never_changes_results(gpu_ctx):
                     assert np.array_equal(a, b)
     finally:
         gpu_ctx.set_autotune(1)
+        gpu_ctx.set_tile_shape(0)
+
+
+def test_cftp_pair_with_equal_chains_runs_one_chain_exactly(gpu_ctx):
+    # top == bottom from the start: every tile of every window takes the
+    # one-chain path of the pair kernels; the sample is the plain walk's.
+    d = rt.LozengeTiler.AlmostHexDomain(10, 2)
+    tmin = rt.LozengeTiler.MinTiling(d)
+    N = rt.side(tmin)
+    P = rt.synthetic_params(N * N, 26)
+    gpu_ctx.set_tinymt_params(P)
+    try:
+        for shape in (1, 2, 3, 6):
+            _say("lozenge cftp equal chains shape", shape)
+            gpu_ctx.set_tile_shape(shape)
+            outs, stats = gpu_ctx.lozenge_cftp_batch(tmin, tmin, [5, 6], rt.SCHED_DOUBLING, 300, 4)
+            for k, m in enumerate((5, 6)):
+                want, ws = o.lozenge_cftp(tmin, tmin, m, P, rt.SCHED_DOUBLING, 300, 4)
+                assert stats[k]["rounds"] == ws["rounds"] == 1
+                assert np.array_equal(outs[k], want)
+    finally:
+        gpu_ctx.set_tile_shape(0)
+
+
+def test_walk_batch_with_more_tiles_than_fit_on_the_chip(gpu_ctx):
+    # 96 Aztec diamonds of order 100 = 768 tiles of 64 rows: many workgroups
+    # start after a neighbouring tile's workgroup has finished its window.
+    # Each chain must still equal its lone walk (8 tiles, all resident) and
+    # the oracle.
+    d = rt.DominoTiler.AztecDiamond(100)
+    t = rt.DominoTiler.MinTiling(d)
+    N = rt.side(t)
+    P = rt.synthetic_params(N * N // 2, 27)
+    gpu_ctx.set_tinymt_params(P)
+    n = 96
+    seeds = [1000 + 17 * k for k in range(n)]
+    try:
+        for shape in (1, 2):
+            gpu_ctx.set_tile_shape(shape)
+            _say("walk batch 96 x Aztec 100 shape", shape)
+            outs = gpu_ctx.domino_walk_batch([t] * n, 600, seeds, rt.DOMINO_HEATBATH,
+                                             use_graph=True)
+            for k in range(0, n, 7):
+                single = gpu_ctx.domino_walk(t, 600, seeds[k], rt.DOMINO_HEATBATH)
+                assert np.array_equal(outs[k], single), k
+            for k in (0, n - 1):
+                want = o.domino_walk(t, 600, seeds[k], P, rt.DOMINO_HEATBATH)
+                assert np.array_equal(outs[k], want), k
+    finally:
         gpu_ctx.set_tile_shape(0)

@@ -56,3 +56,15 @@ def test_bibone_walk_rejects_bad_input(gpu_ctx):
     with pytest.raises(rt.RTError) as e:
         gpu_ctx.bibone_walk(T.IceCreamCone(6, 0), 1, 1)  # 16 < 10*10 parameter sets
     assert e.value.code == -2
+
+
+@pytest.mark.parametrize("a,b,c,steps,seed", [(5, 5, 6, 700, 3), (20, 20, 21, 1500, 8)])
+def test_bibone_walk_from_hexagon_matches_oracle(gpu_ctx, a, b, c, steps, seed):
+    t = T.Hexagon(a, b, c)
+    N = rt.side(t)
+    P = rt.synthetic_params((N // 2) ** 2, seed=seed + 5)
+    gpu_ctx.set_tinymt_params(P)
+    _say("bibone walk hexagon", a, b, c, steps, seed)
+    got = gpu_ctx.bibone_walk(t, steps, seed)
+    want = o.bibone_walk(t, steps, seed, P)
+    assert np.array_equal(got, want)

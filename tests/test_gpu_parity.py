@@ -207,7 +207,7 @@ def test_engines_agree_with_oracle(gpu_ctx, engine):
         gpu_ctx.set_engine(rt.ENGINE_AUTO)
 
 
-@pytest.mark.parametrize("shape", [1, 2, 3])
+@pytest.mark.parametrize("shape", [1, 2, 3, 4, 5, 6])
 def test_tile_shapes_agree_with_oracle(gpu_ctx, shape):
     """Every sweep-block tile shape (64 rows / 8 waves, 64 rows / 16 waves,
     32 rows / 16 waves) on domains spanning several tiles, for a walk of each

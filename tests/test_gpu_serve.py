@@ -90,7 +90,9 @@ def test_tight_budget_queues_and_cancel(gpu_ctx):
     # a budget that holds only one of these at a time
     tmax, _ = _domino(10)
     N = rt.side(tmax)
-    one = 16 * (N - 2) * (N // 2 - 2) + 4 * 2 * N * (N // 2) + 4096
+    # slot footprint: two TinyMT state sets, start + two sweep buffer sets of
+    # both chains' planes (256-byte aligned each)
+    one = 2 * (16 * (N - 2) * (N // 2 - 2) + 256) + 6 * (2 * N * (N // 2) + 256) + 4096
     _say("server tight budget", one)
     srv = rt.Server(gpu_ctx, one, 8)
     try:
