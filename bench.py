@@ -29,10 +29,18 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# Sweep-block kernel instantiation per tile shape (rt_timing.shape_*[k]):
-# k_fused<model 1 = lozenge, 2 chains, waves, rows per lane>.
+# Sweep-block kernel instantiation per launch-shape slot (rt_timing.shape_*[k]):
+# k_fused<model 1 = lozenge, chains per workgroup, waves, rows per lane>; slots
+# 4..6 run CFTP pairs split, one chain per workgroup.
 KERNEL_BY_SHAPE = ["k_fused<1, 2, 8, 8>", "k_fused<1, 2, 16, 4>", "k_fused<1, 2, 16, 2>",
-                   "k_fused<2, 2, 8, 4>"]
+                   "k_fused<2, 2, 8, 4>", "k_fused<1, 1, 8, 8>", "k_fused<1, 1, 16, 4>",
+                   "k_fused<1, 1, 16, 2>", None]
+# Algorithmic bytes per site per launch: a pair workgroup reads the TinyMT
+# state (16 B) and parameters (12 B) once, writes the state (16 B), reads and
+# writes 3 colour planes x 2 chains (12 B) = 56 B; split pairs read state and
+# parameters in both workgroups (2 x 28 B), write the state once (16 B) and
+# the planes as before (12 B) = 84 B.
+ALG_BYTES_BY_SHAPE = [56, 56, 56, 56, 84, 84, 84, 0]
 
 
 def cpu_model():
@@ -157,13 +165,16 @@ def batched_walks(rt, seed=4, nchains=256, steps=10 ** 4):
                          "note": "device time includes the final colour-plane sync"}}
 
 
-def weighted_and_bibone(rt, cftp_seeds=0, steps=10 ** 5):
+def weighted_and_bibone(rt, cftp_seeds=8, cftp_order=24, steps=10 ** 5):
     """Weighted 2-periodic domino walk on the Aztec diamond of order 512
     (a = 1.0, b = 0.5, the reference's Metropolis rule of
     dominokerneltwoperiodic.cl, 10^5 steps from the minimal tiling), the
     bibone walk on the hexagon with sides 200, 201, 200 (10^5 steps), and
-    optionally the two-periodic heat-bath CFTP of the same diamond for
-    master seeds 1..cftp_seeds, batched."""
+    the two-periodic heat-bath CFTP (dominokernelCFTPtwoperiodic.cl, same
+    weights) for master seeds 1..cftp_seeds, batched, on the Aztec diamond
+    of order `cftp_order`: that chain coalesces far more slowly than the
+    uniform one (order 24: 2^16 steps against 2^13; order 64: not within a
+    2^22-step look-back), so order 512 is out of reach of a CFTP run."""
     D, B = rt.DominoTiler, rt.TriangleDimerTiler
     out = {}
     d = D.AztecDiamond(512)
@@ -185,17 +196,18 @@ def weighted_and_bibone(rt, cftp_seeds=0, steps=10 ** 5):
             "flip_attempts_per_s": round(tm["attempts"] / (tm["device_ms"] / 1e3), 1),
             "launches": tm["launches"], "tile_shape": tm["shape"]}
         if cftp_seeds:
-            tmax = D.MaxTiling(d)
+            dc = D.AztecDiamond(cftp_order)
+            cmin, cmax = D.MinTiling(dc), D.MaxTiling(dc)
             w = time.time()
-            outs, stats = ctx.domino_cftp_batch(tmax, tmin, list(range(1, cftp_seeds + 1)),
-                                                rt.SCHED_DOUBLING, 1024, 30,
+            outs, stats = ctx.domino_cftp_batch(cmax, cmin, list(range(1, cftp_seeds + 1)),
+                                                rt.SCHED_DOUBLING, 1024, 16,
                                                 rt.DOMINO_HEATBATH2P, 1.0, 0.5)
             wall = time.time() - w
             out["weighted_cftp"] = {
                 "workload": f"two-periodic heat-bath CFTP (a=1.0, b=0.5) of the Aztec diamond "
-                            f"of order 512, doubling from 2^10, master seeds 1..{cftp_seeds}, "
-                            "batched",
-                "wall_s": round(wall, 2), "samples_per_s": round(cftp_seeds / wall, 5),
+                            f"of order {cftp_order}, doubling from 2^10, master seeds "
+                            f"1..{cftp_seeds}, batched",
+                "wall_s": round(wall, 2), "samples_per_s": round(cftp_seeds / wall, 4),
                 "rounds": [s["rounds"] for s in stats]}
         t = B.Hexagon(200, 200, 201)
         Nb = rt.side(t)
@@ -359,11 +371,17 @@ def profile_rounds(ctx, rt, tmax, tmin, masters, t0, rounds):
         pass  # no coalescence within the cap is expected here
     tm = ctx.last_timing()
     kl = max(tm["kernel_launches"], 1)
+    by_kernel = {}
+    for k, name in enumerate(KERNEL_BY_SHAPE):
+        n = tm["shape_launches"][k]
+        if name and n:
+            by_kernel[name] = {"launches": n, "avg_launch_us": round(tm["shape_ms"][k] * 1e3 / n, 2),
+                               "sites_per_launch": int(tm["shape_sites"][k] / n)}
     print(json.dumps({"profile_rounds": rounds, "samples": len(masters), "wall_s": time.time() - w,
                       "kernel_ms": tm["kernel_ms"], "kernel_launches": tm["kernel_launches"],
                       "avg_launch_us": round(tm["kernel_ms"] * 1e3 / kl, 2),
                       "sites_per_launch": int(tm["kernel_sites"] / kl),
-                      "steps": tm["steps"]}), flush=True)
+                      "steps": tm["steps"], "by_kernel": by_kernel}), flush=True)
 
 
 def cpu_c1(threads):
@@ -403,11 +421,14 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-batch-walks", action="store_true")
     ap.add_argument("--no-c3", action="store_true")
-    ap.add_argument("--c3-cftp", type=int, default=0,
-                    help="also run the two-periodic Aztec-512 CFTP for master seeds 1..K")
+    ap.add_argument("--c3-cftp", type=int, default=8,
+                    help="two-periodic Aztec CFTP for master seeds 1..K (0: skip)")
     ap.add_argument("--profile-rounds", type=int, default=0,
                     help="profiling run: the same batch capped at R doubling rounds (no sample "
                          "needs to finish); prints the sweep kernel's timing only")
+    ap.add_argument("--profile-section", choices=["sweep", "walks"], default=None,
+                    help="profiling run: only the plain Aztec sweep or the 256-chain batch "
+                         "walk section, printed as one JSON line")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="MiB: time a plain device copy of that size (HBM counter calibration)")
     args = ap.parse_args()
@@ -421,6 +442,11 @@ def main():
         return
 
     import randomtilingspublic_amd as rt
+
+    if args.profile_section:
+        fn = plain_sweep if args.profile_section == "sweep" else batched_walks
+        print(json.dumps({"profile_section": args.profile_section, **fn(rt)}), flush=True)
+        return
 
     side = args.side
     d = rt.LozengeTiler.AlmostHexDomain(side, 0)
@@ -467,9 +493,12 @@ def main():
                                       args.max_rounds)
             pw = time.time() - t1
             assert np.array_equal(o1, outs[0])
+            pt = ctx.last_timing()
             rec = {"samples_per_s": 1.0 / pw, "seed": masters[0], "wall_s": round(pw, 3),
                    "rounds": s1["rounds"],
                    "engine": "one launch per colour step" if engine else "sweep-block kernel"}
+            if not engine:
+                rec.update(tile_shape=pt["shape"], split_pairs=bool(pt["split"]))
             if engine:
                 plain = rec
             else:
@@ -489,7 +518,7 @@ def main():
     kms = tm["shape_ms"][shape]
     avg_launch_s = kms / 1e3 / klaunch
     sites_per_launch = tm["shape_sites"][shape] / klaunch
-    alg_bytes = 56.0 * sites_per_launch
+    alg_bytes = float(ALG_BYTES_BY_SHAPE[shape]) * sites_per_launch
     achieved_gbs = alg_bytes / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
     traffic = None
     counters = os.path.join(ROOT, "profiles", "k_fused_lozenge2_hex300_hbm.json")
@@ -503,7 +532,9 @@ def main():
             traffic = int(per_kernel[kname]["hbm_bytes_per_site"] * sites_per_launch)
     roof = {"bound": "hbm", "achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "kernel": kname, "avg_launch_us": round(avg_launch_s * 1e6, 2),
+            "kernel": kname, "alg_bytes_per_site": ALG_BYTES_BY_SHAPE[shape],
+            "kernel_ms_by_shape": [round(x, 1) for x in tm["shape_ms"][:7]],
+            "avg_launch_us": round(avg_launch_s * 1e6, 2),
             "launches": klaunch, "kernel_time_share": round(kms / max(tm["kernel_ms"], 1e-9), 3),
             "sites_per_launch": int(sites_per_launch), "alg_bytes_per_launch": int(alg_bytes),
             "flip_attempts_per_s_in_kernels": round(2 * draws / (tm["kernel_ms"] / 1e3), 1)
@@ -548,6 +579,7 @@ def main():
         "flip_attempts_per_s_device": round(attempts / dev_s, 1),
         "device_s": round(dev_s, 3),
         "rounds": [s["rounds"] for s in stats],
+        "tile_shape_last_round": tm["shape"], "split_pairs_last_round": bool(tm["split"]),
         "plain": plain,
         "plain_fused": plain_fused,
         "batched_vs_plain": round(samples_s / plain["samples_per_s"], 2) if plain else None,

@@ -68,12 +68,13 @@ typedef struct {
   double kernel_ms;   /* time inside the sweep kernels alone (batched calls) */
   long kernel_launches;
   long kernel_sites;  /* sum over those launches of the TinyMT sites they own */
-  /* The same three split by sweep-block tile shape (index shape - 1:
-   * 64 rows / 8 waves, 64 rows / 16 waves, 32 rows / 16 waves, 32 rows /
-   * 8 waves for rect-triangle pairs); no reference counterpart. */
-  double shape_ms[4];
-  long shape_launches[4];
-  long shape_sites[4];
+  /* The same three split by sweep-block launch shape: index 0, 1, 2 =
+   * 64 rows / 8 waves, 64 rows / 16 waves, 32 rows / 16 waves, 3 = 32 rows /
+   * 8 waves for rect-triangle pairs, 4, 5, 6 = shapes 0, 1, 2 with CFTP
+   * pairs split over two workgroups per tile; no reference counterpart. */
+  double shape_ms[8];
+  long shape_launches[8];
+  long shape_sites[8];
   /* Tile shape of the bulk of the last sweep-block job and how it was
    * picked: 0 forced or rule of thumb, 1 tuned in this call, 2 tuned by an
    * earlier call on a job like it. */
@@ -112,6 +113,10 @@ int rt_set_tile_shape(rt_ctx* ctx, int shape);
  * mode 0 = off (rule of thumb on the tile count), 1 = on, 2 = on and forget
  * earlier choices. */
 int rt_set_autotune(rt_ctx* ctx, int mode);
+
+/* The device's CU count and LDS bytes per CU and per workgroup. */
+int rt_device_limits(const rt_ctx* ctx, int* num_cu, size_t* lds_per_cu,
+                     size_t* lds_per_workgroup);
 
 /* ---------------------------------------------------------------- domino */
 /* Flip rules: the four reference kernel files. */
@@ -190,6 +195,15 @@ int rt_domino_cftp_batch(rt_ctx* ctx, const int32_t* tmax, const int32_t* tmin, 
 int rt_domino_walk_batch(rt_ctx* ctx, int32_t* const* tilings, const int* Ns, int nchains,
                          long steps, const long* seeds, const rt_domino_rule* rule,
                          int use_graph);
+/* The same, plus observables computed on the device from each chain's final
+ * colour planes: counts[5*k .. 5*k+4] = vertices of chain k with edge bit
+ * 1, 2, 4, 8 set (DominoTiler.h:28-32; each domino sets two) and rotatable
+ * faces (vertex state 3 or 12).  download = 0 leaves tilings[] as they were
+ * (observables only, no device-to-host copy of the tilings). */
+int rt_domino_walk_batch_observe(rt_ctx* ctx, int32_t* const* tilings, const int* Ns,
+                                 int nchains, long steps, const long* seeds,
+                                 const rt_domino_rule* rule, int use_graph, int download,
+                                 long* counts);
 
 /* -------------------------------------------------------------- serving */
 /* Continuous-batching CFTP server on one context.  A device loop thread

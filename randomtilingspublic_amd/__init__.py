@@ -37,8 +37,8 @@ class _Timing(C.Structure):
     _fields_ = [("device_ms", C.c_double), ("launches", C.c_long),
                 ("steps", C.c_long), ("attempts", C.c_long),
                 ("kernel_ms", C.c_double), ("kernel_launches", C.c_long),
-                ("kernel_sites", C.c_long), ("shape_ms", C.c_double * 4),
-                ("shape_launches", C.c_long * 4), ("shape_sites", C.c_long * 4),
+                ("kernel_sites", C.c_long), ("shape_ms", C.c_double * 8),
+                ("shape_launches", C.c_long * 8), ("shape_sites", C.c_long * 8),
                 ("shape", C.c_int), ("shape_source", C.c_int), ("split", C.c_int)]
 
 
@@ -105,12 +105,17 @@ def lib():
     L.rt_bibone_walk.argtypes = [vp, _I32P, C.c_int, C.c_long, C.c_long]
     L.rt_set_tile_shape.argtypes = [vp, C.c_int]
     L.rt_set_autotune.argtypes = [vp, C.c_int]
+    L.rt_device_limits.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_size_t),
+                                   C.POINTER(C.c_size_t)]
     L.rt_lozenge_cftp_batch.argtypes = [vp, _I32P, _I32P, C.c_int, _U32P, C.c_int, C.c_int,
                                         C.c_long, C.c_int, _I32P, C.c_void_p]
     L.rt_domino_cftp_batch.argtypes = [vp, _I32P, _I32P, C.c_int, _U32P, C.c_int, C.c_int,
                                        C.c_long, C.c_int, C.POINTER(DominoRule), _I32P, C.c_void_p]
     L.rt_domino_walk_batch.argtypes = [vp, C.c_void_p, C.c_void_p, C.c_int, C.c_long, _LONGP,
                                        C.POINTER(DominoRule), C.c_int]
+    L.rt_domino_walk_batch_observe.argtypes = [vp, C.c_void_p, C.c_void_p, C.c_int, C.c_long,
+                                               _LONGP, C.POINTER(DominoRule), C.c_int, C.c_int,
+                                               _LONGP]
     L.rt_server_create.argtypes = [vp, C.c_size_t, C.c_int, C.POINTER(vp)]
     L.rt_server_destroy.argtypes = [vp]
     L.rt_server_destroy.restype = None
@@ -246,6 +251,12 @@ class Context:
         split over two workgroups per tile."""
         self._chk(self._L.rt_set_tile_shape(self._h, shape))
 
+    def device_limits(self):
+        """(CUs, LDS bytes per CU, LDS bytes per workgroup)."""
+        n, a, b = C.c_int(), C.c_size_t(), C.c_size_t()
+        self._chk(self._L.rt_device_limits(self._h, C.byref(n), C.byref(a), C.byref(b)))
+        return n.value, a.value, b.value
+
     def set_autotune(self, mode):
         """Tile-shape autotuner: 0 off, 1 on (default), 2 on and forget
         earlier choices."""
@@ -356,6 +367,22 @@ class Context:
                                                Ns.ctypes.data_as(C.c_void_p), len(ts), steps,
                                                se, C.byref(r), 1 if use_graph else 0))
         return ts
+
+    def domino_walk_batch_observe(self, tilings, steps, seeds, rule=DOMINO_TOGGLE08, a=1.0,
+                                  b=1.0, use_graph=False, download=True):
+        """Batch walk plus device-side observables: returns (tilings or
+        None, counts[nchains, 5]) with the vertices carrying edge bit 1, 2,
+        4, 8 and the rotatable faces of each chain's final tiling."""
+        ts = [_i32(t).copy() for t in tilings]
+        Ns = np.array([side(t) for t in ts], dtype=np.int32)
+        ptrs = (C.c_void_p * len(ts))(*[t.ctypes.data for t in ts])
+        se = np.ascontiguousarray(seeds, dtype=np.int64)
+        r = DominoRule(rule, a, b)
+        counts = np.zeros((len(ts), 5), dtype=np.int64)
+        self._chk(self._L.rt_domino_walk_batch_observe(
+            self._h, C.cast(ptrs, C.c_void_p), Ns.ctypes.data_as(C.c_void_p), len(ts), steps, se,
+            C.byref(r), 1 if use_graph else 0, 1 if download else 0, counts.reshape(-1)))
+        return (ts if download else None), counts
 
     # ---- rectangle-triangle ----
     def recttri_walk(self, t, steps, seed, weights16=None):

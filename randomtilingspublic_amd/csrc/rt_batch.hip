@@ -287,9 +287,10 @@ static int cftp_batch(rt_ctx* ctx, int model, const int32_t* tmax, const int32_t
     acc.kernel_launches += nlaunch;
     acc.kernel_sites += nsites;
     if (job.shape >= 1 && job.shape <= 4) {
-      acc.shape_ms[job.shape - 1] += kms;
-      acc.shape_launches[job.shape - 1] += nlaunch;
-      acc.shape_sites[job.shape - 1] += nsites;
+      const int sl = shape_slot(job.shape, job.split);
+      acc.shape_ms[sl] += kms;
+      acc.shape_launches[sl] += nlaunch;
+      acc.shape_sites[sl] += nsites;
     }
     fused_add_trials(plan, &acc);
     acc.shape = job.shape;
@@ -374,9 +375,65 @@ extern "C" int rt_domino_cftp_batch(rt_ctx* ctx, const int32_t* tmax, const int3
                     stats);
 }
 
+// Observables of a domino tiling read straight off its two device colour
+// planes (N x N/2 bytes each, vertex states of DominoTiler.h:28-32): the
+// number of vertices with each edge bit set (1 up, 2 down, 4 right, 8 left)
+// and of rotatable faces (vertex state 3 or 12: two parallel dominoes).
+// One block row of the grid per chain; 64-bit counters, one atomic per
+// wave and counter.
+__global__ void k_domino_counts(const uint8_t* const* planes, const long* cells,
+                                unsigned long long* counts) {
+  const int c = blockIdx.y;
+  const uint8_t* p = planes[c];
+  const long n = cells[c];  // both planes, contiguous
+  uint32_t k[5] = {0, 0, 0, 0, 0};
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    const uint32_t v = p[i];
+    k[0] += v & 1u;
+    k[1] += (v >> 1) & 1u;
+    k[2] += (v >> 2) & 1u;
+    k[3] += (v >> 3) & 1u;
+    k[4] += (v == 3u || v == 12u) ? 1u : 0u;
+  }
+#pragma unroll
+  for (int q = 0; q < 5; q++) {
+    uint32_t s = k[q];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if ((threadIdx.x & 63) == 0 && s) atomicAdd(&counts[(size_t)c * 5 + q], (unsigned long long)s);
+  }
+}
+
+__global__ void k_zero_u64(unsigned long long* p, long n) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+
+static int walk_batch(rt_ctx* ctx, int32_t* const* tilings, const int* Ns, int nchains, long steps,
+                      const long* seeds, const rt_domino_rule* rule, int use_graph, int download,
+                      long* counts);
+
 extern "C" int rt_domino_walk_batch(rt_ctx* ctx, int32_t* const* tilings, const int* Ns,
                                     int nchains, long steps, const long* seeds,
                                     const rt_domino_rule* rule, int use_graph) {
+  return walk_batch(ctx, tilings, Ns, nchains, steps, seeds, rule, use_graph, 1, nullptr);
+}
+
+extern "C" int rt_domino_walk_batch_observe(rt_ctx* ctx, int32_t* const* tilings, const int* Ns,
+                                            int nchains, long steps, const long* seeds,
+                                            const rt_domino_rule* rule, int use_graph,
+                                            int download, long* counts) {
+  if (!counts) {
+    if (ctx) ctx->err = "counts output missing";
+    return RT_EINVAL;
+  }
+  return walk_batch(ctx, tilings, Ns, nchains, steps, seeds, rule, use_graph, download, counts);
+}
+
+static int walk_batch(rt_ctx* ctx, int32_t* const* tilings, const int* Ns, int nchains, long steps,
+                      const long* seeds, const rt_domino_rule* rule, int use_graph, int download,
+                      long* counts) {
   if (!ctx) return RT_EINVAL;
   if (!tilings || !Ns || !seeds || nchains <= 0 || steps < 0) {
     ctx->err = "bad batch arguments";
@@ -400,7 +457,9 @@ extern "C" int rt_domino_walk_batch(rt_ctx* ctx, int32_t* const* tilings, const 
   int rc = ctx->check_params(need);
   if (rc) return rc;
   HIPCHK_RET(ctx, hipSetDevice(ctx->device));
-  rc = ctx->reserve(bytes + 4096);
+  // tail of the arena: per-chain counters, plane pointers, cell counts
+  const size_t tail = ((size_t)nchains * (5 * 8 + 8 + 8) + 255) & ~(size_t)255;
+  rc = ctx->reserve(bytes + tail + 4096);
   if (rc) return rc;
   FusedJob job;
   job.model = M_DOMINO;
@@ -490,9 +549,10 @@ extern "C" int rt_domino_walk_batch(rt_ctx* ctx, int32_t* const* tilings, const 
     ctx->last.kernel_launches = nlaunch;
     ctx->last.kernel_sites = nsites;
     if (job.shape >= 1 && job.shape <= 4) {
-      ctx->last.shape_ms[job.shape - 1] = kms;
-      ctx->last.shape_launches[job.shape - 1] = nlaunch;
-      ctx->last.shape_sites[job.shape - 1] = nsites;
+      const int sl = shape_slot(job.shape, job.split);
+      ctx->last.shape_ms[sl] = kms;
+      ctx->last.shape_launches[sl] = nlaunch;
+      ctx->last.shape_sites[sl] = nsites;
     }
   }
   fused_add_trials(plan, &ctx->last);
@@ -500,11 +560,34 @@ extern "C" int rt_domino_walk_batch(rt_ctx* ctx, int32_t* const* tilings, const 
   HIPCHK_RET(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
   ctx->last.device_ms = ms;
   for (double t : plan.trial_ms) ctx->last.device_ms += t;  // tuning windows ran before ev0
-  for (int k = 0; k < nchains; k++) {
-    const Geometry g = geometry_of(M_DOMINO, Ns[k]);
-    std::vector<uint8_t> h(2 * (size_t)g.plane);
-    HIPCHK_RET(ctx, hipMemcpy(h.data(), planes[k], h.size(), hipMemcpyDeviceToHost));
-    merge_planes(M_DOMINO, tilings[k], Ns[k], h.data());
+  if (counts) {
+    unsigned long long* d_counts = (unsigned long long*)cur;
+    const uint8_t** d_planes = (const uint8_t**)(cur + (size_t)nchains * 5 * 8);
+    long* d_cells = (long*)(cur + (size_t)nchains * (5 * 8 + 8));
+    std::vector<const uint8_t*> hp(nchains);
+    std::vector<long> hc(nchains);
+    for (int k = 0; k < nchains; k++) {
+      hp[k] = planes[k];
+      hc[k] = 2 * (long)geometry_of(M_DOMINO, Ns[k]).plane;
+    }
+    HIPCHK_RET(ctx, hipMemcpy(d_planes, hp.data(), nchains * sizeof(void*), hipMemcpyHostToDevice));
+    HIPCHK_RET(ctx, hipMemcpy(d_cells, hc.data(), nchains * sizeof(long), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(256), 0, ctx->stream, d_counts, 5L * nchains);
+    hipLaunchKernelGGL(k_domino_counts, dim3(16, (unsigned)nchains), dim3(256), 0, ctx->stream,
+                       d_planes, d_cells, d_counts);
+    HIPCHK_RET(ctx, hipGetLastError());
+    std::vector<unsigned long long> h((size_t)nchains * 5);
+    HIPCHK_RET(ctx, hipMemcpyAsync(h.data(), d_counts, h.size() * 8, hipMemcpyDeviceToHost,
+                                   ctx->stream));
+    HIPCHK_RET(ctx, hipStreamSynchronize(ctx->stream));
+    for (size_t q = 0; q < h.size(); q++) counts[q] = (long)h[q];
   }
+  if (download)
+    for (int k = 0; k < nchains; k++) {
+      const Geometry g = geometry_of(M_DOMINO, Ns[k]);
+      std::vector<uint8_t> h(2 * (size_t)g.plane);
+      HIPCHK_RET(ctx, hipMemcpy(h.data(), planes[k], h.size(), hipMemcpyDeviceToHost));
+      merge_planes(M_DOMINO, tilings[k], Ns[k], h.data());
+    }
   return RT_OK;
 }

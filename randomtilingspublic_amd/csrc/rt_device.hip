@@ -260,6 +260,8 @@ extern "C" int rt_create(rt_ctx** out, int device_ordinal) {
   if (hipGetDeviceProperties(&prop, device_ordinal) == hipSuccess) {
     c->devname = prop.gcnArchName;
     c->num_cu = prop.multiProcessorCount;
+    c->lds_cu = prop.maxSharedMemoryPerMultiProcessor;
+    c->lds_block = prop.sharedMemPerBlock;
   }
   *out = c;
   return RT_OK;
@@ -306,6 +308,15 @@ extern "C" int rt_set_engine(rt_ctx* ctx, int engine) {
 extern "C" int rt_set_tile_shape(rt_ctx* ctx, int shape) {
   if (!ctx || shape < 0 || shape > 6) return RT_EINVAL;
   ctx->tile_shape = shape;
+  return RT_OK;
+}
+
+extern "C" int rt_device_limits(const rt_ctx* ctx, int* num_cu, size_t* lds_per_cu,
+                                size_t* lds_per_workgroup) {
+  if (!ctx) return RT_EINVAL;
+  if (num_cu) *num_cu = ctx->num_cu;
+  if (lds_per_cu) *lds_per_cu = ctx->lds_cu;
+  if (lds_per_workgroup) *lds_per_workgroup = ctx->lds_block;
   return RT_OK;
 }
 
@@ -967,6 +978,10 @@ int cftp_loop(rt_ctx* ctx, uint32_t master_seed, int sched, long t0, int max_rou
     acc.launches += ctx->last.launches;
     acc.steps += ctx->last.steps;
     acc.attempts += ctx->last.attempts;
+    acc.shape = ctx->last.shape;
+    acc.split = ctx->last.split;
+    if (ctx->last.shape_source == 1 || acc.shape_source != 1)
+      acc.shape_source = ctx->last.shape_source;
     if (!differs) {
       if (stats) {
         stats->rounds = r;

@@ -341,9 +341,9 @@ static uint32_t tune_key(const FusedJob& job, long t64) {
   return (uint32_t)job.model | ((uint32_t)job.nc << 4) | ((uint32_t)lg << 8);
 }
 
-constexpr int kTuneMinWindows = 32;  // shorter jobs keep the rule of thumb
-constexpr int kTrialWindows = 2;     // windows per timed trial
-constexpr int kTrialReps = 2;        // trials per candidate shape
+constexpr int kTrialWindows = 4;     // launches per timed trial
+constexpr int kTrialReps = 3;        // trials per candidate shape, interleaved
+constexpr int kTuneMinWindows = 2 * (1 + kTrialReps * 5 * kTrialWindows);  // else rule of thumb
 struct LaunchShape {
   int shape;
   bool split;
@@ -714,10 +714,10 @@ int fused_tune(rt_ctx* ctx, FusedJob& job, FusedPlan* plan) {
   for (size_t k = 0; k < trials.size(); k++) {
     float e = 0;
     HIPCHK_RET(ctx, hipEventElapsedTime(&e, ctx->evt[k], ctx->evt[k + 1]));
-    const int c = trials[k].cand, sh = plan->cand[c].shape;
-    plan->trial_ms[sh - 1] += e;
-    plan->trial_launches[sh - 1] += (long)(trials[k].l1 - trials[k].l0);
-    plan->trial_sites[sh - 1] += plan->sites(trials[k].l0, trials[k].l1);
+    const int c = trials[k].cand, sl = shape_slot(plan->cand[c].shape, plan->cand[c].split);
+    plan->trial_ms[sl] += e;
+    plan->trial_launches[sl] += (long)(trials[k].l1 - trials[k].l0);
+    plan->trial_sites[sl] += plan->sites(trials[k].l0, trials[k].l1);
     if (k == 0) continue;  // the seeding launch is not a fair trial
     ms[c] += e;
     for (size_t x = trials[k].l0; x < trials[k].l1; x++) work[c] += (double)plan->launch_work[x];
@@ -743,7 +743,7 @@ int fused_tune(rt_ctx* ctx, FusedJob& job, FusedPlan* plan) {
 }
 
 void fused_add_trials(const FusedPlan& plan, rt_timing* acc) {
-  for (int s = 0; s < 4; s++) {
+  for (int s = 0; s < 8; s++) {
     acc->kernel_ms += plan.trial_ms[s];
     acc->kernel_launches += plan.trial_launches[s];
     acc->kernel_sites += plan.trial_sites[s];

@@ -187,13 +187,15 @@ struct FusedPlan {
     return n;
   }
   // launches run by fused_tune, by tile shape (index shape - 1)
-  double trial_ms[4] = {0, 0, 0, 0};
-  long trial_launches[4] = {0, 0, 0, 0};
-  long trial_sites[4] = {0, 0, 0, 0};
+  double trial_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  long trial_launches[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  long trial_sites[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
 int fused_upload(rt_ctx* ctx, FusedJob& job, void* scratch, FusedPlan* plan);
 int fused_tune(rt_ctx* ctx, FusedJob& job, FusedPlan* plan);
 int fused_launch(rt_ctx* ctx, const FusedJob& job, const FusedPlan& plan);
+// Slot of a launch shape in rt_timing.shape_*: shape - 1, +4 when split.
+inline int shape_slot(int shape, bool split) { return shape - 1 + (split ? 4 : 0); }
 // Add the plan's tuning launches to per-shape kernel timing.
 void fused_add_trials(const FusedPlan& plan, rt_timing* acc);
 

@@ -43,8 +43,10 @@ struct rt_ctx {
   int engine = 0;
   int tile_shape = 0;  // sweep-block tile shape: 0 auto, else a TileShape
   int autotune = 1;    // time candidate tile shapes on a job's first windows
+  size_t lds_cu = 0;   // LDS bytes per CU
+  size_t lds_block = 0;  // LDS bytes one workgroup may hold
   std::map<uint32_t, int> tuned;  // tuning key (model, chains, tile count) -> shape
-  hipEvent_t evt[16] = {};        // tuner trial boundaries
+  hipEvent_t evt[24] = {};        // tuner trial boundaries
 
   int reserve(size_t bytes);
   int reserve_aux(size_t bytes);

@@ -74,15 +74,15 @@ def test_domino_walk_batch_rect_unions(gpu_ctx):
     P = rt.synthetic_params(big * big // 2, 24)
     gpu_ctx.set_tinymt_params(P)
     seeds = [7 * k + 3 for k in range(len(tilings))]
-    want = [o.domino_walk(t, 900, s, P, rt.DOMINO_HEATBATH) for t, s in zip(tilings, seeds)]
+    want = [o.domino_walk(t, 1800, s, P, rt.DOMINO_HEATBATH) for t, s in zip(tilings, seeds)]
     gpu_ctx.set_autotune(2)
     try:
         for use_graph in (False, True):
             _say("domino walk batch rect unions graph=", use_graph)
-            outs = gpu_ctx.domino_walk_batch(tilings, 900, seeds, rt.DOMINO_HEATBATH,
+            outs = gpu_ctx.domino_walk_batch(tilings, 1800, seeds, rt.DOMINO_HEATBATH,
                                              use_graph=use_graph)
             tm = gpu_ctx.last_timing()
-            # first call tunes (900 steps cut into >= 24 windows), the second reuses it
+            # first call tunes (1800 steps: ~150 launches), the second reuses it
             assert tm["shape_source"] == (2 if use_graph else 1), tm
             assert tm["shape"] in (1, 2, 3)
             for got, w in zip(outs, want):
@@ -104,7 +104,7 @@ def test_tile_shape_choice_never_changes_results(gpu_ctx):
             _say("lozenge cftp batch autotune", mode, "shape", shape)
             gpu_ctx.set_autotune(mode)
             gpu_ctx.set_tile_shape(shape)
-            outs, stats = gpu_ctx.lozenge_cftp_batch(tmax, tmin, masters, rt.SCHED_DOUBLING, 256,
+            outs, stats = gpu_ctx.lozenge_cftp_batch(tmax, tmin, masters, rt.SCHED_DOUBLING, 2048,
                                                      30)
             tm = gpu_ctx.last_timing()
             if shape:
@@ -115,7 +115,7 @@ def test_tile_shape_choice_never_changes_results(gpu_ctx):
             if ref is None:
                 ref = outs
                 for k, m in enumerate(masters):
-                    want, _ = o.lozenge_cftp(tmax, tmin, m, P, rt.SCHED_DOUBLING, 256, 30)
+                    want, _ = o.lozenge_cftp(tmax, tmin, m, P, rt.SCHED_DOUBLING, 2048, 30)
                     assert np.array_equal(outs[k], want)
             else:
                 for a, b in zip(outs, ref):
@@ -172,3 +172,28 @@ def test_walk_batch_with_more_tiles_than_fit_on_the_chip(gpu_ctx):
                 assert np.array_equal(outs[k], want), k
     finally:
         gpu_ctx.set_tile_shape(0)
+
+
+def domino_counts(t):
+    t = np.asarray(t)
+    return [int(((t >> b) & 1).sum()) for b in range(4)] + [int(((t == 3) | (t == 12)).sum())]
+
+
+def test_walk_batch_device_observables(gpu_ctx):
+    tilings = [rt.DominoTiler.MinTiling(rt.DominoTiler.AztecDiamond(n)) for n in (5, 30, 61)]
+    tilings.append(rt.DominoTiler.MinTiling(rt.DominoTiler.RectangleUnion(
+        [(0, 0, 20, 30), (10, 24, 30, 12)])))
+    big = max(rt.side(t) for t in tilings)
+    gpu_ctx.set_tinymt_params(rt.synthetic_params(big * big // 2, 28))
+    seeds = [5, 6, 7, 8]
+    _say("walk batch observables")
+    outs, counts = gpu_ctx.domino_walk_batch_observe(tilings, 700, seeds, rt.DOMINO_HEATBATH)
+    for t, c in zip(outs, counts):
+        assert list(c) == domino_counts(t)
+        assert c[0] == c[1] and c[2] == c[3]  # each domino sets one bit at each end
+    plain = gpu_ctx.domino_walk_batch(tilings, 700, seeds, rt.DOMINO_HEATBATH)
+    for a, b in zip(outs, plain):
+        assert np.array_equal(a, b)
+    none, again = gpu_ctx.domino_walk_batch_observe(tilings, 700, seeds, rt.DOMINO_HEATBATH,
+                                                    use_graph=True, download=False)
+    assert none is None and np.array_equal(again, counts)
