@@ -228,14 +228,17 @@ def weighted_and_bibone(rt, cftp_seeds=8, cftp_order=24, steps=10 ** 5):
     return out
 
 
-def served_stream(rt, seconds=20.0, seed=2024):
+def served_stream(rt, seconds=20.0, seed=2024, dom=(64, 96), loz=(64, 80), t0=1024, burst_n=32,
+                  plain_n=24):
     """A Poisson stream of CFTP requests through the continuous-batching
     server (rt_server), then the same requests one call at a time.  Request
-    mix, drawn from `seed`: domino Aztec diamonds of order 16..48 (heat
-    bath) and lozenge hexagons of side 8..24, doubling from 2^7.  The
-    arrival rate is 70 % of the capacity measured on a burst of 48
+    mix, drawn from `seed`: domino Aztec diamonds of order dom[0]..dom[1]
+    (heat bath) and lozenge hexagons of side loz[0]..loz[1], doubling from
+    t0 (the low end of the served configuration's ranges: its upper sizes
+    take minutes per exact sample; rectangle-triangle has no CFTP).  The
+    arrival rate is 70 % of the capacity measured on a burst of burst_n
     requests.  Reports samples/s, p50/p99 latency (submit to result) and
-    the plain path's samples/s on the same requests."""
+    the plain path's samples/s on the first plain_n requests."""
     rng = np.random.default_rng(seed)
     cache = {}
 
@@ -251,21 +254,23 @@ def served_stream(rt, seconds=20.0, seed=2024):
 
     def draw(k):
         model = int(rng.integers(0, 2))
-        n = int(rng.integers(16, 49)) if model == rt.MODEL_DOMINO else int(rng.integers(8, 25))
+        n = (int(rng.integers(dom[0], dom[1] + 1)) if model == rt.MODEL_DOMINO
+             else int(rng.integers(loz[0], loz[1] + 1)))
         tmax, tmin = domain(model, n)
         return dict(model=model, n=n, tmax=tmax, tmin=tmin, master_seed=10000 + k)
 
-    big = max(rt.side(domain(rt.MODEL_DOMINO, 48)[0]), rt.side(domain(rt.MODEL_LOZENGE, 24)[0]))
+    big = max(rt.side(domain(rt.MODEL_DOMINO, dom[1])[0]),
+              rt.side(domain(rt.MODEL_LOZENGE, loz[1])[0]))
     ctx = rt.Context(0)
     ctx.set_tinymt_params(rt.synthetic_params(big * big, None))
     srv = rt.Server(ctx, 8 << 30, max_active=256)
     try:
         def submit(q):
             return srv.submit(q["model"], q["tmax"], q["tmin"], q["master_seed"],
-                              rt.SCHED_DOUBLING, 128, 32, rt.DOMINO_HEATBATH)
+                              rt.SCHED_DOUBLING, t0, 32, rt.DOMINO_HEATBATH)
 
         # capacity on a burst
-        burst = [draw(k) for k in range(48)]
+        burst = [draw(k) for k in range(burst_n)]
         t0 = time.time()
         ids = [submit(q) for q in burst]
         for i in ids:
@@ -304,30 +309,30 @@ def served_stream(rt, seconds=20.0, seed=2024):
     finally:
         srv.close()
     served = len(stream) / span
-    # The same requests one call at a time (the first 120 of them, bounded):
-    # with the reference's launch shape (one launch per colour step) and with
-    # the sweep-block kernel.
+    # The same requests one call at a time (the first plain_n of them,
+    # bounded): with the reference's launch shape (one launch per colour
+    # step) and with the sweep-block kernel.
     def plain_rate(engine, reqs):
         ctx.set_engine(engine)
         t1 = time.time()
         for q in reqs:
             if q["model"] == rt.MODEL_DOMINO:
-                ctx.domino_cftp(q["tmax"], q["tmin"], q["master_seed"], rt.SCHED_DOUBLING, 128,
+                ctx.domino_cftp(q["tmax"], q["tmin"], q["master_seed"], rt.SCHED_DOUBLING, t0,
                                 32, rt.DOMINO_HEATBATH)
             else:
-                ctx.lozenge_cftp(q["tmax"], q["tmin"], q["master_seed"], rt.SCHED_DOUBLING, 128,
+                ctx.lozenge_cftp(q["tmax"], q["tmin"], q["master_seed"], rt.SCHED_DOUBLING, t0,
                                  32)
         return len(reqs) / (time.time() - t1)
 
-    sub = stream[:120]
+    sub = stream[:plain_n]
     plain = plain_rate(rt.ENGINE_STEP, sub)
     plain_fused = plain_rate(0, sub)
     ctx.set_engine(0)
     ctx.close()
     lat = np.array(lat)
-    return {"workload": "Poisson stream of CFTP requests: domino Aztec order 16-48 (heat bath) "
-                        "and lozenge hexagon side 8-24, doubling from 2^7, rate 70 % of "
-                        "burst capacity, continuous batching",
+    return {"workload": f"Poisson stream of CFTP requests: domino Aztec order {dom[0]}-{dom[1]} "
+                        f"(heat bath) and lozenge hexagon side {loz[0]}-{loz[1]}, doubling from "
+                        f"2^{int(np.log2(t0))}, rate 70 % of burst capacity, continuous batching",
             "requests": len(stream), "span_s": round(span, 2),
             "samples_per_s": round(served, 2), "burst_capacity_per_s": round(cap, 2),
             "p50_latency_s": round(float(np.percentile(lat, 50)), 4),

@@ -101,6 +101,13 @@ __device__ __forceinline__ void swar_step(uint32_t* lds, const SwarRow* row, Tin
 // Tile of 4*W*R rows x 64 columns; wave wv, row slot r, lane quarter q
 // (lane >> 4) own tile row 4*(wv + W*r) + q, word w = lane & 15 (cells
 // 4w .. 4w+3).  Kernel arguments and buffer sets as k_fused.
+//
+// TinyMT states and parameters travel through an LDS staging array, one
+// 32-bit word of every tile cell at a time: the workgroup loads (and
+// stores) it with one lane per column, as k_fused does, and each lane
+// moves its four cells' words with one 16-byte LDS access.  Loading them
+// straight from a lane's four consecutive cells would spread every load
+// instruction over four times as many cache lines.
 template <int NC, int W, int R>
 __global__ __launch_bounds__(64 * W, W >= 16 ? 1 : 2) void k_swar_lozenge(
     const UnitDesc* __restrict__ units, const WorkItem* __restrict__ work, int H, ModelParams mp,
@@ -109,6 +116,7 @@ __global__ __launch_bounds__(64 * W, W >= 16 ? 1 : 2) void k_swar_lozenge(
   constexpr int LH = 4 * W * R;
   constexpr int PLW = (LH + 2) * 16;  // words per plane, with a zero margin row each side
   __shared__ uint32_t lds[NC * 3 * PLW];
+  __shared__ __align__(16) uint32_t stage[LH * kLW];
   __shared__ uint8_t sops[kKMax];
 
   const WorkItem wi = work[blockIdx.x];
@@ -151,7 +159,6 @@ __global__ __launch_bounds__(64 * W, W >= 16 ? 1 : 2) void k_swar_lozenge(
     for (int k = 0; k < 4; k++) {
       const int gc = gc0 + k;
       const bool hs = row_in && gc >= 0 && gc < cols;
-      st[r][k] = Tiny{0, 0, 0, 0, 0, 0, 0};
       if (!hs) continue;
       in |= 0xFFu << (8 * k);
       if (row_int && gc >= 1 && gc <= cols - 2) inter |= 0xFFu << (8 * k);
@@ -161,18 +168,6 @@ __global__ __launch_bounds__(64 * W, W >= 16 ? 1 : 2) void k_swar_lozenge(
 #pragma unroll
         for (int pl = 0; pl < 3; pl++)
           cell[ch][pl] |= (uint32_t)((const uint8_t*)U.plane[ch][pl])[cidx] << (8 * k);
-      const long id = cidx;  // lozenge stream id j + (N/3) i
-      st[r][k].mat1 = U.param[id];
-      st[r][k].mat2 = U.param[U.pstride + id];
-      st[r][k].tmat = U.param[2 * U.pstride + id];
-      if (init) {
-        tiny_init(st[r][k], seed + (uint32_t)id);
-      } else {
-        st[r][k].s0 = U.state[id];
-        st[r][k].s1 = U.state[U.nstream + id];
-        st[r][k].s2 = U.state[2 * U.nstream + id];
-        st[r][k].s3 = U.state[3 * U.nstream + id];
-      }
     }
     row[r].in = in & 0x3F3F3F3Fu;
     row[r].interior = inter;
@@ -181,7 +176,51 @@ __global__ __launch_bounds__(64 * W, W >= 16 ? 1 : 2) void k_swar_lozenge(
 #pragma unroll
       for (int pl = 0; pl < 3; pl++) lds[(ch * 3 + pl) * PLW + row[r].li] = cell[ch][pl];
   }
-  __syncthreads();
+
+  // Stream words (lozenge stream id = cell index j + (N/3) i): parameters,
+  // then the state, or the seeding.
+  auto stage_in = [&](const uint32_t* a, int field) {
+    for (int t = threadIdx.x; t < LH * kLW; t += kThreads) {
+      const int gr = r0 + (t >> 6), gc = c0 + (t & 63);
+      stage[t] = (gr >= 0 && gr < rows && gc >= 0 && gc < cols) ? a[(long)gr * cols + gc] : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int lr = 4 * (wv + W * r) + q;
+      const uint4 v = *(const uint4*)&stage[lr * kLW + 4 * w];
+      const uint32_t x[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        uint32_t* f = field == 0 ? &st[r][k].mat1 : field == 1 ? &st[r][k].mat2
+                    : field == 2 ? &st[r][k].tmat : field == 3 ? &st[r][k].s0
+                    : field == 4 ? &st[r][k].s1 : field == 5 ? &st[r][k].s2 : &st[r][k].s3;
+        *f = x[k];
+      }
+    }
+    __syncthreads();
+  };
+  stage_in(U.param, 0);
+  stage_in(U.param + U.pstride, 1);
+  stage_in(U.param + 2 * U.pstride, 2);
+  if (init) {
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int gr = r0 + 4 * (wv + W * r) + q;
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const int gc = gc0 + k;
+        st[r][k].s0 = st[r][k].s1 = st[r][k].s2 = st[r][k].s3 = 0;
+        if (gr >= 0 && gr < rows && gc >= 0 && gc < cols)
+          tiny_init(st[r][k], seed + (uint32_t)((long)gr * cols + gc));
+      }
+    }
+  } else {
+    stage_in(U.state, 3);
+    stage_in(U.state + U.nstream, 4);
+    stage_in(U.state + 2 * U.nstream, 5);
+    stage_in(U.state + 3 * U.nstream, 6);
+  }
 
   for (int s = 0; s < K; s++) {
     const uint32_t op = sops[s];
@@ -194,6 +233,7 @@ __global__ __launch_bounds__(64 * W, W >= 16 ? 1 : 2) void k_swar_lozenge(
     __syncthreads();
   }
 
+  // Cells of the tile interior back to the destination planes.
 #pragma unroll
   for (int r = 0; r < R; r++) {
     const int lr = 4 * (wv + W * r) + q, gr = r0 + lr;
@@ -214,11 +254,31 @@ __global__ __launch_bounds__(64 * W, W >= 16 ? 1 : 2) void k_swar_lozenge(
 #pragma unroll
         for (int pl = 0; pl < 3; pl++)
           ((uint8_t*)U.plane_b[ch][pl])[cidx] = (uint8_t)(words[ch][pl] >> (8 * k));
-      U.state_b[cidx] = st[r][k].s0;
-      U.state_b[U.nstream + cidx] = st[r][k].s1;
-      U.state_b[2 * U.nstream + cidx] = st[r][k].s2;
-      U.state_b[3 * U.nstream + cidx] = st[r][k].s3;
     }
+  }
+  if (U.no_state_write) return;
+  // States of the tile interior, through the staging array.
+#pragma unroll
+  for (int field = 0; field < 4; field++) {
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int lr = 4 * (wv + W * r) + q;
+      uint4 v;
+      if (field == 0) v = make_uint4(st[r][0].s0, st[r][1].s0, st[r][2].s0, st[r][3].s0);
+      else if (field == 1) v = make_uint4(st[r][0].s1, st[r][1].s1, st[r][2].s1, st[r][3].s1);
+      else if (field == 2) v = make_uint4(st[r][0].s2, st[r][1].s2, st[r][2].s2, st[r][3].s2);
+      else v = make_uint4(st[r][0].s3, st[r][1].s3, st[r][2].s3, st[r][3].s3);
+      *(uint4*)&stage[lr * kLW + 4 * w] = v;
+    }
+    __syncthreads();
+    uint32_t* dst = U.state_b + (long)field * U.nstream;
+    for (int t = threadIdx.x; t < LH * kLW; t += kThreads) {
+      const int lr = t >> 6, lc = t & 63, gr = r0 + lr, gc = c0 + lc;
+      if (lr >= H && lr < H + TH && lc >= H && lc < H + TW && gr >= 0 && gr < rows && gc >= 0 &&
+          gc < cols)
+        dst[(long)gr * cols + gc] = stage[t];
+    }
+    __syncthreads();
   }
 }
 
