@@ -271,12 +271,12 @@ def served_stream(rt, seconds=20.0, seed=2024, dom=(64, 96), loz=(64, 80), t0=10
 
         # capacity on a burst
         burst = [draw(k) for k in range(burst_n)]
-        t0 = time.time()
+        tb = time.time()
         ids = [submit(q) for q in burst]
         for i in ids:
             srv.wait(i, 120.0)
             srv.poll(i)
-        cap = len(burst) / (time.time() - t0)
+        cap = len(burst) / (time.time() - tb)
         rate = 0.7 * cap
         print(f"[bench] served: burst capacity {cap:.1f} samples/s, stream at {rate:.1f}/s",
               file=sys.stderr, flush=True)
@@ -431,9 +431,11 @@ def main():
     ap.add_argument("--profile-rounds", type=int, default=0,
                     help="profiling run: the same batch capped at R doubling rounds (no sample "
                          "needs to finish); prints the sweep kernel's timing only")
-    ap.add_argument("--profile-section", choices=["sweep", "walks"], default=None,
-                    help="profiling run: only the plain Aztec sweep or the 256-chain batch "
-                         "walk section, printed as one JSON line")
+    ap.add_argument("--profile-section", choices=["sweep", "walks", "c3", "served"],
+                    default=None,
+                    help="profiling run: only one section (plain Aztec sweep, 256-chain batch "
+                         "walks, weighted walk / bibone, served stream), printed as one JSON "
+                         "line")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="MiB: time a plain device copy of that size (HBM counter calibration)")
     args = ap.parse_args()
@@ -449,7 +451,8 @@ def main():
     import randomtilingspublic_amd as rt
 
     if args.profile_section:
-        fn = plain_sweep if args.profile_section == "sweep" else batched_walks
+        fn = {"sweep": plain_sweep, "walks": batched_walks, "c3": weighted_and_bibone,
+              "served": served_stream}[args.profile_section]
         print(json.dumps({"profile_section": args.profile_section, **fn(rt)}), flush=True)
         return
 
