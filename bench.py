@@ -436,6 +436,9 @@ def main():
                     help="profiling run: only one section (plain Aztec sweep, 256-chain batch "
                          "walks, weighted walk / bibone, served stream), printed as one JSON "
                          "line")
+    ap.add_argument("--tile-shape", type=int, default=0,
+                    help="force a sweep-block tile shape (rt_set_tile_shape; 0 = autotuned), "
+                         "for profiling one kernel")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="MiB: time a plain device copy of that size (HBM counter calibration)")
     args = ap.parse_args()
@@ -464,6 +467,7 @@ def main():
     params = rt.synthetic_params(N * N, None)
     ctx = rt.Context(0)
     ctx.set_tinymt_params(params)
+    ctx.set_tile_shape(args.tile_shape)
     print(f"[bench] device {ctx.device_name} hexagon {side} N={N} sites/step={P}", file=sys.stderr,
           flush=True)
     for w in range(args.warmup):

@@ -125,7 +125,6 @@ struct FusedJob {
   std::vector<std::vector<uint32_t>> seg_seeds;   // [unit][segment]
   std::vector<std::vector<uint8_t>> ops;          // [unit] concatenated op bytes
   std::vector<OpBuilder> hist;        // [unit]
-  bool adaptive = true;               // single unit: stretch windows to H pulls
   int shape = 0;                      // tile shape, resolved by fused_scratch_bytes
   bool split = false;                 // pairs run as two one-chain units (see below)
   int shape_source = 0;               // 0 rule of thumb / forced, 1 tuned now, 2 tuned before
