@@ -381,26 +381,28 @@ extern "C" int rt_domino_cftp_batch(rt_ctx* ctx, const int32_t* tmax, const int3
 // and of rotatable faces (vertex state 3 or 12: two parallel dominoes).
 // One block row of the grid per chain; 64-bit counters, one atomic per
 // wave and counter.
-__global__ void k_domino_counts(const uint8_t* const* planes, const long* cells,
+__global__ void k_domino_counts(const uint8_t* const* planes, const long* cells, int nchains,
                                 unsigned long long* counts) {
-  const int c = blockIdx.y;
-  const uint8_t* p = planes[c];
-  const long n = cells[c];  // both planes, contiguous
-  uint32_t k[5] = {0, 0, 0, 0, 0};
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (long)gridDim.x * blockDim.x) {
-    const uint32_t v = p[i];
-    k[0] += v & 1u;
-    k[1] += (v >> 1) & 1u;
-    k[2] += (v >> 2) & 1u;
-    k[3] += (v >> 3) & 1u;
-    k[4] += (v == 3u || v == 12u) ? 1u : 0u;
-  }
+  for (int c = blockIdx.y; c < nchains; c += gridDim.y) {
+    const uint8_t* p = planes[c];
+    const long n = cells[c];  // both planes, contiguous
+    uint32_t k[5] = {0, 0, 0, 0, 0};
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long)gridDim.x * blockDim.x) {
+      const uint32_t v = p[i];
+      k[0] += v & 1u;
+      k[1] += (v >> 1) & 1u;
+      k[2] += (v >> 2) & 1u;
+      k[3] += (v >> 3) & 1u;
+      k[4] += (v == 3u || v == 12u) ? 1u : 0u;
+    }
 #pragma unroll
-  for (int q = 0; q < 5; q++) {
-    uint32_t s = k[q];
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-    if ((threadIdx.x & 63) == 0 && s) atomicAdd(&counts[(size_t)c * 5 + q], (unsigned long long)s);
+    for (int q = 0; q < 5; q++) {
+      uint32_t s = k[q];
+      for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+      if ((threadIdx.x & 63) == 0 && s)
+        atomicAdd(&counts[(size_t)c * 5 + q], (unsigned long long)s);
+    }
   }
 }
 
@@ -573,8 +575,8 @@ static int walk_batch(rt_ctx* ctx, int32_t* const* tilings, const int* Ns, int n
     HIPCHK_RET(ctx, hipMemcpy(d_planes, hp.data(), nchains * sizeof(void*), hipMemcpyHostToDevice));
     HIPCHK_RET(ctx, hipMemcpy(d_cells, hc.data(), nchains * sizeof(long), hipMemcpyHostToDevice));
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(256), 0, ctx->stream, d_counts, 5L * nchains);
-    hipLaunchKernelGGL(k_domino_counts, dim3(16, (unsigned)nchains), dim3(256), 0, ctx->stream,
-                       d_planes, d_cells, d_counts);
+    hipLaunchKernelGGL(k_domino_counts, dim3(16, (unsigned)std::min(nchains, 4096)), dim3(256),
+                       0, ctx->stream, d_planes, d_cells, nchains, d_counts);
     HIPCHK_RET(ctx, hipGetLastError());
     std::vector<unsigned long long> h((size_t)nchains * 5);
     HIPCHK_RET(ctx, hipMemcpyAsync(h.data(), d_counts, h.size() * 8, hipMemcpyDeviceToHost,

@@ -449,18 +449,20 @@ struct CopyDesc {
   long bytes;
 };
 
-__global__ void k_copy_list(const CopyDesc* __restrict__ list) {
-  const CopyDesc d = list[blockIdx.y];
+__global__ void k_copy_list(const CopyDesc* __restrict__ list, unsigned n) {
   const long stride = (long)gridDim.x * blockDim.x;
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if ((((uintptr_t)d.src | (uintptr_t)d.dst | (uintptr_t)d.bytes) & 15) == 0) {
-    const uint4* s = (const uint4*)d.src;
-    uint4* o = (uint4*)d.dst;
-    for (long i = t; i < d.bytes / 16; i += stride) o[i] = s[i];
-  } else {
-    const uint8_t* s = (const uint8_t*)d.src;
-    uint8_t* o = (uint8_t*)d.dst;
-    for (long i = t; i < d.bytes; i += stride) o[i] = s[i];
+  for (unsigned k = blockIdx.y; k < n; k += gridDim.y) {
+    const CopyDesc d = list[k];
+    if ((((uintptr_t)d.src | (uintptr_t)d.dst | (uintptr_t)d.bytes) & 15) == 0) {
+      const uint4* s = (const uint4*)d.src;
+      uint4* o = (uint4*)d.dst;
+      for (long i = t; i < d.bytes / 16; i += stride) o[i] = s[i];
+    } else {
+      const uint8_t* s = (const uint8_t*)d.src;
+      uint8_t* o = (uint8_t*)d.dst;
+      for (long i = t; i < d.bytes; i += stride) o[i] = s[i];
+    }
   }
 }
 
@@ -764,8 +766,8 @@ int fused_launch(rt_ctx* ctx, const FusedJob& job, const FusedPlan& plan) {
   for (size_t l = plan.first; l < plan.nlaunch; l++)
     fn(ctx, t.nwork, t.d_units, t.d_work, H, job.mp, t.d_wins + l * t.nunits);
   if (plan.ncopies)
-    hipLaunchKernelGGL(k_copy_list, dim3(64, plan.ncopies), dim3(256), 0, ctx->stream,
-                       plan.d_copies);
+    hipLaunchKernelGGL(k_copy_list, dim3(64, std::min(plan.ncopies, 4096u)), dim3(256), 0,
+                       ctx->stream, plan.d_copies, plan.ncopies);
   HIPCHK_RET(ctx, hipGetLastError());
   count_launches(ctx, plan, plan.first, plan.nlaunch);
   return RT_OK;

@@ -1,6 +1,6 @@
 """The C++ drop-in header (include/rtiling_tilers.hpp) compiles against
 librtiling.so and its static helpers reproduce the reference's outputs;
-on a GPU box the same program also runs a Walk and a CFTP through it."""
+with a GPU present the same program also runs a Walk and a CFTP through it."""
 import os
 import subprocess
 
