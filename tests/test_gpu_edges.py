@@ -1,0 +1,56 @@
+"""Edge cases of the batched paths on the device: a CFTP batch that cannot
+coalesce within its round cap fails with ENOCOUPLE and leaves the context
+usable; zero-step and one-chain batch walks; malformed batch inputs."""
+import sys
+
+import numpy as np
+import pytest
+
+import randomtilingspublic_amd as rt
+from tests import oracle_lib as o
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print("[case]", *a, file=sys.stderr, flush=True)
+
+
+def test_cftp_batch_round_cap_then_context_still_works(gpu_ctx):
+    d = rt.LozengeTiler.AlmostHexDomain(20, 0)
+    tmin, tmax = rt.LozengeTiler.MinTiling(d), rt.LozengeTiler.MaxTiling(d)
+    N = rt.side(tmin)
+    P = rt.synthetic_params(N * N, 31)
+    gpu_ctx.set_tinymt_params(P)
+    _say("lozenge cftp batch capped at 2 rounds")
+    with pytest.raises(rt.RTError) as e:
+        gpu_ctx.lozenge_cftp_batch(tmax, tmin, [1, 2], rt.SCHED_DOUBLING, 4, 2)
+    assert e.value.code == -4  # ENOCOUPLE
+    _say("lozenge cftp batch after the failure")
+    outs, stats = gpu_ctx.lozenge_cftp_batch(tmax, tmin, [1, 2], rt.SCHED_DOUBLING, 1024, 20)
+    for k, m in enumerate((1, 2)):
+        want, _ = o.lozenge_cftp(tmax, tmin, m, P, rt.SCHED_DOUBLING, 1024, 20)
+        assert np.array_equal(outs[k], want)
+
+
+def test_walk_batch_zero_steps_and_single_chain(gpu_ctx):
+    t = rt.DominoTiler.MinTiling(rt.DominoTiler.AztecDiamond(20))
+    N = rt.side(t)
+    P = rt.synthetic_params(N * N // 2, 32)
+    gpu_ctx.set_tinymt_params(P)
+    _say("walk batch zero steps")
+    outs = gpu_ctx.domino_walk_batch([t, t], 0, [1, 2], rt.DOMINO_HEATBATH)
+    assert all(np.array_equal(x, t) for x in outs)
+    _say("walk batch one chain")
+    (one,) = gpu_ctx.domino_walk_batch([t], 333, [9], rt.DOMINO_HEATBATH, use_graph=True)
+    assert np.array_equal(one, o.domino_walk(t, 333, 9, P, rt.DOMINO_HEATBATH))
+
+
+def test_walk_batch_rejects_bad_chains(gpu_ctx):
+    gpu_ctx.set_tinymt_params(rt.synthetic_params(4096, 33))
+    _say("walk batch odd side")
+    with pytest.raises(rt.RTError):
+        gpu_ctx.domino_walk_batch([np.zeros(49, np.int32)], 10, [1])
+    with pytest.raises(rt.RTError):
+        gpu_ctx.domino_walk_batch([rt.DominoTiler.MinTiling(rt.DominoTiler.AztecDiamond(60))],
+                                  10, [1])  # 126 x 63 sites > 4096 parameter sets
