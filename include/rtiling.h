@@ -114,6 +114,13 @@ int rt_set_tile_shape(rt_ctx* ctx, int shape);
  * earlier choices. */
 int rt_set_autotune(rt_ctx* ctx, int mode);
 
+/* Sweep-block launches with no more workgroups than CUs hand each of the
+ * chip's 8 XCDs (which receive workgroups round robin) a contiguous share of
+ * the tile list, so tiles that read each other's halo rows share an L2
+ * (on = 1, the default; no reference counterpart; results never depend on
+ * it). */
+int rt_set_xcd_order(rt_ctx* ctx, int on);
+
 /* The device's CU count and LDS bytes per CU and per workgroup. */
 int rt_device_limits(const rt_ctx* ctx, int* num_cu, size_t* lds_per_cu,
                      size_t* lds_per_workgroup);

@@ -105,6 +105,7 @@ def lib():
     L.rt_bibone_walk.argtypes = [vp, _I32P, C.c_int, C.c_long, C.c_long]
     L.rt_set_tile_shape.argtypes = [vp, C.c_int]
     L.rt_set_autotune.argtypes = [vp, C.c_int]
+    L.rt_set_xcd_order.argtypes = [vp, C.c_int]
     L.rt_device_limits.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_size_t),
                                    C.POINTER(C.c_size_t)]
     L.rt_lozenge_cftp_batch.argtypes = [vp, _I32P, _I32P, C.c_int, _U32P, C.c_int, C.c_int,
@@ -256,6 +257,11 @@ class Context:
         n, a, b = C.c_int(), C.c_size_t(), C.c_size_t()
         self._chk(self._L.rt_device_limits(self._h, C.byref(n), C.byref(a), C.byref(b)))
         return n.value, a.value, b.value
+
+    def set_xcd_order(self, on):
+        """Contiguous share of each launch's tiles per XCD (1, default) or
+        the plain block order (0)."""
+        self._chk(self._L.rt_set_xcd_order(self._h, on))
 
     def set_autotune(self, mode):
         """Tile-shape autotuner: 0 off, 1 on (default), 2 on and forget

@@ -320,6 +320,12 @@ extern "C" int rt_device_limits(const rt_ctx* ctx, int* num_cu, size_t* lds_per_
   return RT_OK;
 }
 
+extern "C" int rt_set_xcd_order(rt_ctx* ctx, int on) {
+  if (!ctx || on < 0 || on > 1) return RT_EINVAL;
+  ctx->xcd_order = on;
+  return RT_OK;
+}
+
 extern "C" int rt_set_autotune(rt_ctx* ctx, int mode) {
   if (!ctx || mode < 0 || mode > 2) return RT_EINVAL;
   if (mode == 2) {

@@ -116,6 +116,16 @@ __device__ __forceinline__ void sweep_step(Cell* lds, const Lanes& L, Tiny* st, 
       lds[ch * NCOL * PL + L.li0 + c * PL + r * W * RS] = (Cell)out[ch][r];
 }
 
+// Work item of this workgroup.  Workgroups go to the 8 XCDs round robin
+// (block b to XCD b % 8); with xcd set, XCD x takes the contiguous x-th
+// eighth of the work list, so tiles that share halo rows share an L2.  The
+// grid is then rounded up to a multiple of 8 and surplus blocks get -1.
+__device__ __forceinline__ int work_index(int nwork, int xcd) {
+  unsigned b = blockIdx.x;
+  if (xcd) b = (b & 7u) * (gridDim.x >> 3) + (b >> 3);
+  return b < (unsigned)nwork ? (int)b : -1;
+}
+
 // The K colour steps of a window on the LDS tile, NCR chains.
 template <int MODEL, int NCR, int W, int R, class Cell, int RS, int PL>
 __device__ __forceinline__ void run_window(Cell* lds, const Lanes& L, Tiny* st,
@@ -143,8 +153,8 @@ __device__ __forceinline__ void run_window(Cell* lds, const Lanes& L, Tiny* st,
 // (rows wv, wv+W, ...), so the loaded tile is (W*R) x 64 cells.
 template <int MODEL, int NC, int W, int R>
 __global__ __launch_bounds__(64 * W, 4) void k_fused(const UnitDesc* __restrict__ units,
-                                                    const WorkItem* __restrict__ work, int H,
-                                                    ModelParams mp,
+                                                    const WorkItem* __restrict__ work, int nwork,
+                                                    int xcd, int H, ModelParams mp,
                                                     const WindowRec* __restrict__ wins) {
   using Cell = typename CellOf<MODEL>::type;
   constexpr int NCOL = MODEL == M_DOMINO ? 2 : 3;
@@ -156,7 +166,9 @@ __global__ __launch_bounds__(64 * W, 4) void k_fused(const UnitDesc* __restrict_
   static_assert((NCOL * PL * sizeof(Cell)) % 4 == 0, "chain planes must be whole words");
   __shared__ uint8_t sops[kKMax];
 
-  const WorkItem wi = work[blockIdx.x];
+  const int b = work_index(nwork, xcd);
+  if (b < 0) return;
+  const WorkItem wi = work[b];
   const WindowRec w = wins[wi.unit];
   const long s0 = w.s0;
   const int K = w.K, init = w.init;
@@ -485,11 +497,21 @@ size_t fused_scratch_bytes(rt_ctx* ctx, FusedJob& job) {
   return bytes;
 }
 
+// XCD order for launches with no more workgroups than CUs (one workgroup
+// per CU): per launch on the side-300 hexagon CFTP it takes a lone sample
+// from 15.3 to 13.3 us and 4 samples (64-row tiles) from 33.2 to 30.1 us;
+// with more workgroups than CUs it was as often slower (+2..5 %) as faster.
+static int xcd_of(const rt_ctx* ctx, unsigned nwork) {
+  return ctx->xcd_order && nwork <= (unsigned)ctx->num_cu ? 1 : 0;
+}
+static unsigned grid_of(int xcd, unsigned nwork) { return xcd ? (nwork + 7u) & ~7u : nwork; }
+
 template <int MODEL, int NC, int W, int R>
 static void launch(rt_ctx* ctx, unsigned nblocks, const UnitDesc* units, const WorkItem* work,
                    int H, const ModelParams& mp, const WindowRec* wins) {
-  hipLaunchKernelGGL((k_fused<MODEL, NC, W, R>), dim3(nblocks), dim3(64 * W), 0, ctx->stream,
-                     units, work, H, mp, wins);
+  const int xcd = xcd_of(ctx, nblocks);
+  hipLaunchKernelGGL((k_fused<MODEL, NC, W, R>), dim3(grid_of(xcd, nblocks)), dim3(64 * W), 0,
+                     ctx->stream, units, work, (int)nblocks, xcd, H, mp, wins);
 }
 
 typedef void (*LaunchFn)(rt_ctx*, unsigned, const UnitDesc*, const WorkItem*, int,
@@ -505,8 +527,9 @@ static LaunchFn pick_byte(int shape) {
 template <int NC, int W, int R>
 static void launch_swar(rt_ctx* ctx, unsigned nblocks, const UnitDesc* units, const WorkItem* work,
                         int H, const ModelParams& mp, const WindowRec* wins) {
-  hipLaunchKernelGGL((k_swar_lozenge<NC, W, R>), dim3(nblocks), dim3(64 * W), 0, ctx->stream,
-                     units, work, H, mp, wins);
+  const int xcd = xcd_of(ctx, nblocks);
+  hipLaunchKernelGGL((k_swar_lozenge<NC, W, R>), dim3(grid_of(xcd, nblocks)), dim3(64 * W), 0,
+                     ctx->stream, units, work, (int)nblocks, xcd, H, mp, wins);
 }
 
 // Lozenge: four cells per lane-word (k_swar_lozenge), same tiles: 64-row

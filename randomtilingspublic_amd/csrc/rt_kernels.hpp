@@ -43,6 +43,7 @@ struct rt_ctx {
   int engine = 0;
   int tile_shape = 0;  // sweep-block tile shape: 0 auto, else a TileShape
   int autotune = 1;    // time candidate tile shapes on a job's first windows
+  int xcd_order = 1;   // each XCD takes a contiguous share of a launch's tiles
   size_t lds_cu = 0;   // LDS bytes per CU
   size_t lds_block = 0;  // LDS bytes one workgroup may hold
   std::map<uint32_t, int> tuned;  // tuning key (model, chains, tile count) -> shape

@@ -110,8 +110,8 @@ __device__ __forceinline__ void swar_step(uint32_t* lds, const SwarRow* row, Tin
 // instruction over four times as many cache lines.
 template <int NC, int W, int R>
 __global__ __launch_bounds__(64 * W, W >= 16 ? 1 : 2) void k_swar_lozenge(
-    const UnitDesc* __restrict__ units, const WorkItem* __restrict__ work, int H, ModelParams mp,
-    const WindowRec* __restrict__ wins) {
+    const UnitDesc* __restrict__ units, const WorkItem* __restrict__ work, int nwork, int xcd,
+    int H, ModelParams mp, const WindowRec* __restrict__ wins) {
   constexpr int kThreads = 64 * W;
   constexpr int LH = 4 * W * R;
   constexpr int PLW = (LH + 2) * 16;  // words per plane, with a zero margin row each side
@@ -119,7 +119,9 @@ __global__ __launch_bounds__(64 * W, W >= 16 ? 1 : 2) void k_swar_lozenge(
   __shared__ __align__(16) uint32_t stage[LH * kLW];
   __shared__ uint8_t sops[kKMax];
 
-  const WorkItem wi = work[blockIdx.x];
+  const int b = work_index(nwork, xcd);
+  if (b < 0) return;
+  const WorkItem wi = work[b];
   const WindowRec wr = wins[wi.unit];
   const long s0 = wr.s0;
   const int K = wr.K, init = wr.init;

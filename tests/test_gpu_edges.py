@@ -54,3 +54,21 @@ def test_walk_batch_rejects_bad_chains(gpu_ctx):
     with pytest.raises(rt.RTError):
         gpu_ctx.domino_walk_batch([rt.DominoTiler.MinTiling(rt.DominoTiler.AztecDiamond(60))],
                                   10, [1])  # 126 x 63 sites > 4096 parameter sets
+
+
+def test_xcd_order_never_changes_results(gpu_ctx):
+    ts = [rt.DominoTiler.MinTiling(rt.DominoTiler.AztecDiamond(n)) for n in (40, 70, 9)]
+    big = max(rt.side(t) for t in ts)
+    P = rt.synthetic_params(big * big // 2, 34)
+    gpu_ctx.set_tinymt_params(P)
+    outs = []
+    try:
+        for on in (0, 1):
+            _say("walk batch xcd order", on)
+            gpu_ctx.set_xcd_order(on)
+            outs.append(gpu_ctx.domino_walk_batch(ts, 500, [3, 4, 5], rt.DOMINO_HEATBATH))
+    finally:
+        gpu_ctx.set_xcd_order(1)
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
+    assert np.array_equal(outs[1][2], o.domino_walk(ts[2], 500, 5, P, rt.DOMINO_HEATBATH))
