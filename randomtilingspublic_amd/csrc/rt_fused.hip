@@ -574,11 +574,18 @@ static int upload_tiles(rt_ctx* ctx, const FusedJob& job, LaunchShape ls, uint8_
     units = job.units;
   }
   std::vector<WorkItem> work;
-  for (size_t u = 0; u < units.size(); u++) {
+  for (size_t u = 0; u < units.size(); u += ls.split ? 2 : 1) {
     int tr, tc;
     tile_counts(units[u], geo.W * geo.R, geo.H, &tr, &tc);
     units[u].tiles_c = tc;
-    for (int t = 0; t < tr * tc; t++) work.push_back(WorkItem{(int)u, t});
+    if (ls.split) units[u + 1].tiles_c = tc;
+    // Split pairs: the two chains' workgroups of a tile sit next to each
+    // other in the list, so they land in the same XCD share and the second
+    // one finds the shared TinyMT states in L2.
+    for (int t = 0; t < tr * tc; t++) {
+      work.push_back(WorkItem{(int)u, t});
+      if (ls.split) work.push_back(WorkItem{(int)u + 1, t});
+    }
   }
   UnitDesc* d_units = (UnitDesc*)p;
   p += (units.size() * sizeof(UnitDesc) + 255) & ~(size_t)255;
