@@ -108,14 +108,24 @@ __device__ __forceinline__ uint32_t lozenge_pull_prev(uint32_t v, const P* p, in
   return v & 0xffu;
 }
 
-// Same, with im = i % 3 precomputed.
+// Bit-field insert: the bits of x under mask M, the rest from y.  Written
+// as the instruction itself: the compiler turns (x & M) | (y & ~M) chains
+// into separate and / or ops.
+template <uint32_t M>
+__device__ __forceinline__ uint32_t bfi(uint32_t x, uint32_t y) {
+  uint32_t r;
+  asm("v_bfi_b32 %0, %1, %2, %3" : "=v"(r) : "n"(M), "v"(x), "v"(y));
+  return r;
+}
+
+// Same, with im = i % 3 precomputed; v is a cell byte.
 template <class P>
 __device__ __forceinline__ uint32_t lozenge_pull_prev_m(uint32_t v, const P* p, int rs, int im,
                                                         int tr) {
   const int p2 = im == ((2 * tr + 1) % 3);
   const int p4 = im == ((2 * tr + 2) % 3);
   const uint32_t a = p[-rs + p2], b = p[-p4], c = p[rs];
-  return (v & ~(2u | 4u | 32u) & 0xffu) | ((a & 16u) >> 3) | ((b & 8u) >> 1) | ((c & 1u) << 5);
+  return bfi<32u>(c << 5, bfi<4u>(b >> 1, bfi<2u>(a >> 3, v)));
 }
 
 template <class P>
@@ -124,7 +134,19 @@ __device__ __forceinline__ uint32_t lozenge_pull_prev2_m(uint32_t v, const P* p,
   const int p8 = im == ((2 * tr) % 3);
   const int p16 = im == ((2 * tr + 1) % 3);
   const uint32_t a = p[-rs], b = p[p8], c = p[rs - p16];
-  return (v & ~(1u | 8u | 16u) & 0xffu) | ((a & 32u) >> 5) | ((b & 4u) << 1) | ((c & 2u) << 3);
+  return bfi<16u>(c << 3, bfi<8u>(b << 1, bfi<1u>(a >> 5, v)));
+}
+
+// lozenge_move split in two: the draw of a cell decides which byte value
+// (25 or 38) a move would change, or none (0x100, no byte equals it); the
+// pair's two chains then apply it with one compare and one select each.
+// hs = 0 (a cell without a TinyMT stream) never moves.
+__device__ __forceinline__ uint32_t lozenge_target(uint32_t b, bool hs) {
+  const uint32_t t = b < kBelow04 ? 25u : (b < kBelow08 ? 38u : 0x100u);
+  return hs ? t : 0x100u;
+}
+__device__ __forceinline__ uint32_t lozenge_apply(uint32_t v, uint32_t t) {
+  return v == t ? (t ^ 0x3Fu) : v;  // 25 <-> 38
 }
 
 // Refresh the {1,8,16} bits of v from colour tr = x-2 (mod 3), i.e. x is

@@ -64,9 +64,16 @@ __device__ __forceinline__ void sweep_step(Cell* lds, const Lanes& L, Tiny* st, 
                                            const UnitDesc& U, const ModelParams& mp) {
   constexpr int NCOL = MODEL == M_DOMINO ? 2 : 3;
   const int c = op & OP_COLOUR;
-  uint32_t b[R];
+  // Lozenge rows of up to 4 cells per lane turn the draw into the move's
+  // target once for both chains; 8-row lanes keep the draw (the targets
+  // would push the pair kernel past 128 VGPRs).
+  constexpr bool kTarget = MODEL == M_LOZENGE && R <= 4;
+  uint32_t b[R];  // the row's draw, or its move target
 #pragma unroll
-  for (int r = 0; r < R; r++) b[r] = tiny_bits(st[r]);
+  for (int r = 0; r < R; r++) {
+    b[r] = tiny_bits(st[r]);
+    if (kTarget) b[r] = lozenge_target(b[r], (L.hsm >> r) & 1u);
+  }
   uint32_t out[NC][R];
 #pragma unroll
   for (int ch = 0; ch < NC; ch++) {
@@ -84,16 +91,18 @@ __device__ __forceinline__ void sweep_step(Cell* lds, const Lanes& L, Tiny* st, 
         }
         nv = domino_move(v, b[r], U.rule, gr, 1 - c, U.dw);
       } else if (MODEL == M_LOZENGE) {
+        // Both pulls, then one select on interm (a cell on the array's
+        // border keeps its bits).
         const int im = (L.im0 + (W % 3) * r) % 3;
-        if (PULL & 1) {
-          const int ca = (c + 2) % 3;
-          const uint32_t pv = lozenge_pull_prev_m(v, P + ca * PL + off, RS, im, ca);
+        if (PULL) {
+          uint32_t pv = v;
+          if (PULL & 1) pv = lozenge_pull_prev_m(pv, P + ((c + 2) % 3) * PL + off, RS, im, (c + 2) % 3);
+          if (PULL & 2) pv = lozenge_pull_prev2_m(pv, P + ((c + 1) % 3) * PL + off, RS, im, (c + 1) % 3);
           v = pick(L.interm, r, pv, v);
         }
-        if (PULL & 2) {
-          const int cb = (c + 1) % 3;
-          const uint32_t pv = lozenge_pull_prev2_m(v, P + cb * PL + off, RS, im, cb);
-          v = pick(L.interm, r, pv, v);
+        if (kTarget) {  // the target already holds the has-stream test
+          out[ch][r] = lozenge_apply(v, b[r]);
+          continue;
         }
         nv = lozenge_move(v, b[r]);
       } else {
