@@ -476,10 +476,14 @@ def main():
     if args.profile_rounds:
         profile_rounds(ctx, rt, tmax, tmin, masters, args.t0, args.profile_rounds)
         return
+    ctx.set_round_callback(lambda r, n, fin, tot: print(
+        f"[bench] round {r} ({n} steps): {fin} of {tot} samples coalesced", file=sys.stderr,
+        flush=True))
     t0 = time.time()
     outs, stats = ctx.lozenge_cftp_batch(tmax, tmin, masters, rt.SCHED_DOUBLING, args.t0,
                                          args.max_rounds)
     wall = time.time() - t0
+    ctx.set_round_callback(None)
     tm = ctx.last_timing()
     dev_s = tm["device_ms"] / 1e3
     draws = tm["attempts"]                  # one TinyMT draw per site per step per sample

@@ -105,7 +105,7 @@ int rt_set_engine(rt_ctx* ctx, int engine);
 int rt_set_tile_shape(rt_ctx* ctx, int shape);
 
 /* Tile-shape autotuner (no reference counterpart; on by default).  With
- * shape 0 above, a sweep-block job of at least 24 launch windows whose
+ * shape 0 above, a sweep-block job of at least 122 launch windows whose
  * kind (model, chains, tile count to a factor of two) has not been tuned
  * on this context runs its first windows split among the candidate
  * shapes, timed with events, and keeps the fastest for the rest of the
@@ -120,6 +120,14 @@ int rt_set_autotune(rt_ctx* ctx, int mode);
  * (on = 1, the default; no reference counterpart; results never depend on
  * it). */
 int rt_set_xcd_order(rt_ctx* ctx, int on);
+
+/* Round callback of the CFTP entry points (no reference counterpart: the
+ * reference's driver loop, AztecDiamondCftpEx.cpp:57-77, reports nothing
+ * until it has coupled).  After each round's coalescence test, in the
+ * calling thread: fn(user, round, steps of the round, samples finished so
+ * far, samples in the call).  fn = NULL turns it off (the default). */
+typedef void (*rt_round_fn)(void* user, int round, long round_steps, int finished, int total);
+int rt_set_round_callback(rt_ctx* ctx, rt_round_fn fn, void* user);
 
 /* The device's CU count and LDS bytes per CU and per workgroup. */
 int rt_device_limits(const rt_ctx* ctx, int* num_cu, size_t* lds_per_cu,

@@ -106,6 +106,7 @@ def lib():
     L.rt_set_tile_shape.argtypes = [vp, C.c_int]
     L.rt_set_autotune.argtypes = [vp, C.c_int]
     L.rt_set_xcd_order.argtypes = [vp, C.c_int]
+    L.rt_set_round_callback.argtypes = [vp, C.c_void_p, C.c_void_p]
     L.rt_device_limits.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_size_t),
                                    C.POINTER(C.c_size_t)]
     L.rt_lozenge_cftp_batch.argtypes = [vp, _I32P, _I32P, C.c_int, _U32P, C.c_int, C.c_int,
@@ -212,6 +213,9 @@ def side(a):
     return n
 
 
+_ROUND_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_long, C.c_int, C.c_int)
+
+
 class Context:
     """One device, one stream, one TinyMT parameter table (rt_ctx)."""
 
@@ -222,6 +226,7 @@ class Context:
         if rc != RT_OK:
             raise RTError(rc, "rt_create failed (no HIP device?)")
         self._h = h
+        self._round_cb = None
         self._L = L
 
     def close(self):
@@ -267,6 +272,17 @@ class Context:
         """Tile-shape autotuner: 0 off, 1 on (default), 2 on and forget
         earlier choices."""
         self._chk(self._L.rt_set_autotune(self._h, mode))
+
+    def set_round_callback(self, fn):
+        """fn(round, round_steps, finished, total) after each CFTP round's
+        coalescence test (rt_set_round_callback); None turns it off."""
+        if fn is None:
+            self._round_cb = None
+            self._chk(self._L.rt_set_round_callback(self._h, None, None))
+            return
+        cb = _ROUND_FN(lambda user, r, steps, fin, tot: fn(r, steps, fin, tot))
+        self._round_cb = cb  # the C side holds only the pointer
+        self._chk(self._L.rt_set_round_callback(self._h, C.cast(cb, C.c_void_p), None))
 
     def set_tinymt_params(self, params3):
         p = np.ascontiguousarray(np.asarray(params3, dtype=np.uint32).reshape(-1, 3).ravel())

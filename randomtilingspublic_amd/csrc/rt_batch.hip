@@ -315,6 +315,7 @@ static int cftp_batch(rt_ctx* ctx, int model, const int32_t* tmax, const int32_t
         stats[k].device_ms = ms_total;
       }
     }
+    if (ctx->round_fn) ctx->round_fn(ctx->round_user, r, tot, ns - remaining, ns);
     if (remaining == 0 || r == max_rounds) break;
     const long ns_len = sched == RT_SCHED_REFERENCE ? steps.back() * 2 : tot;
     steps.insert(steps.begin(), ns_len);

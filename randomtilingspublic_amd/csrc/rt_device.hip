@@ -320,6 +320,13 @@ extern "C" int rt_device_limits(const rt_ctx* ctx, int* num_cu, size_t* lds_per_
   return RT_OK;
 }
 
+extern "C" int rt_set_round_callback(rt_ctx* ctx, rt_round_fn fn, void* user) {
+  if (!ctx) return RT_EINVAL;
+  ctx->round_fn = fn;
+  ctx->round_user = user;
+  return RT_OK;
+}
+
 extern "C" int rt_set_xcd_order(rt_ctx* ctx, int on) {
   if (!ctx || on < 0 || on > 1) return RT_EINVAL;
   ctx->xcd_order = on;
@@ -988,6 +995,7 @@ int cftp_loop(rt_ctx* ctx, uint32_t master_seed, int sched, long t0, int max_rou
     acc.split = ctx->last.split;
     if (ctx->last.shape_source == 1 || acc.shape_source != 1)
       acc.shape_source = ctx->last.shape_source;
+    if (ctx->round_fn) ctx->round_fn(ctx->round_user, r, tot, differs ? 0 : 1, 1);
     if (!differs) {
       if (stats) {
         stats->rounds = r;

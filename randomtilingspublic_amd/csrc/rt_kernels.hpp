@@ -47,6 +47,8 @@ struct rt_ctx {
   size_t lds_cu = 0;   // LDS bytes per CU
   size_t lds_block = 0;  // LDS bytes one workgroup may hold
   std::map<uint32_t, int> tuned;  // tuning key (model, chains, tile count) -> shape
+  rt_round_fn round_fn = nullptr;  // CFTP round-end callback
+  void* round_user = nullptr;
   hipEvent_t evt[24] = {};        // tuner trial boundaries
 
   int reserve(size_t bytes);

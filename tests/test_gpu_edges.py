@@ -1,6 +1,7 @@
 """Edge cases of the batched paths on the device: a CFTP batch that cannot
 coalesce within its round cap fails with ENOCOUPLE and leaves the context
-usable; zero-step and one-chain batch walks; malformed batch inputs."""
+usable; zero-step and one-chain batch walks; malformed batch inputs; the
+XCD tile order; the CFTP round callback."""
 import sys
 
 import numpy as np
@@ -72,3 +73,33 @@ def test_xcd_order_never_changes_results(gpu_ctx):
     for a, b in zip(*outs):
         assert np.array_equal(a, b)
     assert np.array_equal(outs[1][2], o.domino_walk(ts[2], 500, 5, P, rt.DOMINO_HEATBATH))
+
+
+def test_round_callback_reports_every_round(gpu_ctx):
+    d = rt.LozengeTiler.AlmostHexDomain(12, 0)
+    tmin, tmax = rt.LozengeTiler.MinTiling(d), rt.LozengeTiler.MaxTiling(d)
+    N = rt.side(tmin)
+    gpu_ctx.set_tinymt_params(rt.synthetic_params(N * N, 35))
+    calls = []
+    gpu_ctx.set_round_callback(lambda r, n, fin, tot: calls.append((r, n, fin, tot)))
+    try:
+        _say("lozenge cftp batch with a round callback")
+        outs, stats = gpu_ctx.lozenge_cftp_batch(tmax, tmin, [1, 2, 3], rt.SCHED_DOUBLING, 64, 20)
+        R = max(s["rounds"] for s in stats)
+        assert [c[0] for c in calls] == list(range(1, R + 1))
+        assert [c[1] for c in calls] == [64 << (r - 1) for r in range(1, R + 1)]
+        assert all(c[3] == 3 for c in calls)
+        for r, _, fin, _ in calls:
+            assert fin == sum(1 for s in stats if s["rounds"] <= r)
+        calls.clear()
+        _say("lozenge cftp one sample with a round callback")
+        out, st = gpu_ctx.lozenge_cftp(tmax, tmin, 2, rt.SCHED_DOUBLING, 64, 20)
+        assert np.array_equal(out, outs[1])
+        assert [c[0] for c in calls] == list(range(1, st["rounds"] + 1))
+        assert [c[2] for c in calls] == [0] * (st["rounds"] - 1) + [1]
+        calls.clear()
+        gpu_ctx.set_round_callback(None)
+        gpu_ctx.lozenge_cftp(tmax, tmin, 2, rt.SCHED_DOUBLING, 64, 20)
+        assert calls == []
+    finally:
+        gpu_ctx.set_round_callback(None)
