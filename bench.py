@@ -436,6 +436,8 @@ def main():
                     help="profiling run: only one section (plain Aztec sweep, 256-chain batch "
                          "walks, weighted walk / bibone, served stream), printed as one JSON "
                          "line")
+    ap.add_argument("--halo", type=int, default=6,
+                    help="sweep-block halo depth (rt_set_halo; results never depend on it)")
     ap.add_argument("--tile-shape", type=int, default=0,
                     help="force a sweep-block tile shape (rt_set_tile_shape; 0 = autotuned), "
                          "for profiling one kernel")
@@ -468,6 +470,7 @@ def main():
     ctx = rt.Context(0)
     ctx.set_tinymt_params(params)
     ctx.set_tile_shape(args.tile_shape)
+    ctx.set_halo(args.halo)
     print(f"[bench] device {ctx.device_name} hexagon {side} N={N} sites/step={P}", file=sys.stderr,
           flush=True)
     for w in range(args.warmup):

@@ -121,6 +121,14 @@ int rt_set_autotune(rt_ctx* ctx, int mode);
  * it). */
 int rt_set_xcd_order(rt_ctx* ctx, int on);
 
+/* Halo depth H of the sweep-block tiles (no reference counterpart; default
+ * 6, 1..12): a launch runs a window of at most H pulling colour steps of
+ * each unit, tiles overlap by 2H cells.  Larger H means fewer launches and
+ * more redundant halo work.  Results never depend on it.  Setting it
+ * forgets the autotuner's earlier choices; the serving loop keeps 6 and
+ * rectangle-triangle pairs 4. */
+int rt_set_halo(rt_ctx* ctx, int halo);
+
 /* Round callback of the CFTP entry points (no reference counterpart: the
  * reference's driver loop, AztecDiamondCftpEx.cpp:57-77, reports nothing
  * until it has coupled).  After each round's coalescence test, in the

@@ -107,6 +107,7 @@ def lib():
     L.rt_set_autotune.argtypes = [vp, C.c_int]
     L.rt_set_xcd_order.argtypes = [vp, C.c_int]
     L.rt_set_round_callback.argtypes = [vp, C.c_void_p, C.c_void_p]
+    L.rt_set_halo.argtypes = [vp, C.c_int]
     L.rt_device_limits.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_size_t),
                                    C.POINTER(C.c_size_t)]
     L.rt_lozenge_cftp_batch.argtypes = [vp, _I32P, _I32P, C.c_int, _U32P, C.c_int, C.c_int,
@@ -272,6 +273,11 @@ class Context:
         """Tile-shape autotuner: 0 off, 1 on (default), 2 on and forget
         earlier choices."""
         self._chk(self._L.rt_set_autotune(self._h, mode))
+
+    def set_halo(self, halo):
+        """Sweep-block halo depth H (default 6, 1..12): windows of at most H
+        pulling steps; never changes results."""
+        self._chk(self._L.rt_set_halo(self._h, halo))
 
     def set_round_callback(self, fn):
         """fn(round, round_steps, finished, total) after each CFTP round's

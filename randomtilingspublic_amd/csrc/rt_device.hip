@@ -320,6 +320,13 @@ extern "C" int rt_device_limits(const rt_ctx* ctx, int* num_cu, size_t* lds_per_
   return RT_OK;
 }
 
+extern "C" int rt_set_halo(rt_ctx* ctx, int halo) {
+  if (!ctx || halo < 1 || halo > 12) return RT_EINVAL;
+  if (halo != ctx->halo) ctx->tuned.clear();
+  ctx->halo = halo;
+  return RT_OK;
+}
+
 extern "C" int rt_set_round_callback(rt_ctx* ctx, rt_round_fn fn, void* user) {
   if (!ctx) return RT_EINVAL;
   ctx->round_fn = fn;

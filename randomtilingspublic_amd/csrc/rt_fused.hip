@@ -443,7 +443,7 @@ static size_t tiles_bytes(const FusedJob& job, int shape, bool split) {
   size_t work = 0;
   for (const UnitDesc& u : job.units) {
     int tr, tc;
-    tile_counts(u, g.W * g.R, g.H, &tr, &tc);
+    tile_counts(u, g.W * g.R, job.halo, &tr, &tc);
     work += (size_t)tr * tc * per;
   }
   size_t b = ((job.units.size() * per * sizeof(UnitDesc) + 255) & ~(size_t)255) +
@@ -487,10 +487,17 @@ __global__ void k_copy_list(const CopyDesc* __restrict__ list, unsigned n) {
   }
 }
 
-size_t fused_scratch_bytes(rt_ctx* ctx, FusedJob& job) {
-  // Every shape of a model has the same halo, so one window list serves all.
-  cut_windows(job, shape_geo(job.model == M_RECTTRI && job.nc == 2 ? SH_RT8x4 : SH_8x8).H);
+// Halo of a job's tiles: the context's setting, except rect-triangle pairs
+// (32-row int32 tiles, H = 4).  Every shape of a job has the same halo, so
+// one window list serves all.
+static void plan_windows(rt_ctx* ctx, FusedJob& job) {
+  job.halo = job.model == M_RECTTRI && job.nc == 2 ? shape_geo(SH_RT8x4).H : ctx->halo;
+  cut_windows(job, job.halo);
   choose_shape(ctx, job);
+}
+
+size_t fused_scratch_bytes(rt_ctx* ctx, FusedJob& job) {
+  plan_windows(ctx, job);
   long T = 0;
   for (long s : job.seg_steps) T += s;
   const size_t nu = job.units.size();
@@ -585,7 +592,7 @@ static int upload_tiles(rt_ctx* ctx, const FusedJob& job, LaunchShape ls, uint8_
   std::vector<WorkItem> work;
   for (size_t u = 0; u < units.size(); u += ls.split ? 2 : 1) {
     int tr, tc;
-    tile_counts(units[u], geo.W * geo.R, geo.H, &tr, &tc);
+    tile_counts(units[u], geo.W * geo.R, job.halo, &tr, &tc);
     units[u].tiles_c = tc;
     if (ls.split) units[u + 1].tiles_c = tc;
     // Split pairs: the two chains' workgroups of a tile sit next to each
@@ -625,10 +632,7 @@ static int upload_tiles(rt_ctx* ctx, const FusedJob& job, LaunchShape ls, uint8_
 }
 
 int fused_upload(rt_ctx* ctx, FusedJob& job, void* scratch, FusedPlan* plan) {
-  if (job.shape == SH_AUTO) {
-    cut_windows(job, shape_geo(job.model == M_RECTTRI && job.nc == 2 ? SH_RT8x4 : SH_8x8).H);
-    choose_shape(ctx, job);
-  }
+  if (job.shape == SH_AUTO) plan_windows(ctx, job);
   const size_t nu = job.units.size();
   *plan = FusedPlan();
   long T = 0;
@@ -727,7 +731,7 @@ int fused_tune(rt_ctx* ctx, FusedJob& job, FusedPlan* plan) {
   const size_t nc = plan->cand.size();
   const size_t need = 1 + (size_t)kTrialReps * nc * kTrialWindows;
   if (!job.tune || nc < 2 || plan->nlaunch < need) return RT_OK;
-  const int H = shape_geo(job.shape).H;
+  const int H = job.halo;
   // Launch 0 (TinyMT seeding) under the rule-of-thumb shape, then
   // kTrialReps rounds of kTrialWindows launches under each candidate;
   // event k + 1 closes launch group k.
@@ -801,7 +805,7 @@ int fused_launch(rt_ctx* ctx, const FusedJob& job, const FusedPlan& plan) {
   if (plan.cand.empty()) return RT_OK;  // nothing to run
   const FusedPlan::Tiles& t = plan.cand[0];
   LaunchFn fn = pick(ctx, job.model, t.split ? 1 : job.nc, t.shape);
-  const int H = shape_geo(t.shape).H;
+  const int H = job.halo;
   for (size_t l = plan.first; l < plan.nlaunch; l++)
     fn(ctx, t.nwork, t.d_units, t.d_work, H, job.mp, t.d_wins + l * t.nunits);
   if (plan.ncopies)

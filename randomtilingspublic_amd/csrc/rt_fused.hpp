@@ -129,6 +129,7 @@ struct FusedJob {
   bool split = false;                 // pairs run as two one-chain units (see below)
   int shape_source = 0;               // 0 rule of thumb / forced, 1 tuned now, 2 tuned before
   bool tune = false;                  // time the candidate shapes on the first launches
+  int halo = 6;                       // tile halo H: windows hold at most H pulling steps
   // Launch windows per unit, cut by fused_scratch_bytes from the unit's own
   // pull steps: launch l advances unit u by windows[u][l] (nothing once
   // the unit's windows run out), so units with independent schedules do

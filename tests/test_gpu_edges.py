@@ -1,7 +1,7 @@
 """Edge cases of the batched paths on the device: a CFTP batch that cannot
 coalesce within its round cap fails with ENOCOUPLE and leaves the context
 usable; zero-step and one-chain batch walks; malformed batch inputs; the
-XCD tile order; the CFTP round callback."""
+XCD tile order, the halo depth; the CFTP round callback."""
 import sys
 
 import numpy as np
@@ -103,3 +103,43 @@ def test_round_callback_reports_every_round(gpu_ctx):
         assert calls == []
     finally:
         gpu_ctx.set_round_callback(None)
+
+
+@pytest.mark.parametrize("halo", [2, 9])
+def test_halo_depth_never_changes_results(gpu_ctx, halo):
+    # Domains wide enough for two tile columns at either halo (64-column
+    # tiles overlap by 2 * halo): the hexagon of side 100 (69 lozenge
+    # columns) and the Aztec diamond of order 70 (73 domino columns).
+    d30 = rt.LozengeTiler.AlmostHexDomain(30, 0)
+    tmin, tmax = rt.LozengeTiler.MinTiling(d30), rt.LozengeTiler.MaxTiling(d30)
+    big = rt.LozengeTiler.MinTiling(rt.LozengeTiler.AlmostHexDomain(100, 0))
+    P = rt.synthetic_params(rt.side(big) ** 2, 36)
+    want_c = o.lozenge_cftp(tmax, tmin, 4, P, rt.SCHED_DOUBLING, 256, 20)[0]
+    want_w = o.lozenge_walk(big, 300, 8, P)
+    ts = [rt.DominoTiler.MinTiling(rt.DominoTiler.AztecDiamond(n)) for n in (70, 12)]
+    Pd = rt.synthetic_params(rt.side(ts[0]) ** 2 // 2, 37)
+    want_d = [o.domino_walk(t, 700, s, Pd, rt.DOMINO_HEATBATH) for t, s in zip(ts, (6, 7))]
+    gpu_ctx.set_tinymt_params(P)
+    try:
+        gpu_ctx.set_halo(halo)
+        for shape in (2, 6):
+            _say("lozenge cftp batch halo", halo, "shape", shape)
+            gpu_ctx.set_tile_shape(shape)
+            outs, _ = gpu_ctx.lozenge_cftp_batch(tmax, tmin, [4], rt.SCHED_DOUBLING, 256, 20)
+            assert np.array_equal(outs[0], want_c)
+        gpu_ctx.set_tile_shape(0)
+        _say("lozenge walk halo", halo)
+        assert np.array_equal(gpu_ctx.lozenge_walk(big, 300, 8), want_w)
+        gpu_ctx.set_tinymt_params(Pd)
+        gpu_ctx.set_tile_shape(1)
+        _say("domino walk batch halo", halo)
+        outs = gpu_ctx.domino_walk_batch(ts, 700, [6, 7], rt.DOMINO_HEATBATH)
+        for got, w in zip(outs, want_d):
+            assert np.array_equal(got, w)
+    finally:
+        gpu_ctx.set_halo(6)
+        gpu_ctx.set_tile_shape(0)
+    with pytest.raises(rt.RTError):
+        gpu_ctx.set_halo(0)
+    with pytest.raises(rt.RTError):
+        gpu_ctx.set_halo(13)
