@@ -34,13 +34,13 @@ HBM_PEAK_GBS = 8000.0
 # 4..6 run CFTP pairs split, one chain per workgroup.
 KERNEL_BY_SHAPE = ["k_fused<1, 2, 8, 8>", "k_fused<1, 2, 16, 4>", "k_fused<1, 2, 16, 2>",
                    "k_fused<2, 2, 8, 4>", "k_fused<1, 1, 8, 8>", "k_fused<1, 1, 16, 4>",
-                   "k_fused<1, 1, 16, 2>", None]
+                   "k_fused<1, 1, 16, 2>", "k_fused_o8<1>"]
 # Algorithmic bytes per site per launch: a pair workgroup reads the TinyMT
 # state (16 B) and parameters (12 B) once, writes the state (16 B), reads and
 # writes 3 colour planes x 2 chains (12 B) = 56 B; split pairs read state and
 # parameters in both workgroups (2 x 28 B), write the state once (16 B) and
 # the planes as before (12 B) = 84 B.
-ALG_BYTES_BY_SHAPE = [56, 56, 56, 56, 84, 84, 84, 0]
+ALG_BYTES_BY_SHAPE = [56, 56, 56, 56, 84, 84, 84, 56]
 
 
 def cpu_model():
@@ -552,7 +552,7 @@ def main():
     roof = {"bound": "hbm", "achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
             "kernel": kname, "alg_bytes_per_site": ALG_BYTES_BY_SHAPE[shape],
-            "kernel_ms_by_shape": [round(x, 1) for x in tm["shape_ms"][:7]],
+            "kernel_ms_by_shape": [round(x, 1) for x in tm["shape_ms"]],
             "avg_launch_us": round(avg_launch_s * 1e6, 2),
             "launches": klaunch, "kernel_time_share": round(kms / max(tm["kernel_ms"], 1e-9), 3),
             "sites_per_launch": int(sites_per_launch), "alg_bytes_per_launch": int(alg_bytes),

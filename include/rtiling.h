@@ -71,7 +71,8 @@ typedef struct {
   /* The same three split by sweep-block launch shape: index 0, 1, 2 =
    * 64 rows / 8 waves, 64 rows / 16 waves, 32 rows / 16 waves, 3 = 32 rows /
    * 8 waves for rect-triangle pairs, 4, 5, 6 = shapes 0, 1, 2 with CFTP
-   * pairs split over two workgroups per tile; no reference counterpart. */
+   * pairs split over two workgroups per tile, 7 = tile shape 7 below; no
+   * reference counterpart. */
   double shape_ms[8];
   long shape_launches[8];
   long shape_sites[8];
@@ -99,9 +100,11 @@ int rt_set_engine(rt_ctx* ctx, int engine);
  * 64 columns wide: 1 = 64 rows worked by 8 waves x 8 rows per lane, 2 = 64
  * rows by 16 waves x 4 rows, 3 = 32 rows by 16 waves x 2 rows; 4, 5, 6 =
  * shapes 1, 2, 3 with CFTP chain pairs split over two workgroups per tile
- * (each runs one chain, both draw the pair's shared numbers); 0 = chosen
- * per job (autotuner, else a rule of thumb on the tile count).  The serving
- * loop always uses 64-row tiles. */
+ * (each runs one chain, both draw the pair's shared numbers); 7 = shape 2
+ * with pair workgroups at eight waves per SIMD (two per CU, VGPRs capped
+ * at 64; one-chain jobs run shape 2); 0 = chosen per job (autotuner, else a
+ * rule of thumb on the tile count).  The serving loop always uses 64-row
+ * tiles. */
 int rt_set_tile_shape(rt_ctx* ctx, int shape);
 
 /* Tile-shape autotuner (no reference counterpart; on by default).  With

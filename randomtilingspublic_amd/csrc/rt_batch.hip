@@ -286,7 +286,7 @@ static int cftp_batch(rt_ctx* ctx, int model, const int32_t* tmax, const int32_t
     acc.kernel_ms += kms;
     acc.kernel_launches += nlaunch;
     acc.kernel_sites += nsites;
-    if (job.shape >= 1 && job.shape <= 4) {
+    if ((job.shape >= 1 && job.shape <= 4) || job.shape == 7) {
       const int sl = shape_slot(job.shape, job.split);
       acc.shape_ms[sl] += kms;
       acc.shape_launches[sl] += nlaunch;
@@ -551,7 +551,7 @@ static int walk_batch(rt_ctx* ctx, int32_t* const* tilings, const int* Ns, int n
     ctx->last.kernel_ms = kms;
     ctx->last.kernel_launches = nlaunch;
     ctx->last.kernel_sites = nsites;
-    if (job.shape >= 1 && job.shape <= 4) {
+    if ((job.shape >= 1 && job.shape <= 4) || job.shape == 7) {
       const int sl = shape_slot(job.shape, job.split);
       ctx->last.shape_ms[sl] = kms;
       ctx->last.shape_launches[sl] = nlaunch;

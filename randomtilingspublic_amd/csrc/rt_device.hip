@@ -306,7 +306,7 @@ extern "C" int rt_set_engine(rt_ctx* ctx, int engine) {
 }
 
 extern "C" int rt_set_tile_shape(rt_ctx* ctx, int shape) {
-  if (!ctx || shape < 0 || shape > 6) return RT_EINVAL;
+  if (!ctx || shape < 0 || shape > 7) return RT_EINVAL;
   ctx->tile_shape = shape;
   return RT_OK;
 }

@@ -161,10 +161,10 @@ __device__ __forceinline__ void run_window(Cell* lds, const Lanes& L, Tiny* st,
 // W wave64s per workgroup; each lane owns one tile column and R rows
 // (rows wv, wv+W, ...), so the loaded tile is (W*R) x 64 cells.
 template <int MODEL, int NC, int W, int R>
-__global__ __launch_bounds__(64 * W, 4) void k_fused(const UnitDesc* __restrict__ units,
-                                                    const WorkItem* __restrict__ work, int nwork,
-                                                    int xcd, int H, ModelParams mp,
-                                                    const WindowRec* __restrict__ wins) {
+__device__ __forceinline__ void fused_body(const UnitDesc* __restrict__ units,
+                                           const WorkItem* __restrict__ work, int nwork, int xcd,
+                                           int H, const ModelParams& mp,
+                                           const WindowRec* __restrict__ wins) {
   using Cell = typename CellOf<MODEL>::type;
   constexpr int NCOL = MODEL == M_DOMINO ? 2 : 3;
   constexpr int kThreads = 64 * W;
@@ -278,6 +278,29 @@ __global__ __launch_bounds__(64 * W, 4) void k_fused(const UnitDesc* __restrict_
   }
 }
 
+// Four waves per SIMD: a 16-wave workgroup alone on its CU, up to 128
+// VGPRs per lane.
+template <int MODEL, int NC, int W, int R>
+__global__ __launch_bounds__(64 * W, 4) void k_fused(const UnitDesc* __restrict__ units,
+                                                    const WorkItem* __restrict__ work, int nwork,
+                                                    int xcd, int H, ModelParams mp,
+                                                    const WindowRec* __restrict__ wins) {
+  fused_body<MODEL, NC, W, R>(units, work, nwork, xcd, H, mp, wins);
+}
+
+// 64-row chain-pair tiles (16 waves x 4 rows) at eight waves per SIMD: two
+// workgroups per CU, VGPRs capped at 64 (about 25 more spill to scratch).
+// Pays once a launch has several tiles per CU: 32 hexagon-300 CFTP samples
+// 137 us per launch against 155 for split pairs; 4 samples 34 against 28
+// for the plain 16x4 pairs.
+template <int MODEL>
+__global__ __launch_bounds__(1024, 8) void k_fused_o8(const UnitDesc* __restrict__ units,
+                                                     const WorkItem* __restrict__ work, int nwork,
+                                                     int xcd, int H, ModelParams mp,
+                                                     const WindowRec* __restrict__ wins) {
+  fused_body<MODEL, 2, 16, 4>(units, work, nwork, xcd, H, mp, wins);
+}
+
 }  // namespace rt
 
 #include "swar_lozenge.hpp"
@@ -314,11 +337,14 @@ long OpBuilder::append(const std::vector<uint8_t>& colours, std::vector<uint8_t>
 struct Geo {
   int W, R, H;
 };
-enum TileShape { SH_AUTO = 0, SH_8x8 = 1, SH_16x4 = 2, SH_16x2 = 3, SH_RT8x4 = 4 };
+// SH_16x4_O8 keeps the 16x4 geometry with pair workgroups at eight waves
+// per SIMD (k_fused_o8); one-chain jobs run it as SH_16x4.
+enum TileShape { SH_AUTO = 0, SH_8x8 = 1, SH_16x4 = 2, SH_16x2 = 3, SH_RT8x4 = 4, SH_16x4_O8 = 7 };
 
 static Geo shape_geo(int shape) {
   switch (shape) {
-    case SH_16x4: return Geo{16, 4, 6};
+    case SH_16x4:
+    case SH_16x4_O8: return Geo{16, 4, 6};
     case SH_16x2: return Geo{16, 2, 6};
     case SH_RT8x4: return Geo{8, 4, 4};
     default: return Geo{8, 8, 6};
@@ -364,14 +390,15 @@ static uint32_t tune_key(const FusedJob& job, long t64) {
 
 constexpr int kTrialWindows = 4;     // launches per timed trial
 constexpr int kTrialReps = 3;        // trials per candidate shape, interleaved
-constexpr int kTuneMinWindows = 2 * (1 + kTrialReps * 5 * kTrialWindows);  // else rule of thumb
+constexpr int kTuneMinWindows = 2 * (1 + kTrialReps * 6 * kTrialWindows);  // else rule of thumb
 struct LaunchShape {
   int shape;
   bool split;
 };
-static const LaunchShape kTuneShapes[5] = {
-    {SH_8x8, false}, {SH_16x4, false}, {SH_16x2, false}, {SH_16x4, true}, {SH_16x2, true}};
-static int tune_count(const FusedJob& job) { return job.nc == 2 ? 5 : 3; }
+static const LaunchShape kTuneShapes[6] = {{SH_8x8, false},  {SH_16x4, false},
+                                           {SH_16x2, false}, {SH_16x4, true},
+                                           {SH_16x2, true},  {SH_16x4_O8, false}};
+static int tune_count(const FusedJob& job) { return job.nc == 2 ? 6 : 3; }
 
 static bool tunable(const rt_ctx* ctx, const FusedJob& job) {
   return ctx->autotune && ctx->tile_shape == SH_AUTO && job.model != M_RECTTRI &&
@@ -384,6 +411,10 @@ static void choose_shape(const rt_ctx* ctx, FusedJob& job) {
   job.shape_source = 0;
   if (job.model == M_RECTTRI) {
     job.shape = job.nc == 2 ? SH_RT8x4 : SH_8x8;
+    return;
+  }
+  if (ctx->tile_shape == SH_16x4_O8) {
+    job.shape = job.nc == 2 ? SH_16x4_O8 : SH_16x4;
     return;
   }
   if (ctx->tile_shape != SH_AUTO) {  // 1..3, or 4..6 = the same with pairs split
@@ -533,8 +564,17 @@ static void launch(rt_ctx* ctx, unsigned nblocks, const UnitDesc* units, const W
 typedef void (*LaunchFn)(rt_ctx*, unsigned, const UnitDesc*, const WorkItem*, int,
                          const ModelParams&, const WindowRec*);
 
+template <int MODEL>
+static void launch_o8(rt_ctx* ctx, unsigned nblocks, const UnitDesc* units, const WorkItem* work,
+                      int H, const ModelParams& mp, const WindowRec* wins) {
+  const int xcd = xcd_of(ctx, nblocks);
+  hipLaunchKernelGGL((k_fused_o8<MODEL>), dim3(grid_of(xcd, nblocks)), dim3(1024), 0,
+                     ctx->stream, units, work, (int)nblocks, xcd, H, mp, wins);
+}
+
 template <int MODEL, int NC>
 static LaunchFn pick_byte(int shape) {
+  if (shape == SH_16x4_O8) return NC == 2 ? launch_o8<MODEL> : launch<MODEL, NC, 16, 4>;
   if (shape == SH_16x4) return launch<MODEL, NC, 16, 4>;
   if (shape == SH_16x2) return launch<MODEL, NC, 16, 2>;
   return launch<MODEL, NC, 8, 8>;

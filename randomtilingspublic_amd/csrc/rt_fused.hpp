@@ -194,8 +194,11 @@ struct FusedPlan {
 int fused_upload(rt_ctx* ctx, FusedJob& job, void* scratch, FusedPlan* plan);
 int fused_tune(rt_ctx* ctx, FusedJob& job, FusedPlan* plan);
 int fused_launch(rt_ctx* ctx, const FusedJob& job, const FusedPlan& plan);
-// Slot of a launch shape in rt_timing.shape_*: shape - 1, +4 when split.
-inline int shape_slot(int shape, bool split) { return shape - 1 + (split ? 4 : 0); }
+// Slot of a launch shape in rt_timing.shape_*: shape - 1, +4 when split;
+// 7 for 16x4 pairs at eight waves per SIMD (shape 7).
+inline int shape_slot(int shape, bool split) {
+  return shape == 7 ? 7 : shape - 1 + (split ? 4 : 0);
+}
 // Add the plan's tuning launches to per-shape kernel timing.
 void fused_add_trials(const FusedPlan& plan, rt_timing* acc);
 

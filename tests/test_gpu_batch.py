@@ -100,7 +100,8 @@ def test_tile_shape_choice_never_changes_results(gpu_ctx):
     masters = [3, 8, 13]
     ref = None
     try:
-        for mode, shape in [(0, 0), (2, 0), (1, 0), (0, 1), (0, 2), (0, 3), (0, 4), (0, 5), (0, 6)]:
+        for mode, shape in [(0, 0), (2, 0), (1, 0), (0, 1), (0, 2), (0, 3), (0, 4), (0, 5), (0, 6),
+                            (0, 7)]:
             _say("lozenge cftp batch autotune", mode, "shape", shape)
             gpu_ctx.set_autotune(mode)
             gpu_ctx.set_tile_shape(shape)
@@ -108,8 +109,8 @@ def test_tile_shape_choice_never_changes_results(gpu_ctx):
                                                      30)
             tm = gpu_ctx.last_timing()
             if shape:
-                assert tm["shape"] == (shape - 3 if shape > 3 else shape)
-                assert tm["shape_source"] == 0 and tm["split"] == (shape > 3)
+                assert tm["shape"] == (shape - 3 if 3 < shape < 7 else shape)
+                assert tm["shape_source"] == 0 and tm["split"] == (3 < shape < 7)
             if mode == 2:
                 assert tm["shape_source"] == 1, tm
             if ref is None:
@@ -134,7 +135,7 @@ def test_cftp_pair_with_equal_chains_runs_one_chain_exactly(gpu_ctx):
     P = rt.synthetic_params(N * N, 26)
     gpu_ctx.set_tinymt_params(P)
     try:
-        for shape in (1, 2, 3, 6):
+        for shape in (1, 2, 3, 6, 7):
             _say("lozenge cftp equal chains shape", shape)
             gpu_ctx.set_tile_shape(shape)
             outs, stats = gpu_ctx.lozenge_cftp_batch(tmin, tmin, [5, 6], rt.SCHED_DOUBLING, 300, 4)

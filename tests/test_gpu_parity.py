@@ -207,11 +207,12 @@ def test_engines_agree_with_oracle(gpu_ctx, engine):
         gpu_ctx.set_engine(rt.ENGINE_AUTO)
 
 
-@pytest.mark.parametrize("shape", [1, 2, 3, 4, 5, 6])
+@pytest.mark.parametrize("shape", [1, 2, 3, 4, 5, 6, 7])
 def test_tile_shapes_agree_with_oracle(gpu_ctx, shape):
     """Every sweep-block tile shape (64 rows / 8 waves, 64 rows / 16 waves,
-    32 rows / 16 waves) on domains spanning several tiles, for a walk of each
-    byte-cell model and a batched CFTP."""
+    32 rows / 16 waves; pairs split; 64 rows / 16 waves at eight waves per
+    SIMD) on domains spanning several tiles, for a walk of each byte-cell
+    model and a batched CFTP of each."""
     gpu_ctx.set_tile_shape(shape)
     try:
         d, tmin, tmax = _domino(45)
@@ -235,6 +236,16 @@ def test_tile_shapes_agree_with_oracle(gpu_ctx, shape):
         outs, stats = gpu_ctx.lozenge_cftp_batch(lmax, lmin, [4, 5, 6], rt.SCHED_DOUBLING, 64, 40)
         for k, master in enumerate([4, 5, 6]):
             w, ws = o.lozenge_cftp(lmax, lmin, master, PL, rt.SCHED_DOUBLING, 64, 40)
+            assert np.array_equal(outs[k], w)
+            assert stats[k]["rounds"] == ws["rounds"]
+        d, tmin, tmax = _domino(7)
+        N = rt.side(tmin)
+        P = _params(N * N // 2, seed=15)
+        gpu_ctx.set_tinymt_params(P)
+        _say("shape", shape, "domino-7 cftp batch")
+        outs, stats = gpu_ctx.domino_cftp_batch(tmax, tmin, [2, 3], rt.SCHED_DOUBLING, 64, 30)
+        for k, master in enumerate([2, 3]):
+            w, ws = o.domino_cftp(tmax, tmin, master, P, rt.SCHED_DOUBLING, 64, 30)
             assert np.array_equal(outs[k], w)
             assert stats[k]["rounds"] == ws["rounds"]
     finally:
