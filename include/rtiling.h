@@ -108,7 +108,7 @@ int rt_set_engine(rt_ctx* ctx, int engine);
 int rt_set_tile_shape(rt_ctx* ctx, int shape);
 
 /* Tile-shape autotuner (no reference counterpart; on by default).  With
- * shape 0 above, a sweep-block job of at least 122 launch windows whose
+ * shape 0 above, a sweep-block job of at least 146 launch windows whose
  * kind (model, chains, tile count to a factor of two) has not been tuned
  * on this context runs its first windows split among the candidate
  * shapes, timed with events, and keeps the fastest for the rest of the
