@@ -136,7 +136,8 @@ int rt_set_halo(rt_ctx* ctx, int halo);
  * reference's driver loop, AztecDiamondCftpEx.cpp:57-77, reports nothing
  * until it has coupled).  After each round's coalescence test, in the
  * calling thread: fn(user, round, steps of the round, samples finished so
- * far, samples in the call).  fn = NULL turns it off (the default). */
+ * far, samples in the call).  fn must not call into the library with the
+ * same context.  fn = NULL turns it off (the default). */
 typedef void (*rt_round_fn)(void* user, int round, long round_steps, int finished, int total);
 int rt_set_round_callback(rt_ctx* ctx, rt_round_fn fn, void* user);
 
