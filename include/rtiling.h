@@ -108,11 +108,11 @@ int rt_set_engine(rt_ctx* ctx, int engine);
 int rt_set_tile_shape(rt_ctx* ctx, int shape);
 
 /* Tile-shape autotuner (no reference counterpart; on by default).  With
- * shape 0 above, a sweep-block job of at least 146 launch windows whose
- * kind (model, chains, tile count to a factor of two) has not been tuned
- * on this context runs its first windows split among the candidate
- * shapes, timed with events, and keeps the fastest for the rest of the
- * job and for later jobs of that kind.  Results never depend on the shape.
+ * shape 0 above, a sweep-block job of at least 194 launch windows (98
+ * for one-chain walks) whose kind (model, chains, tile count to a factor
+ * of two) has not been tuned on this context runs its first windows split
+ * among the candidate shapes, timed with events, and keeps the fastest for
+ * the rest of the job and for later jobs of that kind.  Results never depend on the shape.
  * mode 0 = off (rule of thumb on the tile count), 1 = on, 2 = on and forget
  * earlier choices. */
 int rt_set_autotune(rt_ctx* ctx, int mode);

@@ -59,21 +59,27 @@ __device__ __forceinline__ uint32_t pick(uint32_t mask_bits, int r, uint32_t yes
 // read and the plane written differ, but the compiler cannot see that from
 // runtime colour offsets, and stores issued between reads would serialise
 // every row behind an LDS round trip.
-template <int MODEL, int NC, int W, int R, int PULL, class Cell, int RS, int PL>
-__device__ __forceinline__ void sweep_step(Cell* lds, const Lanes& L, Tiny* st, uint32_t op,
-                                           const UnitDesc& U, const ModelParams& mp) {
-  constexpr int NCOL = MODEL == M_DOMINO ? 2 : 3;
-  const int c = op & OP_COLOUR;
-  // Lozenge rows of up to 4 cells per lane turn the draw into the move's
-  // target once for both chains; 8-row lanes keep the draw (the targets
-  // would push the pair kernel past 128 VGPRs).
-  constexpr bool kTarget = MODEL == M_LOZENGE && R <= 4;
-  uint32_t b[R];  // the row's draw, or its move target
+// Lozenge rows of up to 4 cells per lane turn the draw into the move's
+// target once for both chains; 8-row lanes keep the draw (the targets would
+// push the pair kernel past 128 VGPRs).
+template <int MODEL, int R>
+constexpr bool kTarget = MODEL == M_LOZENGE && R <= 4;
+
+// The next step's draw of every row (its move target for kTarget).
+template <int MODEL, int R>
+__device__ __forceinline__ void step_draws(Tiny* st, const Lanes& L, uint32_t* b) {
 #pragma unroll
   for (int r = 0; r < R; r++) {
     b[r] = tiny_bits(st[r]);
-    if (kTarget) b[r] = lozenge_target(b[r], (L.hsm >> r) & 1u);
+    if (kTarget<MODEL, R>) b[r] = lozenge_target(b[r], (L.hsm >> r) & 1u);
   }
+}
+
+template <int MODEL, int NC, int W, int R, int PULL, class Cell, int RS, int PL>
+__device__ __forceinline__ void sweep_step(Cell* lds, const Lanes& L, const uint32_t* b,
+                                           uint32_t op, const UnitDesc& U, const ModelParams& mp) {
+  constexpr int NCOL = MODEL == M_DOMINO ? 2 : 3;
+  const int c = op & OP_COLOUR;
   uint32_t out[NC][R];
 #pragma unroll
   for (int ch = 0; ch < NC; ch++) {
@@ -100,7 +106,7 @@ __device__ __forceinline__ void sweep_step(Cell* lds, const Lanes& L, Tiny* st, 
           if (PULL & 2) pv = lozenge_pull_prev2_m(pv, P + ((c + 1) % 3) * PL + off, RS, im, (c + 1) % 3);
           v = pick(L.interm, r, pv, v);
         }
-        if (kTarget) {  // the target already holds the has-stream test
+        if (kTarget<MODEL, R>) {  // the target already holds the has-stream test
           out[ch][r] = lozenge_apply(v, b[r]);
           continue;
         }
@@ -135,24 +141,41 @@ __device__ __forceinline__ int work_index(int nwork, int xcd) {
   return b < (unsigned)nwork ? (int)b : -1;
 }
 
-// The K colour steps of a window on the LDS tile, NCR chains.
-template <int MODEL, int NCR, int W, int R, class Cell, int RS, int PL>
+// The K colour steps of a window on the LDS tile, NCR chains.  PIPE: the
+// next step's op byte and draws are taken before the barrier, so a step
+// starts with its LDS loads instead of an op read and the TinyMT
+// arithmetic.  Measured per launch on the side-300 hexagon CFTP: 16x4
+// pairs at eight waves per SIMD (32 samples) 151 -> 139 us, but 16x4 pairs
+// at four waves (4 samples) 28.0 -> 29.6 and 16x4 split pairs (32 samples)
+// 155 -> 160, so only the eight-wave kernel takes it.
+template <int MODEL, int NCR, int W, int R, bool PIPE, class Cell, int RS, int PL>
 __device__ __forceinline__ void run_window(Cell* lds, const Lanes& L, Tiny* st,
                                            const uint8_t* sops, int K, const UnitDesc& U,
                                            const ModelParams& mp) {
+  if (K <= 0) return;
+  uint32_t b[R];
+  if (PIPE) step_draws<MODEL, R>(st, L, b);
+  uint32_t op = sops[0];
   for (int s = 0; s < K; s++) {
-    const uint32_t op = sops[s];
+    if (!PIPE) {
+      op = sops[s];
+      step_draws<MODEL, R>(st, L, b);
+    }
     const int pull = (op >> 2) & 3;
     if (MODEL == M_LOZENGE) {
       switch (pull) {
-        case 0: sweep_step<MODEL, NCR, W, R, 0, Cell, RS, PL>(lds, L, st, op, U, mp); break;
-        case 1: sweep_step<MODEL, NCR, W, R, 1, Cell, RS, PL>(lds, L, st, op, U, mp); break;
-        case 2: sweep_step<MODEL, NCR, W, R, 2, Cell, RS, PL>(lds, L, st, op, U, mp); break;
-        default: sweep_step<MODEL, NCR, W, R, 3, Cell, RS, PL>(lds, L, st, op, U, mp); break;
+        case 0: sweep_step<MODEL, NCR, W, R, 0, Cell, RS, PL>(lds, L, b, op, U, mp); break;
+        case 1: sweep_step<MODEL, NCR, W, R, 1, Cell, RS, PL>(lds, L, b, op, U, mp); break;
+        case 2: sweep_step<MODEL, NCR, W, R, 2, Cell, RS, PL>(lds, L, b, op, U, mp); break;
+        default: sweep_step<MODEL, NCR, W, R, 3, Cell, RS, PL>(lds, L, b, op, U, mp); break;
       }
     } else {
-      if (pull & 1) sweep_step<MODEL, NCR, W, R, 1, Cell, RS, PL>(lds, L, st, op, U, mp);
-      else sweep_step<MODEL, NCR, W, R, 0, Cell, RS, PL>(lds, L, st, op, U, mp);
+      if (pull & 1) sweep_step<MODEL, NCR, W, R, 1, Cell, RS, PL>(lds, L, b, op, U, mp);
+      else sweep_step<MODEL, NCR, W, R, 0, Cell, RS, PL>(lds, L, b, op, U, mp);
+    }
+    if (PIPE && s + 1 < K) {
+      op = sops[s + 1];
+      step_draws<MODEL, R>(st, L, b);
     }
     __syncthreads();
   }
@@ -160,7 +183,7 @@ __device__ __forceinline__ void run_window(Cell* lds, const Lanes& L, Tiny* st,
 
 // W wave64s per workgroup; each lane owns one tile column and R rows
 // (rows wv, wv+W, ...), so the loaded tile is (W*R) x 64 cells.
-template <int MODEL, int NC, int W, int R>
+template <int MODEL, int NC, int W, int R, bool PIPE>
 __device__ __forceinline__ void fused_body(const UnitDesc* __restrict__ units,
                                            const WorkItem* __restrict__ work, int nwork, int xcd,
                                            int H, const ModelParams& mp,
@@ -253,8 +276,8 @@ __device__ __forceinline__ void fused_body(const UnitDesc* __restrict__ units,
     for (int k = threadIdx.x; k < kWords; k += kThreads) diff |= w32[k] != w32[kWords + k];
     one = !__syncthreads_or(diff);
   }
-  if (kOne && one) run_window<MODEL, 1, W, R, Cell, RS, PL>(lds, L, st, sops, K, U, mp);
-  else run_window<MODEL, NC, W, R, Cell, RS, PL>(lds, L, st, sops, K, U, mp);
+  if (kOne && one) run_window<MODEL, 1, W, R, PIPE, Cell, RS, PL>(lds, L, st, sops, K, U, mp);
+  else run_window<MODEL, NC, W, R, PIPE, Cell, RS, PL>(lds, L, st, sops, K, U, mp);
 
 #pragma unroll
   for (int r = 0; r < R; r++) {
@@ -285,7 +308,7 @@ __global__ __launch_bounds__(64 * W, 4) void k_fused(const UnitDesc* __restrict_
                                                     const WorkItem* __restrict__ work, int nwork,
                                                     int xcd, int H, ModelParams mp,
                                                     const WindowRec* __restrict__ wins) {
-  fused_body<MODEL, NC, W, R>(units, work, nwork, xcd, H, mp, wins);
+  fused_body<MODEL, NC, W, R, false>(units, work, nwork, xcd, H, mp, wins);
 }
 
 // 64-row chain-pair tiles (16 waves x 4 rows) at eight waves per SIMD: two
@@ -298,7 +321,7 @@ __global__ __launch_bounds__(1024, 8) void k_fused_o8(const UnitDesc* __restrict
                                                      const WorkItem* __restrict__ work, int nwork,
                                                      int xcd, int H, ModelParams mp,
                                                      const WindowRec* __restrict__ wins) {
-  fused_body<MODEL, 2, 16, 4>(units, work, nwork, xcd, H, mp, wins);
+  fused_body<MODEL, 2, 16, 4, true>(units, work, nwork, xcd, H, mp, wins);
 }
 
 }  // namespace rt
@@ -389,8 +412,10 @@ static uint32_t tune_key(const FusedJob& job, long t64) {
 }
 
 constexpr int kTrialWindows = 4;     // launches per timed trial
-constexpr int kTrialReps = 3;        // trials per candidate shape, interleaved
-constexpr int kTuneMinWindows = 2 * (1 + kTrialReps * 6 * kTrialWindows);  // else rule of thumb
+// Trials per candidate shape, interleaved; the first round of trials is
+// not counted: a kernel's first launch can pay one-off costs (the scratch
+// setup of a kernel that spills), which would bias a 4-launch trial.
+constexpr int kTrialReps = 4;
 struct LaunchShape {
   int shape;
   bool split;
@@ -399,6 +424,11 @@ static const LaunchShape kTuneShapes[6] = {{SH_8x8, false},  {SH_16x4, false},
                                            {SH_16x2, false}, {SH_16x4, true},
                                            {SH_16x2, true},  {SH_16x4_O8, false}};
 static int tune_count(const FusedJob& job) { return job.nc == 2 ? 6 : 3; }
+// A job is tuned when at least half its launches run on the chosen shape
+// (else the rule of thumb): 194 launches for chain pairs, 98 for one chain.
+static size_t tune_min_windows(const FusedJob& job) {
+  return 2 * (1 + (size_t)kTrialReps * tune_count(job) * kTrialWindows);
+}
 
 static bool tunable(const rt_ctx* ctx, const FusedJob& job) {
   return ctx->autotune && ctx->tile_shape == SH_AUTO && job.model != M_RECTTRI &&
@@ -432,7 +462,7 @@ static void choose_shape(const rt_ctx* ctx, FusedJob& job) {
     job.shape_source = 2;
     return;
   }
-  job.tune = job.nlaunch >= (size_t)kTuneMinWindows;
+  job.tune = job.nlaunch >= tune_min_windows(job);
 }
 
 // Launch windows of each unit: at most kKMax steps and at most H of the
@@ -803,7 +833,7 @@ int fused_tune(rt_ctx* ctx, FusedJob& job, FusedPlan* plan) {
     plan->trial_ms[sl] += e;
     plan->trial_launches[sl] += (long)(trials[k].l1 - trials[k].l0);
     plan->trial_sites[sl] += plan->sites(trials[k].l0, trials[k].l1);
-    if (k == 0) continue;  // the seeding launch is not a fair trial
+    if (k <= nc) continue;  // the seeding launch and the first round are not fair trials
     ms[c] += e;
     for (size_t x = trials[k].l0; x < trials[k].l1; x++) work[c] += (double)plan->launch_work[x];
   }

@@ -50,7 +50,7 @@ struct rt_ctx {
   std::map<uint32_t, int> tuned;  // tuning key (model, chains, tile count) -> shape
   rt_round_fn round_fn = nullptr;  // CFTP round-end callback
   void* round_user = nullptr;
-  hipEvent_t evt[24] = {};        // tuner trial boundaries
+  hipEvent_t evt[32] = {};        // tuner trial boundaries
 
   int reserve(size_t bytes);
   int reserve_aux(size_t bytes);
