@@ -311,17 +311,16 @@ __global__ __launch_bounds__(64 * W, 4) void k_fused(const UnitDesc* __restrict_
   fused_body<MODEL, NC, W, R, false>(units, work, nwork, xcd, H, mp, wins);
 }
 
-// 64-row chain-pair tiles (16 waves x 4 rows) at eight waves per SIMD: two
-// workgroups per CU, VGPRs capped at 64 (about 25 more spill to scratch).
-// Pays once a launch has several tiles per CU: 32 hexagon-300 CFTP samples
-// 137 us per launch against 155 for split pairs; 4 samples 34 against 28
-// for the plain 16x4 pairs.
+// 48-row chain-pair tiles (16 waves x 3 rows) at eight waves per SIMD: two
+// workgroups per CU within 64 VGPRs, with no spills in the step loop (4
+// rows per lane would spill ~25 VGPRs to scratch every step).  Pays once a
+// launch has several tiles per CU.
 template <int MODEL>
 __global__ __launch_bounds__(1024, 8) void k_fused_o8(const UnitDesc* __restrict__ units,
                                                      const WorkItem* __restrict__ work, int nwork,
                                                      int xcd, int H, ModelParams mp,
                                                      const WindowRec* __restrict__ wins) {
-  fused_body<MODEL, 2, 16, 4, true>(units, work, nwork, xcd, H, mp, wins);
+  fused_body<MODEL, 2, 16, 3, true>(units, work, nwork, xcd, H, mp, wins);
 }
 
 }  // namespace rt
@@ -360,14 +359,14 @@ long OpBuilder::append(const std::vector<uint8_t>& colours, std::vector<uint8_t>
 struct Geo {
   int W, R, H;
 };
-// SH_16x4_O8 keeps the 16x4 geometry with pair workgroups at eight waves
-// per SIMD (k_fused_o8); one-chain jobs run it as SH_16x4.
-enum TileShape { SH_AUTO = 0, SH_8x8 = 1, SH_16x4 = 2, SH_16x2 = 3, SH_RT8x4 = 4, SH_16x4_O8 = 7 };
+// SH_16x3_O8: 48-row tiles of chain pairs at eight waves per SIMD
+// (k_fused_o8); one-chain jobs run SH_16x4 instead.
+enum TileShape { SH_AUTO = 0, SH_8x8 = 1, SH_16x4 = 2, SH_16x2 = 3, SH_RT8x4 = 4, SH_16x3_O8 = 7 };
 
 static Geo shape_geo(int shape) {
   switch (shape) {
-    case SH_16x4:
-    case SH_16x4_O8: return Geo{16, 4, 6};
+    case SH_16x4: return Geo{16, 4, 6};
+    case SH_16x3_O8: return Geo{16, 3, 6};
     case SH_16x2: return Geo{16, 2, 6};
     case SH_RT8x4: return Geo{8, 4, 4};
     default: return Geo{8, 8, 6};
@@ -422,7 +421,7 @@ struct LaunchShape {
 };
 static const LaunchShape kTuneShapes[6] = {{SH_8x8, false},  {SH_16x4, false},
                                            {SH_16x2, false}, {SH_16x4, true},
-                                           {SH_16x2, true},  {SH_16x4_O8, false}};
+                                           {SH_16x2, true},  {SH_16x3_O8, false}};
 static int tune_count(const FusedJob& job) { return job.nc == 2 ? 6 : 3; }
 // A job is tuned when at least half its launches run on the chosen shape
 // (else the rule of thumb): 194 launches for chain pairs, 98 for one chain.
@@ -443,8 +442,8 @@ static void choose_shape(const rt_ctx* ctx, FusedJob& job) {
     job.shape = job.nc == 2 ? SH_RT8x4 : SH_8x8;
     return;
   }
-  if (ctx->tile_shape == SH_16x4_O8) {
-    job.shape = job.nc == 2 ? SH_16x4_O8 : SH_16x4;
+  if (ctx->tile_shape == SH_16x3_O8) {
+    job.shape = job.nc == 2 ? SH_16x3_O8 : SH_16x4;
     return;
   }
   if (ctx->tile_shape != SH_AUTO) {  // 1..3, or 4..6 = the same with pairs split
@@ -604,7 +603,9 @@ static void launch_o8(rt_ctx* ctx, unsigned nblocks, const UnitDesc* units, cons
 
 template <int MODEL, int NC>
 static LaunchFn pick_byte(int shape) {
-  if (shape == SH_16x4_O8) return NC == 2 ? launch_o8<MODEL> : launch<MODEL, NC, 16, 4>;
+  // (one-chain jobs are given SH_16x4 instead; the 16x3 kernel keeps the
+  // tile geometry right should one get here)
+  if (shape == SH_16x3_O8) return NC == 2 ? launch_o8<MODEL> : launch<MODEL, NC, 16, 3>;
   if (shape == SH_16x4) return launch<MODEL, NC, 16, 4>;
   if (shape == SH_16x2) return launch<MODEL, NC, 16, 2>;
   return launch<MODEL, NC, 8, 8>;
