@@ -4,8 +4,9 @@
 Default workload (the headline configuration): uniform lozenge tilings of
 the hexagon with sides a=b=c=300, exact samples by coupling from the past
 (top = MaxTiling, bottom = MinTiling, look-back doubling from 2^10 steps),
-one sample per master seed, seeds 1..K, all K samples advanced together by
-the batched fused kernel.  A "step" of this benchmark is one exact sample
+one sample per master seed, seeds 1..K (K = 32 by default, the full
+configuration), all K samples advanced together by the batched fused
+kernel.  A "step" of this benchmark is one exact sample
 (one master seed); --steps K seeds are timed, --warmup W short walks run
 first.  Synthetic TinyMT parameter table (the TINYMT32J set for every
 stream; the reference's 2^20-line table is not in its checkout).
@@ -31,7 +32,8 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # Sweep-block kernel instantiation per launch-shape slot (rt_timing.shape_*[k]):
 # k_fused<model 1 = lozenge, chains per workgroup, waves, rows per lane>; slots
-# 4..6 run CFTP pairs split, one chain per workgroup.
+# 4..6 run CFTP pairs split, one chain per workgroup; slot 7 is the 48-row
+# pair kernel at eight waves per SIMD.
 KERNEL_BY_SHAPE = ["k_fused<1, 2, 8, 8>", "k_fused<1, 2, 16, 4>", "k_fused<1, 2, 16, 2>",
                    "k_fused<2, 2, 8, 4>", "k_fused<1, 1, 8, 8>", "k_fused<1, 1, 16, 4>",
                    "k_fused<1, 1, 16, 2>", "k_fused_o8<1>"]
@@ -415,7 +417,8 @@ def cpu_c1(threads):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=4, help="CFTP samples (master seeds 1..K)")
+    ap.add_argument("--steps", type=int, default=32,
+                    help="CFTP samples (master seeds 1..K; 32 = the full configuration)")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--side", type=int, default=300)
     ap.add_argument("--t0", type=int, default=1024)
