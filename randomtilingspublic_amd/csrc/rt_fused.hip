@@ -442,8 +442,9 @@ static void choose_shape(const rt_ctx* ctx, FusedJob& job) {
     job.shape = job.nc == 2 ? SH_RT8x4 : SH_8x8;
     return;
   }
-  if (ctx->tile_shape == SH_16x3_O8) {
-    job.shape = job.nc == 2 ? SH_16x3_O8 : SH_16x4;
+  if (ctx->tile_shape == SH_16x3_O8) {  // chain pairs of the byte-cell kernels only
+    const bool swar = job.model == M_LOZENGE && ctx->engine == RT_ENGINE_FUSED_SWAR;
+    job.shape = job.nc == 2 && !swar ? SH_16x3_O8 : SH_16x4;
     return;
   }
   if (ctx->tile_shape != SH_AUTO) {  // 1..3, or 4..6 = the same with pairs split
@@ -624,6 +625,7 @@ static void launch_swar(rt_ctx* ctx, unsigned nblocks, const UnitDesc* units, co
 template <int NC>
 static LaunchFn pick_swar(int shape) {
   if (shape == SH_16x2) return launch_swar<NC, 8, 1>;
+  if (shape == SH_16x3_O8) return nullptr;  // no 48-row SWAR kernel (choose_shape never asks)
   return launch_swar<NC, 16, 1>;
 }
 
@@ -820,6 +822,10 @@ int fused_tune(rt_ctx* ctx, FusedJob& job, FusedPlan* plan) {
   for (size_t k = 0; k < trials.size(); k++) {
     const FusedPlan::Tiles& t = plan->cand[trials[k].cand];
     LaunchFn fn = pick(ctx, job.model, t.split ? 1 : job.nc, t.shape);
+    if (!fn) {
+      ctx->err = "no sweep-block kernel for this tile shape";
+      return RT_EINVAL;
+    }
     for (size_t x = trials[k].l0; x < trials[k].l1; x++)
       fn(ctx, t.nwork, t.d_units, t.d_work, H, job.mp, t.d_wins + x * t.nunits);
     HIPCHK_RET(ctx, hipGetLastError());
@@ -876,6 +882,10 @@ int fused_launch(rt_ctx* ctx, const FusedJob& job, const FusedPlan& plan) {
   if (plan.cand.empty()) return RT_OK;  // nothing to run
   const FusedPlan::Tiles& t = plan.cand[0];
   LaunchFn fn = pick(ctx, job.model, t.split ? 1 : job.nc, t.shape);
+  if (!fn) {
+    ctx->err = "no sweep-block kernel for this tile shape";
+    return RT_EINVAL;
+  }
   const int H = job.halo;
   for (size_t l = plan.first; l < plan.nlaunch; l++)
     fn(ctx, t.nwork, t.d_units, t.d_work, H, job.mp, t.d_wins + l * t.nunits);

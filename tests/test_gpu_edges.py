@@ -1,7 +1,8 @@
 """Edge cases of the batched paths on the device: a CFTP batch that cannot
 coalesce within its round cap fails with ENOCOUPLE and leaves the context
 usable; zero-step and one-chain batch walks; malformed batch inputs; the
-XCD tile order, the halo depth; the CFTP round callback."""
+XCD tile order, the halo depth; the CFTP round callback; forced tile
+shapes under the four-cells-per-word lozenge kernel."""
 import sys
 
 import numpy as np
@@ -143,3 +144,27 @@ def test_halo_depth_never_changes_results(gpu_ctx, halo):
         gpu_ctx.set_halo(0)
     with pytest.raises(rt.RTError):
         gpu_ctx.set_halo(13)
+
+
+@pytest.mark.parametrize("shape", [0, 2, 3, 6, 7])
+def test_swar_engine_cftp_batch_with_forced_shapes(gpu_ctx, shape):
+    # The four-cells-per-word lozenge kernel has 64- and 32-row tiles only;
+    # a forced shape without a SWAR kernel (7) falls back to 64-row tiles.
+    d = rt.LozengeTiler.AlmostHexDomain(12, 0)
+    tmin, tmax = rt.LozengeTiler.MinTiling(d), rt.LozengeTiler.MaxTiling(d)
+    N = rt.side(tmin)
+    P = rt.synthetic_params(N * N, 38)
+    gpu_ctx.set_tinymt_params(P)
+    try:
+        gpu_ctx.set_engine(rt.ENGINE_FUSED_SWAR)
+        gpu_ctx.set_tile_shape(shape)
+        _say("swar lozenge cftp batch shape", shape)
+        outs, _ = gpu_ctx.lozenge_cftp_batch(tmax, tmin, [2, 3], rt.SCHED_DOUBLING, 256, 20)
+        for k, m in enumerate((2, 3)):
+            want, _ = o.lozenge_cftp(tmax, tmin, m, P, rt.SCHED_DOUBLING, 256, 20)
+            assert np.array_equal(outs[k], want)
+        if shape == 7:
+            assert gpu_ctx.last_timing()["shape"] == 2
+    finally:
+        gpu_ctx.set_engine(rt.ENGINE_AUTO)
+        gpu_ctx.set_tile_shape(0)
