@@ -101,9 +101,9 @@ int rt_set_engine(rt_ctx* ctx, int engine);
  * rows by 16 waves x 4 rows, 3 = 32 rows by 16 waves x 2 rows; 4, 5, 6 =
  * shapes 1, 2, 3 with CFTP chain pairs split over two workgroups per tile
  * (each runs one chain, both draw the pair's shared numbers); 7 = 48 rows
- * by 16 waves x 3 rows with chain-pair workgroups at eight waves per SIMD
- * (two per CU, VGPRs capped at 64; one-chain jobs run shape 2); 0 = chosen
- * per job (autotuner, else a rule of thumb on the tile count).  The serving loop always uses 64-row
+ * by 16 waves x 3 rows at eight waves per SIMD (two workgroups per CU,
+ * VGPRs capped at 64); 0 = chosen per job (autotuner, else a rule of
+ * thumb on the tile count).  The serving loop always uses 64-row
  * tiles. */
 int rt_set_tile_shape(rt_ctx* ctx, int shape);
 

@@ -255,8 +255,8 @@ class Context:
     def set_tile_shape(self, shape):
         """Sweep-block tile shape: 0 auto, 1 = 64 rows / 8 waves, 2 = 64 rows /
         16 waves, 3 = 32 rows / 16 waves; 4..6 = 1..3 with CFTP chain pairs
-        split over two workgroups per tile; 7 = 48 rows / 16 waves with pair
-        workgroups at eight waves per SIMD."""
+        split over two workgroups per tile; 7 = 48 rows / 16 waves at eight
+        waves per SIMD."""
         self._chk(self._L.rt_set_tile_shape(self._h, shape))
 
     def device_limits(self):

@@ -323,6 +323,15 @@ __global__ __launch_bounds__(1024, 8) void k_fused_o8(const UnitDesc* __restrict
   fused_body<MODEL, 2, 16, 3, true>(units, work, nwork, xcd, H, mp, wins);
 }
 
+// The same tiles for one-chain jobs (walks, batched walks).
+template <int MODEL>
+__global__ __launch_bounds__(1024, 8) void k_walk_o8(const UnitDesc* __restrict__ units,
+                                                    const WorkItem* __restrict__ work, int nwork,
+                                                    int xcd, int H, ModelParams mp,
+                                                    const WindowRec* __restrict__ wins) {
+  fused_body<MODEL, 1, 16, 3, true>(units, work, nwork, xcd, H, mp, wins);
+}
+
 }  // namespace rt
 
 #include "swar_lozenge.hpp"
@@ -359,8 +368,8 @@ long OpBuilder::append(const std::vector<uint8_t>& colours, std::vector<uint8_t>
 struct Geo {
   int W, R, H;
 };
-// SH_16x3_O8: 48-row tiles of chain pairs at eight waves per SIMD
-// (k_fused_o8); one-chain jobs run SH_16x4 instead.
+// SH_16x3_O8: 48-row tiles at eight waves per SIMD (k_fused_o8 for chain
+// pairs, k_walk_o8 for one chain).
 enum TileShape { SH_AUTO = 0, SH_8x8 = 1, SH_16x4 = 2, SH_16x2 = 3, SH_RT8x4 = 4, SH_16x3_O8 = 7 };
 
 static Geo shape_geo(int shape) {
@@ -419,10 +428,16 @@ struct LaunchShape {
   int shape;
   bool split;
 };
-static const LaunchShape kTuneShapes[6] = {{SH_8x8, false},  {SH_16x4, false},
-                                           {SH_16x2, false}, {SH_16x4, true},
-                                           {SH_16x2, true},  {SH_16x3_O8, false}};
-static int tune_count(const FusedJob& job) { return job.nc == 2 ? 6 : 3; }
+// Candidates: the first tune_count(job) of the job's list.
+static const LaunchShape kTunePairs[6] = {{SH_8x8, false},  {SH_16x4, false},
+                                          {SH_16x2, false}, {SH_16x4, true},
+                                          {SH_16x2, true},  {SH_16x3_O8, false}};
+static const LaunchShape kTuneOne[4] = {
+    {SH_8x8, false}, {SH_16x4, false}, {SH_16x2, false}, {SH_16x3_O8, false}};
+static int tune_count(const FusedJob& job) { return job.nc == 2 ? 6 : 4; }
+static LaunchShape tune_shape(const FusedJob& job, int c) {
+  return job.nc == 2 ? kTunePairs[c] : kTuneOne[c];
+}
 // A job is tuned when at least half its launches run on the chosen shape
 // (else the rule of thumb): 194 launches for chain pairs, 98 for one chain.
 static size_t tune_min_windows(const FusedJob& job) {
@@ -442,9 +457,9 @@ static void choose_shape(const rt_ctx* ctx, FusedJob& job) {
     job.shape = job.nc == 2 ? SH_RT8x4 : SH_8x8;
     return;
   }
-  if (ctx->tile_shape == SH_16x3_O8) {  // chain pairs of the byte-cell kernels only
+  if (ctx->tile_shape == SH_16x3_O8) {  // byte-cell kernels only (no 48-row SWAR kernel)
     const bool swar = job.model == M_LOZENGE && ctx->engine == RT_ENGINE_FUSED_SWAR;
-    job.shape = job.nc == 2 && !swar ? SH_16x3_O8 : SH_16x4;
+    job.shape = swar ? SH_16x4 : SH_16x3_O8;
     return;
   }
   if (ctx->tile_shape != SH_AUTO) {  // 1..3, or 4..6 = the same with pairs split
@@ -566,7 +581,7 @@ size_t fused_scratch_bytes(rt_ctx* ctx, FusedJob& job) {
                  ((job.nlaunch * nu * sizeof(WindowRec) + 255) & ~(size_t)255) + 1024;
   if (job.tune)
     for (int c = 0; c < tune_count(job); c++)
-      bytes += tiles_bytes(job, kTuneShapes[c].shape, kTuneShapes[c].split);
+      bytes += tiles_bytes(job, tune_shape(job, c).shape, tune_shape(job, c).split);
   else
     bytes += tiles_bytes(job, job.shape, job.split);
   for (const UnitDesc& u : job.units) bytes += second_set_bytes(job, u);
@@ -594,19 +609,21 @@ static void launch(rt_ctx* ctx, unsigned nblocks, const UnitDesc* units, const W
 typedef void (*LaunchFn)(rt_ctx*, unsigned, const UnitDesc*, const WorkItem*, int,
                          const ModelParams&, const WindowRec*);
 
-template <int MODEL>
+template <int MODEL, int NC>
 static void launch_o8(rt_ctx* ctx, unsigned nblocks, const UnitDesc* units, const WorkItem* work,
                       int H, const ModelParams& mp, const WindowRec* wins) {
   const int xcd = xcd_of(ctx, nblocks);
-  hipLaunchKernelGGL((k_fused_o8<MODEL>), dim3(grid_of(xcd, nblocks)), dim3(1024), 0,
-                     ctx->stream, units, work, (int)nblocks, xcd, H, mp, wins);
+  if (NC == 2)
+    hipLaunchKernelGGL((k_fused_o8<MODEL>), dim3(grid_of(xcd, nblocks)), dim3(1024), 0,
+                       ctx->stream, units, work, (int)nblocks, xcd, H, mp, wins);
+  else
+    hipLaunchKernelGGL((k_walk_o8<MODEL>), dim3(grid_of(xcd, nblocks)), dim3(1024), 0,
+                       ctx->stream, units, work, (int)nblocks, xcd, H, mp, wins);
 }
 
 template <int MODEL, int NC>
 static LaunchFn pick_byte(int shape) {
-  // (one-chain jobs are given SH_16x4 instead; the 16x3 kernel keeps the
-  // tile geometry right should one get here)
-  if (shape == SH_16x3_O8) return NC == 2 ? launch_o8<MODEL> : launch<MODEL, NC, 16, 3>;
+  if (shape == SH_16x3_O8) return launch_o8<MODEL, NC>;
   if (shape == SH_16x4) return launch<MODEL, NC, 16, 4>;
   if (shape == SH_16x2) return launch<MODEL, NC, 16, 2>;
   return launch<MODEL, NC, 8, 8>;
@@ -778,7 +795,7 @@ int fused_upload(rt_ctx* ctx, FusedJob& job, void* scratch, FusedPlan* plan) {
   plan->cand.push_back(t);
   if (job.tune)
     for (int c = 0; c < tune_count(job); c++) {
-      const LaunchShape ls = kTuneShapes[c];
+      const LaunchShape ls = tune_shape(job, c);
       if (ls.shape == first.shape && ls.split == first.split) continue;
       rc = upload_tiles(ctx, job, ls, p, d_wins, wins, &t);
       if (rc) return rc;

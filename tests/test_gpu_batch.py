@@ -84,7 +84,7 @@ def test_domino_walk_batch_rect_unions(gpu_ctx):
             tm = gpu_ctx.last_timing()
             # first call tunes (1800 steps: ~150 launches), the second reuses it
             assert tm["shape_source"] == (2 if use_graph else 1), tm
-            assert tm["shape"] in (1, 2, 3)
+            assert tm["shape"] in (1, 2, 3, 7)
             for got, w in zip(outs, want):
                 assert np.array_equal(got, w)
     finally:
