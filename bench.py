@@ -448,7 +448,8 @@ def main():
                     help="MiB: time a plain device copy of that size (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
+        print(json.dumps({"error": "the benchmark runs on one GPU only (independent samples "
+                                   "shard across GPUs without communication; not measured here)",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
