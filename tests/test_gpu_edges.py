@@ -123,7 +123,7 @@ def test_halo_depth_never_changes_results(gpu_ctx, halo):
     gpu_ctx.set_tinymt_params(P)
     try:
         gpu_ctx.set_halo(halo)
-        for shape in (2, 6):
+        for shape in (2, 6, 7):
             _say("lozenge cftp batch halo", halo, "shape", shape)
             gpu_ctx.set_tile_shape(shape)
             outs, _ = gpu_ctx.lozenge_cftp_batch(tmax, tmin, [4], rt.SCHED_DOUBLING, 256, 20)
@@ -132,11 +132,12 @@ def test_halo_depth_never_changes_results(gpu_ctx, halo):
         _say("lozenge walk halo", halo)
         assert np.array_equal(gpu_ctx.lozenge_walk(big, 300, 8), want_w)
         gpu_ctx.set_tinymt_params(Pd)
-        gpu_ctx.set_tile_shape(1)
-        _say("domino walk batch halo", halo)
-        outs = gpu_ctx.domino_walk_batch(ts, 700, [6, 7], rt.DOMINO_HEATBATH)
-        for got, w in zip(outs, want_d):
-            assert np.array_equal(got, w)
+        for shape in (1, 7):
+            gpu_ctx.set_tile_shape(shape)
+            _say("domino walk batch halo", halo, "shape", shape)
+            outs = gpu_ctx.domino_walk_batch(ts, 700, [6, 7], rt.DOMINO_HEATBATH)
+            for got, w in zip(outs, want_d):
+                assert np.array_equal(got, w)
     finally:
         gpu_ctx.set_halo(6)
         gpu_ctx.set_tile_shape(0)
