@@ -439,6 +439,13 @@ def main():
                     help="profiling run: only one section (plain Aztec sweep, 256-chain batch "
                          "walks, weighted walk / bibone, served stream), printed as one JSON "
                          "line")
+    ap.add_argument("--serve-dom", type=int, nargs=2, default=[64, 96], metavar=("LO", "HI"),
+                    help="served stream: domino Aztec orders")
+    ap.add_argument("--serve-loz", type=int, nargs=2, default=[64, 80], metavar=("LO", "HI"),
+                    help="served stream: lozenge hexagon sides")
+    ap.add_argument("--serve-seconds", type=float, default=20.0)
+    ap.add_argument("--serve-plain", type=int, default=24,
+                    help="served stream: requests timed on the plain paths")
     ap.add_argument("--halo", type=int, default=6,
                     help="sweep-block halo depth (rt_set_halo; results never depend on it)")
     ap.add_argument("--tile-shape", type=int, default=0,
@@ -459,10 +466,14 @@ def main():
 
     import randomtilingspublic_amd as rt
 
+    def serve():
+        return served_stream(rt, seconds=args.serve_seconds, dom=tuple(args.serve_dom),
+                             loz=tuple(args.serve_loz), plain_n=args.serve_plain)
+
     if args.profile_section:
-        fn = {"sweep": plain_sweep, "walks": batched_walks, "c3": weighted_and_bibone,
-              "served": served_stream}[args.profile_section]
-        print(json.dumps({"profile_section": args.profile_section, **fn(rt)}), flush=True)
+        fn = {"sweep": lambda: plain_sweep(rt), "walks": lambda: batched_walks(rt),
+              "c3": lambda: weighted_and_bibone(rt), "served": serve}[args.profile_section]
+        print(json.dumps({"profile_section": args.profile_section, **fn()}), flush=True)
         return
 
     side = args.side
@@ -573,7 +584,7 @@ def main():
         print(f"[bench] weighted / bibone {c3}", file=sys.stderr, flush=True)
     served = None
     if not args.no_serve:
-        served = served_stream(rt)
+        served = serve()
         print(f"[bench] served {served}", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
