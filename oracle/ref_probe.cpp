@@ -1,14 +1,19 @@
 // ref_probe.cpp -- extern "C" entry points over the reference's own host
-// statics, compiled together with the reference's sources into
+// statics, its TinyMT host generator and its parameter-file reader,
+// compiled together with the reference's sources into
 // oracle/_ref/libref_host.so (see oracle/Makefile).  Test infrastructure:
 // only tests/golden scripts and CPU tests load it, to pin the engine's
 // host helpers against the reference implementation itself.
 #include <cstdint>
 #include <cstring>
+#include <exception>
+#include <fstream>
 #include <string>
 #include <vector>
 
 #include "Domino/DominoTiler.h"
+#include "TinyMT/file_reader.h"
+#include "TinyMT/tinymt32.h"
 #include "Lozenge/LozengeTiler.h"
 #include "RectTriangle/RectTriangleTiler.h"
 #include "TriangleDimer/TriangleDimerTiler.h"
@@ -130,5 +135,43 @@ int ref_svg(const char* model, const char* what, const int32_t* t, long nt, cons
 long ref_bibone_dimer_indicator(const int32_t* t, long nt, int c, int32_t* out, long cap) {
   std::vector<int> v = vec(t, nt);
   return give(TriangleDimerTiler::DimerIndicator(v, c), out, cap);
+}
+// The reference's host-side generator (src/TinyMT/tinymt32.h:72-136,
+// 205-208): n draws of tinymt32_generate_float01 / _uint32 from the state
+// s7 = {s0, s1, s2, s3, mat1, mat2, tmat}; the state advances.
+void ref_h_tinymt_float01(uint32_t* s7, float* out, int n) {
+  tinymt32_t t;
+  for (int k = 0; k < 4; k++) t.status[k] = s7[k];
+  t.mat1 = s7[4];
+  t.mat2 = s7[5];
+  t.tmat = s7[6];
+  for (int k = 0; k < n; k++) out[k] = tinymt32_generate_float01(&t);
+  for (int k = 0; k < 4; k++) s7[k] = t.status[k];
+}
+void ref_h_tinymt_uint32(uint32_t* s7, uint32_t* out, int n) {
+  tinymt32_t t;
+  for (int k = 0; k < 4; k++) t.status[k] = s7[k];
+  t.mat1 = s7[4];
+  t.mat2 = s7[5];
+  t.tmat = s7[6];
+  for (int k = 0; k < n; k++) out[k] = tinymt32_generate_uint32(&t);
+  for (int k = 0; k < 4; k++) s7[k] = t.status[k];
+}
+// The reference's parameter-file reader (src/TinyMT/file_reader.cpp:51-70,
+// 103-166): `count` successive file_reader::get calls, as get_params_buffer
+// makes them (22-38).  Returns 0, -2 when the reader throws (format error,
+// comma not found; the reference then ends the program) or -3 when the
+// file does not open.
+int ref_parse_params(const char* path, long count, uint32_t* out3) {
+  std::ifstream probe(path);
+  if (!probe) return -3;
+  probe.close();
+  try {
+    tinymt::file_reader fr(path);
+    for (long k = 0; k < count; k++) fr.get(&out3[3 * k], &out3[3 * k + 1], &out3[3 * k + 2]);
+  } catch (const std::exception&) {
+    return -2;
+  }
+  return 0;
 }
 }

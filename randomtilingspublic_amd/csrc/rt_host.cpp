@@ -3,6 +3,7 @@
 // function helpers that the reference keeps as static class methods.
 #include <stdint.h>
 
+#include <cerrno>
 #include <climits>
 #include <cmath>
 #include <cstdio>
@@ -72,8 +73,15 @@ int parse_tinymt_file(const char* path, size_t count, uint32_t* out3, std::strin
       if (err) *err = "TinyMT parameter file: comma not found at set " + std::to_string(k);
       return RT_EPARAMS;
     }
+    // A field past unsigned long's range is a format error, as in
+    // file_reader::get_params (errno checked after the three fields).
+    errno = 0;
     for (int f = 0; f < 3; f++)
       out3[3 * k + f] = (uint32_t)std::strtoul(line.c_str() + pos[3 + f], nullptr, 16);
+    if (errno != 0) {
+      if (err) *err = "TinyMT parameter file: value out of range at set " + std::to_string(k);
+      return RT_EPARAMS;
+    }
   }
   return RT_OK;
 }

@@ -9,6 +9,13 @@
   reference sources.  Skipped when that library is not built.
 * reference_outputs/*.txt: SaveMatrix files the reference checkout ships
   in Examples/ExampleOuts (copied data, not code).
+* reference_tinymt.npz: tinymt32_init states and tinymt32_single01 /
+  tinymt32_uint32 draws of the reference's own tinymt32.clh (compiled for
+  the host into oracle/_ref/libref_host.so) on random parameters, seeds and
+  states.
+* reference_parser.json: what the reference's file_reader returns (or
+  that it throws) on the parameter-file layouts of
+  tests/test_reference_pins.py.
 """
 import ctypes as C
 import json
@@ -97,6 +104,45 @@ def ref_host():
     np.savez_compressed(os.path.join(HERE, "reference_host.npz"), **arrs)
 
 
+def ref_tinymt():
+    sys.path.insert(0, ROOT)
+    from tests import ref_lib as R
+    if not R.available():
+        print("oracle/_ref/libref_host.so not built; skipping reference_tinymt.npz")
+        return
+    rng = np.random.default_rng(20240601)
+    init_in = rng.integers(0, 1 << 32, size=(400, 4), dtype=np.uint64).astype(np.uint32)
+    init_out = np.stack([R.cl_init(m1, m2, tm, sd)[:4] for sd, m1, m2, tm in init_in])
+    draw_in = rng.integers(0, 1 << 32, size=(400, 7), dtype=np.uint64).astype(np.uint32)
+    f, u = [], []
+    for s7 in draw_in:
+        f.append(R.cl_single01(s7.copy(), 8))
+        u.append(R.cl_uint32(s7.copy(), 8))
+    np.savez_compressed(os.path.join(HERE, "reference_tinymt.npz"), init_in=init_in,
+                        init_out=init_out, draw_in=draw_in, draw_f32=np.stack(f),
+                        draw_u32=np.stack(u))
+
+
+def ref_parser():
+    sys.path.insert(0, ROOT)
+    from tests import ref_lib as R
+    from tests.test_reference_pins import parser_cases
+    if not R.available():
+        print("oracle/_ref/libref_host.so not built; skipping reference_parser.json")
+        return
+    out = []
+    with tempfile.TemporaryDirectory() as d:
+        for name, text, count in parser_cases():
+            path = os.path.join(d, name + ".txt")
+            with open(path, "wb") as f:
+                f.write(text.encode())
+            rc, p = R.parse_params(path, count)
+            out.append({"name": name, "text": text, "count": count, "rc": rc,
+                        "params": p.ravel().tolist() if rc == 0 else [0] * (3 * count)})
+    with open(os.path.join(HERE, "reference_parser.json"), "w") as f:
+        json.dump(out, f, indent=1)
+
+
 def svg_digests():
     """SHA-256 of the reference's example SVGs whose inputs are deterministic
     (Examples/ExampleOuts), keyed by the call that made them.  One line of
@@ -136,5 +182,7 @@ def svg_digests():
 if __name__ == "__main__":
     schedules()
     ref_host()
+    ref_tinymt()
+    ref_parser()
     svg_digests()
     print("golden fixtures written", file=sys.stderr)
