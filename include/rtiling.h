@@ -181,6 +181,10 @@ typedef struct {
   double device_ms;  /* device time over all rounds */
 } rt_cftp_stats;
 
+/* rule: RT_DOMINO_HEATBATH or RT_DOMINO_HEATBATH2P only (the reference's
+ * CFTP kernels, dominokernelCFTP*.cl; NULL = heat bath); the 0.8 toggle and
+ * the two-periodic Metropolis rule are not monotone couplings, so they
+ * return RT_EINVAL here, in rt_domino_cftp_batch and in rt_server_submit. */
 int rt_domino_cftp(rt_ctx* ctx, const int32_t* tmax, const int32_t* tmin,
                    int N, uint32_t master_seed, int sched, long t0,
                    int max_rounds, const rt_domino_rule* rule, int32_t* out,
@@ -214,6 +218,17 @@ int rt_domino_cftp_batch(rt_ctx* ctx, const int32_t* tmax, const int32_t* tmin, 
                          const uint32_t* master_seeds, int nsamples, int sched, long t0,
                          int max_rounds, const rt_domino_rule* rule, int32_t* out,
                          rt_cftp_stats* stats);
+
+/* Both coupled chains of each sample after exactly `rounds` rounds of the
+ * CFTP schedule, coalesced or not: top_out / bot_out (nsamples*N*N each)
+ * equal LozengeTiler::Walk over the round's segment list (seeds newest
+ * first, AztecDiamondCftpEx.cpp:57-76) from tmax / tmin.  For coupling
+ * diagnostics and for checking the batched kernels against plain walks;
+ * no reference counterpart.  stats[k].rounds = rounds. */
+int rt_lozenge_cftp_batch_rounds(rt_ctx* ctx, const int32_t* tmax, const int32_t* tmin, int N,
+                                 const uint32_t* master_seeds, int nsamples, int sched, long t0,
+                                 int rounds, int32_t* top_out, int32_t* bot_out,
+                                 rt_cftp_stats* stats);
 
 /* Batched domino walks on domains of uneven size: chain k is
  * DominoTiler::Walk(tilings[k], steps, seeds[k]) with side Ns[k]; all

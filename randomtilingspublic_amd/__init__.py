@@ -112,6 +112,9 @@ def lib():
                                    C.POINTER(C.c_size_t)]
     L.rt_lozenge_cftp_batch.argtypes = [vp, _I32P, _I32P, C.c_int, _U32P, C.c_int, C.c_int,
                                         C.c_long, C.c_int, _I32P, C.c_void_p]
+    L.rt_lozenge_cftp_batch_rounds.argtypes = [vp, _I32P, _I32P, C.c_int, _U32P, C.c_int,
+                                               C.c_int, C.c_long, C.c_int, _I32P, _I32P,
+                                               C.c_void_p]
     L.rt_domino_cftp_batch.argtypes = [vp, _I32P, _I32P, C.c_int, _U32P, C.c_int, C.c_int,
                                        C.c_long, C.c_int, C.POINTER(DominoRule), _I32P, C.c_void_p]
     L.rt_domino_walk_batch.argtypes = [vp, C.c_void_p, C.c_void_p, C.c_int, C.c_long, _LONGP,
@@ -378,6 +381,23 @@ class Context:
                            max_rounds=40):
         return self._cftp_batch(self._L.rt_lozenge_cftp_batch, tmax, tmin, masters, sched, t0,
                                 max_rounds)
+
+    def lozenge_cftp_batch_rounds(self, tmax, tmin, masters, sched=SCHED_DOUBLING, t0=1024,
+                                  rounds=1):
+        """Both chains of every sample after exactly `rounds` CFTP rounds:
+        (tops, bots, stats)."""
+        tmax, tmin = _i32(tmax), _i32(tmin)
+        masters = np.ascontiguousarray(masters, dtype=np.uint32)
+        n = len(masters)
+        tops = np.zeros(n * len(tmax), dtype=np.int32)
+        bots = np.zeros_like(tops)
+        st = (CftpStats * n)()
+        self._chk(self._L.rt_lozenge_cftp_batch_rounds(self._h, tmax, tmin, side(tmax), masters,
+                                                       n, sched, t0, rounds, tops, bots,
+                                                       C.cast(st, C.c_void_p)))
+        stats = [{"rounds": s.rounds, "total_steps": s.total_steps, "steps_run": s.steps_run,
+                  "device_ms": s.device_ms} for s in st]
+        return tops.reshape(n, len(tmax)), bots.reshape(n, len(tmax)), stats
 
     def domino_cftp_batch(self, tmax, tmin, masters, sched=SCHED_DOUBLING, t0=128,
                           max_rounds=40, rule=DOMINO_HEATBATH, a=1.0, b=1.0):

@@ -130,10 +130,16 @@ struct Sample {
 
 }  // namespace
 
+// fixed_rounds > 0: every sample runs exactly that many rounds, coalesced
+// or not, and out / bot_out receive its top and bottom chains after the
+// last round (both planes of the pair, merged back to N x N tilings).
 static int cftp_batch(rt_ctx* ctx, int model, const int32_t* tmax, const int32_t* tmin, int N,
                       const uint32_t* masters, int ns, int sched, long t0, int max_rounds,
-                      const ModelParams& mp, int32_t* out, rt_cftp_stats* stats) {
-  if (!tmax || !tmin || !out || !masters || ns <= 0) {
+                      const ModelParams& mp, int32_t* out, rt_cftp_stats* stats,
+                      int fixed_rounds = 0, int32_t* bot_out = nullptr) {
+  const bool fixed = fixed_rounds > 0;
+  if (fixed) max_rounds = fixed_rounds;
+  if (!tmax || !tmin || !out || !masters || ns <= 0 || (fixed && !bot_out)) {
     ctx->err = "bad batch arguments";
     return RT_EINVAL;
   }
@@ -303,7 +309,13 @@ static int cftp_batch(rt_ctx* ctx, int model, const int32_t* tmax, const int32_t
     std::vector<uint8_t> h(pbytes);
     for (size_t u = 0; u < act.size(); u++) {
       const int k = act[u];
-      if (flags[u]) continue;
+      if (fixed) {
+        if (r < fixed_rounds) continue;
+        HIPCHK_RET(ctx, hipMemcpy(h.data(), bot[k], pbytes, hipMemcpyDeviceToHost));
+        merge_planes(model, bot_out + (size_t)k * N * N, N, h.data());
+      } else if (flags[u]) {
+        continue;
+      }
       HIPCHK_RET(ctx, hipMemcpy(h.data(), top[k], pbytes, hipMemcpyDeviceToHost));
       merge_planes(model, out + (size_t)k * N * N, N, h.data());
       S[k].done = true;
@@ -331,6 +343,7 @@ static int cftp_batch(rt_ctx* ctx, int model, const int32_t* tmax, const int32_t
   }
   acc.device_ms = ms_total;
   ctx->last = acc;
+  if (fixed) return RT_OK;
   if (remaining > 0) {
     for (int k = 0; k < ns; k++)
       if (!S[k].done && stats) {
@@ -358,13 +371,27 @@ extern "C" int rt_lozenge_cftp_batch(rt_ctx* ctx, const int32_t* tmax, const int
                     out, stats);
 }
 
+extern "C" int rt_lozenge_cftp_batch_rounds(rt_ctx* ctx, const int32_t* tmax, const int32_t* tmin,
+                                            int N, const uint32_t* masters, int nsamples,
+                                            int sched, long t0, int rounds, int32_t* top_out,
+                                            int32_t* bot_out, rt_cftp_stats* stats) {
+  if (!ctx) return RT_EINVAL;
+  if (rounds <= 0) {
+    ctx->err = "rounds must be positive";
+    return RT_EINVAL;
+  }
+  ModelParams mp{};
+  return cftp_batch(ctx, M_LOZENGE, tmax, tmin, N, masters, nsamples, sched, t0, rounds, mp,
+                    top_out, stats, rounds, bot_out);
+}
+
 extern "C" int rt_domino_cftp_batch(rt_ctx* ctx, const int32_t* tmax, const int32_t* tmin, int N,
                                     const uint32_t* masters, int nsamples, int sched, long t0,
                                     int max_rounds, const rt_domino_rule* rule, int32_t* out,
                                     rt_cftp_stats* stats) {
   if (!ctx) return RT_EINVAL;
-  if (rule && (rule->kind < 0 || rule->kind > 3)) {
-    ctx->err = "unknown domino rule";
+  if (!domino_cftp_rule(rule)) {
+    ctx->err = "CFTP needs a heat-bath rule (RT_DOMINO_HEATBATH or RT_DOMINO_HEATBATH2P)";
     return RT_EINVAL;
   }
   ModelParams mp{};

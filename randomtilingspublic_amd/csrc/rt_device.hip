@@ -1039,6 +1039,10 @@ extern "C" int rt_domino_cftp(rt_ctx* ctx, const int32_t* tmax, const int32_t* t
   if (!ctx) return RT_EINVAL;
   int rc = domino_check(ctx, tmax, N, rule);
   if (rc) return rc;
+  if (!domino_cftp_rule(rule)) {
+    ctx->err = "CFTP needs a heat-bath rule (RT_DOMINO_HEATBATH or RT_DOMINO_HEATBATH2P)";
+    return RT_EINVAL;
+  }
   if (!tmin || !out) {
     ctx->err = "null tiling";
     return RT_EINVAL;

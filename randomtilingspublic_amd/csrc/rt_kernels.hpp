@@ -81,6 +81,15 @@ int recttri_device_walk(rt_ctx* ctx, TriArrays32* a, int nc, int N,
                         const RTWeights& W, uint32_t* state, int* last_out);
 RTWeights recttri_weights(const float* w16);
 
+// CFTP runs the heat-bath rules only (dominokernelCFTP.cl,
+// dominokernelCFTPtwoperiodic.cl, the kernels the reference's CFTP driver
+// uses): their top/bottom coupling is monotone, so coalescence of the two
+// extreme chains is an exact sample.  The 0.8 toggle and the two-periodic
+// Metropolis rule are not monotone couplings.  NULL means heat bath.
+inline bool domino_cftp_rule(const rt_domino_rule* rule) {
+  return !rule || rule->kind == RT_DOMINO_HEATBATH || rule->kind == RT_DOMINO_HEATBATH2P;
+}
+
 // Final lazy sync of the colour planes (step-path kernels).
 int launch_domino_sync(rt_ctx* ctx, DominoArrays* a, int nc, int N, int last);
 int launch_recttri_sync(rt_ctx* ctx, TriArrays32* a, int nc, int N, int last);

@@ -19,10 +19,11 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
-#include <deque>
+#include <list>
 #include <map>
 #include <mutex>
 #include <random>
@@ -56,7 +57,8 @@ struct Req {
   ModelParams mp{};
   double t_submit = 0, t_start = 0, t_done = 0;
   int status = RT_EAGAIN;
-  bool cancel = false;
+  // set by rt_server_cancel (any thread), read by the loop thread
+  std::atomic<bool> cancel{false};
   bool active = false;
   std::vector<int32_t> out;
   rt_cftp_stats st{};
@@ -167,7 +169,10 @@ struct Server {
   std::string err;
   int loop_rc = RT_OK;
   uint64_t next_id = 1;
-  std::deque<Req*> queue;
+  // Waiting requests in arrival order.  A list: the admission loop keeps an
+  // iterator across the unlocked admit() call, and a concurrent
+  // rt_server_submit's push_back must not invalidate it.
+  std::list<Req*> queue;
   std::map<uint64_t, Req*> reqs;
   // loop-thread state
   std::vector<Slot*> active;
@@ -549,7 +554,7 @@ static void loop_main(Server* S) {
         g.unlock();
         int rc = admit(S, r);
         g.lock();
-            if (rc == RT_OK) {
+        if (rc == RT_OK) {
           it = S->queue.erase(it);
         } else if (rc == RT_EAGAIN) {
           ++it;
@@ -659,6 +664,7 @@ extern "C" int rt_server_submit(rt_server* sv, const rt_request* q, uint64_t* id
   if (q->t0 <= 0 || q->max_rounds <= 0 ||
       (q->sched != RT_SCHED_REFERENCE && q->sched != RT_SCHED_DOUBLING))
     return RT_EINVAL;
+  if (model == M_DOMINO && !domino_cftp_rule(&q->rule)) return RT_EINVAL;
   int ncol;
   long plane, nstream, cols;
   const size_t bytes = footprint(model, N, &ncol, &plane, &nstream, &cols);

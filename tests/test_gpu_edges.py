@@ -147,6 +147,63 @@ def test_halo_depth_never_changes_results(gpu_ctx, halo):
         gpu_ctx.set_halo(13)
 
 
+@pytest.mark.parametrize("halo", [3, 9])
+def test_swar_engine_halo_depth_never_changes_results(gpu_ctx, halo):
+    # The four-cells-per-word lozenge kernel at halo depths other than 6,
+    # on the side-100 hexagon (two tile columns at either depth).
+    d = rt.LozengeTiler.AlmostHexDomain(100, 0)
+    tmin, tmax = rt.LozengeTiler.MinTiling(d), rt.LozengeTiler.MaxTiling(d)
+    N = rt.side(tmin)
+    P = rt.synthetic_params(N * N, 39)
+    want = o.lozenge_walk(tmin, 300, 8, P)
+    steps, seeds = [96, 64], [11, 12]
+    want_seg = o.lozenge_walk_segments(tmax, steps, seeds, P)
+    gpu_ctx.set_tinymt_params(P)
+    try:
+        gpu_ctx.set_engine(rt.ENGINE_FUSED_SWAR)
+        gpu_ctx.set_halo(halo)
+        for shape in (2, 3):
+            gpu_ctx.set_tile_shape(shape)
+            _say("swar lozenge walk halo", halo, "shape", shape)
+            assert np.array_equal(gpu_ctx.lozenge_walk(tmin, 300, 8), want)
+            assert np.array_equal(gpu_ctx.lozenge_walk_segments(tmax, steps, seeds), want_seg)
+    finally:
+        gpu_ctx.set_halo(6)
+        gpu_ctx.set_engine(rt.ENGINE_AUTO)
+        gpu_ctx.set_tile_shape(0)
+
+
+def test_domino_cftp_rejects_non_monotone_rules(gpu_ctx):
+    # The 0.8 toggle and the two-periodic Metropolis rule are not monotone
+    # couplings: coalescence of the extreme chains would not be exact.
+    d = rt.DominoTiler.AztecDiamond(4)
+    tmin, tmax = rt.DominoTiler.MinTiling(d), rt.DominoTiler.MaxTiling(d)
+    N = rt.side(tmin)
+    gpu_ctx.set_tinymt_params(rt.synthetic_params(N * N // 2, 40))
+    _say("domino cftp rule checks")
+    for rule in (rt.DOMINO_TOGGLE08, rt.DOMINO_METRO2P, 7, -1):
+        with pytest.raises(rt.RTError) as e:
+            gpu_ctx.domino_cftp(tmax, tmin, 1, rt.SCHED_DOUBLING, 64, 10, rule)
+        assert e.value.code == -1
+        with pytest.raises(rt.RTError) as e:
+            gpu_ctx.domino_cftp_batch(tmax, tmin, [1, 2], rt.SCHED_DOUBLING, 64, 10, rule)
+        assert e.value.code == -1
+    srv = rt.Server(gpu_ctx, 64 << 20, max_active=4)
+    try:
+        for rule in (rt.DOMINO_TOGGLE08, rt.DOMINO_METRO2P, 9):
+            with pytest.raises(rt.RTError) as e:
+                srv.submit(rt.MODEL_DOMINO, tmax, tmin, 1, rt.SCHED_DOUBLING, 64, 10, rule)
+            assert e.value.code == -1
+        rid = srv.submit(rt.MODEL_DOMINO, tmax, tmin, 1, rt.SCHED_DOUBLING, 64, 10,
+                         rt.DOMINO_HEATBATH2P, 1.0, 0.5)
+        srv.wait(rid, 60.0)
+        assert srv.poll(rid)[0] == 0
+    finally:
+        srv.close()
+    for rule in (rt.DOMINO_HEATBATH, rt.DOMINO_HEATBATH2P):
+        gpu_ctx.domino_cftp(tmax, tmin, 1, rt.SCHED_DOUBLING, 64, 10, rule)
+
+
 @pytest.mark.parametrize("shape", [0, 2, 3, 6, 7])
 def test_swar_engine_cftp_batch_with_forced_shapes(gpu_ctx, shape):
     # The four-cells-per-word lozenge kernel has 64- and 32-row tiles only;

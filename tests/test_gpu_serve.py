@@ -112,3 +112,47 @@ def test_tight_budget_queues_and_cancel(gpu_ctx):
         assert rc in (-9, 0)  # cancelled unless it had already finished
     finally:
         srv.close()
+
+
+def test_submitters_on_two_threads(gpu_ctx):
+    """Requests submitted from two host threads while the loop admits and
+    retires others: every result equals the request run alone, none is lost
+    or admitted twice."""
+    import threading
+    base = _mix()
+    reqs = []
+    for rep in range(3):
+        for q in base:
+            reqs.append(dict(q, master_seed=q["master_seed"] + 1000 * rep))
+    big = max(rt.side(q["tmax"]) for q in reqs)
+    gpu_ctx.set_tinymt_params(rt.synthetic_params(big * big, 33))
+    alone = [_alone(gpu_ctx, q)[0] for q in reqs]
+    _say("server, two submitting threads,", len(reqs), "requests")
+    srv = rt.Server(gpu_ctx, 64 << 20, 6)
+    ids = [None] * len(reqs)
+    errors = []
+
+    def submit(part):
+        try:
+            for k in range(part, len(reqs), 2):
+                ids[k] = srv.submit(max_rounds=32, **reqs[k])
+                time.sleep(0.002 * (k % 3))
+        except Exception as e:  # reported below
+            errors.append(e)
+
+    try:
+        th = [threading.Thread(target=submit, args=(p,)) for p in (0, 1)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        assert not errors
+        assert len(set(ids)) == len(reqs)
+        for rid, want in zip(ids, alone):
+            assert srv.wait(rid, 120.0)
+            rc, got, _, _ = srv.poll(rid)
+            assert rc == 0 and np.array_equal(got, want)
+        s = srv.stats()
+        assert s["completed"] == len(reqs) and s["queued"] == 0 and s["status"] == 0
+    finally:
+        srv.close()
