@@ -39,7 +39,13 @@ HBM_PEAK_GBS = 8000.0
 # pair kernel at eight waves per SIMD.
 KERNEL_BY_SHAPE = ["k_fused<1, 2, 8, 8>", "k_fused<1, 2, 16, 4>", "k_fused<1, 2, 16, 2>",
                    "k_fused<2, 2, 8, 4>", "k_fused<1, 1, 8, 8>", "k_fused<1, 1, 16, 4>",
-                   "k_fused<1, 1, 16, 2>", "k_fused_o8<1>"]
+                   "k_fused<1, 1, 16, 2>", "k_pair_lozenge"]
+
+
+def set_halo_kernels(halo):
+    """Slot 7 runs k_pair_lozenge (both chains packed per LDS word) when the
+    halo is a multiple of 3, else k_fused_o8<1>."""
+    KERNEL_BY_SHAPE[7] = "k_pair_lozenge" if halo % 3 == 0 else "k_fused_o8<1>"
 # Algorithmic bytes per site per launch: a pair workgroup reads the TinyMT
 # state (16 B) and parameters (12 B) once, writes the state (16 B), reads and
 # writes 3 colour planes x 2 chains (12 B) = 56 B; split pairs read state and
@@ -121,6 +127,8 @@ def plain_sweep(rt, order=3000, steps=1000):
     return {"workload": f"domino Aztec diamond order {order} (N={N}), heat-bath, {steps} steps, "
                         "one launch per colour step",
             "kernel": "k_domino_step<1>", "us_per_step": round(tm["device_ms"] * 1e3 / steps, 2),
+            "sites_per_launch": (N - 2) * (N // 2 - 2),
+            "avg_launch_us": round(tm["device_ms"] * 1e3 / steps, 2),
             "flip_attempts_per_s": round(att_s, 1),
             "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
@@ -173,6 +181,7 @@ def batched_walks(rt, seed=4, nchains=256, steps=10 ** 4):
             "flip_attempts_per_s": round(att_s, 1),
             "tile_shape": tm["shape"], "tuned_on_warmup": warm["shape_source"] == 1,
             "launches": launches,
+            "sites_per_launch": int(tm["kernel_sites"] / max(tm["kernel_launches"], 1)),
             "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
                          "alg_bytes_per_site_per_launch": 48,
@@ -360,24 +369,229 @@ def served_stream(rt, seconds=20.0, seed=2024, dom=(64, 96), loz=(64, 80), t0=10
 
 
 def copy_probe(mib):
-    """Plain device-to-device copy of `mib` MiB, five times: a kernel whose
-    HBM bytes are known (mib MiB read, mib MiB written per launch), to scale
-    the FETCH_SIZE / WRITE_SIZE counters."""
-    import torch
-    n = mib * (1 << 20) // 4
-    x = torch.arange(n, dtype=torch.int32, device="cuda")
-    y = torch.empty_like(x)
-    torch.cuda.synchronize()
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-    ev[0].record()
+    """Plain device-to-device copy of `mib` MiB (hipMemcpy, which runs the
+    runtime's copy kernel), five times: a kernel whose HBM bytes are known
+    (mib MiB read, mib MiB written per launch), to scale the FETCH_SIZE /
+    WRITE_SIZE counters."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = mib * (1 << 20)
+    src, dst = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(src), C.c_size_t(n)) == 0
+    assert hip.hipMalloc(C.byref(dst), C.c_size_t(n)) == 0
+    assert hip.hipMemset(src, 0x5A, C.c_size_t(n)) == 0
+    ev = [C.c_void_p(), C.c_void_p()]
+    for e in ev:
+        assert hip.hipEventCreate(C.byref(e)) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipEventRecord(ev[0], None)
     for _ in range(5):
-        y.copy_(x)
-    ev[1].record()
-    torch.cuda.synchronize()
-    ms = ev[0].elapsed_time(ev[1]) / 5
-    assert torch.equal(x, y)
-    print(json.dumps({"copy_probe_mib": mib, "bytes_read": n * 4, "bytes_written": n * 4,
-                      "avg_ms": round(ms, 4), "GBs": round(2 * n * 4 / ms / 1e6, 1)}), flush=True)
+        assert hip.hipMemcpy(dst, src, C.c_size_t(n), 3) == 0  # hipMemcpyDeviceToDevice
+    hip.hipEventRecord(ev[1], None)
+    assert hip.hipEventSynchronize(ev[1]) == 0
+    ms = C.c_float()
+    hip.hipEventElapsedTime(C.byref(ms), ev[0], ev[1])
+    ms = ms.value / 5
+    probe = (C.c_ubyte * 4096)()
+    assert hip.hipMemcpy(probe, C.c_void_p(dst.value + n - 4096), C.c_size_t(4096), 2) == 0
+    assert all(b == 0x5A for b in probe)
+    for e in ev:
+        hip.hipEventDestroy(e)
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(json.dumps({"copy_probe_mib": mib, "bytes_read": n, "bytes_written": n,
+                      "avg_ms": round(ms, 4), "GBs": round(2 * n / ms / 1e6, 1)}), flush=True)
+
+
+# Profiling workloads of --rocprof-profile: name -> (bench.py arguments,
+# kernel whose dispatches the summary is about, algorithmic bytes per site
+# per dispatch of that kernel).
+PROFILE_WORKLOADS = {
+    "hex300_cftp_32seeds": (["--profile-rounds", "6", "--samples", "32", "--tile-shape", "7"],
+                            "k_pair_lozenge", 56),
+    "aztec3000_step": (["--profile-section", "sweep"], "k_domino_step<1>", 48),
+    "batch256_walk": (["--profile-section", "walks"], "k_walk_o8<0>", 48),
+}
+PROFILE_TEXT = {
+    "hex300_cftp_32seeds": "lozenge hexagon a=b=c=300 (N=609, 123 627 sites per plane), batched "
+                           "CFTP of master seeds 1..32, doubling from 2^10, stopped after 6 "
+                           "doubling rounds, tile shape 7 (48-row chain-pair tiles at eight waves "
+                           "per SIMD); 56 B per site: TinyMT state 16 B read + 16 B written, "
+                           "parameters 12 B, 3 colour planes x 2 chains read and written",
+    "aztec3000_step": "plain heat-bath domino sweep of the Aztec diamond of order 3000 "
+                      "(N=6006), 1000 steps, one launch per colour step; 48 B per attempt: "
+                      "TinyMT state 16 + 16 B, parameters 12 B, cells 4 B",
+    "batch256_walk": "256 heat-bath domino chains on uneven domains (seed 4), 10^4 steps, one "
+                     "hipGraph, autotuned tile shape; 48 B per site per launch: TinyMT state "
+                     "16 + 16 B, parameters 12 B, 2 colour planes read and written",
+}
+
+
+def _short_kernel(name):
+    """'void rt::k_fused_o8<1>(rt::UnitDesc const*, ...)' -> 'k_fused_o8<1>'."""
+    n = name.strip().strip('"')
+    if n.startswith("void "):
+        n = n[5:]
+    depth, cut = 0, len(n)
+    for i, ch in enumerate(n):  # drop the argument list (outside template brackets)
+        if ch == "<":
+            depth += 1
+        elif ch == ">":
+            depth -= 1
+        elif ch == "(" and depth == 0:
+            cut = i
+            break
+    n = n[:cut]
+    return n.replace("rt::", "").replace("(anonymous namespace)::", "").strip()
+
+
+def _csv_rows(root, suffix):
+    import csv
+    import glob
+    rows = []
+    for path in sorted(glob.glob(os.path.join(root, "**", "*" + suffix), recursive=True)):
+        with open(path, newline="") as f:
+            rows += list(csv.DictReader(f))
+    return rows
+
+
+def _kernel_stats(root):
+    out = {}
+    for r in _csv_rows(root, "kernel_stats.csv"):
+        k = _short_kernel(r["Name"])
+        out[k] = {"calls": int(r["Calls"]), "total_ms": float(r["TotalDurationNs"]) / 1e6,
+                  "avg_us": float(r["AverageNs"]) / 1e3, "min_us": float(r["MinNs"]) / 1e3,
+                  "max_us": float(r["MaxNs"]) / 1e3, "pct": float(r["Percentage"])}
+    return out
+
+
+def _counter_dispatches(root, counter):
+    """Per kernel, the counter's value of every dispatch (rows of one
+    dispatch summed)."""
+    per = {}
+    for r in _csv_rows(root, "counter_collection.csv"):
+        if r.get("Counter_Name") != counter:
+            continue
+        k = _short_kernel(r["Kernel_Name"])
+        d = r.get("Dispatch_Id") or r.get("Correlation_Id")
+        per.setdefault(k, {})
+        per[k][d] = per[k].get(d, 0.0) + float(r["Counter_Value"])
+    return {k: list(v.values()) for k, v in per.items()}
+
+
+def _counter_per_dispatch(root, counter):
+    """Counter value per dispatch, averaged over each kernel's dispatches."""
+    return {k: (sum(v) / len(v), len(v)) for k, v in _counter_dispatches(root, counter).items()}
+
+
+def _copy_kib(root, counter, copies=5):
+    """The counter per 1 GiB copy: the mean of the `copies` largest copy
+    dispatches (the probe's small read-back uses the copy kernel too)."""
+    vals = [v for k, vs in _counter_dispatches(root, counter).items()
+            if "copyBuffer" in k for v in vs]
+    vals = sorted(vals)[-copies:]
+    return sum(vals) / len(vals)
+
+
+def rocprof_profile(name, outdir, summarize_only=False):
+    """Runs rocprofv3 on one profiling workload of this benchmark (kernel
+    trace; FETCH_SIZE and WRITE_SIZE, each in a run of its own; the same two
+    counters on a plain 1 GiB device copy, whose HBM bytes are known, to
+    scale them) and writes profiles/<kernel>_<workload>.md and that kernel's
+    entry of profiles/kernels_hbm.json (HBM bytes per site per dispatch,
+    which the headline line's roofline.traffic uses).  Raw rocprofv3 output
+    stays in `outdir`."""
+    args, kernel, alg = PROFILE_WORKLOADS[name]
+    me = os.path.abspath(__file__)
+    os.makedirs(outdir, exist_ok=True)
+
+    def run(tag, prof, bench_args, timeout):
+        d = os.path.join(outdir, f"{name}_{tag}")
+        cmd = ["rocprofv3", *prof, "--output-format", "csv", "-d", d, "-o", "run", "--",
+               sys.executable, me, *bench_args]
+        stdout_file = os.path.join(d, "stdout.txt")
+        if not summarize_only:
+            print(f"[profile] {name}: {' '.join(cmd)}", file=sys.stderr, flush=True)
+            r = subprocess.run(cmd, cwd="/tmp", env=dict(os.environ, TMPDIR="/tmp"),
+                               capture_output=True, text=True, timeout=timeout)
+            last = [l for l in r.stdout.splitlines() if l.startswith("{")]
+            if r.returncode != 0 and not last:
+                sys.stderr.write(r.stdout[-2000:] + r.stderr[-2000:])
+                raise RuntimeError(f"rocprofv3 {tag} failed ({r.returncode})")
+            os.makedirs(d, exist_ok=True)
+            with open(stdout_file, "w") as f:
+                f.write(r.stdout)
+        with open(stdout_file) as f:
+            last = [l for l in f.read().splitlines() if l.startswith("{")]
+        return d, json.loads(last[-1]) if last else None, " ".join(
+            ["rocprofv3", *prof, "--output-format csv", "-d", f"<dir>/{name}_{tag}", "-o run --",
+             "python3 bench.py", *bench_args])
+
+    kt_dir, out, kt_cmd = run("kt", ["--kernel-trace", "--stats"], args, 600)
+    f_dir, _, f_cmd = run("fetch", ["--pmc", "FETCH_SIZE"], args, 600)
+    w_dir, _, w_cmd = run("write", ["--pmc", "WRITE_SIZE"], args, 600)
+    cf_dir, _, cf_cmd = run("copy_fetch", ["--pmc", "FETCH_SIZE"], ["--copy-probe", "1024"], 300)
+    cw_dir, _, cw_cmd = run("copy_write", ["--pmc", "WRITE_SIZE"], ["--copy-probe", "1024"], 300)
+    stats = _kernel_stats(kt_dir)
+    fetch, write = _counter_per_dispatch(f_dir, "FETCH_SIZE"), _counter_per_dispatch(w_dir,
+                                                                                       "WRITE_SIZE")
+    copy = "__amd_rocclr_copyBuffer"
+    moved_kib = 1024 * 1024  # 1 GiB read and 1 GiB written per copy
+    f_factor = moved_kib / _copy_kib(cf_dir, "FETCH_SIZE")
+    w_factor = moved_kib / _copy_kib(cw_dir, "WRITE_SIZE")
+    if kernel not in stats:
+        raise RuntimeError(f"{kernel} not in the kernel trace: {sorted(stats)}")
+    # sites per dispatch of the kernel, from the workload's own report
+    if "by_kernel" in out:
+        sites = next(v["sites_per_launch"] for k, v in out["by_kernel"].items())
+        ev_us = next(v["avg_launch_us"] for k, v in out["by_kernel"].items())
+    else:
+        sites, ev_us = out["sites_per_launch"], out.get("avg_launch_us")
+    st = stats[kernel]
+    fk, fn = fetch[kernel]
+    wk, wn = write[kernel]
+    hbm = (fk * f_factor + wk * w_factor) * 1024
+    alg_b = alg * sites
+    lines = [f"# {kernel} -- {name}", "",
+             f"Workload: `python3 bench.py {' '.join(args)}`: {PROFILE_TEXT[name]}.",
+             "", "Commands (made by `python3 bench.py --rocprof-profile " + name +
+             " <dir>`; each counter in a run of its own):", "", "```",
+             "cd /tmp && export TMPDIR=/tmp", kt_cmd, f_cmd, w_cmd, cf_cmd, cw_cmd, "```", "",
+             "Kernel trace (`kernel_stats.csv`, top kernels):", "",
+             "| Kernel | Calls | Total ms | Avg us | Min us | Max us | % |",
+             "|---|---|---|---|---|---|---|"]
+    for k, v in sorted(stats.items(), key=lambda kv: -kv[1]["total_ms"])[:6]:
+        lines.append(f"| `{k}` | {v['calls']} | {v['total_ms']:.1f} | {v['avg_us']:.2f} | "
+                     f"{v['min_us']:.2f} | {v['max_us']:.2f} | {v['pct']:.2f} |")
+    lines += ["", f"`{kernel}`: {st['calls']} dispatches, average {st['avg_us']:.2f} us"
+              + (f" (HIP events in the same program: {ev_us:.2f} us per launch)." if ev_us else "."),
+              f"Algorithmic bytes: {alg} B per site x {sites} sites per dispatch = "
+              f"{alg_b / 1e6:.2f} MB, {alg_b / st['avg_us'] / 1e3:.0f} GB/s = "
+              f"{100 * alg_b / st['avg_us'] / 1e3 / HBM_PEAK_GBS:.1f} % of 8 TB/s.", "",
+              f"HBM counters per dispatch, scaled by a plain 1 GiB device copy "
+              f"(`{copy}`: FETCH_SIZE factor {f_factor:.4f}, WRITE_SIZE factor {w_factor:.4f}):", "",
+              "| Dispatches (fetch / write) | FETCH_SIZE KiB | WRITE_SIZE KiB | scaled HBM bytes | "
+              "bytes per site |", "|---|---|---|---|---|",
+              f"| {fn} / {wn} | {fk:.1f} | {wk:.1f} | {hbm / 1e6:.2f} MB | {hbm / sites:.1f} |", "",
+              f"Measured HBM traffic {hbm / st['avg_us'] / 1e3:.0f} GB/s "
+              f"({100 * hbm / st['avg_us'] / 1e3 / HBM_PEAK_GBS:.1f} % of peak)."]
+    md = f"{kernel.replace('<', '_').replace('>', '').replace(', ', '_')}_{name}.md"
+    with open(os.path.join(ROOT, "profiles", md), "w") as f:
+        f.write("\n".join(lines) + "\n")
+    jpath = os.path.join(ROOT, "profiles", "kernels_hbm.json")
+    table = json.load(open(jpath)) if os.path.exists(jpath) else {}
+    # entries of kernels this mode profiles only
+    profiled = {k for _, k, _ in PROFILE_WORKLOADS.values()}
+    table = {k: v for k, v in table.items() if k in profiled}
+    table[kernel] = {"workload": name, "fetch_kib_per_dispatch": round(fk, 2),
+                     "write_kib_per_dispatch": round(wk, 2), "fetch_factor": round(f_factor, 5),
+                     "write_factor": round(w_factor, 5), "sites_per_dispatch": sites,
+                     "hbm_bytes_per_site": round(hbm / sites, 3),
+                     "avg_launch_us": round(st["avg_us"], 2), "profile": md}
+    with open(jpath, "w") as f:
+        json.dump(table, f, indent=1)
+    print(json.dumps({"rocprof_profile": name, "kernel": kernel, "avg_us": st["avg_us"],
+                      "hbm_bytes_per_site": hbm / sites, "summary": md}), flush=True)
 
 
 def profile_rounds(ctx, rt, tmax, tmin, masters, t0, rounds):
@@ -522,6 +736,12 @@ def main():
     ap.add_argument("--tile-shape", type=int, default=0,
                     help="force a sweep-block tile shape (rt_set_tile_shape; 0 = autotuned), "
                          "for profiling one kernel")
+    ap.add_argument("--rocprof-profile", nargs=2, metavar=("WORKLOAD", "OUTDIR"), default=None,
+                    help="run rocprofv3 (kernel trace, FETCH_SIZE, WRITE_SIZE, copy calibration) "
+                         f"on one of {sorted(PROFILE_WORKLOADS)} and write its summary under "
+                         "profiles/; raw output in OUTDIR")
+    ap.add_argument("--summarize-only", action="store_true",
+                    help="with --rocprof-profile: only re-read the raw output in OUTDIR")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="MiB: time a plain device copy of that size (HBM counter calibration)")
     args = ap.parse_args()
@@ -531,6 +751,9 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
+    if args.rocprof_profile:
+        rocprof_profile(*args.rocprof_profile, summarize_only=args.summarize_only)
+        return
     if args.copy_probe:
         copy_probe(args.copy_probe)
         return
@@ -547,6 +770,7 @@ def main():
         print(json.dumps({"profile_section": args.profile_section, **fn()}), flush=True)
         return
 
+    set_halo_kernels(args.halo)
     side = args.side
     d = rt.LozengeTiler.AlmostHexDomain(side, 0)
     tmin, tmax = rt.LozengeTiler.MinTiling(d), rt.LozengeTiler.MaxTiling(d)
@@ -626,7 +850,7 @@ def main():
     alg_bytes = float(ALG_BYTES_BY_SHAPE[shape]) * sites_per_launch
     achieved_gbs = alg_bytes / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
     traffic = None
-    counters = os.path.join(ROOT, "profiles", "k_fused_lozenge2_hex300_hbm.json")
+    counters = os.path.join(ROOT, "profiles", "kernels_hbm.json")
     if os.path.exists(counters) and side == 300:
         # FETCH_SIZE / WRITE_SIZE per dispatch of this kernel on this workload,
         # scaled by a plain copy's readings, as bytes per site: times this

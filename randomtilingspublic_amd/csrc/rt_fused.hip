@@ -335,6 +335,7 @@ __global__ __launch_bounds__(1024, 8) void k_walk_o8(const UnitDesc* __restrict_
 }  // namespace rt
 
 #include "swar_lozenge.hpp"
+#include "pair_lozenge.hpp"
 
 namespace rt {
 
@@ -621,8 +622,24 @@ static void launch_o8(rt_ctx* ctx, unsigned nblocks, const UnitDesc* units, cons
                        ctx->stream, units, work, (int)nblocks, xcd, H, mp, wins);
 }
 
+// Lozenge chain pairs on tile shape 7: both chains packed into one LDS word
+// per cell (k_pair_lozenge) when the halo is a multiple of 3, which that
+// kernel's compile-time row residues need; k_fused_o8<1> otherwise.
+static void launch_pair_lozenge(rt_ctx* ctx, unsigned nblocks, const UnitDesc* units,
+                                const WorkItem* work, int H, const ModelParams& mp,
+                                const WindowRec* wins) {
+  const int xcd = xcd_of(ctx, nblocks);
+  if (H % 3 == 0)
+    hipLaunchKernelGGL(k_pair_lozenge, dim3(grid_of(xcd, nblocks)), dim3(1024), 0, ctx->stream,
+                       units, work, (int)nblocks, xcd, H, mp, wins);
+  else
+    hipLaunchKernelGGL((k_fused_o8<M_LOZENGE>), dim3(grid_of(xcd, nblocks)), dim3(1024), 0,
+                       ctx->stream, units, work, (int)nblocks, xcd, H, mp, wins);
+}
+
 template <int MODEL, int NC>
 static LaunchFn pick_byte(int shape) {
+  if (shape == SH_16x3_O8 && MODEL == M_LOZENGE && NC == 2) return launch_pair_lozenge;
   if (shape == SH_16x3_O8) return launch_o8<MODEL, NC>;
   if (shape == SH_16x4) return launch<MODEL, NC, 16, 4>;
   if (shape == SH_16x2) return launch<MODEL, NC, 16, 2>;

@@ -15,15 +15,23 @@ struct Tiny {
   uint32_t mat1, mat2, tmat;
 };
 
+// a ^ b ^ c in one instruction (gfx950's three-input bit operation with
+// the parity truth table 0x96); the compiler splits some of these chains.
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+  uint32_t r;
+  asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0x96" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  return r;
+}
+
 __device__ __forceinline__ void tiny_next(Tiny& t) {
   uint32_t x = (t.s0 & 0x7fffffffu) ^ t.s1 ^ t.s2;
   uint32_t y = t.s3;
   x ^= x << 1;
-  y ^= (y >> 1) ^ x;
+  y = xor3(y, y >> 1, x);
   const uint32_t m = 0u - (y & 1u);
   t.s0 = t.s1;
   t.s1 = t.s2 ^ (m & t.mat1);
-  t.s2 = x ^ (y << 10) ^ (m & t.mat2);
+  t.s2 = xor3(x, y << 10, m & t.mat2);
   t.s3 = y;
 }
 
@@ -31,7 +39,7 @@ __device__ __forceinline__ void tiny_next(Tiny& t) {
 __device__ __forceinline__ uint32_t tiny_bits(Tiny& t) {
   tiny_next(t);
   const uint32_t t1 = t.s0 + (t.s2 >> 8);
-  return t.s3 ^ t1 ^ ((0u - (t1 & 1u)) & t.tmat);
+  return xor3(t.s3, t1, (0u - (t1 & 1u)) & t.tmat);
 }
 
 // The float the reference draws from tempered bits: ((b >> 9) | 1.0f) - 1.0f,
