@@ -162,6 +162,21 @@ typedef struct {
  * DominoTiler.h:89). Needs >= N*N/2 TinyMT parameter sets. */
 int rt_domino_walk(rt_ctx* ctx, int32_t* tiling, int N, long steps,
                    long seed, const rt_domino_rule* rule);
+/* rt_domino_walk with no device or host allocation in the call: no
+ * reference counterpart; for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and result as rt_domino_walk, in the reference's launch
+ * shape (one launch per colour step).  All device memory of the call -- the
+ * TinyMT states (16 B per stream), the two colour planes (N*N bytes) and
+ * the int32 tiling staged for the on-device split and merge (4*N*N bytes),
+ * each rounded up to 256 bytes -- lives in one fixed 16 MiB array per GPU
+ * (N up to 1136, i.e. Aztec diamonds up to order 565); larger N returns
+ * RT_EINVAL.  The array keeps the last call's contents and is shared by
+ * every context on that GPU: calls that overlap on one GPU race.  Needs
+ * the TinyMT parameter table set beforehand (rt_set_tinymt_params). */
+int rt_domino_walk_noalloc(rt_ctx* ctx, int32_t* tiling, int N, long steps,
+                           long seed, const rt_domino_rule* rule);
+
 /* DominoTiler::Walk(tiling&, vector<long> steps, vector<long> seeds)
  * (DominoTiler.cpp:125-231, DominoTiler.h:92). */
 int rt_domino_walk_segments(rt_ctx* ctx, int32_t* tiling, int N,

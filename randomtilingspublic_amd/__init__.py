@@ -93,6 +93,8 @@ def lib():
     L.rt_last_timing.argtypes = [vp, C.POINTER(_Timing)]
     L.rt_set_engine.argtypes = [vp, C.c_int]
     L.rt_domino_walk.argtypes = [vp, _I32P, C.c_int, C.c_long, C.c_long, C.POINTER(DominoRule)]
+    L.rt_domino_walk_noalloc.argtypes = [vp, _I32P, C.c_int, C.c_long, C.c_long,
+                                         C.POINTER(DominoRule)]
     L.rt_domino_walk_segments.argtypes = [vp, _I32P, C.c_int, _LONGP, _LONGP, C.c_int,
                                           C.POINTER(DominoRule)]
     L.rt_domino_cftp.argtypes = [vp, _I32P, _I32P, C.c_int, C.c_uint32, C.c_int, C.c_long,
@@ -316,6 +318,13 @@ class Context:
         t = _i32(t).copy()
         r = DominoRule(rule, a, b)
         self._chk(self._L.rt_domino_walk(self._h, t, side(t), steps, seed, C.byref(r)))
+        return t
+
+    def domino_walk_noalloc(self, t, steps, seed, rule=DOMINO_TOGGLE08, a=1.0, b=1.0):
+        """domino_walk through the entry point that allocates nothing."""
+        t = _i32(t).copy()
+        r = DominoRule(rule, a, b)
+        self._chk(self._L.rt_domino_walk_noalloc(self._h, t, side(t), steps, seed, C.byref(r)))
         return t
 
     def domino_walk_segments(self, t, steps, seeds, rule=DOMINO_TOGGLE08, a=1.0, b=1.0):

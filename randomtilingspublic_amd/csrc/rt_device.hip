@@ -191,6 +191,46 @@ __global__ void k_recttri_sync(TriArrays32 c0, TriArrays32 c1, int N, int last) 
 
 }  // namespace rt
 
+// ------------------------------------------------- allocation-free walk
+// Device memory of rt_domino_walk_noalloc: one 16 MiB array per GPU, laid
+// out per call as TinyMT states (16 B per stream) | both colour planes
+// (N*N bytes) | the int32 tiling staged for the on-device split and merge
+// (4*N*N bytes), each part 256-byte aligned.  Host code never uses the
+// array's name: rt_create looks up its address on the context's device
+// (hipGetSymbolAddress) and keeps it in the context.
+constexpr size_t kNoAllocBytes = (size_t)16 << 20;
+static __device__ __align__(256) uint8_t scratch[kNoAllocBytes];
+
+namespace rt {
+
+static size_t noalloc_round(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Offsets of the three parts for side N; returns the bytes used.
+static size_t noalloc_layout(int N, size_t* planes_at, size_t* tiling_at) {
+  const size_t P = (size_t)(N - 2) * (size_t)(N / 2 - 2);
+  *planes_at = noalloc_round(16 * P);
+  *tiling_at = *planes_at + noalloc_round((size_t)N * N);
+  return *tiling_at + noalloc_round(4 * (size_t)N * N);
+}
+
+// Split / merge of DominoTiler.cpp:24-33, 109-114 on the device: cell
+// (i, j) of the int32 tiling <-> byte (i, j/2) of colour (i+j) % 2.
+__global__ void k_domino_split_dev(const int32_t* t, int N, uint8_t* black, uint8_t* white) {
+  const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= (long)N * N) return;
+  const int i = (int)(id / N), j = (int)(id % N);
+  ((i + j) % 2 == 0 ? black : white)[(long)i * (N / 2) + j / 2] = (uint8_t)t[id];
+}
+
+__global__ void k_domino_merge_dev(int32_t* t, int N, const uint8_t* black, const uint8_t* white) {
+  const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= (long)N * N) return;
+  const int i = (int)(id / N), j = (int)(id % N);
+  t[id] = (int32_t)(int8_t)((i + j) % 2 == 0 ? black : white)[(long)i * (N / 2) + j / 2];
+}
+
+}  // namespace rt
+
 // ------------------------------------------------------------------ context
 #define HIPCHK HIPCHK_RET
 
@@ -253,6 +293,12 @@ extern "C" int rt_create(rt_ctx** out, int device_ordinal) {
       }() ||
       hipMalloc(&c->d_flag, 64 * sizeof(uint32_t)) != hipSuccess ||
       hipHostMalloc(&c->h_flag, 64 * sizeof(uint32_t)) != hipSuccess) {
+    delete c;
+    return RT_EDEVICE;
+  }
+  // this device's copy of the allocation-free walk's scratch array
+  if (hipGetSymbolAddress(&c->noalloc_scratch, HIP_SYMBOL(scratch)) != hipSuccess ||
+      !c->noalloc_scratch) {
     delete c;
     return RT_EDEVICE;
   }
@@ -663,6 +709,69 @@ extern "C" int rt_domino_walk_segments(rt_ctx* ctx, int32_t* tiling, int N, cons
     s.seeds.push_back(seeds[k]);
   }
   return domino_walk_impl(ctx, tiling, N, s, rule);
+}
+
+extern "C" int rt_domino_walk_noalloc(rt_ctx* ctx, int32_t* tiling, int N, long steps,
+                                      long seed, const rt_domino_rule* rule) {
+  if (!ctx) return RT_EINVAL;
+  if (steps < 0) {
+    ctx->err = "negative step count";
+    return RT_EINVAL;
+  }
+  int rc = domino_check(ctx, tiling, N, rule);
+  if (rc) return rc;
+  size_t planes_at, tiling_at;
+  const size_t need = noalloc_layout(N, &planes_at, &tiling_at);
+  if (need > kNoAllocBytes) {
+    ctx->err = "rt_domino_walk_noalloc: side " + std::to_string(N) + " needs " +
+               std::to_string(need) + " bytes of device scratch, the fixed array holds " +
+               std::to_string(kNoAllocBytes);
+    return RT_EINVAL;
+  }
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  uint8_t* base = (uint8_t*)ctx->noalloc_scratch;
+  uint32_t* state = (uint32_t*)base;
+  DominoArrays a;
+  a.col[0] = base + planes_at;
+  a.col[1] = a.col[0] + (size_t)N * (N / 2);
+  int32_t* staged = (int32_t*)(base + tiling_at);
+  const long cells = (long)N * N;
+  const long P = (long)(N - 2) * (N / 2 - 2);
+  const int rk = rule ? rule->kind : RT_DOMINO_TOGGLE08;
+  const DominoWeights w = domino_weights(rule);
+  const StreamTable S = stream_table(ctx, state, P);
+  begin_timing(ctx);
+  HIPCHK(ctx, hipMemcpyAsync(staged, tiling, 4 * (size_t)cells, hipMemcpyHostToDevice,
+                             ctx->stream));
+  hipLaunchKernelGGL(k_domino_split_dev, dim3(blocks_for(cells, 256)), dim3(256), 0, ctx->stream,
+                     staged, N, a.col[0], a.col[1]);
+  // DominoTiler::Walk(t, steps, seed): mt19937(seed), first output = TinyMT
+  // base seed, then one colour per step (out % 2 == 1: black, index 0).
+  std::mt19937 g((uint32_t)seed);
+  rc = launch_init(ctx, S, P, (uint32_t)g());
+  if (rc) return rc;
+  int prev = -1;
+  for (long s = 0; s < steps; s++) {
+    const int c = (g() % 2 == 1) ? 0 : 1;
+    hipLaunchKernelGGL(k_domino_step<1>, dim3(blocks_for(P, 256)), dim3(256), 0, ctx->stream, a,
+                       a, S, N, c, prev >= 0 && prev != c ? 1 : 0, rk, w);
+    prev = c;
+  }
+  if (prev >= 0)
+    hipLaunchKernelGGL(k_domino_sync<1>, dim3(blocks_for(P, 256)), dim3(256), 0, ctx->stream, a,
+                       a, N, prev);
+  hipLaunchKernelGGL(k_domino_merge_dev, dim3(blocks_for(cells, 256)), dim3(256), 0, ctx->stream,
+                     staged, N, a.col[0], a.col[1]);
+  HIPCHK(ctx, hipGetLastError());
+  rc = end_timing(ctx);
+  if (rc) return rc;
+  ctx->last.launches = steps + (prev >= 0 ? 1 : 0) + 3;
+  ctx->last.steps = steps;
+  ctx->last.attempts = steps * P;
+  HIPCHK(ctx, hipMemcpyAsync(tiling, staged, 4 * (size_t)cells, hipMemcpyDeviceToHost,
+                             ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return RT_OK;
 }
 
 // ----------------------------------------------------------------- lozenge

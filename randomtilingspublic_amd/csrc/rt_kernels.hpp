@@ -51,6 +51,7 @@ struct rt_ctx {
   rt_round_fn round_fn = nullptr;  // CFTP round-end callback
   void* round_user = nullptr;
   hipEvent_t evt[32] = {};        // tuner trial boundaries
+  void* noalloc_scratch = nullptr;  // this device's rt_domino_walk_noalloc array
 
   int reserve(size_t bytes);
   int reserve_aux(size_t bytes);
