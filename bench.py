@@ -251,14 +251,13 @@ def weighted_and_bibone(rt, cftp_seeds=8, cftp_order=24, steps=10 ** 5):
     return out
 
 
-def served_stream(rt, seconds=20.0, seed=2024, dom=(64, 96), loz=(64, 80), t0=1024, burst_n=32,
-                  plain_n=24):
+def served_stream(rt, seconds=40.0, seed=2024, dom=(64, 256), loz=(64, 160), t0=1024, burst_n=32,
+                  plain_n=6):
     """A Poisson stream of CFTP requests through the continuous-batching
     server (rt_server), then the same requests one call at a time.  Request
     mix, drawn from `seed`: domino Aztec diamonds of order dom[0]..dom[1]
     (heat bath) and lozenge hexagons of side loz[0]..loz[1], doubling from
-    t0 (the low end of the served configuration's ranges: its upper sizes
-    take minutes per exact sample; rectangle-triangle has no CFTP).  The
+    t0 (rectangle-triangle has no CFTP: its coupling is not monotone).  The
     arrival rate is 70 % of the capacity measured on a burst of burst_n
     requests.  Reports samples/s, p50/p99 latency (submit to result) and
     the plain path's samples/s on the first plain_n requests."""
@@ -292,13 +291,26 @@ def served_stream(rt, seconds=20.0, seed=2024, dom=(64, 96), loz=(64, 80), t0=10
             return srv.submit(q["model"], q["tmax"], q["tmin"], q["master_seed"],
                               rt.SCHED_DOUBLING, t0, 32, rt.DOMINO_HEATBATH)
 
+        def finish(i, what):
+            """Poll request i until it is done (a progress line per minute)."""
+            t1, last = time.time(), time.time()
+            while True:
+                r = srv.poll(i)
+                if r is not None:
+                    assert r[0] == 0, r[0]
+                    return r
+                srv.wait(i, 1.0)
+                if time.time() - last > 60:
+                    last = time.time()
+                    print(f"[bench] served: {what} still running after {time.time() - t1:.0f} s",
+                          file=sys.stderr, flush=True)
+
         # capacity on a burst
         burst = [draw(k) for k in range(burst_n)]
         tb = time.time()
         ids = [submit(q) for q in burst]
-        for i in ids:
-            srv.wait(i, 120.0)
-            srv.poll(i)
+        for n, i in enumerate(ids):
+            finish(i, f"burst request {n}")
         cap = len(burst) / (time.time() - tb)
         rate = 0.7 * cap
         print(f"[bench] served: burst capacity {cap:.1f} samples/s, stream at {rate:.1f}/s",
@@ -308,6 +320,7 @@ def served_stream(rt, seconds=20.0, seed=2024, dom=(64, 96), loz=(64, 80), t0=10
         start = time.time()
         nxt = start
         k = 1000
+        last_report = start
         while time.time() - start < seconds:
             now = time.time()
             while nxt <= now:
@@ -315,6 +328,8 @@ def served_stream(rt, seconds=20.0, seed=2024, dom=(64, 96), loz=(64, 80), t0=10
                 k += 1
                 pending[submit(q)] = q
                 stream.append(q)
+                print(f"[bench] served: request {len(stream)} arrives (model {q['model']}, "
+                      f"size {q['n']})", file=sys.stderr, flush=True)
                 nxt += rng.exponential(1.0 / rate)
             for i in list(pending):
                 r = srv.poll(i)
@@ -322,12 +337,19 @@ def served_stream(rt, seconds=20.0, seed=2024, dom=(64, 96), loz=(64, 80), t0=10
                     assert r[0] == 0
                     lat.append(r[3])
                     del pending[i]
+                    print(f"[bench] served: a request completed after {r[3]:.1f} s",
+                          file=sys.stderr, flush=True)
+            if now - last_report > 60:
+                last_report = now
+                st = srv.stats()
+                print(f"[bench] served: {st['completed']} completed, {len(pending)} in flight, "
+                      f"{st['ticks']} device ticks", file=sys.stderr, flush=True)
             time.sleep(0.002)
-        for i in list(pending):
-            srv.wait(i, 120.0)
-            r = srv.poll(i)
-            assert r[0] == 0
+        for n, i in enumerate(list(pending)):
+            r = finish(i, f"stream request {n} of {len(pending)} left")
             lat.append(r[3])
+            print(f"[bench] served: drained {n + 1} of {len(pending)}", file=sys.stderr,
+                  flush=True)
         span = time.time() - start
     finally:
         srv.close()
@@ -338,7 +360,9 @@ def served_stream(rt, seconds=20.0, seed=2024, dom=(64, 96), loz=(64, 80), t0=10
     def plain_rate(engine, reqs):
         ctx.set_engine(engine)
         t1 = time.time()
-        for q in reqs:
+        for n, q in enumerate(reqs):
+            print(f"[bench] served: plain request {n} (model {q['model']}, size {q['n']}), "
+                  f"engine {engine}", file=sys.stderr, flush=True)
             if q["model"] == rt.MODEL_DOMINO:
                 ctx.domino_cftp(q["tmax"], q["tmin"], q["master_seed"], rt.SCHED_DOUBLING, t0,
                                 32, rt.DOMINO_HEATBATH)
@@ -348,8 +372,8 @@ def served_stream(rt, seconds=20.0, seed=2024, dom=(64, 96), loz=(64, 80), t0=10
         return len(reqs) / (time.time() - t1)
 
     sub = stream[:plain_n]
-    plain = plain_rate(rt.ENGINE_STEP, sub)
-    plain_fused = plain_rate(0, sub)
+    plain = plain_rate(rt.ENGINE_STEP, sub) if sub else None
+    plain_fused = plain_rate(0, sub) if sub else None
     ctx.set_engine(0)
     ctx.close()
     lat = np.array(lat)
@@ -357,15 +381,15 @@ def served_stream(rt, seconds=20.0, seed=2024, dom=(64, 96), loz=(64, 80), t0=10
                         f"(heat bath) and lozenge hexagon side {loz[0]}-{loz[1]}, doubling from "
                         f"2^{int(np.log2(t0))}, rate 70 % of burst capacity, continuous batching",
             "requests": len(stream), "span_s": round(span, 2),
-            "samples_per_s": round(served, 2), "burst_capacity_per_s": round(cap, 2),
+            "samples_per_s": round(served, 4), "burst_capacity_per_s": round(cap, 4),
             "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
             "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
-            "plain_samples_per_s": round(plain, 2),
-            "plain_fused_samples_per_s": round(plain_fused, 2),
+            "plain_samples_per_s": round(plain, 4) if plain else None,
+            "plain_fused_samples_per_s": round(plain_fused, 4) if plain_fused else None,
             "plain_requests": len(sub),
-            "served_vs_plain": round(served / plain, 2),
-            "capacity_vs_plain": round(cap / plain, 2),
-            "capacity_vs_plain_fused": round(cap / plain_fused, 2)}
+            "served_vs_plain": round(served / plain, 2) if plain else None,
+            "capacity_vs_plain": round(cap / plain, 2) if plain else None,
+            "capacity_vs_plain_fused": round(cap / plain_fused, 2) if plain_fused else None}
 
 
 def copy_probe(mib):
@@ -724,13 +748,15 @@ def main():
                     help="profiling run: only one section (plain Aztec sweep, 256-chain batch "
                          "walks, weighted walk / bibone, served stream), printed as one JSON "
                          "line")
-    ap.add_argument("--serve-dom", type=int, nargs=2, default=[64, 96], metavar=("LO", "HI"),
+    ap.add_argument("--serve-dom", type=int, nargs=2, default=[64, 256], metavar=("LO", "HI"),
                     help="served stream: domino Aztec orders")
-    ap.add_argument("--serve-loz", type=int, nargs=2, default=[64, 80], metavar=("LO", "HI"),
+    ap.add_argument("--serve-loz", type=int, nargs=2, default=[64, 160], metavar=("LO", "HI"),
                     help="served stream: lozenge hexagon sides")
-    ap.add_argument("--serve-seconds", type=float, default=20.0)
-    ap.add_argument("--serve-plain", type=int, default=24,
+    ap.add_argument("--serve-seconds", type=float, default=40.0)
+    ap.add_argument("--serve-plain", type=int, default=6,
                     help="served stream: requests timed on the plain paths")
+    ap.add_argument("--serve-burst", type=int, default=32,
+                    help="served stream: requests of the burst that measures capacity")
     ap.add_argument("--halo", type=int, default=6,
                     help="sweep-block halo depth (rt_set_halo; results never depend on it)")
     ap.add_argument("--tile-shape", type=int, default=0,
@@ -762,7 +788,8 @@ def main():
 
     def serve():
         return served_stream(rt, seconds=args.serve_seconds, dom=tuple(args.serve_dom),
-                             loz=tuple(args.serve_loz), plain_n=args.serve_plain)
+                             loz=tuple(args.serve_loz), plain_n=args.serve_plain,
+                             burst_n=args.serve_burst)
 
     if args.profile_section:
         fn = {"sweep": lambda: plain_sweep(rt), "walks": lambda: batched_walks(rt),

@@ -102,9 +102,10 @@ int rt_set_engine(rt_ctx* ctx, int engine);
  * shapes 1, 2, 3 with CFTP chain pairs split over two workgroups per tile
  * (each runs one chain, both draw the pair's shared numbers); 7 = 48 rows
  * by 16 waves x 3 rows at eight waves per SIMD (two workgroups per CU,
- * VGPRs capped at 64); 0 = chosen per job (autotuner, else a rule of
- * thumb on the tile count).  The serving loop always uses 64-row
- * tiles. */
+ * VGPRs capped at 64; lozenge chain pairs there hold both chains in one
+ * LDS word per cell when the halo is a multiple of 3); 0 = chosen per job
+ * (autotuner, else a rule of thumb on the tile count).  The serving loop
+ * picks 64-row tiles while its live tiles fit the CUs, shape 7 beyond. */
 int rt_set_tile_shape(rt_ctx* ctx, int shape);
 
 /* Tile-shape autotuner (no reference counterpart; on by default).  With
@@ -261,6 +262,17 @@ int rt_domino_walk_batch_observe(rt_ctx* ctx, int32_t* const* tilings, const int
                                  int nchains, long steps, const long* seeds,
                                  const rt_domino_rule* rule, int use_graph, int download,
                                  long* counts);
+
+/* The same, plus each chain's height function computed on the device from
+ * its final colour planes: heights[k] (Ns[k]^2 int32) receives
+ * DominoTiler::TilingToHeightfunc(t_k, TilingToDomain(t_k))
+ * (DominoTiler.cpp:555-593, 595-609): 0 at the first boundary vertex in
+ * row-major order, INT_MAX at vertices the reference's search does not
+ * reach.  counts may be NULL; download = 0 leaves tilings[] untouched. */
+int rt_domino_walk_batch_heights(rt_ctx* ctx, int32_t* const* tilings, const int* Ns,
+                                 int nchains, long steps, const long* seeds,
+                                 const rt_domino_rule* rule, int use_graph, int download,
+                                 long* counts, int32_t* const* heights);
 
 /* -------------------------------------------------------------- serving */
 /* Continuous-batching CFTP server on one context.  A device loop thread

@@ -121,6 +121,9 @@ def lib():
                                        C.c_long, C.c_int, C.POINTER(DominoRule), _I32P, C.c_void_p]
     L.rt_domino_walk_batch.argtypes = [vp, C.c_void_p, C.c_void_p, C.c_int, C.c_long, _LONGP,
                                        C.POINTER(DominoRule), C.c_int]
+    L.rt_domino_walk_batch_heights.argtypes = [vp, C.c_void_p, C.c_void_p, C.c_int, C.c_long,
+                                               _LONGP, C.POINTER(DominoRule), C.c_int, C.c_int,
+                                               _LONGP, C.c_void_p]
     L.rt_domino_walk_batch_observe.argtypes = [vp, C.c_void_p, C.c_void_p, C.c_int, C.c_long,
                                                _LONGP, C.POINTER(DominoRule), C.c_int, C.c_int,
                                                _LONGP]
@@ -441,6 +444,26 @@ class Context:
             self._h, C.cast(ptrs, C.c_void_p), Ns.ctypes.data_as(C.c_void_p), len(ts), steps, se,
             C.byref(r), 1 if use_graph else 0, 1 if download else 0, counts.reshape(-1)))
         return (ts if download else None), counts
+
+    def domino_walk_batch_heights(self, tilings, steps, seeds, rule=DOMINO_TOGGLE08, a=1.0,
+                                  b=1.0, use_graph=False, download=True):
+        """Batch walk plus each chain's height function computed on the
+        device (DominoTiler::TilingToHeightfunc of the final tiling on its
+        own domain): returns (tilings or None, counts[nchains, 5],
+        heights list of N x N int32 arrays flattened like a tiling)."""
+        ts = [_i32(t).copy() for t in tilings]
+        Ns = np.array([side(t) for t in ts], dtype=np.int32)
+        ptrs = (C.c_void_p * len(ts))(*[t.ctypes.data for t in ts])
+        hs = [np.zeros(len(t), dtype=np.int32) for t in ts]
+        hptrs = (C.c_void_p * len(hs))(*[h.ctypes.data for h in hs])
+        se = np.ascontiguousarray(seeds, dtype=np.int64)
+        r = DominoRule(rule, a, b)
+        counts = np.zeros((len(ts), 5), dtype=np.int64)
+        self._chk(self._L.rt_domino_walk_batch_heights(
+            self._h, C.cast(ptrs, C.c_void_p), Ns.ctypes.data_as(C.c_void_p), len(ts), steps, se,
+            C.byref(r), 1 if use_graph else 0, 1 if download else 0, counts.reshape(-1),
+            C.cast(hptrs, C.c_void_p)))
+        return (ts if download else None), counts, hs
 
     # ---- rectangle-triangle ----
     def recttri_walk(self, t, steps, seed, weights16=None):

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cstring>
 #include <random>
 #include <thread>
@@ -434,6 +435,151 @@ __global__ void k_domino_counts(const uint8_t* const* planes, const long* cells,
   }
 }
 
+// ---------------------------------------------------------------------
+// Height function of a domino tiling on the device, equal to
+// DominoTiler::TilingToHeightfunc(t, TilingToDomain(t)) (DominoTiler.cpp:
+// 555-593, 595-631): heights 0 at the first boundary vertex in row-major
+// order, spread over every edge no domino crosses whose two ends are
+// vertices of the region and one of whose two faces is in the domain,
+// changing by (east/west ? +1 : -1) * ((i+j) even ? +1 : -1) from (i, j)
+// (DominoTiler.cpp:335-341, 578-586); vertices not reached keep INT_MAX.
+// For a tiling, heights do not depend on the path (every face loop sums to
+// zero), so the breadth-first search becomes: (1) per row, prefix sums
+// along runs of traversable east edges; (2) per chain, offsets of those
+// runs linked through traversable north-south edges, by sweeps over the
+// rows (down, then up, repeated until a sweep adds nothing); (3) height =
+// run offset + prefix.  The domain and vertex classes come from the tiling
+// itself (TilingToDomain / DomainToVertices).
+struct HfChain {
+  const uint8_t* planes;  // black N x N/2, then white N x N/2
+  int N;
+  int first;              // row-major index of the first boundary vertex
+  int32_t* rr;            // N x N: prefix along the run; heights at the end
+  int32_t* run;           // N x N: column where the vertex's run starts, -1 outside
+  int32_t* off;           // N x N: offset of the run starting at (i, j)
+};
+
+constexpr int32_t kHfUnknown = INT32_MIN;
+
+__device__ __forceinline__ uint32_t hf_cell(const HfChain& c, int i, int j) {
+  const int H = c.N / 2;
+  return c.planes[(size_t)((i + j) & 1) * c.N * H + (size_t)i * H + j / 2];
+}
+
+// Face (fi, fj) of the (N-1) x (N-1) face grid is in the domain when a
+// domino covers it (TilingToDomain, DominoTiler.cpp:595-609); faces off
+// the grid are not.
+__device__ __forceinline__ bool hf_face(const HfChain& c, int fi, int fj) {
+  const int M = c.N - 1;
+  if (fi < 0 || fj < 0 || fi >= M || fj >= M) return false;
+  return (hf_cell(c, fi, fj) & (2u + 8u)) || (hf_cell(c, fi + 1, fj + 1) & (1u + 4u));
+}
+
+// DomainToVertices (DominoTiler.cpp:611-631): 2 boundary, 1 interior, 0
+// outside; grid-edge vertices look at their faces clamped into the grid.
+__device__ __forceinline__ int hf_status(const HfChain& c, int i, int j) {
+  const int M = c.N - 1;
+  if (i == 0 || j == 0 || i == M || j == M) {
+    const int a = i - (i == M), b = i - 1 + (i == 0), x = j - (j == M), y = j - 1 + (j == 0);
+    return (hf_face(c, a, x) || hf_face(c, b, x) || hf_face(c, a, y) || hf_face(c, b, y)) ? 2 : 0;
+  }
+  const bool f0 = hf_face(c, i - 1, j - 1), f1 = hf_face(c, i - 1, j), f2 = hf_face(c, i, j - 1),
+             f3 = hf_face(c, i, j);
+  if (f0 && f1 && f2 && f3) return 1;
+  return (f0 || f1 || f2 || f3) ? 2 : 0;
+}
+
+__device__ __forceinline__ int hf_sign(int i, int j) { return ((i + j) & 1) ? -1 : 1; }
+
+// (1) one thread per row: runs, prefixes, first boundary vertex.
+__global__ void k_hf_rows(HfChain* chains, int nchains) {
+  for (int k = blockIdx.y; k < nchains; k += gridDim.y) {
+    HfChain& c = chains[k];
+    const int N = c.N, i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) continue;
+    int start = 0, r = 0, first = INT32_MAX;
+    bool cont = false;
+    int st = hf_status(c, i, 0);
+    for (int j = 0; j < N; j++) {
+      const size_t v = (size_t)i * N + j;
+      const int st_next = j + 1 < N ? hf_status(c, i, j + 1) : 0;
+      c.off[v] = kHfUnknown;
+      if (st == 0) {
+        c.run[v] = -1;
+        cont = false;
+      } else {
+        if (st == 2 && first == INT32_MAX) first = (int)v;
+        if (!cont) {
+          start = j;
+          r = 0;
+        }
+        c.run[v] = start;
+        c.rr[v] = r;
+        // east edge: no domino across (bit 8 of (i, j)), a face above or below
+        cont = st_next > 0 && !(hf_cell(c, i, j) & 8u) && (hf_face(c, i - 1, j) || hf_face(c, i, j));
+        r += hf_sign(i, j);
+      }
+      st = st_next;
+    }
+    if (first != INT32_MAX) atomicMin(&c.first, first);
+  }
+}
+
+// (2) one workgroup per chain: run offsets through north-south edges.
+__global__ void k_hf_link(HfChain* chains, int nchains, int max_sweeps) {
+  for (int k = blockIdx.x; k < nchains; k += gridDim.x) {
+    HfChain& c = chains[k];
+    const int N = c.N;
+    const int f = c.first;
+    if (f == INT32_MAX) continue;  // no boundary vertex: nothing is reached
+    if (threadIdx.x == 0) {
+      const int fi = f / N;
+      c.off[(size_t)fi * N + c.run[f]] = -c.rr[f];  // height 0 at the first boundary vertex
+    }
+    __syncthreads();
+    for (int sweep = 0; sweep < max_sweeps; sweep++) {
+      int changed = 0;
+      for (int pass = 0; pass < 2; pass++) {  // down, then up
+        for (int q = 0; q + 1 < N; q++) {
+          const int i = pass == 0 ? q : N - 2 - q;  // edge between rows i and i+1
+          for (int j = threadIdx.x; j < N; j += blockDim.x) {
+            const size_t a = (size_t)i * N + j, b = a + N;
+            const int ra = c.run[a], rb = c.run[b];
+            if (ra < 0 || rb < 0) continue;
+            if (!(hf_face(c, i, j - 1) || hf_face(c, i, j))) continue;
+            int32_t* oa = &c.off[(size_t)i * N + ra];
+            int32_t* ob = &c.off[(size_t)(i + 1) * N + rb];
+            if (pass == 0) {  // down from (i, j): bit 2, step -sign(i, j)
+              if (*oa == kHfUnknown || *ob != kHfUnknown || (hf_cell(c, i, j) & 2u)) continue;
+              *ob = *oa + c.rr[a] - hf_sign(i, j) - c.rr[b];
+            } else {  // up from (i+1, j): bit 1, step -sign(i+1, j)
+              if (*ob == kHfUnknown || *oa != kHfUnknown || (hf_cell(c, i + 1, j) & 1u)) continue;
+              *oa = *ob + c.rr[b] - hf_sign(i + 1, j) - c.rr[a];
+            }
+            changed = 1;
+          }
+          __syncthreads();
+        }
+      }
+      if (!__syncthreads_or(changed)) break;
+    }
+  }
+}
+
+// (3) heights.
+__global__ void k_hf_final(HfChain* chains, int nchains) {
+  for (int k = blockIdx.y; k < nchains; k += gridDim.y) {
+    HfChain& c = chains[k];
+    const long n = (long)c.N * c.N;
+    for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+         v += (long)gridDim.x * blockDim.x) {
+      const int r = c.run[v];
+      const int32_t o = r >= 0 ? c.off[(v / c.N) * c.N + r] : kHfUnknown;
+      c.rr[v] = o == kHfUnknown ? INT32_MAX : o + c.rr[v];
+    }
+  }
+}
+
 __global__ void k_zero_u64(unsigned long long* p, long n) {
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (long)gridDim.x * blockDim.x)
@@ -442,7 +588,7 @@ __global__ void k_zero_u64(unsigned long long* p, long n) {
 
 static int walk_batch(rt_ctx* ctx, int32_t* const* tilings, const int* Ns, int nchains, long steps,
                       const long* seeds, const rt_domino_rule* rule, int use_graph, int download,
-                      long* counts);
+                      long* counts, int32_t* const* heights = nullptr);
 
 extern "C" int rt_domino_walk_batch(rt_ctx* ctx, int32_t* const* tilings, const int* Ns,
                                     int nchains, long steps, const long* seeds,
@@ -461,9 +607,27 @@ extern "C" int rt_domino_walk_batch_observe(rt_ctx* ctx, int32_t* const* tilings
   return walk_batch(ctx, tilings, Ns, nchains, steps, seeds, rule, use_graph, download, counts);
 }
 
+extern "C" int rt_domino_walk_batch_heights(rt_ctx* ctx, int32_t* const* tilings, const int* Ns,
+                                            int nchains, long steps, const long* seeds,
+                                            const rt_domino_rule* rule, int use_graph,
+                                            int download, long* counts,
+                                            int32_t* const* heights) {
+  if (!heights) {
+    if (ctx) ctx->err = "heights output missing";
+    return RT_EINVAL;
+  }
+  for (int k = 0; k < nchains && heights; k++)
+    if (!heights[k]) {
+      if (ctx) ctx->err = "heights output of a chain missing";
+      return RT_EINVAL;
+    }
+  return walk_batch(ctx, tilings, Ns, nchains, steps, seeds, rule, use_graph, download, counts,
+                    heights);
+}
+
 static int walk_batch(rt_ctx* ctx, int32_t* const* tilings, const int* Ns, int nchains, long steps,
                       const long* seeds, const rt_domino_rule* rule, int use_graph, int download,
-                      long* counts) {
+                      long* counts, int32_t* const* heights) {
   if (!ctx) return RT_EINVAL;
   if (!tilings || !Ns || !seeds || nchains <= 0 || steps < 0) {
     ctx->err = "bad batch arguments";
@@ -611,6 +775,41 @@ static int walk_batch(rt_ctx* ctx, int32_t* const* tilings, const int* Ns, int n
                                    ctx->stream));
     HIPCHK_RET(ctx, hipStreamSynchronize(ctx->stream));
     for (size_t q = 0; q < h.size(); q++) counts[q] = (long)h[q];
+  }
+  if (heights) {
+    // workspace in the aux block (the walk's launch tables are done with):
+    // chain table, then rr | run | off per chain
+    size_t bytes = ((size_t)nchains * sizeof(HfChain) + 255) & ~(size_t)255;
+    std::vector<size_t> at(nchains);
+    int maxN = 0;
+    for (int k = 0; k < nchains; k++) {
+      at[k] = bytes;
+      bytes += 3 * ((4 * (size_t)Ns[k] * Ns[k] + 255) & ~(size_t)255);
+      maxN = std::max(maxN, Ns[k]);
+    }
+    HIPCHK_RET(ctx, hipStreamSynchronize(ctx->stream));
+    rc = ctx->reserve_aux(bytes);
+    if (rc) return rc;
+    uint8_t* base = (uint8_t*)ctx->aux;
+    std::vector<HfChain> hc(nchains);
+    for (int k = 0; k < nchains; k++) {
+      const size_t nn = (4 * (size_t)Ns[k] * Ns[k] + 255) & ~(size_t)255;
+      hc[k] = HfChain{planes[k], Ns[k], INT32_MAX, (int32_t*)(base + at[k]),
+                      (int32_t*)(base + at[k] + nn), (int32_t*)(base + at[k] + 2 * nn)};
+    }
+    HfChain* d_chains = (HfChain*)base;
+    HIPCHK_RET(ctx, hipMemcpy(d_chains, hc.data(), nchains * sizeof(HfChain),
+                              hipMemcpyHostToDevice));
+    const unsigned gy = (unsigned)std::min(nchains, 4096);
+    hipLaunchKernelGGL(k_hf_rows, dim3((unsigned)((maxN + 63) / 64), gy), dim3(64), 0,
+                       ctx->stream, d_chains, nchains);
+    hipLaunchKernelGGL(k_hf_link, dim3((unsigned)std::min(nchains, 4096)), dim3(1024), 0,
+                       ctx->stream, d_chains, nchains, 64);
+    hipLaunchKernelGGL(k_hf_final, dim3(64, gy), dim3(256), 0, ctx->stream, d_chains, nchains);
+    HIPCHK_RET(ctx, hipGetLastError());
+    for (int k = 0; k < nchains; k++)
+      HIPCHK_RET(ctx, hipMemcpy(heights[k], hc[k].rr, 4 * (size_t)Ns[k] * Ns[k],
+                                hipMemcpyDeviceToHost));
   }
   if (download)
     for (int k = 0; k < nchains; k++) {

@@ -31,34 +31,25 @@ struct Bi {
   int M;
 };
 
-struct Cell {
-  int k, i, j;
+// Cell accessor of the one-launch-per-phase kernels: the planes in HBM.
+struct GAcc {
+  const Bi& B;
+  __device__ int M() const { return B.M; }
+  __device__ int get(int pl, int i, int j) const { return B.p[pl][i * B.M + j]; }
+  __device__ void set(int pl, int i, int j, int v) const { B.p[pl][i * B.M + j] = v; }
 };
 
-__device__ __forceinline__ bool cell_of(const Bi& B, Cell& c) {
-  c.k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c.k >= B.M * B.M) return false;
-  c.i = c.k / B.M;
-  c.j = c.k - c.i * B.M;
-  return true;
+template <class A>
+__device__ __forceinline__ bool interior(const A& a, int i, int j) {
+  return i > 0 && j > 0 && i < a.M() - 1 && j < a.M() - 1;
 }
 
-__device__ __forceinline__ bool interior(const Bi& B, const Cell& c) {
-  return c.i > 0 && c.j > 0 && c.i < B.M - 1 && c.j < B.M - 1;
-}
-
-__device__ __forceinline__ int at(const Bi& B, int pl, int i, int j) { return B.p[pl][i * B.M + j]; }
 __device__ __forceinline__ int bt(int x, int k) { return (x >> k) & 1; }
 __device__ __forceinline__ int fld(int x, int k) { return (x >> (2 * k)) & 3; }
 __device__ __forceinline__ int mod3(int x) { return ((x % 3) + 3) % 3; }
 
 // One draw of stream k; true when rd < 0.5, i.e. the tempered word < 2^31.
-__device__ __forceinline__ bool coin(const StreamTable& S, int k) {
-  Tiny t = tiny_load(S, k);
-  const uint32_t b = tiny_bits(t);
-  tiny_store(S, k, t);
-  return b < kBelow05;
-}
+__device__ __forceinline__ bool coin(Tiny& t) { return tiny_bits(t) < kBelow05; }
 
 __global__ void k_bi_init(StreamTable S, int n, uint32_t seed) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -71,78 +62,69 @@ __global__ void k_bi_init(StreamTable S, int n, uint32_t seed) {
   tiny_store(S, k, t);
 }
 
-// ---------------------------------------------------------------- rotations
+// ------------------------------------------------------------- rotations
 // RotateLozenges (triangledimerkernel.cl:4-22): sublattice parity of
 // orientation o is i (H), j (L) or i+j (R); 5 <-> 10 with probability 1/2.
-__global__ void k_bi_rot_loz(Bi B, StreamTable S, int o, int c) {
-  Cell x;
-  if (!cell_of(B, x)) return;
-  const bool heads = coin(S, x.k);
-  const int par = (o == 0 ? x.i : (o == 1 ? x.j : x.i + x.j)) & 1;
-  int32_t* p = B.p[EH + o] + x.k;
-  const int v = *p;
-  if (heads && par == c && (v == 5 || v == 10)) *p = 15 - v;
+template <class A>
+__device__ __forceinline__ void ph_rot_loz(const A& a, int i, int j, bool heads, int o, int c) {
+  const int par = (o == 0 ? i : (o == 1 ? j : i + j)) & 1;
+  const int v = a.get(EH + o, i, j);
+  if (heads && par == c && (v == 5 || v == 10)) a.set(EH + o, i, j, 15 - v);
 }
 
 // RotateTriangles (190-211): class (j-i) mod 3; 45 <-> 54.
-__global__ void k_bi_rot_tri(Bi B, StreamTable S, int pl, int c) {
-  Cell x;
-  if (!cell_of(B, x)) return;
-  const bool heads = coin(S, x.k);
-  int32_t* p = B.p[pl] + x.k;
-  const int v = *p;
-  if (heads && mod3(x.j - x.i) == c && (v == 45 || v == 54)) *p = 99 - v;
+template <class A>
+__device__ __forceinline__ void ph_rot_tri(const A& a, int i, int j, bool heads, int pl, int c) {
+  const int v = a.get(pl, i, j);
+  if (heads && mod3(j - i) == c && (v == 45 || v == 54)) a.set(pl, i, j, 99 - v);
 }
 
 // RotateButterflys (330-352): slice i, j or i+j (mod 3) == p1; the flip
 // 85 <-> 170 happens on the cells whose parity (j for H and R, i for L)
 // equals p2.
-__global__ void k_bi_rot_bf(Bi B, StreamTable S, int o, int p1, int p2) {
-  Cell x;
-  if (!cell_of(B, x)) return;
-  const bool heads = coin(S, x.k);
-  const int slice = (o == 0 ? x.i : (o == 1 ? x.j : x.i + x.j)) % 3;
-  const int q = (o == 1 ? x.i : x.j) & 1;
-  int32_t* p = B.p[BH + o] + x.k;
-  const int v = *p;
-  if (heads && slice == p1 && q == p2 && (v == 85 || v == 170)) *p = 255 - v;
+template <class A>
+__device__ __forceinline__ void ph_rot_bf(const A& a, int i, int j, bool heads, int o, int p1,
+                                          int p2) {
+  const int slice = (o == 0 ? i : (o == 1 ? j : i + j)) % 3;
+  const int q = (o == 1 ? i : j) & 1;
+  const int v = a.get(BH + o, i, j);
+  if (heads && slice == p1 && q == p2 && (v == 85 || v == 170)) a.set(BH + o, i, j, 255 - v);
 }
 
-// ------------------------------------------------------------ lozenge moves
+// ---------------------------------------------------------- lozenge moves
 // UpdateLozengesFlipped (24-34): cells of the other parity rebuilt from
 // their four neighbours a, b, c, d (bit 2 of a, bit 3 of b, bit 0 of c and
 // bit 1 of d, in that order).
-__global__ void k_bi_loz_refresh(Bi B, int o, int c) {
-  Cell x;
-  if (!cell_of(B, x) || !interior(B, x)) return;
-  const int par = (o == 0 ? x.i : (o == 1 ? x.j : x.i + x.j)) & 1;
+template <class A>
+__device__ __forceinline__ void ph_loz_refresh(const A& a, int i, int j, int o, int c) {
+  if (!interior(a, i, j)) return;
+  const int par = (o == 0 ? i : (o == 1 ? j : i + j)) & 1;
   if (par == c) return;
-  const int pl = EH + o, i = x.i, j = x.j;
+  const int pl = EH + o;
   const int s = o & 1;            // column shift of the a / c neighbours
   const int r = o == 0 ? 1 : 0;   // row shift of the b / d neighbours
-  const int a = at(B, pl, i - 1, j + s), b = at(B, pl, i - r, j + 1);
-  const int cc = at(B, pl, i + 1, j - s), d = at(B, pl, i + r, j - 1);
-  B.p[pl][x.k] = bt(a, 2) | bt(b, 3) << 1 | bt(cc, 0) << 2 | bt(d, 1) << 3;
+  const int na = a.get(pl, i - 1, j + s), nb = a.get(pl, i - r, j + 1);
+  const int nc = a.get(pl, i + 1, j - s), nd = a.get(pl, i + r, j - 1);
+  a.set(pl, i, j, bt(na, 2) | bt(nb, 3) << 1 | bt(nc, 0) << 2 | bt(nd, 1) << 3);
 }
 
 // UpdateLozenges0/1/2 (36-79): the two other lozenge orientations take the
 // dimer bits they share with the rotated orientation o.
-__global__ void k_bi_loz_share(Bi B, int o) {
-  Cell x;
-  if (!cell_of(B, x) || !interior(B, x)) return;
-  const int i = x.i, j = x.j, k = x.k;
+template <class A>
+__device__ __forceinline__ void ph_loz_share(const A& a, int i, int j, int o) {
+  if (!interior(a, i, j)) return;
   if (o == 0) {
-    const int dn = at(B, EH, i + 1, j);
-    B.p[EL][k] = (B.p[EL][k] & ~10) | bt(dn, 0) << 1 | bt(at(B, EH, i + 1, j - 1), 0) << 3;
-    B.p[ER][k] = (B.p[ER][k] & ~10) | (dn & 2) | (at(B, EH, i, j) & 8);
+    const int dn = a.get(EH, i + 1, j);
+    a.set(EL, i, j, (a.get(EL, i, j) & ~10) | bt(dn, 0) << 1 | bt(a.get(EH, i + 1, j - 1), 0) << 3);
+    a.set(ER, i, j, (a.get(ER, i, j) & ~10) | (dn & 2) | (a.get(EH, i, j) & 8));
   } else if (o == 1) {
-    const int me = at(B, EL, i, j);
-    B.p[ER][k] = (B.p[ER][k] & ~5) | (me & 1) | bt(at(B, EL, i + 1, j), 0) << 2;
-    B.p[EH][k] = (B.p[EH][k] & ~5) | bt(at(B, EL, i - 1, j), 1) | bt(me, 1) << 2;
+    const int me = a.get(EL, i, j);
+    a.set(ER, i, j, (a.get(ER, i, j) & ~5) | (me & 1) | bt(a.get(EL, i + 1, j), 0) << 2);
+    a.set(EH, i, j, (a.get(EH, i, j) & ~5) | bt(a.get(EL, i - 1, j), 1) | bt(me, 1) << 2);
   } else {
-    const int lf = at(B, ER, i, j - 1);
-    B.p[EH][k] = (B.p[EH][k] & ~10) | (at(B, ER, i - 1, j) & 2) | bt(lf, 1) << 3;
-    B.p[EL][k] = (B.p[EL][k] & ~5) | (at(B, ER, i, j) & 1) | (lf & 4);
+    const int lf = a.get(ER, i, j - 1);
+    a.set(EH, i, j, (a.get(EH, i, j) & ~10) | (a.get(ER, i - 1, j) & 2) | bt(lf, 1) << 3);
+    a.set(EL, i, j, (a.get(EL, i, j) & ~5) | (a.get(ER, i, j) & 1) | (lf & 4));
   }
 }
 
@@ -158,35 +140,34 @@ __device__ __forceinline__ int pack4(int f0, int f1, int f2, int f3, int c) {
 // Everything rebuilt from the lozenge planes alone, selected by `mask`:
 // bit 0 tU (UpdateTriangleUFromLozenges, 97-119), bits 1..3 bH, bL, bR
 // (UpdateButterflys{H,L,R}FromLozenge, 121-188).
-__global__ void k_bi_from_loz(Bi B, int mask) {
-  Cell x;
-  if (!cell_of(B, x) || !interior(B, x)) return;
-  const int i = x.i, j = x.j, k = x.k;
-  const int h = at(B, EH, i, j), l = at(B, EL, i, j), r = at(B, ER, i, j);
-  const int hd = at(B, EH, i + 1, j), lr = at(B, EL, i, j + 1);
+template <class A>
+__device__ __forceinline__ void ph_from_loz(const A& a, int i, int j, int mask) {
+  if (!interior(a, i, j)) return;
+  const int h = a.get(EH, i, j), l = a.get(EL, i, j), r = a.get(ER, i, j);
+  const int hd = a.get(EH, i + 1, j), lr = a.get(EL, i, j + 1);
   if (mask & 1) {
     const int L = bt(l, 0) + 2 * bt(h, 3) + 3 * bt(hd, 0);
     const int R = bt(lr, 0) + 2 * bt(hd, 1) + 3 * bt(lr, 1);
     const int V = bt(lr, 2) + 2 * bt(hd, 3) + 3 * bt(hd, 2);
-    B.p[TU][k] = L + 4 * R + 16 * V;
+    a.set(TU, i, j, L + 4 * R + 16 * V);
   }
   if (mask & 2) {
-    const int ru = at(B, ER, i - 1, j), lu = at(B, EL, i - 1, j + 1);
-    B.p[BH][k] = pack4(wing(ru, 3, 0, h, 0), wing(lu, 0, 1, h, 1), wing(r, 1, 2, h, 2),
-                       wing(l, 2, 3, h, 3), bt(lu, 2));
+    const int ru = a.get(ER, i - 1, j), lu = a.get(EL, i - 1, j + 1);
+    a.set(BH, i, j, pack4(wing(ru, 3, 0, h, 0), wing(lu, 0, 1, h, 1), wing(r, 1, 2, h, 2),
+                          wing(l, 2, 3, h, 3), bt(lu, 2)));
   }
   if (mask & 4) {
-    const int hdl = at(B, EH, i + 1, j - 1), rl = at(B, ER, i, j - 1);
-    B.p[BL][k] = pack4(wing(h, 0, 1, l, 0), wing(r, 1, 2, l, 1), wing(hdl, 2, 3, l, 2),
-                       wing(rl, 3, 0, l, 3), bt(r, 3));
+    const int hdl = a.get(EH, i + 1, j - 1), rl = a.get(ER, i, j - 1);
+    a.set(BL, i, j, pack4(wing(h, 0, 1, l, 0), wing(r, 1, 2, l, 1), wing(hdl, 2, 3, l, 2),
+                          wing(rl, 3, 0, l, 3), bt(r, 3)));
   }
   if (mask & 8) {
-    B.p[BR][k] = pack4(wing(h, 0, 1, r, 0), wing(lr, 0, 1, r, 1), wing(hd, 2, 3, r, 2),
-                       wing(l, 2, 3, r, 3), bt(lr, 3));
+    a.set(BR, i, j, pack4(wing(h, 0, 1, r, 0), wing(lr, 0, 1, r, 1), wing(hd, 2, 3, r, 2),
+                          wing(l, 2, 3, r, 3), bt(lr, 3)));
   }
 }
 
-// ----------------------------------------------------------- triangle moves
+// --------------------------------------------------------- triangle moves
 // tri_code(x) = 1, 2 or 3 when x's fields say the shared slot is used in
 // the way UpdateTriangles (257-278) tests, summed as the reference sums.
 __device__ __forceinline__ int tri_code(int x) {
@@ -195,59 +176,57 @@ __device__ __forceinline__ int tri_code(int x) {
 
 // UpdateTriangles: dir 0 rebuilds tD from tU (vertical neighbour above),
 // dir 1 rebuilds tU from tD (neighbour below).
-__global__ void k_bi_tri_pair(Bi B, int dir) {
-  Cell x;
-  if (!cell_of(B, x) || !interior(B, x)) return;
+template <class A>
+__device__ __forceinline__ void ph_tri_pair(const A& a, int i, int j, int dir) {
+  if (!interior(a, i, j)) return;
   const int src = dir ? TD : TU, dst = dir ? TU : TD;
-  const int i = x.i, j = x.j;
-  const int a = at(B, src, i, j - 1 + dir), b = at(B, src, i, j + dir);
-  const int v = at(B, src, i + (dir ? 1 : -1), j);
-  B.p[dst][x.k] = tri_code(a) + 4 * tri_code(b) + 16 * tri_code(v);
+  const int na = a.get(src, i, j - 1 + dir), nb = a.get(src, i, j + dir);
+  const int v = a.get(src, i + (dir ? 1 : -1), j);
+  a.set(dst, i, j, tri_code(na) + 4 * tri_code(nb) + 16 * tri_code(v));
 }
 
 // UpdateTrianglesFlipped0/1 (213-255): the two classes that did not rotate
 // take their fields from three class-c neighbours.  For class c+1 (p = 1)
 // the fields come from (f1, f2, f0) of the neighbours, for class c+2 from
 // (f2, f0, f1).
-__global__ void k_bi_tri_refresh(Bi B, int down, int c) {
-  Cell x;
-  if (!cell_of(B, x) || !interior(B, x)) return;
-  const int cls = mod3(x.j - x.i);
+template <class A>
+__device__ __forceinline__ void ph_tri_refresh(const A& a, int i, int j, int down, int c) {
+  if (!interior(a, i, j)) return;
+  const int cls = mod3(j - i);
   if (cls == c) return;
   const int p = cls == (c + 1) % 3;
-  const int pl = down ? TD : TU, i = x.i, j = x.j;
-  int a, b, d;
+  const int pl = down ? TD : TU;
+  int na, nb, nd;
   if (!down) {
-    a = at(B, pl, i - 1 + p, j - p);
-    b = at(B, pl, i - p, j + 1);
-    d = at(B, pl, i + 1, j - 1 + p);
+    na = a.get(pl, i - 1 + p, j - p);
+    nb = a.get(pl, i - p, j + 1);
+    nd = a.get(pl, i + 1, j - 1 + p);
   } else {
-    a = at(B, pl, i + 1 - p, j - 1);
-    b = at(B, pl, i + p, j + 1 - p);
-    d = at(B, pl, i - 1, j + p);
+    na = a.get(pl, i + 1 - p, j - 1);
+    nb = a.get(pl, i + p, j + 1 - p);
+    nd = a.get(pl, i - 1, j + p);
   }
-  const int g0 = p ? fld(a, 1) : fld(a, 2);
-  const int g1 = p ? fld(b, 2) : fld(b, 0);
-  const int g2 = p ? fld(d, 0) : fld(d, 1);
-  B.p[pl][x.k] = g0 + 4 * g1 + 16 * g2;
+  const int g0 = p ? fld(na, 1) : fld(na, 2);
+  const int g1 = p ? fld(nb, 2) : fld(nb, 0);
+  const int g2 = p ? fld(nd, 0) : fld(nd, 1);
+  a.set(pl, i, j, g0 + 4 * g1 + 16 * g2);
 }
 
 // UpdateLozenge{H,L,R}FromTriangles (280-328).
-__global__ void k_bi_loz_from_tri(Bi B) {
-  Cell x;
-  if (!cell_of(B, x) || !interior(B, x)) return;
-  const int i = x.i, j = x.j, k = x.k;
-  const int d = at(B, TD, i, j);
-  const int uu = at(B, TU, i - 1, j), ul = at(B, TU, i, j - 1), u = at(B, TU, i, j);
-  B.p[EH][k] = (fld(d, 2) == 2) | (fld(d, 2) == 3) << 1 | (fld(uu, 2) == 3) << 2 |
-               (fld(uu, 2) == 2) << 3;
-  B.p[EL][k] = (fld(ul, 1) == 1) | (fld(ul, 1) == 3) << 1 | (fld(d, 0) == 1) << 2 |
-               (fld(d, 0) == 2) << 3;
-  B.p[ER][k] = (fld(u, 0) == 1) | (fld(d, 1) == 3) << 1 | (fld(d, 1) == 1) << 2 |
-               (fld(u, 0) == 2) << 3;
+template <class A>
+__device__ __forceinline__ void ph_loz_from_tri(const A& a, int i, int j) {
+  if (!interior(a, i, j)) return;
+  const int d = a.get(TD, i, j);
+  const int uu = a.get(TU, i - 1, j), ul = a.get(TU, i, j - 1), u = a.get(TU, i, j);
+  a.set(EH, i, j, (fld(d, 2) == 2) | (fld(d, 2) == 3) << 1 | (fld(uu, 2) == 3) << 2 |
+                      (fld(uu, 2) == 2) << 3);
+  a.set(EL, i, j, (fld(ul, 1) == 1) | (fld(ul, 1) == 3) << 1 | (fld(d, 0) == 1) << 2 |
+                      (fld(d, 0) == 2) << 3);
+  a.set(ER, i, j, (fld(u, 0) == 1) | (fld(d, 1) == 3) << 1 | (fld(d, 1) == 1) << 2 |
+                      (fld(u, 0) == 2) << 3);
 }
 
-// ---------------------------------------------------------- butterfly moves
+// -------------------------------------------------------- butterfly moves
 // Slice and per-orientation neighbour offsets of the butterfly refresh
 // kernels (355-649).  Orientation o: 0 H (slice i mod 3), 1 L (j mod 3),
 // 2 R ((i+j) mod 3).
@@ -262,53 +241,53 @@ __device__ __forceinline__ int flag(int x) { return bt(x, 8); }
 
 // Same slice, other parity (UpdateButterflysFlipped{H,L,R}1): the wings
 // turn with their two in-slice neighbours n1 (before) and n2 (after).
-__global__ void k_bi_bf_same(Bi B, int o, int p1, int p2) {
-  Cell x;
-  if (!cell_of(B, x) || !interior(B, x)) return;
-  const int i = x.i, j = x.j, pl = BH + o;
+template <class A>
+__device__ __forceinline__ void ph_bf_same(const A& a, int i, int j, int o, int p1, int p2) {
+  if (!interior(a, i, j)) return;
+  const int pl = BH + o;
   const int q = (o == 1 ? i : j) & 1;
   if (bf_slice(o, i, j) != p1 || q == p2) return;
-  const int c = flag(B.p[pl][x.k]);
+  const int c = flag(a.get(pl, i, j));
   int v;
   if (o == 2) {
-    const int n1 = at(B, pl, i + 1, j - 1), n2 = at(B, pl, i - 1, j + 1);
+    const int n1 = a.get(pl, i + 1, j - 1), n2 = a.get(pl, i - 1, j + 1);
     v = pack4(rot2(fld(n2, 3)), rot1(fld(n2, 2)), rot2(fld(n1, 1)), rot1(fld(n1, 0)), c);
   } else {
-    const int n1 = o == 0 ? at(B, pl, i, j - 1) : at(B, pl, i - 1, j);
-    const int n2 = o == 0 ? at(B, pl, i, j + 1) : at(B, pl, i + 1, j);
+    const int n1 = o == 0 ? a.get(pl, i, j - 1) : a.get(pl, i - 1, j);
+    const int n2 = o == 0 ? a.get(pl, i, j + 1) : a.get(pl, i + 1, j);
     v = pack4(rot1(fld(n1, 1)), rot2(fld(n2, 0)), rot1(fld(n2, 3)), rot2(fld(n1, 2)), c);
   }
-  B.p[pl][x.k] = v;
+  a.set(pl, i, j, v);
 }
 
 // Adjacent slices (UpdateButterflysFlipped{H,L,R}2{1,2,3}).  Stage 1 and 3
 // work on slice p1+1 (H) / p1+2 (L, R), stage 2 on the remaining one.
 // H and L read the same fields from differently placed neighbours.
-__global__ void k_bi_bf_stage(Bi B, int o, int p1, int stage) {
-  Cell x;
-  if (!cell_of(B, x) || !interior(B, x)) return;
-  const int i = x.i, j = x.j, pl = BH + o;
+template <class A>
+__device__ __forceinline__ void ph_bf_stage(const A& a, int i, int j, int o, int p1, int stage) {
+  if (!interior(a, i, j)) return;
+  const int pl = BH + o;
   const int s13 = o == 0 ? (p1 + 1) % 3 : (p1 + 2) % 3;
   const int s2 = o == 0 ? (p1 + 2) % 3 : (p1 + 1) % 3;
   const int sl = bf_slice(o, i, j);
   if (sl != (stage == 2 ? s2 : s13)) return;
-  const int v = B.p[pl][x.k];
+  const int v = a.get(pl, i, j);
   int f0 = fld(v, 0), f1 = fld(v, 1), f2 = fld(v, 2), f3 = fld(v, 3), c = flag(v);
   if (o != 2) {
     // neighbours: u1, u2 above (H: (i-1,j),(i-1,j+1); L: (i-1,j+1),(i,j+1)),
     //             d1, d2 below (H: (i+1,j),(i+1,j-1); L: (i+1,j-1),(i,j-1)).
     const bool H = o == 0;
     if (stage == 1) {
-      const int u1 = H ? at(B, pl, i - 1, j) : at(B, pl, i - 1, j + 1);
-      const int u2 = H ? at(B, pl, i - 1, j + 1) : at(B, pl, i, j + 1);
+      const int u1 = H ? a.get(pl, i - 1, j) : a.get(pl, i - 1, j + 1);
+      const int u2 = H ? a.get(pl, i - 1, j + 1) : a.get(pl, i, j + 1);
       c = fld(u1, 2) == 2;
       if (f0 != 2) f0 = is3(u1, 3) + 3 * is3(u1, 2);
       if (f1 != 1) f1 = 3 * is3(u2, 3) + 2 * is3(u2, 2);
     } else if (stage == 2) {
-      const int u1 = H ? at(B, pl, i - 1, j) : at(B, pl, i - 1, j + 1);
-      const int u2 = H ? at(B, pl, i - 1, j + 1) : at(B, pl, i, j + 1);
-      const int d1 = H ? at(B, pl, i + 1, j) : at(B, pl, i + 1, j - 1);
-      const int d2 = H ? at(B, pl, i + 1, j - 1) : at(B, pl, i, j - 1);
+      const int u1 = H ? a.get(pl, i - 1, j) : a.get(pl, i - 1, j + 1);
+      const int u2 = H ? a.get(pl, i - 1, j + 1) : a.get(pl, i, j + 1);
+      const int d1 = H ? a.get(pl, i + 1, j) : a.get(pl, i + 1, j - 1);
+      const int d2 = H ? a.get(pl, i + 1, j - 1) : a.get(pl, i, j - 1);
       c = fld(d1, 0) == 2;
       const int n0 = flag(u1) ? 2 : (f0 != 2) * f0;
       const int n1 = flag(u2) ? 1 : (f1 != 1) * f1;
@@ -317,21 +296,21 @@ __global__ void k_bi_bf_stage(Bi B, int o, int p1, int stage) {
       f0 = n0;
       f1 = n1;
     } else {
-      const int d1 = H ? at(B, pl, i + 1, j) : at(B, pl, i + 1, j - 1);
-      const int d2 = H ? at(B, pl, i + 1, j - 1) : at(B, pl, i, j - 1);
+      const int d1 = H ? a.get(pl, i + 1, j) : a.get(pl, i + 1, j - 1);
+      const int d2 = H ? a.get(pl, i + 1, j - 1) : a.get(pl, i, j - 1);
       f2 = flag(d1) ? 2 : (f2 != 2) * f2;
       f3 = flag(d2) ? 1 : (f3 != 1) * f3;
     }
   } else {
     // R: neighbours up (i-1,j), right (i,j+1), down (i+1,j), left (i,j-1).
     if (stage == 1) {
-      const int rt = at(B, pl, i, j + 1), dn = at(B, pl, i + 1, j);
+      const int rt = a.get(pl, i, j + 1), dn = a.get(pl, i + 1, j);
       c = fld(dn, 0) == 1;
       if (f1 != 2) f1 = is3(rt, 0) + 3 * is3(rt, 3);
       if (f2 != 1) f2 = 3 * is3(dn, 0) + 2 * is3(dn, 3);
     } else if (stage == 2) {
-      const int up = at(B, pl, i - 1, j), rt = at(B, pl, i, j + 1);
-      const int dn = at(B, pl, i + 1, j), lf = at(B, pl, i, j - 1);
+      const int up = a.get(pl, i - 1, j), rt = a.get(pl, i, j + 1);
+      const int dn = a.get(pl, i + 1, j), lf = a.get(pl, i, j - 1);
       c = fld(up, 2) == 1;
       if (f0 != 1) f0 = 3 * is3(up, 2) + 2 * is3(up, 1);
       const int n1 = flag(rt) ? 2 : (f1 != 2) * f1;
@@ -340,21 +319,21 @@ __global__ void k_bi_bf_stage(Bi B, int o, int p1, int stage) {
       f1 = n1;
       f2 = n2;
     } else {
-      const int up = at(B, pl, i - 1, j), lf = at(B, pl, i, j - 1);
+      const int up = a.get(pl, i - 1, j), lf = a.get(pl, i, j - 1);
       f0 = flag(up) ? 1 : (f0 != 1) * f0;
       f3 = flag(lf) ? 2 : (f3 != 2) * f3;
     }
   }
-  B.p[pl][x.k] = pack4(f0, f1, f2, f3, c);
+  a.set(pl, i, j, pack4(f0, f1, f2, f3, c));
 }
 
 // UpdateLozengeFromButterflys{H,L,R} (652-749): the three lozenge planes at
 // a butterfly cell of orientation o.  `full` is the lozenge of the
 // butterfly's own orientation, the two others read one field pair + flag.
-__global__ void k_bi_loz_from_bf(Bi B, int o) {
-  Cell x;
-  if (!cell_of(B, x) || !interior(B, x)) return;
-  const int v = B.p[BH + o][x.k];
+template <class A>
+__device__ __forceinline__ void ph_loz_from_bf(const A& a, int i, int j, int o) {
+  if (!interior(a, i, j)) return;
+  const int v = a.get(BH + o, i, j);
   const int f0 = fld(v, 0), f1 = fld(v, 1), f2 = fld(v, 2), f3 = fld(v, 3), c = flag(v);
   const int full = (f0 == 3) | (f1 == 3) << 1 | (f2 == 3) << 2 | (f3 == 3) << 3;
   int h, l, r;
@@ -371,15 +350,219 @@ __global__ void k_bi_loz_from_bf(Bi B, int o) {
     h = (f0 == 1) | (f0 == 2) << 1 | c << 2 | (f3 == 3) << 3;
     l = (f0 == 3) | c << 1 | (f3 == 1) << 2 | (f3 == 2) << 3;
   }
-  B.p[EH][x.k] = h;
-  B.p[EL][x.k] = l;
-  B.p[ER][x.k] = r;
+  a.set(EH, i, j, h);
+  a.set(EL, i, j, l);
+  a.set(ER, i, j, r);
+}
+
+// ------------------------------------- one launch per phase (engine 1)
+// Each kernel: one work item per plane cell, the phase above on it.
+__device__ __forceinline__ bool cell_of(const Bi& B, int& i, int& j) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= B.M * B.M) return false;
+  i = k / B.M;
+  j = k - i * B.M;
+  return true;
+}
+
+#define BI_KERNEL(name, phase, params, ...)  \
+  __global__ void name params {              \
+    int i, j;                                \
+    if (!cell_of(B, i, j)) return;           \
+    phase(GAcc{B}, i, j, __VA_ARGS__);       \
+  }
+
+__device__ __forceinline__ bool gcoin(const StreamTable& S, int k) {
+  Tiny t = tiny_load(S, k);
+  const bool h = coin(t);
+  tiny_store(S, k, t);
+  return h;
+}
+
+__global__ void k_bi_rot_loz(Bi B, StreamTable S, int o, int c) {
+  int i, j;
+  if (!cell_of(B, i, j)) return;
+  ph_rot_loz(GAcc{B}, i, j, gcoin(S, i * B.M + j), o, c);
+}
+__global__ void k_bi_rot_tri(Bi B, StreamTable S, int pl, int c) {
+  int i, j;
+  if (!cell_of(B, i, j)) return;
+  ph_rot_tri(GAcc{B}, i, j, gcoin(S, i * B.M + j), pl, c);
+}
+__global__ void k_bi_rot_bf(Bi B, StreamTable S, int o, int p1, int p2) {
+  int i, j;
+  if (!cell_of(B, i, j)) return;
+  ph_rot_bf(GAcc{B}, i, j, gcoin(S, i * B.M + j), o, p1, p2);
+}
+BI_KERNEL(k_bi_loz_refresh, ph_loz_refresh, (Bi B, int o, int c), o, c)
+BI_KERNEL(k_bi_loz_share, ph_loz_share, (Bi B, int o), o)
+BI_KERNEL(k_bi_from_loz, ph_from_loz, (Bi B, int mask), mask)
+BI_KERNEL(k_bi_tri_pair, ph_tri_pair, (Bi B, int dir), dir)
+BI_KERNEL(k_bi_tri_refresh, ph_tri_refresh, (Bi B, int down, int c), down, c)
+__global__ void k_bi_loz_from_tri(Bi B) {
+  int i, j;
+  if (!cell_of(B, i, j)) return;
+  ph_loz_from_tri(GAcc{B}, i, j);
+}
+BI_KERNEL(k_bi_bf_same, ph_bf_same, (Bi B, int o, int p1, int p2), o, p1, p2)
+BI_KERNEL(k_bi_bf_stage, ph_bf_stage, (Bi B, int o, int p1, int stage), o, p1, stage)
+BI_KERNEL(k_bi_loz_from_bf, ph_loz_from_bf, (Bi B, int o), o)
+#undef BI_KERNEL
+
+// ------------------------------------------ sweep blocks (engine 0, 2)
+// One workgroup owns a tile of LW x LW plane cells (an interior of side
+// LW - 2H plus an H-cell halo), holds all eight planes of it in LDS as
+// 16-bit values (every plane value is below 512) and the TinyMT states of
+// its cells in VGPRs, and runs every phase of a window of steps with one
+// workgroup barrier between phases.  Every phase reads neighbours at most
+// one cell away (rotations and UpdateLozengeFromButterflys none), so a step
+// makes the cells within its radius -- 4 for lozenge and triangle steps, 6
+// for butterfly steps -- of the tile's edge stale; a window whose radii sum
+// to at most H leaves the interior exact.  Halo cells draw from their
+// owners' streams, redundantly, so every draw is the reference's.  Planes
+// and states are double-buffered (read one set, write the other), as in
+// k_fused.
+constexpr int kBiMaxK = 16;  // steps per window
+// Tile geometry of the walk: 48 x 48 loaded cells, halo 8 (32 x 32
+// interior), 768 threads (three cells each).  Measured on the bibone
+// hexagon 200, 201, 200 (406 x 406 plane cells), device time per step:
+// 13.7 us against 14.7 for 64 x 64 tiles with halo 12 (1024 threads),
+// 13.8 with halo 16, 17.7 for 48 x 48 with halo 12, 17.7 for 32 x 32 with
+// halo 8 (256 threads); 21.8 for one launch per phase.
+constexpr int kBiLW = 48, kBiH = 8, kBiThreads = 768;
+
+__host__ __device__ constexpr int bi_radius(int type) { return type == 2 ? 6 : 4; }
+
+// Cell accessor over an LDS tile of LW x LW cells with a one-cell zero
+// margin; (r0, c0) is the plane cell of tile position (0, 0).
+template <int LW>
+struct TAcc {
+  static constexpr int RS = LW + 2, PL = RS * RS;
+  uint16_t* l;
+  int r0, c0, m;
+  __device__ int M() const { return m; }
+  __device__ int get(int pl, int i, int j) const {
+    return l[pl * PL + (i - r0 + 1) * RS + (j - c0 + 1)];
+  }
+  __device__ void set(int pl, int i, int j, int v) const {
+    l[pl * PL + (i - r0 + 1) * RS + (j - c0 + 1)] = (uint16_t)v;
+  }
+};
+
+struct BiMove {
+  uint8_t type, r1, r2, p1, p2;
+};
+
+// Tile geometry: LW x LW loaded cells, halo H, T threads; thread t owns
+// tile cells (t / LW + (T / LW) q, t % LW), q < LW * LW / T: one column per
+// lane, rows T / LW apart.  The four state words of its cells stay in
+// VGPRs; the streams' parameters, read only by the rotation draws, sit in
+// LDS.
+template <int LW>
+__device__ __forceinline__ bool bi_coin(uint32_t* s, const uint32_t* sp, int lc) {
+  Tiny t{s[0], s[1], s[2], s[3], sp[lc], sp[LW * LW + lc], sp[2 * LW * LW + lc]};
+  const bool h = coin(t);
+  s[0] = t.s0;
+  s[1] = t.s1;
+  s[2] = t.s2;
+  s[3] = t.s3;
+  return h;
+}
+
+template <int LW, int H, int T>
+__global__ __launch_bounds__(T) void k_bi_sweep(
+    const int32_t* __restrict__ pin, int32_t* __restrict__ pout, const uint32_t* __restrict__ sin,
+    uint32_t* __restrict__ sout, const uint32_t* __restrict__ param, long pstride, int M,
+    int tiles_c, const BiMove* __restrict__ moves, int K) {
+  constexpr int TH = LW - 2 * H;  // interior rows and columns
+  constexpr int kCells = LW * LW / T, kRowStep = T / LW;
+  constexpr int PLS = TAcc<LW>::PL;
+  static_assert(T % LW == 0 && (LW * LW) % T == 0, "tile rows split evenly over the threads");
+  __shared__ uint16_t lds[NPLANE * PLS];
+  __shared__ uint32_t sp[3 * LW * LW];
+  __shared__ BiMove smv[kBiMaxK];
+  const long n = (long)M * M;
+  const int tr = blockIdx.x / tiles_c, tc = blockIdx.x % tiles_c;
+  const int r0 = tr * TH - H, c0 = tc * TH - H;
+  for (int k = threadIdx.x; k < NPLANE * PLS; k += T) lds[k] = 0;
+  if (threadIdx.x < K) smv[threadIdx.x] = moves[threadIdx.x];
+  __syncthreads();
+  const TAcc<LW> a{lds, r0, c0, M};
+  const int lj = threadIdx.x % LW, li0 = threadIdx.x / LW;
+  const int gj = c0 + lj;
+  const bool col_in = gj >= 0 && gj < M;
+  uint32_t st[kCells][4];
+#pragma unroll
+  for (int q = 0; q < kCells; q++) {
+    const int gi = r0 + li0 + kRowStep * q, lc = (li0 + kRowStep * q) * LW + lj;
+    st[q][0] = st[q][1] = st[q][2] = st[q][3] = 0;
+    sp[lc] = sp[LW * LW + lc] = sp[2 * LW * LW + lc] = 0;
+    if (col_in && gi >= 0 && gi < M) {
+      const long k = (long)gi * M + gj;
+#pragma unroll
+      for (int pl = 0; pl < NPLANE; pl++) a.set(pl, gi, gj, pin[pl * n + k]);
+#pragma unroll
+      for (int w = 0; w < 4; w++) st[q][w] = sin[w * n + k];
+      sp[lc] = param[k];
+      sp[LW * LW + lc] = param[pstride + k];
+      sp[2 * LW * LW + lc] = param[2 * pstride + k];
+    }
+  }
+  __syncthreads();
+#define BI_PHASE(call)                                                      \
+  do {                                                                      \
+    _Pragma("unroll") for (int q = 0; q < kCells; q++) {                  \
+      const int gi = r0 + li0 + kRowStep * q;                                     \
+      if (col_in && gi >= 0 && gi < M) call;                                \
+    }                                                                       \
+    __syncthreads();                                                        \
+  } while (0)
+#define BI_COIN bi_coin<LW>(st[q], sp, (li0 + kRowStep * q) * LW + lj)
+  for (int s = 0; s < K; s++) {
+    const BiMove m = smv[s];
+    if (m.type == 0) {
+      BI_PHASE(ph_rot_loz(a, gi, gj, BI_COIN, m.r1, m.r2));
+      BI_PHASE(ph_loz_refresh(a, gi, gj, m.r1, m.r2));
+      BI_PHASE(ph_loz_share(a, gi, gj, m.r1));
+      BI_PHASE(ph_from_loz(a, gi, gj, 15));
+      BI_PHASE(ph_tri_pair(a, gi, gj, 0));
+    } else if (m.type == 1) {
+      const int down = m.r2 == 1;
+      BI_PHASE(ph_rot_tri(a, gi, gj, BI_COIN, down ? (int)TD : (int)TU, m.r1));
+      BI_PHASE(ph_tri_refresh(a, gi, gj, down, m.r1));
+      BI_PHASE(ph_tri_pair(a, gi, gj, down));
+      BI_PHASE(ph_loz_from_tri(a, gi, gj));
+      BI_PHASE(ph_from_loz(a, gi, gj, 14));
+    } else {
+      const int o = m.r1;
+      BI_PHASE(ph_rot_bf(a, gi, gj, BI_COIN, o, m.p1, m.p2));
+      BI_PHASE(ph_bf_same(a, gi, gj, o, m.p1, m.p2));
+      BI_PHASE(ph_bf_stage(a, gi, gj, o, m.p1, 1));
+      BI_PHASE(ph_bf_stage(a, gi, gj, o, m.p1, 2));
+      BI_PHASE(ph_bf_stage(a, gi, gj, o, m.p1, 3));
+      BI_PHASE(ph_loz_from_bf(a, gi, gj, o));
+      BI_PHASE(ph_from_loz(a, gi, gj, 1 | (14 & ~(2 << o))));
+      BI_PHASE(ph_tri_pair(a, gi, gj, 0));
+    }
+  }
+#undef BI_COIN
+#undef BI_PHASE
+#pragma unroll
+  for (int q = 0; q < kCells; q++) {
+    const int li = li0 + kRowStep * q, gi = r0 + li;
+    if (!col_in || gi < 0 || gi >= M || li < H || li >= H + TH || lj < H ||
+        lj >= H + TH)
+      continue;
+    const long k = (long)gi * M + gj;
+#pragma unroll
+    for (int pl = 0; pl < NPLANE; pl++) pout[pl * n + k] = a.get(pl, gi, gj);
+#pragma unroll
+    for (int w = 0; w < 4; w++) sout[w * n + k] = st[q][w];
+  }
 }
 
 // ------------------------------------------------------------------ host
-struct Move {
-  uint8_t type, r1, r2, p1, p2;
-};
+using Move = BiMove;
 
 // The reference's schedule (TriangleDimerTiler.cpp:272-298, 447-448): one
 // default_random_engine seeded with `seed`; per step rFlipType, r1 ~ U{0,2},
@@ -446,6 +629,41 @@ int bibone_run(rt_ctx* ctx, const Bi& B, const StreamTable& S, const std::vector
   return RT_OK;
 }
 
+// Sweep-block walk: windows of consecutive steps whose radii sum to at most
+// H (and at most kBiMaxK steps), one launch per window over all tiles,
+// reading one buffer set and writing the other.  `a` and `b` are the two
+// sets (eight planes then four state words per cell); returns the set that
+// holds the result.
+template <int LW, int H, int T>
+int bibone_sweep(rt_ctx* ctx, int M, int32_t* a, int32_t* b, BiMove* d_moves,
+                 const std::vector<Move>& sched, int32_t** result, long* launches_out) {
+  constexpr int TH = LW - 2 * H;
+  const long n = (long)M * M;
+  const int tiles = (M + TH - 1) / TH;
+  int32_t* cur = a;
+  int32_t* nxt = b;
+  long launches = 0;
+  size_t s0 = 0;
+  while (s0 < sched.size()) {
+    int K = 0, rad = 0;
+    while (s0 + K < sched.size() && K < kBiMaxK && rad + bi_radius(sched[s0 + K].type) <= H) {
+      rad += bi_radius(sched[s0 + K].type);
+      K++;
+    }
+    hipLaunchKernelGGL((k_bi_sweep<LW, H, T>), dim3((unsigned)(tiles * tiles)), dim3(T), 0,
+                       ctx->stream, cur, nxt, (const uint32_t*)(cur + NPLANE * n),
+                       (uint32_t*)(nxt + NPLANE * n), ctx->d_param, (long)ctx->nparam, M, tiles,
+                       d_moves + s0, K);
+    launches++;
+    std::swap(cur, nxt);
+    s0 += K;
+  }
+  HIPCHK_RET(ctx, hipGetLastError());
+  *result = cur;
+  *launches_out = launches;
+  return RT_OK;
+}
+
 }  // namespace
 }  // namespace rt
 
@@ -468,7 +686,11 @@ extern "C" int rt_bibone_walk(rt_ctx* ctx, int32_t* tiling, int N, long steps, l
   if (rc) return rc;
   HIPCHK_RET(ctx, hipSetDevice(ctx->device));
   const size_t plane_bytes = n * sizeof(int32_t);
-  rc = ctx->reserve(NPLANE * plane_bytes + 4 * n * sizeof(uint32_t) + 256);
+  // two buffer sets of eight planes + four TinyMT words per cell (the
+  // sweep blocks read one and write the other), then the move list
+  const size_t set_bytes = (NPLANE + 4) * plane_bytes;
+  const size_t move_bytes = ((size_t)steps * sizeof(Move) + 255) & ~(size_t)255;
+  rc = ctx->reserve(2 * set_bytes + move_bytes + 512);
   if (rc) return rc;
 
   // Split into the eight planes on the host (TriangleDimerTiler.cpp:122-148).
@@ -502,14 +724,24 @@ extern "C" int rt_bibone_walk(rt_ctx* ctx, int32_t* tiling, int N, long steps, l
   ctx->last = rt_timing{};
   HIPCHK_RET(ctx, hipMemcpyAsync(base, host.data(), NPLANE * plane_bytes, hipMemcpyHostToDevice,
                                  ctx->stream));
+  BiMove* d_moves = (BiMove*)((uint8_t*)ctx->arena + 2 * set_bytes);
+  if (steps > 0)
+    HIPCHK_RET(ctx, hipMemcpyAsync(d_moves, sched.data(), (size_t)steps * sizeof(Move),
+                                   hipMemcpyHostToDevice, ctx->stream));
   HIPCHK_RET(ctx, hipEventRecord(ctx->ev0, ctx->stream));
   hipLaunchKernelGGL(k_bi_init, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, S,
                      (int)n, (uint32_t)seed);
   long launches = 0;
-  rc = bibone_run(ctx, B, S, sched, &launches);
+  int32_t* result = base;
+  if (ctx->engine == RT_ENGINE_STEP)
+    rc = bibone_run(ctx, B, S, sched, &launches);
+  else
+    rc = bibone_sweep<kBiLW, kBiH, kBiThreads>(
+        ctx, M, base, (int32_t*)((uint8_t*)ctx->arena + set_bytes), d_moves, sched, &result,
+        &launches);
   if (rc) return rc;
   HIPCHK_RET(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-  HIPCHK_RET(ctx, hipMemcpyAsync(host.data(), base, 3 * plane_bytes, hipMemcpyDeviceToHost,
+  HIPCHK_RET(ctx, hipMemcpyAsync(host.data(), result, 3 * plane_bytes, hipMemcpyDeviceToHost,
                                  ctx->stream));
   HIPCHK_RET(ctx, hipStreamSynchronize(ctx->stream));
   float ms = 0;

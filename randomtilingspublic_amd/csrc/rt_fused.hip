@@ -931,21 +931,31 @@ int fused_launch(rt_ctx* ctx, const FusedJob& job, const FusedPlan& plan) {
   return RT_OK;
 }
 
-int fused_launch_windows(rt_ctx* ctx, int model, int nc, const ModelParams& mp,
+int fused_serve_shape(const rt_ctx* ctx, int model, int nc, long tiles64) {
+  if (model == M_RECTTRI) return nc == 2 ? SH_RT8x4 : SH_8x8;
+  // Few tiles: one 16-wave workgroup per CU on 64-row tiles; more tiles
+  // than CUs: the 48-row tiles at eight waves per SIMD (two workgroups per
+  // CU), which the batch tuner picks from about 16 hexagon-300 samples up.
+  return tiles64 <= ctx->num_cu ? SH_16x4 : SH_16x3_O8;
+}
+
+int fused_launch_windows(rt_ctx* ctx, int model, int nc, int shape, const ModelParams& mp,
                          const UnitDesc* d_units, const WorkItem* d_work, unsigned nwork,
                          const WindowRec* d_wins) {
   if (nwork == 0) return RT_OK;
-  // 64-row tiles (fused_tiles); 16 waves when there are no more tiles than CUs.
-  const int shape = model == M_RECTTRI ? (nc == 2 ? SH_RT8x4 : SH_8x8)
-                    : nwork <= (unsigned)ctx->num_cu ? SH_16x4 : SH_8x8;
-  pick(ctx, model, nc, shape)(ctx, nwork, d_units, d_work, shape_geo(shape).H, mp, d_wins);
+  LaunchFn fn = pick(ctx, model, nc, shape);
+  if (!fn) {
+    ctx->err = "no sweep-block kernel for this tile shape";
+    return RT_EINVAL;
+  }
+  fn(ctx, nwork, d_units, d_work, shape_geo(shape).H, mp, d_wins);
   HIPCHK_RET(ctx, hipGetLastError());
   ctx->last.launches += 1;
   return RT_OK;
 }
 
-void fused_tiles(int model, int nc, int rows, int cols, int* tiles_r, int* tiles_c) {
-  const Geo g = shape_geo(model == M_RECTTRI && nc == 2 ? SH_RT8x4 : SH_8x8);
+void fused_tiles(int shape, int rows, int cols, int* tiles_r, int* tiles_c) {
+  const Geo g = shape_geo(shape);
   UnitDesc u{};
   u.rows = rows;
   u.cols = cols;

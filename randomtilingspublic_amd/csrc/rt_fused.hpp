@@ -202,13 +202,16 @@ inline int shape_slot(int shape, bool split) {
 // Add the plan's tuning launches to per-shape kernel timing.
 void fused_add_trials(const FusedPlan& plan, rt_timing* acc);
 
-// One launch over `nwork` work items where each unit follows its own
-// window (wins[unit]); model / chains / params as in `job` (units unused).
-int fused_launch_windows(rt_ctx* ctx, int model, int nc, const ModelParams& mp,
+// Tile shape for launches over `tiles64` 64-row tiles of (model, nc)
+// units whose windows are cut by the caller (the serving loop, halo 6).
+int fused_serve_shape(const rt_ctx* ctx, int model, int nc, long tiles64);
+// One launch over `nwork` work items of tile shape `shape` where each unit
+// follows its own window (wins[unit]); model / chains / params as in a job.
+int fused_launch_windows(rt_ctx* ctx, int model, int nc, int shape, const ModelParams& mp,
                          const UnitDesc* d_units, const WorkItem* d_work, unsigned nwork,
                          const WindowRec* d_wins);
-// Tile counts of a unit under the job geometry of (model, nc).
-void fused_tiles(int model, int nc, int rows, int cols, int* tiles_r, int* tiles_c);
+// Tile counts of a unit under a tile shape's geometry.
+void fused_tiles(int shape, int rows, int cols, int* tiles_r, int* tiles_c);
 // Bring every colour plane of every unit up to date (end of a walk).
 int fused_sync(rt_ctx* ctx, const FusedJob& job);
 

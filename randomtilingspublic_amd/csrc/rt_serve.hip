@@ -125,6 +125,7 @@ struct Arena {
 };
 
 constexpr int kMaxSlots = 256;
+constexpr int SH_SERVE_COUNT = 1;  // tile shape whose tile count picks the launch shape (64 rows)
 constexpr int kOpsPerUnit = 64;  // = the fused kernel's window limit
 // Windows per tick: each tick launches kSub sweep kernels back to back from
 // one upload of window tables, so the host work per tick (tables, event,
@@ -182,6 +183,7 @@ struct Server {
   WorkItem* d_work[2] = {nullptr, nullptr};
   size_t work_cap[2] = {0, 0};
   unsigned nwork[2] = {0, 0};
+  int shape[2] = {0, 0};            // tile shape of each model's launches
   std::vector<Slot*> unit_slot[2];  // unit index -> slot, per model
   WindowRec* d_wins = nullptr;      // [buf][sub][model][kMaxSlots]
   uint8_t* d_ops = nullptr;         // [buf][sub][model][kMaxSlots][kOpsPerUnit]
@@ -320,6 +322,15 @@ static int upload_tables(Server* S) {
     std::vector<UnitDesc> units;
     std::vector<WorkItem> work;
     S->unit_slot[m].clear();
+    // tile shape from the live tile count (64-row tiles)
+    long t64 = 0;
+    for (Slot* s : S->active) {
+      if (s->req->model != model) continue;
+      int tr, tc;
+      fused_tiles(SH_SERVE_COUNT, s->req->N, (int)s->cols, &tr, &tc);
+      t64 += (long)tr * tc;
+    }
+    S->shape[m] = fused_serve_shape(S->ctx, model, 2, t64);
     for (Slot* s : S->active) {
       if (s->req->model != model) continue;
       UnitDesc d{};
@@ -341,7 +352,7 @@ static int upload_tables(Server* S) {
       const int u = (int)units.size();
       d.ops = S->d_ops;  // window s0 = ops_at(buffer, sub, model, unit)
       int tr, tc;
-      fused_tiles(model, 2, d.rows, d.cols, &tr, &tc);
+      fused_tiles(S->shape[m], d.rows, d.cols, &tr, &tc);
       d.tiles_c = tc;
       for (int t = 0; t < tr * tc; t++) work.push_back(WorkItem{u, t});
       units.push_back(d);
@@ -490,7 +501,8 @@ static int run_tick(Server* S) {
       if (S->unit_slot[m].empty()) continue;
       const int model = m == 0 ? M_DOMINO : M_LOZENGE;
       ModelParams mp = S->unit_slot[m][0]->req->mp;
-      int rc = fused_launch_windows(ctx, model, 2, mp, S->d_units[m], S->d_work[m], S->nwork[m],
+      int rc = fused_launch_windows(ctx, model, 2, S->shape[m], mp, S->d_units[m], S->d_work[m],
+                                    S->nwork[m],
                                     S->d_wins + wins_at(b, k, m));
       if (rc) return fail(S, ctx->err, rc);
     }

@@ -18,16 +18,24 @@ def _say(*a):
     print("[case]", *a, file=sys.stderr, flush=True)
 
 
+@pytest.mark.parametrize("engine", [rt.ENGINE_AUTO, rt.ENGINE_STEP])
 @pytest.mark.parametrize("n,m,steps,seed", [(6, 0, 0, 1), (6, 0, 1, 2), (6, 0, 64, 3),
                                             (9, 2, 500, 11), (13, 5, 1500, 4), (30, 0, 3000, 77),
-                                            (70, 9, 1200, 123)])
-def test_bibone_walk_matches_oracle(gpu_ctx, n, m, steps, seed):
+                                            (70, 9, 1200, 123), (150, 20, 700, 5)])
+def test_bibone_walk_matches_oracle(gpu_ctx, n, m, steps, seed, engine):
+    """Engine 0: sweep blocks (all phases of a few steps per launch on LDS
+    tiles; the cone of side 150 has 11 x 11 tiles); engine 1: one launch per
+    phase."""
     t = T.IceCreamCone(n, m)
     N = rt.side(t)
     P = rt.synthetic_params((N // 2) ** 2, seed=seed + 1)
     gpu_ctx.set_tinymt_params(P)
-    _say("bibone walk", n, m, steps, seed)
-    got = gpu_ctx.bibone_walk(t, steps, seed)
+    _say("bibone walk", n, m, steps, seed, "engine", engine)
+    try:
+        gpu_ctx.set_engine(engine)
+        got = gpu_ctx.bibone_walk(t, steps, seed)
+    finally:
+        gpu_ctx.set_engine(rt.ENGINE_AUTO)
     want = o.bibone_walk(t, steps, seed, P)
     assert np.array_equal(got, want)
     tm = gpu_ctx.last_timing()
