@@ -188,17 +188,20 @@ def batched_walks(rt, seed=4, nchains=256, steps=10 ** 4):
                          "note": "device time includes the final colour-plane sync"}}
 
 
-def weighted_and_bibone(rt, cftp_seeds=8, cftp_order=24, steps=10 ** 5):
+def weighted_and_bibone(rt, cftp_seeds=8, cftp_order=24, steps=10 ** 5, rounds512=7):
     """Weighted 2-periodic domino walk on the Aztec diamond of order 512
     (a = 1.0, b = 0.5, the reference's Metropolis rule of
     dominokerneltwoperiodic.cl, 10^5 steps from the minimal tiling), the
     bibone walk on the hexagon with sides 200, 201, 200 (10^5 steps), and
     the two-periodic heat-bath CFTP (dominokernelCFTPtwoperiodic.cl, same
-    weights) for master seeds 1..cftp_seeds, batched, on the Aztec diamond
-    of order `cftp_order`: that chain coalesces far more slowly than the
-    uniform one (order 24: 2^16 steps against 2^13; order 64: not within a
-    2^22-step look-back), so order 512 is out of reach of a CFTP run."""
-    D, B = rt.DominoTiler, rt.TriangleDimerTiler
+    weights) for master seeds 1..cftp_seeds, batched: to completion on the
+    Aztec diamond of order `cftp_order`, and on order 512 for `rounds512`
+    doubling rounds (look-back 2^10 .. 2^(9+rounds512)), reporting after
+    each round how many vertex states still differ between the coupled
+    chains.  That chain coalesces far more slowly than the uniform one
+    (order 24: 2^16 steps against 2^13; order 64: not within a 2^22-step
+    look-back), so order 512 does not coalesce within reach."""
+    D = rt.DominoTiler
     out = {}
     d = D.AztecDiamond(512)
     tmin = D.MinTiling(d)
@@ -232,23 +235,66 @@ def weighted_and_bibone(rt, cftp_seeds=8, cftp_order=24, steps=10 ** 5):
                             f"1..{cftp_seeds}, batched",
                 "wall_s": round(wall, 2), "samples_per_s": round(cftp_seeds / wall, 4),
                 "rounds": [s["rounds"] for s in stats]}
-        t = B.Hexagon(200, 200, 201)
-        Nb = rt.side(t)
-        ctx.set_tinymt_params(rt.synthetic_params((Nb // 2) ** 2, None))
-        ctx.bibone_walk(t, 200, 1)
-        w = time.time()
-        ctx.bibone_walk(t, steps, 1)
-        wall = time.time() - w
-        tm = ctx.last_timing()
-        out["bibone_walk"] = {
-            "workload": f"bibone hexagon sides 200, 201, 200 (tiling side {Nb}), {steps} steps",
-            "wall_s": round(wall, 3), "device_s": round(tm["device_ms"] / 1e3, 3),
-            "us_per_step": round(tm["device_ms"] * 1e3 / steps, 3),
-            "flip_attempts_per_s": round(tm["attempts"] / (tm["device_ms"] / 1e3), 1),
-            "launches": tm["launches"]}
+        if cftp_seeds and rounds512:
+            # tmin / d are the order-512 diamond of the walk above
+            tmax = D.MaxTiling(d)
+            masters = list(range(1, cftp_seeds + 1))
+            differ, dev_ms, att = [], 0.0, 0
+            for r in range(1, rounds512 + 1):
+                tops, bots, st = ctx.domino_cftp_batch_rounds(
+                    tmax, tmin, masters, rt.SCHED_DOUBLING, 1024, r, rt.DOMINO_HEATBATH2P,
+                    1.0, 0.5)
+                differ.append(int(np.count_nonzero(tops != bots)))
+                tm = ctx.last_timing()
+                dev_ms += tm["device_ms"]
+                att += tm["attempts"]
+            out["weighted_cftp_512"] = {
+                "workload": f"two-periodic heat-bath CFTP (a=1.0, b=0.5) of the Aztec diamond "
+                            f"of order 512 (N={N}), master seeds 1..{cftp_seeds}, batched, "
+                            f"doubling from 2^10, rounds 1..{rounds512} (each call reruns "
+                            "rounds 1..r)",
+                "differing_vertices_after_round": differ,
+                "vertices": cftp_seeds * N * N,
+                "coalesced": differ[-1] == 0,
+                "device_s": round(dev_ms / 1e3, 3),
+                "flip_attempts_per_s": round(att / (dev_ms / 1e3), 1)}
+        out["bibone_walk"] = bibone_walk(rt, ctx, steps)
     finally:
         ctx.close()
     return out
+
+
+def bibone_walk(rt, ctx, steps=10 ** 5):
+    """The bibone walk on the hexagon with sides 200, 201, 200 (tiling side
+    812, 406 x 406 plane cells), `steps` steps from the hexagon's tiling.
+    Algorithmic bytes per plane cell per launch of the sweep-block kernel:
+    eight int32 planes read and written (64 B), TinyMT state read and written
+    (32 B) and parameters read (12 B) = 108 B."""
+    B = rt.TriangleDimerTiler
+    t = B.Hexagon(200, 200, 201)
+    Nb = rt.side(t)
+    M = Nb // 2
+    ctx.set_tinymt_params(rt.synthetic_params(M * M, None))
+    ctx.bibone_walk(t, 200, 1)
+    w = time.time()
+    ctx.bibone_walk(t, steps, 1)
+    wall = time.time() - w
+    tm = ctx.last_timing()
+    return {"workload": f"bibone hexagon sides 200, 201, 200 (tiling side {Nb}), {steps} steps",
+            "wall_s": round(wall, 3), "device_s": round(tm["device_ms"] / 1e3, 3),
+            "us_per_step": round(tm["device_ms"] * 1e3 / steps, 3),
+            "flip_attempts_per_s": round(tm["attempts"] / (tm["device_ms"] / 1e3), 1),
+            "launches": tm["launches"], "tile_shape": tm["shape"], "sites_per_launch": M * M,
+            "avg_launch_us": round(tm["device_ms"] * 1e3 / tm["launches"], 2)}
+
+
+def bibone_section(rt, shape=0):
+    ctx = rt.Context(0)
+    try:
+        ctx.set_tile_shape(shape)
+        return bibone_walk(rt, ctx)
+    finally:
+        ctx.close()
 
 
 def served_stream(rt, seconds=40.0, seed=2024, dom=(64, 256), loz=(64, 160), t0=1024, burst_n=32,
@@ -435,6 +481,7 @@ PROFILE_WORKLOADS = {
                             "k_pair_lozenge", 56),
     "aztec3000_step": (["--profile-section", "sweep"], "k_domino_step<1>", 48),
     "batch256_walk": (["--profile-section", "walks"], "k_walk_o8<0>", 48),
+    "bibone_hex200": (["--profile-section", "bibone"], "k_bi_sweep<48, 8, 768>", 108),
 }
 PROFILE_TEXT = {
     "hex300_cftp_32seeds": "lozenge hexagon a=b=c=300 (N=609, 123 627 sites per plane), batched "
@@ -448,6 +495,10 @@ PROFILE_TEXT = {
     "batch256_walk": "256 heat-bath domino chains on uneven domains (seed 4), 10^4 steps, one "
                      "hipGraph, autotuned tile shape; 48 B per site per launch: TinyMT state "
                      "16 + 16 B, parameters 12 B, 2 colour planes read and written",
+    "bibone_hex200": "bibone walk on the hexagon with sides 200, 201, 200 (tiling side 812, "
+                     "406 x 406 plane cells), 10^5 steps, sweep blocks of 48 x 48 cells with "
+                     "halo 8; 108 B per plane cell per launch: eight int32 planes read and "
+                     "written, TinyMT state 16 + 16 B, parameters 12 B",
 }
 
 
@@ -517,6 +568,96 @@ def _copy_kib(root, counter, copies=5):
     return sum(vals) / len(vals)
 
 
+def _rocprof_run(name, outdir, tag, prof, bench_args, timeout, summarize_only):
+    """One rocprofv3 run of this script with `bench_args`; raw output in
+    outdir/<name>_<tag>.  Returns that folder, the program's last JSON line
+    and the command as the summary shows it."""
+    me = os.path.abspath(__file__)
+    d = os.path.join(os.path.abspath(outdir), f"{name}_{tag}")  # rocprofv3 runs in /tmp
+    cmd = ["rocprofv3", *prof, "--output-format", "csv", "-d", d, "-o", "run", "--",
+           sys.executable, me, *bench_args]
+    stdout_file = os.path.join(d, "stdout.txt")
+    if not summarize_only:
+        print(f"[profile] {name}: {' '.join(cmd)}", file=sys.stderr, flush=True)
+        r = subprocess.run(cmd, cwd="/tmp", env=dict(os.environ, TMPDIR="/tmp"),
+                           capture_output=True, text=True, timeout=timeout)
+        last = [l for l in r.stdout.splitlines() if l.startswith("{")]
+        if r.returncode != 0 and not last:
+            sys.stderr.write(r.stdout[-2000:] + r.stderr[-2000:])
+            raise RuntimeError(f"rocprofv3 {tag} failed ({r.returncode})")
+        os.makedirs(d, exist_ok=True)
+        with open(stdout_file, "w") as f:
+            f.write(r.stdout)
+    with open(stdout_file) as f:
+        last = [l for l in f.read().splitlines() if l.startswith("{")]
+    return d, json.loads(last[-1]) if last else None, " ".join(
+        ["rocprofv3", *prof, "--output-format csv", "-d", f"<dir>/{name}_{tag}", "-o run --",
+         "python3 bench.py", *bench_args])
+
+
+def _workload_launch(out):
+    """Sites per dispatch and the HIP-event average launch time that the
+    profiled workload reports."""
+    if "by_kernel" in out:
+        v = next(iter(out["by_kernel"].values()))
+        return v["sites_per_launch"], v["avg_launch_us"]
+    return out["sites_per_launch"], out.get("avg_launch_us")
+
+
+# Instruction and LDS counters of --rocprof-sq (eight SQ counters, one pass).
+SQ_COUNTERS = ["SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_INSTS_LDS", "SQ_LDS_BANK_CONFLICT",
+               "SQ_ACTIVE_INST_VALU", "SQ_ACTIVE_INST_LDS", "SQ_BUSY_CYCLES", "SQ_WAVE_CYCLES"]
+# VALU issue capacity: 256 CUs x 4 SIMDs at 2.4 GHz, a wave64 VALU
+# instruction holding a 16-lane SIMD for 4 cycles.
+SIMD_WAVE_INSTS_PER_US = 256 * 4 * 2400 / 4
+
+
+def rocprof_sq(name, outdir, summarize_only=False):
+    """Runs rocprofv3 on one profiling workload with the SQ instruction and
+    LDS counters (one pass) and a kernel trace (for the dispatch time), and
+    writes profiles/<kernel>_<workload>_sq.md: per dispatch of the
+    workload's kernel, wave-level VALU / SALU / LDS instruction counts, VALU
+    lane-instructions per site, the share of the chip's VALU issue cycles
+    the kernel uses and LDS bank conflicts."""
+    args, kernel, _ = PROFILE_WORKLOADS[name]
+    os.makedirs(outdir, exist_ok=True)
+    kt_dir, out, kt_cmd = _rocprof_run(name, outdir, "sqkt", ["--kernel-trace", "--stats"], args,
+                                       600, summarize_only)
+    sq_dir, _, sq_cmd = _rocprof_run(name, outdir, "sq", ["--pmc", *SQ_COUNTERS], args, 600,
+                                     summarize_only)
+    stats = _kernel_stats(kt_dir)
+    if kernel not in stats:
+        raise RuntimeError(f"{kernel} not in the kernel trace: {sorted(stats)}")
+    sites, ev_us = _workload_launch(out)
+    avg_us = stats[kernel]["avg_us"]
+    per = {c: _counter_per_dispatch(sq_dir, c).get(kernel, (0.0, 0)) for c in SQ_COUNTERS}
+    valu, nd = per["SQ_INSTS_VALU"]
+    lds = per["SQ_INSTS_LDS"][0]
+    lines = [f"# {kernel} -- {name}, instruction and LDS counters", "",
+             f"Workload: `python3 bench.py {' '.join(args)}`: {PROFILE_TEXT[name]}.", "",
+             f"Commands (made by `python3 bench.py --rocprof-sq {name} <dir>`):", "", "```",
+             "cd /tmp && export TMPDIR=/tmp", kt_cmd, sq_cmd, "```", "",
+             f"Per dispatch of `{kernel}` (means over {nd} dispatches; kernel trace of the same "
+             f"workload: {stats[kernel]['calls']} dispatches, average {avg_us:.2f} us"
+             + (f", HIP events {ev_us:.2f} us" if ev_us else "") + "):", "",
+             "| Counter | per dispatch |", "|---|---|"]
+    for c in SQ_COUNTERS:
+        lines.append(f"| {c} | {per[c][0] / 1e6:.3f} M |")
+    lines += ["",
+              f"* VALU lane-instructions per site per dispatch: {64 * valu / sites:.1f} "
+              f"({sites} sites per dispatch).",
+              f"* VALU issue: {100 * valu / (SIMD_WAVE_INSTS_PER_US * avg_us):.1f} % of the "
+              "chip's SIMD cycles over the dispatch (256 CUs x 4 SIMDs x 2.4 GHz, 4 cycles "
+              "per wave64 VALU instruction).",
+              f"* LDS instructions: {lds / 1e6:.3f} M per dispatch, "
+              f"{per['SQ_LDS_BANK_CONFLICT'][0] / 1e6:.3f} M bank-conflict cycles."]
+    md = f"{kernel.replace('<', '_').replace('>', '').replace(', ', '_')}_{name}_sq.md"
+    with open(os.path.join(ROOT, "profiles", md), "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print(json.dumps({"rocprof_sq": name, "kernel": kernel, "avg_us": avg_us,
+                      "valu_per_dispatch": valu, "summary": md}), flush=True)
+
+
 def rocprof_profile(name, outdir, summarize_only=False):
     """Runs rocprofv3 on one profiling workload of this benchmark (kernel
     trace; FETCH_SIZE and WRITE_SIZE, each in a run of its own; the same two
@@ -526,30 +667,10 @@ def rocprof_profile(name, outdir, summarize_only=False):
     which the headline line's roofline.traffic uses).  Raw rocprofv3 output
     stays in `outdir`."""
     args, kernel, alg = PROFILE_WORKLOADS[name]
-    me = os.path.abspath(__file__)
     os.makedirs(outdir, exist_ok=True)
 
     def run(tag, prof, bench_args, timeout):
-        d = os.path.join(outdir, f"{name}_{tag}")
-        cmd = ["rocprofv3", *prof, "--output-format", "csv", "-d", d, "-o", "run", "--",
-               sys.executable, me, *bench_args]
-        stdout_file = os.path.join(d, "stdout.txt")
-        if not summarize_only:
-            print(f"[profile] {name}: {' '.join(cmd)}", file=sys.stderr, flush=True)
-            r = subprocess.run(cmd, cwd="/tmp", env=dict(os.environ, TMPDIR="/tmp"),
-                               capture_output=True, text=True, timeout=timeout)
-            last = [l for l in r.stdout.splitlines() if l.startswith("{")]
-            if r.returncode != 0 and not last:
-                sys.stderr.write(r.stdout[-2000:] + r.stderr[-2000:])
-                raise RuntimeError(f"rocprofv3 {tag} failed ({r.returncode})")
-            os.makedirs(d, exist_ok=True)
-            with open(stdout_file, "w") as f:
-                f.write(r.stdout)
-        with open(stdout_file) as f:
-            last = [l for l in f.read().splitlines() if l.startswith("{")]
-        return d, json.loads(last[-1]) if last else None, " ".join(
-            ["rocprofv3", *prof, "--output-format csv", "-d", f"<dir>/{name}_{tag}", "-o run --",
-             "python3 bench.py", *bench_args])
+        return _rocprof_run(name, outdir, tag, prof, bench_args, timeout, summarize_only)
 
     kt_dir, out, kt_cmd = run("kt", ["--kernel-trace", "--stats"], args, 600)
     f_dir, _, f_cmd = run("fetch", ["--pmc", "FETCH_SIZE"], args, 600)
@@ -566,11 +687,7 @@ def rocprof_profile(name, outdir, summarize_only=False):
     if kernel not in stats:
         raise RuntimeError(f"{kernel} not in the kernel trace: {sorted(stats)}")
     # sites per dispatch of the kernel, from the workload's own report
-    if "by_kernel" in out:
-        sites = next(v["sites_per_launch"] for k, v in out["by_kernel"].items())
-        ev_us = next(v["avg_launch_us"] for k, v in out["by_kernel"].items())
-    else:
-        sites, ev_us = out["sites_per_launch"], out.get("avg_launch_us")
+    sites, ev_us = _workload_launch(out)
     st = stats[kernel]
     fk, fn = fetch[kernel]
     wk, wn = write[kernel]
@@ -743,7 +860,7 @@ def main():
     ap.add_argument("--profile-rounds", type=int, default=0,
                     help="profiling run: the same batch capped at R doubling rounds (no sample "
                          "needs to finish); prints the sweep kernel's timing only")
-    ap.add_argument("--profile-section", choices=["sweep", "walks", "c3", "served"],
+    ap.add_argument("--profile-section", choices=["sweep", "walks", "c3", "bibone", "served"],
                     default=None,
                     help="profiling run: only one section (plain Aztec sweep, 256-chain batch "
                          "walks, weighted walk / bibone, served stream), printed as one JSON "
@@ -766,8 +883,12 @@ def main():
                     help="run rocprofv3 (kernel trace, FETCH_SIZE, WRITE_SIZE, copy calibration) "
                          f"on one of {sorted(PROFILE_WORKLOADS)} and write its summary under "
                          "profiles/; raw output in OUTDIR")
+    ap.add_argument("--rocprof-sq", nargs=2, metavar=("WORKLOAD", "OUTDIR"), default=None,
+                    help="run rocprofv3 with instruction and LDS counters (SQ_*) on one "
+                         "profiling workload and write profiles/<kernel>_<workload>_sq.md")
     ap.add_argument("--summarize-only", action="store_true",
-                    help="with --rocprof-profile: only re-read the raw output in OUTDIR")
+                    help="with --rocprof-profile / --rocprof-sq: only re-read the raw output "
+                         "in OUTDIR")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="MiB: time a plain device copy of that size (HBM counter calibration)")
     args = ap.parse_args()
@@ -779,6 +900,9 @@ def main():
 
     if args.rocprof_profile:
         rocprof_profile(*args.rocprof_profile, summarize_only=args.summarize_only)
+        return
+    if args.rocprof_sq:
+        rocprof_sq(*args.rocprof_sq, summarize_only=args.summarize_only)
         return
     if args.copy_probe:
         copy_probe(args.copy_probe)
@@ -793,7 +917,8 @@ def main():
 
     if args.profile_section:
         fn = {"sweep": lambda: plain_sweep(rt), "walks": lambda: batched_walks(rt),
-              "c3": lambda: weighted_and_bibone(rt), "served": serve}[args.profile_section]
+              "c3": lambda: weighted_and_bibone(rt), "bibone": lambda: bibone_section(rt, args.tile_shape),
+              "served": serve}[args.profile_section]
         print(json.dumps({"profile_section": args.profile_section, **fn()}), flush=True)
         return
 

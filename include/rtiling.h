@@ -245,6 +245,16 @@ int rt_lozenge_cftp_batch_rounds(rt_ctx* ctx, const int32_t* tmax, const int32_t
                                  const uint32_t* master_seeds, int nsamples, int sched, long t0,
                                  int rounds, int32_t* top_out, int32_t* bot_out,
                                  rt_cftp_stats* stats);
+/* The domino counterpart: top_out / bot_out equal DominoTiler::Walk over
+ * the round's segment list (DominoTiler.cpp:125-231, seeds newest first)
+ * with the heat-bath rule `rule` (RT_DOMINO_HEATBATH or
+ * RT_DOMINO_HEATBATH2P, dominokernelCFTP*.cl).  Lets a caller run a bounded
+ * number of rounds of a CFTP whose coupling time is out of reach and watch
+ * the chains approach each other; no reference counterpart. */
+int rt_domino_cftp_batch_rounds(rt_ctx* ctx, const int32_t* tmax, const int32_t* tmin, int N,
+                                const uint32_t* master_seeds, int nsamples, int sched, long t0,
+                                int rounds, const rt_domino_rule* rule, int32_t* top_out,
+                                int32_t* bot_out, rt_cftp_stats* stats);
 
 /* Batched domino walks on domains of uneven size: chain k is
  * DominoTiler::Walk(tilings[k], steps, seeds[k]) with side Ns[k]; all
@@ -336,7 +346,14 @@ int rt_recttri_walk(rt_ctx* ctx, int32_t* tiling, int N, long steps,
  * states are rebuilt from it on entry, as the reference does.  Same
  * schedule (default_random_engine(seed): move type, r1, r2 and, for
  * butterfly steps, p1, p2) and TinyMT streams (id j + (N/2)*i, seeded
- * seed + id).  Needs >= (N/2)^2 parameter sets. */
+ * seed + id).  Needs >= (N/2)^2 parameter sets.  Engines: RT_ENGINE_AUTO
+ * and RT_ENGINE_FUSED_SWAR run starts that rt_bibone_edge_form accepts on
+ * one bit per lattice edge (each move a cycle flip, H/2 steps per launch;
+ * rt_set_tile_shape 1-5 picks its tile, 0 the default, reported in
+ * rt_timing.shape) and other starts on the eight planes in sweep blocks
+ * (rt_timing.shape 0); RT_ENGINE_FUSED forces the eight-plane sweep
+ * blocks, RT_ENGINE_STEP one launch per reference kernel.  Results never
+ * depend on the engine. */
 int rt_bibone_walk(rt_ctx* ctx, int32_t* tiling, int N, long steps, long seed);
 
 /* ---------------------------------------------- host-side domain helpers */
@@ -392,6 +409,14 @@ rt_vec* rt_bibone_butterfly_flips(const int32_t* t, size_t n);   /* 875-921 */
 rt_vec* rt_bibone_tiling_to_vertices(const int32_t* t, size_t n); /* 923-949 */
 rt_vec* rt_bibone_tiling_to_domain(const int32_t* t, size_t n);  /* 951-982 */
 rt_vec* rt_bibone_dimer_indicator(const int32_t* t, size_t n, int c); /* 35-95 */
+/* Edge form of a bibone tiling (N x N, n = N*N): out[(N/2)*(N/2)] gets one
+ * byte per plane cell (i, j), bit 0 = dimer on (i,j)-(i,j+1), bit 1 on
+ * (i,j)-(i+1,j), bit 2 on (i,j+1)-(i+1,j) (vertex coordinates).  Returns
+ * RT_EINVAL when rt_bibone_walk would not run this start on its edge form
+ * (inconsistent cells, not a matching, a covered vertex within two cells
+ * of the border); such starts run on the eight planes.  No reference
+ * counterpart: the reference keeps only the eight planes. */
+int rt_bibone_edge_form(const int32_t* t, size_t n, uint8_t* out);
 
 /* SVG writers (drop-in drawing helpers; files are the reference's byte for
  * byte).  Return RT_OK or RT_EIO.  A NULL domain in a *_dimer_svg call means

@@ -119,6 +119,9 @@ def lib():
                                                C.c_void_p]
     L.rt_domino_cftp_batch.argtypes = [vp, _I32P, _I32P, C.c_int, _U32P, C.c_int, C.c_int,
                                        C.c_long, C.c_int, C.POINTER(DominoRule), _I32P, C.c_void_p]
+    L.rt_domino_cftp_batch_rounds.argtypes = [vp, _I32P, _I32P, C.c_int, _U32P, C.c_int,
+                                              C.c_int, C.c_long, C.c_int, C.POINTER(DominoRule),
+                                              _I32P, _I32P, C.c_void_p]
     L.rt_domino_walk_batch.argtypes = [vp, C.c_void_p, C.c_void_p, C.c_int, C.c_long, _LONGP,
                                        C.POINTER(DominoRule), C.c_int]
     L.rt_domino_walk_batch_heights.argtypes = [vp, C.c_void_p, C.c_void_p, C.c_int, C.c_long,
@@ -172,6 +175,7 @@ def lib():
         f = getattr(L, name)
         f.argtypes = args
         f.restype = vp
+    L.rt_bibone_edge_form.argtypes = [_I32P, C.c_size_t, C.c_void_p]
     for name in ["rt_domino_domain_svg", "rt_domino_tiling_svg", "rt_domino_maya_svg",
                  "rt_lozenge_domain_svg", "rt_lozenge_tiling_svg", "rt_recttri_tiling_svg",
                  "rt_bibone_domain_svg", "rt_bibone_tiling_svg"]:
@@ -394,22 +398,33 @@ class Context:
         return self._cftp_batch(self._L.rt_lozenge_cftp_batch, tmax, tmin, masters, sched, t0,
                                 max_rounds)
 
-    def lozenge_cftp_batch_rounds(self, tmax, tmin, masters, sched=SCHED_DOUBLING, t0=1024,
-                                  rounds=1):
-        """Both chains of every sample after exactly `rounds` CFTP rounds:
-        (tops, bots, stats)."""
+    def _cftp_rounds(self, fn, tmax, tmin, masters, sched, t0, rounds, *rule):
         tmax, tmin = _i32(tmax), _i32(tmin)
         masters = np.ascontiguousarray(masters, dtype=np.uint32)
         n = len(masters)
         tops = np.zeros(n * len(tmax), dtype=np.int32)
         bots = np.zeros_like(tops)
         st = (CftpStats * n)()
-        self._chk(self._L.rt_lozenge_cftp_batch_rounds(self._h, tmax, tmin, side(tmax), masters,
-                                                       n, sched, t0, rounds, tops, bots,
-                                                       C.cast(st, C.c_void_p)))
+        self._chk(fn(self._h, tmax, tmin, side(tmax), masters, n, sched, t0, rounds, *rule, tops,
+                     bots, C.cast(st, C.c_void_p)))
         stats = [{"rounds": s.rounds, "total_steps": s.total_steps, "steps_run": s.steps_run,
                   "device_ms": s.device_ms} for s in st]
         return tops.reshape(n, len(tmax)), bots.reshape(n, len(tmax)), stats
+
+    def lozenge_cftp_batch_rounds(self, tmax, tmin, masters, sched=SCHED_DOUBLING, t0=1024,
+                                  rounds=1):
+        """Both chains of every sample after exactly `rounds` CFTP rounds:
+        (tops, bots, stats)."""
+        return self._cftp_rounds(self._L.rt_lozenge_cftp_batch_rounds, tmax, tmin, masters,
+                                 sched, t0, rounds)
+
+    def domino_cftp_batch_rounds(self, tmax, tmin, masters, sched=SCHED_DOUBLING, t0=128,
+                                 rounds=1, rule=DOMINO_HEATBATH, a=1.0, b=1.0):
+        """Domino counterpart of lozenge_cftp_batch_rounds (heat-bath rules
+        only): (tops, bots, stats)."""
+        r = DominoRule(rule, a, b)
+        return self._cftp_rounds(self._L.rt_domino_cftp_batch_rounds, tmax, tmin, masters,
+                                 sched, t0, rounds, C.byref(r))
 
     def domino_cftp_batch(self, tmax, tmin, masters, sched=SCHED_DOUBLING, t0=128,
                           max_rounds=40, rule=DOMINO_HEATBATH, a=1.0, b=1.0):
@@ -742,6 +757,17 @@ class TriangleDimerTiler:
     def DimerIndicator(t, c):
         t = _i32(t)
         return _take_vec(lib().rt_bibone_dimer_indicator(t, len(t), c))
+
+    @staticmethod
+    def EdgeForm(t):
+        """One byte per plane cell (bit 0 h, 1 l, 2 r dimer edges), as
+        rt_bibone_walk runs the start on its edge form; None when the start
+        runs on the eight planes instead (no reference counterpart)."""
+        t = _i32(t)
+        M = side(t) // 2
+        out = np.zeros(M * M, dtype=np.uint8)
+        rc = lib().rt_bibone_edge_form(t, len(t), out.ctypes.data_as(C.c_void_p))
+        return out.reshape(M, M) if rc == 0 else None
 
     @staticmethod
     def DomainToSVG(d, path):

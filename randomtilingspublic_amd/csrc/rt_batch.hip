@@ -404,6 +404,29 @@ extern "C" int rt_domino_cftp_batch(rt_ctx* ctx, const int32_t* tmax, const int3
                     stats);
 }
 
+extern "C" int rt_domino_cftp_batch_rounds(rt_ctx* ctx, const int32_t* tmax, const int32_t* tmin,
+                                           int N, const uint32_t* masters, int nsamples, int sched,
+                                           long t0, int rounds, const rt_domino_rule* rule,
+                                           int32_t* top_out, int32_t* bot_out,
+                                           rt_cftp_stats* stats) {
+  if (!ctx) return RT_EINVAL;
+  if (rounds <= 0) {
+    ctx->err = "rounds must be positive";
+    return RT_EINVAL;
+  }
+  if (!domino_cftp_rule(rule)) {
+    ctx->err = "CFTP needs a heat-bath rule (RT_DOMINO_HEATBATH or RT_DOMINO_HEATBATH2P)";
+    return RT_EINVAL;
+  }
+  ModelParams mp{};
+  mp.rule = rule ? rule->kind : RT_DOMINO_HEATBATH;
+  const double a = rule ? rule->a : 1.0, b = rule ? rule->b : 1.0;
+  mp.dw.wa2 = (float)(a * a);
+  mp.dw.wb2 = (float)(b * b);
+  return cftp_batch(ctx, M_DOMINO, tmax, tmin, N, masters, nsamples, sched, t0, rounds, mp,
+                    top_out, stats, rounds, bot_out);
+}
+
 // Observables of a domino tiling read straight off its two device colour
 // planes (N x N/2 bytes each, vertex states of DominoTiler.h:28-32): the
 // number of vertices with each edge bit set (1 up, 2 down, 4 right, 8 left)

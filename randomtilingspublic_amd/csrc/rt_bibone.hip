@@ -14,17 +14,21 @@
 //   butterfly rotate(b_o) -> four refresh stages of b_o -> e* -> {tU, other b} -> tD
 // Every kernel writes only cells it does not read, so one launch per phase
 // is exact; phases that only read the lozenge planes share a launch.
+// Three engines: one launch per phase (ENGINE_STEP), all phases of a few
+// steps per launch on LDS tiles of the eight planes (ENGINE_FUSED, and
+// starts the edge form does not take), and the edge form below, one bit per
+// lattice edge and one cycle flip per move (the default).
 #include <hip/hip_runtime.h>
 
 #include <random>
 #include <vector>
 
+#include "bibone_edges.hpp"
+#include "rt_internal.hpp"
 #include "rt_kernels.hpp"
 
 namespace rt {
 namespace {
-
-enum Plane { EH = 0, EL, ER, TU, TD, BH, BL, BR, NPLANE };
 
 struct Bi {
   int32_t* p[NPLANE];
@@ -409,7 +413,7 @@ BI_KERNEL(k_bi_bf_stage, ph_bf_stage, (Bi B, int o, int p1, int stage), o, p1, s
 BI_KERNEL(k_bi_loz_from_bf, ph_loz_from_bf, (Bi B, int o), o)
 #undef BI_KERNEL
 
-// ------------------------------------------ sweep blocks (engine 0, 2)
+// ---------------------------- eight-plane sweep blocks (engine 2, fallback)
 // One workgroup owns a tile of LW x LW plane cells (an interior of side
 // LW - 2H plus an H-cell halo), holds all eight planes of it in LDS as
 // 16-bit values (every plane value is below 512) and the TinyMT states of
@@ -423,6 +427,12 @@ BI_KERNEL(k_bi_loz_from_bf, ph_loz_from_bf, (Bi B, int o), o)
 // and states are double-buffered (read one set, write the other), as in
 // k_fused.
 constexpr int kBiMaxK = 16;  // steps per window
+// Edge-form tile geometry used when rt_set_tile_shape leaves it at 0: 64 x
+// 64 cells, halo 16 (8 steps per launch), 1024 threads.  Device time per
+// step on the bibone hexagon 200, 201, 200 (406 x 406 plane cells): 2.43 us
+// against 2.74 for 48 x 48 / halo 8 (shape 2), 3.22 for 64 x 64 / halo 8
+// (3), 3.31 for 40 x 40 / halo 8 (5), 4.39 for 32 x 32 / halo 4 (1).
+constexpr int kBiEdgeDefaultShape = 4;
 // Tile geometry of the walk: 48 x 48 loaded cells, halo 8 (32 x 32
 // interior), 768 threads (three cells each).  Measured on the bibone
 // hexagon 200, 201, 200 (406 x 406 plane cells), device time per step:
@@ -561,6 +571,234 @@ __global__ __launch_bounds__(T) void k_bi_sweep(
   }
 }
 
+// ------------------------------------------------ edge form (engine 0, 3)
+// The eight planes above are redundant: on every tiling the walk reaches
+// from a consistent start, each plane value is a function of the dimer
+// cover, i.e. of one occupation bit per edge of the triangular lattice
+// (bibone_edges.hpp), and each move of TriangleDimerTiler::Walk flips one
+// cycle of that cover: a lozenge rotation (5 <-> 10) a 4-cycle, a triangle
+// rotation (45 <-> 54) a 6-cycle, a butterfly rotation (85 <-> 170) an
+// 8-cycle.  The reference's refresh kernels only copy those edge changes
+// into the other planes.  The edge-form walk keeps the three edge families
+// as three LDS byte planes of a tile (one byte per edge, 0 or 1) and runs a
+// step as: one draw per cell of the array, the rotation test of the heads
+// cells in the step's class (are the edges of cycle state A, or of state
+// B, all set), a barrier, the flipped cycles' edges written with their new
+// values (cycles of one class share no edge, so plain byte stores do not
+// race), a barrier.  A cell reads and writes edges at most one cell away,
+// so a step makes two cells of the tile's edge stale and a window of H / 2
+// steps leaves the interior of an H-cell halo exact.  HBM keeps one byte
+// per plane cell (bit 0 h, 1 l, 2 r) and the TinyMT state.  The walk takes
+// this form when bibone_edge_form accepts the start (rt_bibone_host.cpp);
+// other starts run on the eight planes.
+// Per-cell lattice classes, computed once: bit 0 i & 1, bit 1 j & 1,
+// bit 2 (i + j) & 1, bits 3-4 (j - i) mod 3, bits 5-6 i mod 3, bits 7-8
+// j mod 3, bits 9-10 (i + j) mod 3.
+__device__ __forceinline__ uint32_t edge_classes(int i, int j) {
+  return (uint32_t)(i & 1) | (uint32_t)(j & 1) << 1 | (uint32_t)((i + j) & 1) << 2 |
+         (uint32_t)mod3(j - i) << 3 | (uint32_t)(i % 3) << 5 | (uint32_t)(j % 3) << 7 |
+         (uint32_t)((i + j) % 3) << 9;
+}
+
+// Code of a move: 0..2 lozenge of orientation r1, 3 / 4 triangle up / down,
+// 5..7 butterfly of orientation r1.
+__host__ __device__ inline int edge_code(const BiMove& m) {
+  return m.type == 0 ? m.r1 : (m.type == 1 ? 3 + (m.r2 == 1) : 5 + m.r1);
+}
+
+// On a matching every rotation test is "these edges all hold a dimer": a
+// lozenge is 5 when its bits 0 and 2 are set (bits 1 and 3 share their
+// vertices, so they are clear), a triangle cell 45 when its fields' slots
+// 1, 3, 2 are set, a butterfly 85 when every wing's slot 1 is set (the flag
+// edge shares a vertex with a wing).  EdgeCycle lists the move's cycle:
+// n / 2 edges of state A (5, 45, 85) then n / 2 of state B (10, 54, 170); a
+// flip sets the other state's edges and clears its own.
+struct EdgeCycle {
+  int n;
+  ESlot s[8];
+};
+__host__ __device__ constexpr EdgeCycle edge_cycle(int code) {
+  if (code < 3) {
+    const ESlot* l = kELoz[code];
+    return {4, {l[0], l[2], l[1], l[3]}};
+  }
+  if (code < 5) {
+    const auto& f = kETri[code - 3];
+    return {6, {f[0][0], f[1][2], f[2][1], f[0][1], f[1][0], f[2][2]}};
+  }
+  const auto& f = kEBf[code - 5];
+  return {8, {f[0][0], f[1][0], f[2][0], f[3][0], f[0][1], f[1][1], f[2][1], f[3][1]}};
+}
+
+// Whether a cell with classes `cl` is in the class of move m.
+template <int CODE>
+__device__ __forceinline__ bool edge_in_class(uint32_t cl, const BiMove& m) {
+  if constexpr (CODE < 3) {
+    return (int)((cl >> (CODE == 0 ? 0 : (CODE == 1 ? 1 : 2))) & 1) == m.r2;
+  } else if constexpr (CODE < 5) {
+    return (int)((cl >> 3) & 3) == m.r1;
+  } else {
+    constexpr int o = CODE - 5;
+    const int slice = (cl >> (o == 0 ? 5 : (o == 1 ? 7 : 9))) & 3;
+    const int q = (cl >> (o == 1 ? 0 : 1)) & 1;
+    return slice == m.p1 && q == m.p2;
+  }
+}
+
+template <int LW, int H, int T>
+struct EdgeTile {
+  static constexpr int RS = LW + 2;                 // LDS row: one margin cell each side
+  static constexpr int PL = RS * RS;                // one edge family (h, l, r) per plane
+  static constexpr int kBytes = (3 * PL + 3) & ~3;
+  static constexpr int kCells = LW * LW / T, kRowStep = T / LW;
+  static constexpr int TH = LW - 2 * H;             // interior side
+  static constexpr int K = H / 2;                   // steps per window
+  static_assert(T % LW == 0 && (LW * LW) % T == 0, "tile rows split evenly over the threads");
+  static_assert(H % 2 == 0 && K <= kBiMaxK, "halo covers whole steps");
+  // LDS offset of slot s from the byte of cell (li - 1, lj - 1), h plane
+  static constexpr int off(const ESlot& s) { return s.bit * PL + (s.di + 1) * RS + (s.dj + 1); }
+};
+
+// Rotation tests of move class CODE on the thread's cells: bit q of the
+// result is set when cell q flips (heads, in the class, cycle in state A
+// or B); bit 16 + q says it was in state A.  `base` is the LDS byte of
+// cell 0's up-left neighbour in the h plane.
+template <int CODE, int LW, int H, int T>
+__device__ __forceinline__ uint32_t edge_decide(const uint8_t* base, uint32_t heads,
+                                                const uint32_t (&cls)[EdgeTile<LW, H, T>::kCells],
+                                                const BiMove& m) {
+  using G = EdgeTile<LW, H, T>;
+  constexpr EdgeCycle c = edge_cycle(CODE);
+  uint32_t flip = 0;
+#pragma unroll
+  for (int q = 0; q < G::kCells; q++) {
+    const uint8_t* b = base + q * G::kRowStep * G::RS;
+    uint32_t a = 1, bb = 1;
+#pragma unroll
+    for (int k = 0; k < c.n / 2; k++) {
+      a &= b[G::off(c.s[k])];
+      bb &= b[G::off(c.s[c.n / 2 + k])];
+    }
+    if (((heads >> q) & 1) && edge_in_class<CODE>(cls[q], m) && (a | bb))
+      flip |= 1u << q | a << (16 + q);
+  }
+  return flip;
+}
+
+template <int CODE, int LW, int H, int T>
+__device__ __forceinline__ void edge_apply(uint8_t* base, uint32_t flip) {
+  using G = EdgeTile<LW, H, T>;
+  constexpr EdgeCycle c = edge_cycle(CODE);
+#pragma unroll
+  for (int q = 0; q < G::kCells; q++) {
+    if (!((flip >> q) & 1)) continue;
+    uint8_t* b = base + q * G::kRowStep * G::RS;
+    const uint8_t was_a = (uint8_t)((flip >> (16 + q)) & 1);
+#pragma unroll
+    for (int k = 0; k < c.n / 2; k++) {
+      b[G::off(c.s[k])] = was_a ^ 1;
+      b[G::off(c.s[c.n / 2 + k])] = was_a;
+    }
+  }
+}
+
+template <int LW, int H, int T>
+__global__ __launch_bounds__(T) void k_bi_edge(const uint8_t* __restrict__ ein,
+                                               uint8_t* __restrict__ eout,
+                                               const uint32_t* __restrict__ sin,
+                                               uint32_t* __restrict__ sout,
+                                               const uint32_t* __restrict__ param, long pstride,
+                                               long sstride, int M, int tiles_c,
+                                               const BiMove* __restrict__ moves, int K) {
+  using G = EdgeTile<LW, H, T>;
+  __shared__ uint32_t ew[G::kBytes / 4];
+  __shared__ BiMove smv[kBiMaxK];
+  uint8_t* e = (uint8_t*)ew;
+  const int tr = blockIdx.x / tiles_c, tc = blockIdx.x % tiles_c;
+  const int r0 = tr * G::TH - H, c0 = tc * G::TH - H;
+  for (int k = threadIdx.x; k < G::kBytes / 4; k += T) ew[k] = 0;
+  if (threadIdx.x < K) smv[threadIdx.x] = moves[threadIdx.x];
+  __syncthreads();
+  const int lj = threadIdx.x % LW, li0 = threadIdx.x / LW;
+  const int gj = c0 + lj;
+  // byte of cell (li0, lj) in the h plane, and of its up-left neighbour
+  const int p0 = (li0 + 1) * G::RS + lj + 1;
+  uint8_t* base = e + p0 - G::RS - 1;
+  uint32_t st[G::kCells][4], pm[G::kCells][3], cls[G::kCells];
+  uint32_t valid = 0;
+#pragma unroll
+  for (int q = 0; q < G::kCells; q++) {
+    const int gi = r0 + li0 + G::kRowStep * q;
+    st[q][0] = st[q][1] = st[q][2] = st[q][3] = 0;
+    pm[q][0] = pm[q][1] = pm[q][2] = 0;
+    cls[q] = 0;
+    if (gj >= 0 && gj < M && gi >= 0 && gi < M) {
+      valid |= 1u << q;
+      const long k = (long)gi * M + gj;
+      const uint32_t v = ein[k];
+      const int p = p0 + q * G::kRowStep * G::RS;
+      e[p] = v & 1;
+      e[G::PL + p] = (v >> 1) & 1;
+      e[2 * G::PL + p] = (v >> 2) & 1;
+#pragma unroll
+      for (int w = 0; w < 4; w++) st[q][w] = sin[w * sstride + k];
+      pm[q][0] = param[k];
+      pm[q][1] = param[pstride + k];
+      pm[q][2] = param[2 * pstride + k];
+      cls[q] = edge_classes(gi, gj);
+    }
+  }
+  __syncthreads();
+  for (int s = 0; s < K; s++) {
+    const BiMove m = smv[s];
+    // one draw per cell of the array, whatever the move
+    uint32_t heads = 0;
+#pragma unroll
+    for (int q = 0; q < G::kCells; q++) {
+      Tiny t{st[q][0], st[q][1], st[q][2], st[q][3], pm[q][0], pm[q][1], pm[q][2]};
+      if (coin(t) && ((valid >> q) & 1)) heads |= 1u << q;
+      st[q][0] = t.s0;
+      st[q][1] = t.s1;
+      st[q][2] = t.s2;
+      st[q][3] = t.s3;
+    }
+    const int code = edge_code(m);
+    uint32_t flip;
+    switch (code) {
+      case 0: flip = edge_decide<0, LW, H, T>(base, heads, cls, m); break;
+      case 1: flip = edge_decide<1, LW, H, T>(base, heads, cls, m); break;
+      case 2: flip = edge_decide<2, LW, H, T>(base, heads, cls, m); break;
+      case 3: flip = edge_decide<3, LW, H, T>(base, heads, cls, m); break;
+      case 4: flip = edge_decide<4, LW, H, T>(base, heads, cls, m); break;
+      case 5: flip = edge_decide<5, LW, H, T>(base, heads, cls, m); break;
+      case 6: flip = edge_decide<6, LW, H, T>(base, heads, cls, m); break;
+      default: flip = edge_decide<7, LW, H, T>(base, heads, cls, m); break;
+    }
+    __syncthreads();
+    switch (code) {
+      case 0: edge_apply<0, LW, H, T>(base, flip); break;
+      case 1: edge_apply<1, LW, H, T>(base, flip); break;
+      case 2: edge_apply<2, LW, H, T>(base, flip); break;
+      case 3: edge_apply<3, LW, H, T>(base, flip); break;
+      case 4: edge_apply<4, LW, H, T>(base, flip); break;
+      case 5: edge_apply<5, LW, H, T>(base, flip); break;
+      case 6: edge_apply<6, LW, H, T>(base, flip); break;
+      default: edge_apply<7, LW, H, T>(base, flip); break;
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < G::kCells; q++) {
+    const int li = li0 + G::kRowStep * q, gi = r0 + li;
+    if (!((valid >> q) & 1) || li < H || li >= H + G::TH || lj < H || lj >= H + G::TH) continue;
+    const long k = (long)gi * M + gj;
+    const int p = p0 + q * G::kRowStep * G::RS;
+    eout[k] = (uint8_t)(e[p] | e[G::PL + p] << 1 | e[2 * G::PL + p] << 2);
+#pragma unroll
+    for (int w = 0; w < 4; w++) sout[w * sstride + k] = st[q][w];
+  }
+}
+
 // ------------------------------------------------------------------ host
 using Move = BiMove;
 
@@ -664,6 +902,46 @@ int bibone_sweep(rt_ctx* ctx, int M, int32_t* a, int32_t* b, BiMove* d_moves,
   return RT_OK;
 }
 
+// Edge-form walk: windows of G::K steps, one launch per window over all
+// tiles; `a` / `b` are the two buffer sets (edge bytes, then four TinyMT
+// words per cell at offset `soff`).  Returns the set holding the result.
+template <int LW, int H, int T>
+int bibone_edge_sweep(rt_ctx* ctx, int M, uint8_t* a, uint8_t* b, size_t soff, BiMove* d_moves,
+                      long steps, uint8_t** result, long* launches_out) {
+  using G = EdgeTile<LW, H, T>;
+  const long n = (long)M * M;
+  const int tiles = (M + G::TH - 1) / G::TH;
+  uint8_t* cur = a;
+  uint8_t* nxt = b;
+  long launches = 0;
+  for (long s0 = 0; s0 < steps; s0 += G::K) {
+    const int K = (int)std::min<long>(G::K, steps - s0);
+    hipLaunchKernelGGL((k_bi_edge<LW, H, T>), dim3((unsigned)(tiles * tiles)), dim3(T), 0,
+                       ctx->stream, cur, nxt, (const uint32_t*)(cur + soff),
+                       (uint32_t*)(nxt + soff), ctx->d_param, (long)ctx->nparam, n, M, tiles,
+                       d_moves + s0, K);
+    launches++;
+    std::swap(cur, nxt);
+  }
+  HIPCHK_RET(ctx, hipGetLastError());
+  *result = cur;
+  *launches_out = launches;
+  return RT_OK;
+}
+
+// Tile geometries of the edge form, by rt_set_tile_shape value (0: the
+// default).
+int bibone_edge_run(rt_ctx* ctx, int shape, int M, uint8_t* a, uint8_t* b, size_t soff,
+                    BiMove* d_moves, long steps, uint8_t** result, long* launches) {
+  switch (shape) {
+    case 1: return bibone_edge_sweep<32, 4, 256>(ctx, M, a, b, soff, d_moves, steps, result, launches);
+    case 2: return bibone_edge_sweep<48, 8, 768>(ctx, M, a, b, soff, d_moves, steps, result, launches);
+    case 3: return bibone_edge_sweep<64, 8, 1024>(ctx, M, a, b, soff, d_moves, steps, result, launches);
+    case 5: return bibone_edge_sweep<40, 8, 320>(ctx, M, a, b, soff, d_moves, steps, result, launches);
+    default: return bibone_edge_sweep<64, 16, 1024>(ctx, M, a, b, soff, d_moves, steps, result, launches);
+  }
+}
+
 }  // namespace
 }  // namespace rt
 
@@ -686,45 +964,39 @@ extern "C" int rt_bibone_walk(rt_ctx* ctx, int32_t* tiling, int N, long steps, l
   if (rc) return rc;
   HIPCHK_RET(ctx, hipSetDevice(ctx->device));
   const size_t plane_bytes = n * sizeof(int32_t);
-  // two buffer sets of eight planes + four TinyMT words per cell (the
-  // sweep blocks read one and write the other), then the move list
+  // two buffer sets (the sweep blocks read one and write the other) of
+  // eight planes + four TinyMT words per cell, then the move list; the
+  // edge form uses the front of each set (edge bytes, then the state)
   const size_t set_bytes = (NPLANE + 4) * plane_bytes;
   const size_t move_bytes = ((size_t)steps * sizeof(Move) + 255) & ~(size_t)255;
   rc = ctx->reserve(2 * set_bytes + move_bytes + 512);
   if (rc) return rc;
 
-  // Split into the eight planes on the host (TriangleDimerTiler.cpp:122-148).
-  std::vector<int> t(tiling, tiling + (size_t)N * N);
-  const std::vector<int> tri = bibone_triangle_flips(t, N);
-  const std::vector<int> bf = bibone_butterfly_flips(t, N);
-  std::vector<int32_t> host(NPLANE * n, 0);
-  auto P = [&](int pl, int i, int j) -> int32_t& { return host[pl * n + (size_t)i * M + j]; };
-  for (int i = 0; i < N; i++)
-    for (int j = 0; j < N; j++) {
-      const int pl = (i % 2 == 0) ? (j % 2 ? 0 : -1) : (j % 2 ? 2 : 1);
-      if (pl < 0) continue;
-      P(EH + pl, i / 2, j / 2) = t[(size_t)i * N + j];
-      P(BH + pl, i / 2, j / 2) = bf[(size_t)i * N + j];
-    }
-  for (int i = 0; i < M; i++)
-    for (int j = 0; j < N; j++) P(j % 2 ? TU : TD, i, j / 2) = tri[(size_t)i * N + j];
+  // The eight planes of the start (TriangleDimerTiler.cpp:122-148) and,
+  // where it applies, its edge form.
+  const std::vector<int> t(tiling, tiling + (size_t)N * N);
+  std::vector<int32_t> host = bibone_planes(t, N);
+  const bool edge_engine = ctx->engine == RT_ENGINE_AUTO || ctx->engine == RT_ENGINE_FUSED_SWAR;
+  std::vector<uint8_t> edges;
+  if (edge_engine) edges = bibone_edge_form(host, M);
 
-  Bi B;
-  B.M = M;
-  int32_t* base = (int32_t*)ctx->arena;
-  for (int pl = 0; pl < NPLANE; pl++) B.p[pl] = base + pl * n;
-  uint32_t* state = (uint32_t*)(base + NPLANE * n);
+  uint8_t* set_a = (uint8_t*)ctx->arena;
+  uint8_t* set_b = set_a + set_bytes;
+  const size_t soff = (n + 255) & ~(size_t)255;  // edge form: state after the edge bytes
   StreamTable S;
-  S.state = state;
+  S.state = (uint32_t*)(edges.empty() ? set_a + NPLANE * plane_bytes : set_a + soff);
   S.param = ctx->d_param;
   S.stride = (long)n;
   S.pstride = (long)ctx->nparam;
 
   const std::vector<Move> sched = bibone_schedule(seed, steps);
   ctx->last = rt_timing{};
-  HIPCHK_RET(ctx, hipMemcpyAsync(base, host.data(), NPLANE * plane_bytes, hipMemcpyHostToDevice,
-                                 ctx->stream));
-  BiMove* d_moves = (BiMove*)((uint8_t*)ctx->arena + 2 * set_bytes);
+  if (edges.empty())
+    HIPCHK_RET(ctx, hipMemcpyAsync(set_a, host.data(), NPLANE * plane_bytes,
+                                   hipMemcpyHostToDevice, ctx->stream));
+  else
+    HIPCHK_RET(ctx, hipMemcpyAsync(set_a, edges.data(), n, hipMemcpyHostToDevice, ctx->stream));
+  BiMove* d_moves = (BiMove*)(set_a + 2 * set_bytes);
   if (steps > 0)
     HIPCHK_RET(ctx, hipMemcpyAsync(d_moves, sched.data(), (size_t)steps * sizeof(Move),
                                    hipMemcpyHostToDevice, ctx->stream));
@@ -732,17 +1004,29 @@ extern "C" int rt_bibone_walk(rt_ctx* ctx, int32_t* tiling, int N, long steps, l
   hipLaunchKernelGGL(k_bi_init, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, S,
                      (int)n, (uint32_t)seed);
   long launches = 0;
-  int32_t* result = base;
-  if (ctx->engine == RT_ENGINE_STEP)
-    rc = bibone_run(ctx, B, S, sched, &launches);
-  else
-    rc = bibone_sweep<kBiLW, kBiH, kBiThreads>(
-        ctx, M, base, (int32_t*)((uint8_t*)ctx->arena + set_bytes), d_moves, sched, &result,
-        &launches);
-  if (rc) return rc;
-  HIPCHK_RET(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-  HIPCHK_RET(ctx, hipMemcpyAsync(host.data(), result, 3 * plane_bytes, hipMemcpyDeviceToHost,
-                                 ctx->stream));
+  int shape = 0;
+  if (!edges.empty()) {
+    shape = ctx->tile_shape ? ctx->tile_shape : kBiEdgeDefaultShape;
+    uint8_t* result = set_a;
+    rc = bibone_edge_run(ctx, shape, M, set_a, set_b, soff, d_moves, steps, &result, &launches);
+    if (rc) return rc;
+    HIPCHK_RET(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    HIPCHK_RET(ctx, hipMemcpyAsync(edges.data(), result, n, hipMemcpyDeviceToHost, ctx->stream));
+  } else {
+    Bi B;
+    B.M = M;
+    for (int pl = 0; pl < NPLANE; pl++) B.p[pl] = (int32_t*)set_a + pl * n;
+    int32_t* result = (int32_t*)set_a;
+    if (ctx->engine == RT_ENGINE_STEP)
+      rc = bibone_run(ctx, B, S, sched, &launches);
+    else
+      rc = bibone_sweep<kBiLW, kBiH, kBiThreads>(ctx, M, (int32_t*)set_a, (int32_t*)set_b,
+                                                 d_moves, sched, &result, &launches);
+    if (rc) return rc;
+    HIPCHK_RET(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    HIPCHK_RET(ctx, hipMemcpyAsync(host.data(), result, 3 * plane_bytes, hipMemcpyDeviceToHost,
+                                   ctx->stream));
+  }
   HIPCHK_RET(ctx, hipStreamSynchronize(ctx->stream));
   float ms = 0;
   HIPCHK_RET(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
@@ -750,12 +1034,17 @@ extern "C" int rt_bibone_walk(rt_ctx* ctx, int32_t* tiling, int N, long steps, l
   ctx->last.launches = launches + 1;
   ctx->last.steps = steps;
   ctx->last.attempts = steps * (long)n;
+  ctx->last.shape = shape;
 
-  // Merge the lozenge-type planes back (TriangleDimerTiler.cpp:563-577).
+  // Merge the lozenge-type cells back (TriangleDimerTiler.cpp:563-577).
+  if (!edges.empty()) {
+    bibone_edge_merge(edges, M, tiling);
+    return RT_OK;
+  }
   for (int i = 0; i < N; i++)
     for (int j = 0; j < N; j++) {
-      const int pl = (i % 2 == 0) ? (j % 2 ? 0 : -1) : (j % 2 ? 2 : 1);
-      if (pl >= 0) tiling[(size_t)i * N + j] = P(EH + pl, i / 2, j / 2);
+      const int o = (i % 2 == 0) ? (j % 2 ? 0 : -1) : (j % 2 ? 2 : 1);
+      if (o >= 0) tiling[(size_t)i * N + j] = host[o * n + (size_t)(i / 2) * M + j / 2];
     }
   return RT_OK;
 }

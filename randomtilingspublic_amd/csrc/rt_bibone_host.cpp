@@ -12,6 +12,7 @@
 #include <cmath>
 #include <vector>
 
+#include "bibone_edges.hpp"
 #include "rt_internal.hpp"
 
 namespace rt {
@@ -274,6 +275,140 @@ std::vector<int> bibone_dimer_indicator(const tiling& t, int N, int c) {
   for (int i = row_par; i < N; i += 2)
     for (int j = 1; j < N; j += 2) out[(size_t)(i / 2) * M + j / 2] = bit(t[(size_t)i * N + j], b);
   return out;
+}
+
+// The eight planes the walk runs on (TriangleDimerTiler.cpp:122-148):
+// eH, eL, eR from the tiling's (even, odd), (odd, even) and (odd, odd)
+// cells, tU / tD from the triangle flips (odd / even columns), bH, bL, bR
+// from the butterfly flips at the lozenge cells.  8 x M x M, M = N / 2.
+std::vector<int32_t> bibone_planes(const tiling& t, int N) {
+  const int M = N / 2;
+  const size_t n = (size_t)M * M;
+  const std::vector<int> tri = bibone_triangle_flips(t, N);
+  const std::vector<int> bf = bibone_butterfly_flips(t, N);
+  std::vector<int32_t> pl(NPLANE * n, 0);
+  auto P = [&](int p, int i, int j) -> int32_t& { return pl[p * n + (size_t)i * M + j]; };
+  for (int i = 0; i < N; i++)
+    for (int j = 0; j < N; j++) {
+      const int o = (i % 2 == 0) ? (j % 2 ? 0 : -1) : (j % 2 ? 2 : 1);
+      if (o < 0) continue;
+      P(EH + o, i / 2, j / 2) = t[(size_t)i * N + j];
+      P(BH + o, i / 2, j / 2) = bf[(size_t)i * N + j];
+    }
+  for (int i = 0; i < M; i++)
+    for (int j = 0; j < N; j++) P(j % 2 ? TU : TD, i, j / 2) = tri[(size_t)i * N + j];
+  return pl;
+}
+
+namespace {
+
+// Edge bit b of cell (i, j), zero outside the M x M grid.
+struct EdgeBits {
+  const std::vector<uint8_t>& e;
+  int M;
+  int operator()(int i, int j, int b) const {
+    return (i >= 0 && j >= 0 && i < M && j < M) ? (e[(size_t)i * M + j] >> b) & 1 : 0;
+  }
+  int slot(int i, int j, const ESlot& s) const { return (*this)(i + s.di, j + s.dj, s.bit); }
+  int field(int i, int j, const ESlot* f) const {
+    return slot(i, j, f[0]) + 2 * slot(i, j, f[1]) + 3 * slot(i, j, f[2]);
+  }
+  int lozenge(int o, int i, int j) const {
+    const ESlot* s = kELoz[o];
+    return slot(i, j, s[0]) | slot(i, j, s[1]) << 1 | slot(i, j, s[2]) << 2 |
+           slot(i, j, s[3]) << 3;
+  }
+};
+
+}  // namespace
+
+// Edge form of a start given as its eight planes (bibone_planes): one byte
+// per plane cell, or an empty vector when the edge-form walk would not
+// reproduce the eight-plane walk from it.  It does when
+//  * the lozenge cells agree on every shared edge (they are the edges' own),
+//  * the edges form a matching inside the M x M vertex grid,
+//  * the triangle and butterfly start states are the edges' (zero on the
+//    outer ring of cells), and
+//  * every edge that a cell state of the outer ring reads touches an
+//    uncovered vertex.
+// The reference never refreshes the ring (its update kernels skip it), so
+// its ring states keep their start values; the last condition makes the
+// edges they stand for empty for good (a flip moves dimers only among
+// covered vertices), so those values stay what the edges say.
+std::vector<uint8_t> bibone_edge_form(const std::vector<int32_t>& pl, int M) {
+  const size_t n = (size_t)M * M;
+  auto P = [&](int p, int i, int j) { return pl[p * n + (size_t)i * M + j]; };
+  std::vector<uint8_t> e(n);
+  for (int i = 0; i < M; i++)
+    for (int j = 0; j < M; j++) {
+      const int h = P(EH, i, j);
+      e[(size_t)i * M + j] = (uint8_t)((P(ER, i, j) & 1) << kEdgeH | ((h >> 3) & 1) << kEdgeL |
+                                       ((h >> 2) & 1) << kEdgeR);
+    }
+  const EdgeBits E{e, M};
+  std::vector<int> deg(n, 0);
+  for (int i = 0; i < M; i++)
+    for (int j = 0; j < M; j++) {
+      // endpoints of h(i,j), l(i,j), r(i,j) must lie in the grid
+      if ((E(i, j, kEdgeH) && j + 1 >= M) || (E(i, j, kEdgeL) && i + 1 >= M) ||
+          (E(i, j, kEdgeR) && (i + 1 >= M || j + 1 >= M)))
+        return {};
+      deg[(size_t)i * M + j] = E(i, j, kEdgeH) + E(i, j - 1, kEdgeH) + E(i, j, kEdgeL) +
+                               E(i - 1, j, kEdgeL) + E(i, j - 1, kEdgeR) + E(i - 1, j, kEdgeR);
+      if (deg[(size_t)i * M + j] > 1) return {};
+    }
+  auto covered = [&](int a, int b) {
+    return a >= 0 && b >= 0 && a < M && b < M && deg[(size_t)a * M + b] > 0;
+  };
+  // edge (a, b, bit) stays empty: one endpoint uncovered (or off the grid)
+  auto empty_for_good = [&](int a, int b, int bit) {
+    if (bit == kEdgeH) return !covered(a, b) || !covered(a, b + 1);
+    if (bit == kEdgeL) return !covered(a, b) || !covered(a + 1, b);
+    return !covered(a, b + 1) || !covered(a + 1, b);
+  };
+  auto ring_ok = [&](int i, int j) {
+    for (int o = 0; o < 3; o++)
+      for (const ESlot& s : kELoz[o])
+        if (!empty_for_good(i + s.di, j + s.dj, s.bit)) return false;
+    for (const auto& f : kETri)
+      for (const auto& row : f)
+        for (const ESlot& s : row)
+          if (!empty_for_good(i + s.di, j + s.dj, s.bit)) return false;
+    for (const auto& f : kEBf)
+      for (const auto& row : f)
+        for (const ESlot& s : row)
+          if (!empty_for_good(i + s.di, j + s.dj, s.bit)) return false;
+    return true;
+  };
+  for (int i = 0; i < M; i++)
+    for (int j = 0; j < M; j++) {
+      for (int o = 0; o < 3; o++)
+        if (E.lozenge(o, i, j) != P(EH + o, i, j)) return {};
+      const bool ring = i == 0 || j == 0 || i == M - 1 || j == M - 1;
+      if (ring && !ring_ok(i, j)) return {};
+      int want[5];
+      for (int d = 0; d < 2; d++)
+        want[d] = E.field(i, j, kETri[d][0]) + 4 * E.field(i, j, kETri[d][1]) +
+                  16 * E.field(i, j, kETri[d][2]);
+      for (int o = 0; o < 3; o++)
+        want[2 + o] = E.field(i, j, kEBf[o][0]) + 4 * E.field(i, j, kEBf[o][1]) +
+                      16 * E.field(i, j, kEBf[o][2]) + 64 * E.field(i, j, kEBf[o][3]) +
+                      256 * E.slot(i, j, kEBf[o][4][0]);
+      for (int k = 0; k < 5; k++)
+        if (P(TU + k, i, j) != (ring ? 0 : want[k])) return {};
+    }
+  return e;
+}
+
+// Writes the lozenge cells of an N x N tiling (N = 2M) from an edge form;
+// the tiling's other cells are left as they are.
+void bibone_edge_merge(const std::vector<uint8_t>& e, int M, int32_t* tiling) {
+  const int N = 2 * M;
+  const EdgeBits E{e, M};
+  for (int i = 0; i < M; i++)
+    for (int j = 0; j < M; j++)
+      for (int o = 0; o < 3; o++)
+        tiling[(size_t)(2 * i + (o != 0)) * N + 2 * j + (o != 1)] = E.lozenge(o, i, j);
 }
 
 }  // namespace rt

@@ -936,6 +936,15 @@ RT_BIBONE_STATIC(rt_bibone_tiling_to_vertices, bibone_tiling_to_vertices)
 RT_BIBONE_STATIC(rt_bibone_tiling_to_domain, bibone_tiling_to_domain)
 #undef RT_BIBONE_STATIC
 
+int rt_bibone_edge_form(const int32_t* t, size_t n, uint8_t* out) {
+  int N;
+  if (!t || !out || !bibone_side(n, &N)) return RT_EINVAL;
+  const std::vector<uint8_t> e = bibone_edge_form(bibone_planes(from_c(t, n), N), N / 2);
+  if (e.empty()) return RT_EINVAL;
+  std::copy(e.begin(), e.end(), out);
+  return RT_OK;
+}
+
 rt_vec* rt_bibone_dimer_indicator(const int32_t* t, size_t n, int c) {
   int N;
   if (!t || !bibone_side(n, &N)) return nullptr;

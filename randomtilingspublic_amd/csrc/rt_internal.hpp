@@ -64,5 +64,11 @@ std::vector<int> bibone_butterfly_flips(const tiling& t, int N);
 std::vector<int> bibone_tiling_to_vertices(const tiling& t, int N);
 domain bibone_tiling_to_domain(const tiling& t, int N);
 std::vector<int> bibone_dimer_indicator(const tiling& t, int N, int c);
+// The eight planes of the walk from a tiling, the edge form of such a start
+// (empty when the edge-form walk does not apply to it) and the lozenge
+// cells written back from an edge form (rt_bibone_host.cpp).
+std::vector<int32_t> bibone_planes(const tiling& t, int N);
+std::vector<uint8_t> bibone_edge_form(const std::vector<int32_t>& planes, int M);
+void bibone_edge_merge(const std::vector<uint8_t>& e, int M, int32_t* tiling);
 
 }  // namespace rt

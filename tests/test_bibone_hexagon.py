@@ -110,3 +110,32 @@ def test_hexagon_walk_keeps_the_domain():
     h, l, r = decode(got)
     assert np.array_equal(degrees(h, l, r), hexagon_vertices(6, 6, 7, N // 2))
     assert (got != t).sum() > 20
+
+
+@pytest.mark.parametrize("start", ["cone9_2", "cone30_0", "hex7_7_8", "hex10_4_9", "butterfly1"])
+def test_edge_form_is_the_decoded_matching(start):
+    """rt_bibone_edge_form (the start of the walk's edge-form engine) gives
+    the three dimer-edge families that decode() reads off the cells."""
+    kind, *args = start.replace("cone", "cone_").replace("hex", "hex_").replace(
+        "butterfly", "bf_").split("_")
+    args = [int(x) for x in args]
+    t = {"cone": T.IceCreamCone, "hex": T.Hexagon, "bf": T.TestButterfly}[kind](*args)
+    e = T.EdgeForm(t)
+    assert e is not None
+    h, l, r = decode(t)
+    assert np.array_equal(e, h | l << 1 | r << 2)
+
+
+def test_edge_form_refuses_starts_it_does_not_reproduce():
+    t = np.asarray(T.Hexagon(6, 6, 7))
+    N = rt.side(t)
+    # a covered vertex on the second vertex row: the reference's unrefreshed
+    # outer ring of cells would see it
+    up = np.roll(t.reshape(N, N), -2, axis=0).ravel()
+    assert T.EdgeForm(up) is None
+    # two lozenge cells that disagree on a shared edge
+    bad = t.copy().reshape(N, N)
+    i, j = np.argwhere(bad[0::2, 1::2] != 0)[0]
+    bad[2 * i, 2 * j + 1] ^= 1
+    assert T.EdgeForm(bad.ravel()) is None
+    assert T.EdgeForm(t) is not None

@@ -47,6 +47,40 @@ def test_domino_cftp_batch_equals_singles(gpu_ctx):
             assert np.array_equal(outs[k], want)
 
 
+def test_domino_cftp_rounds_match_oracle_segments(gpu_ctx):
+    """rt_domino_cftp_batch_rounds: after r doubling rounds both chains of
+    every sample equal the oracle's segmented walks from MaxTiling and
+    MinTiling (DominoTiler.cpp:125-231, segment lengths 2^(r-2) t0, ...,
+    t0, t0 and seeds newest first, AztecDiamondCftpEx.cpp:57-76); at the
+    round where the CFTP batch reports coalescence the chains are equal and
+    equal to its sample."""
+    d = rt.DominoTiler.AztecDiamond(10)
+    tmin, tmax = rt.DominoTiler.MinTiling(d), rt.DominoTiler.MaxTiling(d)
+    N = rt.side(tmin)
+    P = rt.synthetic_params(N * N // 2, 23)
+    gpu_ctx.set_tinymt_params(P)
+    masters = [1, 2, 7]
+    rule = (rt.DOMINO_HEATBATH2P, 1.0, 0.5)
+    _say("domino cftp rounds vs oracle segments")
+    tops, bots, stats = gpu_ctx.domino_cftp_batch_rounds(tmax, tmin, masters, rt.SCHED_DOUBLING,
+                                                         64, 3, *rule)
+    assert all(s["rounds"] == 3 for s in stats)
+    for k, m in enumerate(masters):
+        seeds = [int(x) for x in o.mt19937(m, 3)][::-1]
+        steps = [128, 64, 64]
+        assert np.array_equal(tops[k], o.domino_walk_segments(tmax, steps, seeds, P, *rule)), m
+        assert np.array_equal(bots[k], o.domino_walk_segments(tmin, steps, seeds, P, *rule)), m
+    outs, cs = gpu_ctx.domino_cftp_batch(tmax, tmin, masters, rt.SCHED_DOUBLING, 64, 30, *rule)
+    for k, m in enumerate(masters):
+        r = cs[k]["rounds"]
+        t, b, _ = gpu_ctx.domino_cftp_batch_rounds(tmax, tmin, [m], rt.SCHED_DOUBLING, 64, r,
+                                                   *rule)
+        assert np.array_equal(t[0], outs[k]) and np.array_equal(b[0], outs[k]), m
+    with pytest.raises(rt.RTError):
+        gpu_ctx.domino_cftp_batch_rounds(tmax, tmin, [1], rt.SCHED_DOUBLING, 64, 2,
+                                         rt.DOMINO_METRO2P, 1.0, 0.5)
+
+
 @pytest.mark.parametrize("use_graph", [False, True])
 def test_domino_walk_batch_uneven(gpu_ctx, use_graph):
     orders = [3, 17, 40, 9, 64, 25]

@@ -1,7 +1,10 @@
 """Bibone walk on the GPU against the oracle: bit-exact lozenge tilings for
 the same TinyMT parameters, seed and step count (TriangleDimerTiler::Walk,
-TriangleDimerTiler.cpp:101-695), from the reference's IceCreamCone starts,
-plus the device result's own invariants at a larger size."""
+TriangleDimerTiler.cpp:101-695), from the reference's IceCreamCone starts
+and from hexagons, on all three engines (edge form, eight-plane sweep
+blocks, one launch per reference kernel), every edge-form tile shape, and a
+start that the edge form does not take; plus the device result's own
+invariants at a larger size."""
 import sys
 
 import numpy as np
@@ -18,14 +21,15 @@ def _say(*a):
     print("[case]", *a, file=sys.stderr, flush=True)
 
 
-@pytest.mark.parametrize("engine", [rt.ENGINE_AUTO, rt.ENGINE_STEP])
+@pytest.mark.parametrize("engine", [rt.ENGINE_AUTO, rt.ENGINE_FUSED, rt.ENGINE_STEP])
 @pytest.mark.parametrize("n,m,steps,seed", [(6, 0, 0, 1), (6, 0, 1, 2), (6, 0, 64, 3),
                                             (9, 2, 500, 11), (13, 5, 1500, 4), (30, 0, 3000, 77),
                                             (70, 9, 1200, 123), (150, 20, 700, 5)])
 def test_bibone_walk_matches_oracle(gpu_ctx, n, m, steps, seed, engine):
-    """Engine 0: sweep blocks (all phases of a few steps per launch on LDS
-    tiles; the cone of side 150 has 11 x 11 tiles); engine 1: one launch per
-    phase."""
+    """ENGINE_AUTO: the edge form (one bit per lattice edge, a cycle flip
+    per move, 8 steps per launch on 64 x 64 tiles; the cone of side 150 has
+    5 x 5 tiles); ENGINE_FUSED: the eight planes in sweep blocks; ENGINE_STEP:
+    one launch per reference kernel."""
     t = T.IceCreamCone(n, m)
     N = rt.side(t)
     P = rt.synthetic_params((N // 2) ** 2, seed=seed + 1)
@@ -40,6 +44,7 @@ def test_bibone_walk_matches_oracle(gpu_ctx, n, m, steps, seed, engine):
     assert np.array_equal(got, want)
     tm = gpu_ctx.last_timing()
     assert tm["steps"] == steps and tm["attempts"] == steps * (N // 2) ** 2
+    assert (tm["shape"] != 0) == (engine == rt.ENGINE_AUTO)
 
 
 def test_bibone_walk_large_cone_invariants(gpu_ctx):
@@ -76,3 +81,37 @@ def test_bibone_walk_from_hexagon_matches_oracle(gpu_ctx, a, b, c, steps, seed):
     got = gpu_ctx.bibone_walk(t, steps, seed)
     want = o.bibone_walk(t, steps, seed, P)
     assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("shape", [1, 2, 3, 4, 5])
+def test_bibone_edge_tile_shapes_match_oracle(gpu_ctx, shape):
+    """Every edge-form tile geometry (32/48/64/40-cell tiles, halo 4-16, so
+    2-8 steps per launch) on a hexagon of 30 x 30 plane cells per side."""
+    t = T.Hexagon(20, 19, 20)
+    N = rt.side(t)
+    P = rt.synthetic_params((N // 2) ** 2, seed=31)
+    gpu_ctx.set_tinymt_params(P)
+    _say("bibone edge tile shape", shape)
+    try:
+        gpu_ctx.set_tile_shape(shape)
+        got = gpu_ctx.bibone_walk(t, 1001, 6)
+        assert gpu_ctx.last_timing()["shape"] == shape
+    finally:
+        gpu_ctx.set_tile_shape(0)
+    assert np.array_equal(got, o.bibone_walk(t, 1001, 6, P))
+
+
+def test_bibone_walk_start_off_the_edge_form(gpu_ctx):
+    """A hexagon moved up by one vertex row touches the outer ring of cells
+    that the reference never refreshes: the edge form refuses it and the
+    walk runs on the eight planes, still equal to the oracle."""
+    t = np.asarray(T.Hexagon(6, 6, 7))
+    N = rt.side(t)
+    t = np.roll(t.reshape(N, N), -2, axis=0).ravel()
+    assert T.EdgeForm(t) is None
+    P = rt.synthetic_params((N // 2) ** 2, seed=32)
+    gpu_ctx.set_tinymt_params(P)
+    _say("bibone start off the edge form")
+    got = gpu_ctx.bibone_walk(t, 900, 4)
+    assert gpu_ctx.last_timing()["shape"] == 0
+    assert np.array_equal(got, o.bibone_walk(t, 900, 4, P))
