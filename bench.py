@@ -266,10 +266,10 @@ def weighted_and_bibone(rt, cftp_seeds=8, cftp_order=24, steps=10 ** 5, rounds51
 
 def bibone_walk(rt, ctx, steps=10 ** 5):
     """The bibone walk on the hexagon with sides 200, 201, 200 (tiling side
-    812, 406 x 406 plane cells), `steps` steps from the hexagon's tiling.
-    Algorithmic bytes per plane cell per launch of the sweep-block kernel:
-    eight int32 planes read and written (64 B), TinyMT state read and written
-    (32 B) and parameters read (12 B) = 108 B."""
+    812, 406 x 406 plane cells), `steps` steps from the hexagon's tiling, on
+    the edge form (k_bi_edge).  Algorithmic bytes per plane cell per launch:
+    the edge byte read and written (2 B), TinyMT state read and written
+    (32 B) and parameters read (12 B) = 46 B."""
     B = rt.TriangleDimerTiler
     t = B.Hexagon(200, 200, 201)
     Nb = rt.side(t)
@@ -288,11 +288,11 @@ def bibone_walk(rt, ctx, steps=10 ** 5):
             "avg_launch_us": round(tm["device_ms"] * 1e3 / tm["launches"], 2)}
 
 
-def bibone_section(rt, shape=0):
+def bibone_section(rt, shape=0, steps=10 ** 5):
     ctx = rt.Context(0)
     try:
         ctx.set_tile_shape(shape)
-        return bibone_walk(rt, ctx)
+        return bibone_walk(rt, ctx, steps)
     finally:
         ctx.close()
 
@@ -481,7 +481,8 @@ PROFILE_WORKLOADS = {
                             "k_pair_lozenge", 56),
     "aztec3000_step": (["--profile-section", "sweep"], "k_domino_step<1>", 48),
     "batch256_walk": (["--profile-section", "walks"], "k_walk_o8<0>", 48),
-    "bibone_hex200": (["--profile-section", "bibone"], "k_bi_sweep<48, 8, 768>", 108),
+    "bibone_hex200": (["--profile-section", "bibone", "--bibone-steps", "20000"],
+                      "k_bi_edge<64, 16, 1024>", 46),
 }
 PROFILE_TEXT = {
     "hex300_cftp_32seeds": "lozenge hexagon a=b=c=300 (N=609, 123 627 sites per plane), batched "
@@ -496,15 +497,16 @@ PROFILE_TEXT = {
                      "hipGraph, autotuned tile shape; 48 B per site per launch: TinyMT state "
                      "16 + 16 B, parameters 12 B, 2 colour planes read and written",
     "bibone_hex200": "bibone walk on the hexagon with sides 200, 201, 200 (tiling side 812, "
-                     "406 x 406 plane cells), 10^5 steps, sweep blocks of 48 x 48 cells with "
-                     "halo 8; 108 B per plane cell per launch: eight int32 planes read and "
-                     "written, TinyMT state 16 + 16 B, parameters 12 B",
+                     "406 x 406 plane cells), 2 x 10^4 steps on the edge form (one bit per "
+                     "lattice edge), 64 x 64-cell tiles with halo 16, 8 steps per launch; 46 B "
+                     "per plane cell per launch: the edge byte read and written, TinyMT state "
+                     "16 + 16 B, parameters 12 B",
 }
 
 
 def _short_kernel(name):
     """'void rt::k_fused_o8<1>(rt::UnitDesc const*, ...)' -> 'k_fused_o8<1>'."""
-    n = name.strip().strip('"')
+    n = name.strip().strip('"').replace("(anonymous namespace)::", "")
     if n.startswith("void "):
         n = n[5:]
     depth, cut = 0, len(n)
@@ -874,6 +876,8 @@ def main():
                     help="served stream: requests timed on the plain paths")
     ap.add_argument("--serve-burst", type=int, default=32,
                     help="served stream: requests of the burst that measures capacity")
+    ap.add_argument("--bibone-steps", type=int, default=10 ** 5,
+                    help="bibone profiling section: walk steps")
     ap.add_argument("--halo", type=int, default=6,
                     help="sweep-block halo depth (rt_set_halo; results never depend on it)")
     ap.add_argument("--tile-shape", type=int, default=0,
@@ -917,7 +921,7 @@ def main():
 
     if args.profile_section:
         fn = {"sweep": lambda: plain_sweep(rt), "walks": lambda: batched_walks(rt),
-              "c3": lambda: weighted_and_bibone(rt), "bibone": lambda: bibone_section(rt, args.tile_shape),
+              "c3": lambda: weighted_and_bibone(rt), "bibone": lambda: bibone_section(rt, args.tile_shape, args.bibone_steps),
               "served": serve}[args.profile_section]
         print(json.dumps({"profile_section": args.profile_section, **fn()}), flush=True)
         return

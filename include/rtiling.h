@@ -349,7 +349,7 @@ int rt_recttri_walk(rt_ctx* ctx, int32_t* tiling, int N, long steps,
  * seed + id).  Needs >= (N/2)^2 parameter sets.  Engines: RT_ENGINE_AUTO
  * and RT_ENGINE_FUSED_SWAR run starts that rt_bibone_edge_form accepts on
  * one bit per lattice edge (each move a cycle flip, H/2 steps per launch;
- * rt_set_tile_shape 1-5 picks its tile, 0 the default, reported in
+ * rt_set_tile_shape 1-7 picks its tile, 0 the default, reported in
  * rt_timing.shape) and other starts on the eight planes in sweep blocks
  * (rt_timing.shape 0); RT_ENGINE_FUSED forces the eight-plane sweep
  * blocks, RT_ENGINE_STEP one launch per reference kernel.  Results never

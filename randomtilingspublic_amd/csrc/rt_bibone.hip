@@ -430,8 +430,11 @@ constexpr int kBiMaxK = 16;  // steps per window
 // Edge-form tile geometry used when rt_set_tile_shape leaves it at 0: 64 x
 // 64 cells, halo 16 (8 steps per launch), 1024 threads.  Device time per
 // step on the bibone hexagon 200, 201, 200 (406 x 406 plane cells): 2.43 us
-// against 2.74 for 48 x 48 / halo 8 (shape 2), 3.22 for 64 x 64 / halo 8
-// (3), 3.31 for 40 x 40 / halo 8 (5), 4.39 for 32 x 32 / halo 4 (1).
+// against 2.74 for 48 x 48 / halo 8 (shape 2), 2.75 for 48 x 48 / halo 12
+// (6, 289 tiles against 169), 3.22 for 64 x 64 / halo 8 (3), 3.31 for
+// 40 x 40 / halo 8 (5), 3.95 for 48 x 48 / halo 12 at 576 threads (7) and
+// 4.39 for 32 x 32 / halo 4 (1).  VALU issue is a third of the chip's
+// SIMD cycles: one 1024-thread workgroup per CU, two barriers per step.
 constexpr int kBiEdgeDefaultShape = 4;
 // Tile geometry of the walk: 48 x 48 loaded cells, halo 8 (32 x 32
 // interior), 768 threads (three cells each).  Measured on the bibone
@@ -938,6 +941,8 @@ int bibone_edge_run(rt_ctx* ctx, int shape, int M, uint8_t* a, uint8_t* b, size_
     case 2: return bibone_edge_sweep<48, 8, 768>(ctx, M, a, b, soff, d_moves, steps, result, launches);
     case 3: return bibone_edge_sweep<64, 8, 1024>(ctx, M, a, b, soff, d_moves, steps, result, launches);
     case 5: return bibone_edge_sweep<40, 8, 320>(ctx, M, a, b, soff, d_moves, steps, result, launches);
+    case 6: return bibone_edge_sweep<48, 12, 768>(ctx, M, a, b, soff, d_moves, steps, result, launches);
+    case 7: return bibone_edge_sweep<48, 12, 576>(ctx, M, a, b, soff, d_moves, steps, result, launches);
     default: return bibone_edge_sweep<64, 16, 1024>(ctx, M, a, b, soff, d_moves, steps, result, launches);
   }
 }

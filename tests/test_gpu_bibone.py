@@ -83,10 +83,10 @@ def test_bibone_walk_from_hexagon_matches_oracle(gpu_ctx, a, b, c, steps, seed):
     assert np.array_equal(got, want)
 
 
-@pytest.mark.parametrize("shape", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("shape", [1, 2, 3, 4, 5, 6, 7])
 def test_bibone_edge_tile_shapes_match_oracle(gpu_ctx, shape):
-    """Every edge-form tile geometry (32/48/64/40-cell tiles, halo 4-16, so
-    2-8 steps per launch) on a hexagon of 30 x 30 plane cells per side."""
+    """Every edge-form tile geometry (32/40/48/64-cell tiles, halo 4-16, so
+    2-8 steps per launch) on the hexagon 20, 19, 20 (44 x 44 plane cells)."""
     t = T.Hexagon(20, 19, 20)
     N = rt.side(t)
     P = rt.synthetic_params((N // 2) ** 2, seed=31)
