@@ -581,11 +581,13 @@ def _rocprof_run(name, outdir, tag, prof, bench_args, timeout, summarize_only):
     stdout_file = os.path.join(d, "stdout.txt")
     if not summarize_only:
         print(f"[profile] {name}: {' '.join(cmd)}", file=sys.stderr, flush=True)
+        # stdout (the JSON line) is captured; the program's progress lines
+        # and the profiler's messages go on to this process's stderr
         r = subprocess.run(cmd, cwd="/tmp", env=dict(os.environ, TMPDIR="/tmp"),
-                           capture_output=True, text=True, timeout=timeout)
+                           stdout=subprocess.PIPE, text=True, timeout=timeout)
         last = [l for l in r.stdout.splitlines() if l.startswith("{")]
         if r.returncode != 0 and not last:
-            sys.stderr.write(r.stdout[-2000:] + r.stderr[-2000:])
+            sys.stderr.write(r.stdout[-2000:])
             raise RuntimeError(f"rocprofv3 {tag} failed ({r.returncode})")
         os.makedirs(d, exist_ok=True)
         with open(stdout_file, "w") as f:
@@ -658,6 +660,71 @@ def rocprof_sq(name, outdir, summarize_only=False):
         f.write("\n".join(lines) + "\n")
     print(json.dumps({"rocprof_sq": name, "kernel": kernel, "avg_us": avg_us,
                       "valu_per_dispatch": valu, "summary": md}), flush=True)
+
+
+# The driver's own invocation of this benchmark, traced by --rocprof-bench.
+BENCH_DEFAULT_ARGS = ["--gpus", "1", "--steps", "20", "--warmup", "5"]
+
+
+def rocprof_bench(outdir, summarize_only=False):
+    """Runs the default benchmark command under rocprofv3 --kernel-trace
+    --stats and writes profiles/bench_default_kernel_trace.md: the kernel
+    statistics of the whole run, and the headline kernel's dispatches inside
+    the headline section (every dispatch before the plain path's first
+    k_lozenge_step), whose average duration is the one the JSON line's
+    roofline uses (roofline.avg_launch_us, HIP events on the same
+    launches).  Raw output stays in `outdir` (the dispatch trace of the
+    full run is hundreds of MB)."""
+    import csv
+    name = "bench_default"
+    d, out, cmd = _rocprof_run(name, outdir, "kt", ["--kernel-trace", "--stats"],
+                               BENCH_DEFAULT_ARGS, 1100, summarize_only)
+    roof = out["roofline"]
+    kernel = roof["kernel"]
+    stats = _kernel_stats(d)
+    trace = []
+    import glob
+    for path in glob.glob(os.path.join(d, "**", "*kernel_trace.csv"), recursive=True):
+        with open(path, newline="") as f:
+            for r in csv.DictReader(f):
+                trace.append((int(r["Dispatch_Id"]), _short_kernel(r["Kernel_Name"]),
+                              int(r["End_Timestamp"]) - int(r["Start_Timestamp"])))
+    trace.sort()
+    head = []
+    for _, k, ns in trace:
+        if k.startswith("k_lozenge_step"):
+            break
+        if k == kernel:
+            head.append(ns)
+    avg_head = sum(head) / len(head) / 1e3
+    lines = [f"# {kernel} -- the default benchmark command, kernel trace", "",
+             f"Workload: `python3 bench.py {' '.join(BENCH_DEFAULT_ARGS)}` (the whole default "
+             f"run: headline batch `{out['config']['workload']}`, then the plain paths, C1, "
+             "the plain sweep, the batch walks, the weighted / bibone section, the served "
+             "stream and the CPU baseline).", "",
+             "Command (made by `python3 bench.py --rocprof-bench <dir>`):", "", "```",
+             "cd /tmp && export TMPDIR=/tmp", cmd, "```", "",
+             "Kernel statistics of the whole run (`kernel_stats.csv`, top kernels by time):", "",
+             "| Kernel | Calls | Total ms | Avg us | Min us | Max us | % |",
+             "|---|---|---|---|---|---|---|"]
+    for k, v in sorted(stats.items(), key=lambda kv: -kv[1]["total_ms"])[:10]:
+        lines.append(f"| `{k}` | {v['calls']} | {v['total_ms']:.1f} | {v['avg_us']:.2f} | "
+                     f"{v['min_us']:.2f} | {v['max_us']:.2f} | {v['pct']:.2f} |")
+    lines += ["",
+              f"Headline section (dispatches before the plain path's first `k_lozenge_step`): "
+              f"{len(head)} dispatches of `{kernel}`, average {avg_head:.2f} us by the trace; "
+              f"the same run's JSON line: {roof['launches']} launches, avg_launch_us "
+              f"{roof['avg_launch_us']} (HIP events), value {out['value']} {out['unit']}, "
+              f"roofline frac {roof['frac']}.",
+              f"Over the whole run the tracer counts {stats.get(kernel, {}).get('calls', 0)} "
+              f"dispatches of `{kernel}`, average "
+              f"{stats.get(kernel, {}).get('avg_us', 0.0):.2f} us (the served stream launches "
+              "it too, on smaller hexagons)."]
+    with open(os.path.join(ROOT, "profiles", "bench_default_kernel_trace.md"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print(json.dumps({"rocprof_bench": kernel, "headline_dispatches": len(head),
+                      "trace_avg_us": avg_head, "events_avg_us": roof["avg_launch_us"]}),
+          flush=True)
 
 
 def rocprof_profile(name, outdir, summarize_only=False):
@@ -890,6 +957,9 @@ def main():
     ap.add_argument("--rocprof-sq", nargs=2, metavar=("WORKLOAD", "OUTDIR"), default=None,
                     help="run rocprofv3 with instruction and LDS counters (SQ_*) on one "
                          "profiling workload and write profiles/<kernel>_<workload>_sq.md")
+    ap.add_argument("--rocprof-bench", metavar="OUTDIR", default=None,
+                    help="run the default benchmark command under rocprofv3 --kernel-trace "
+                         "--stats and write profiles/bench_default_kernel_trace.md")
     ap.add_argument("--summarize-only", action="store_true",
                     help="with --rocprof-profile / --rocprof-sq: only re-read the raw output "
                          "in OUTDIR")
@@ -907,6 +977,9 @@ def main():
         return
     if args.rocprof_sq:
         rocprof_sq(*args.rocprof_sq, summarize_only=args.summarize_only)
+        return
+    if args.rocprof_bench:
+        rocprof_bench(args.rocprof_bench, summarize_only=args.summarize_only)
         return
     if args.copy_probe:
         copy_probe(args.copy_probe)

@@ -127,3 +127,45 @@ def test_param_file_malformed(tmp_path):
     with pytest.raises(rt.RTError) as e:
         rt.parse_tinymt_file(tmp_path / "missing.txt", 1)
     assert e.value.code == -3
+
+
+@pytest.mark.parametrize("model", ["domino0", "domino1", "domino2", "domino3", "lozenge",
+                                   "recttri"])
+def test_segmented_walks_compose(model):
+    """The segmented Walk re-seeds both generators per segment and never
+    splits or merges between segments (DominoTiler.cpp:125-231,
+    LozengeTiler.cpp:95-159, RectTriangleTiler.cpp's segment loop): it is
+    the segments run one after the other, seeds in list order.  For lozenge
+    and rectangle-triangle a one-segment list is the plain Walk.  The domino
+    plain Walk seeds TinyMT with the first mt19937(seed) output and takes its
+    colours from the outputs after it (DominoTiler.cpp:51-70), a segment
+    seeds TinyMT with the seed itself (155-160), so there segments compose
+    with segments only."""
+    if model.startswith("domino"):
+        rule = int(model[-1])
+        t = rt.DominoTiler.MaxTiling(rt.DominoTiler.AztecDiamond(9))
+        P = rt.synthetic_params(len(t) // 2, 41)
+        seg = lambda x, st, se: o.domino_walk_segments(x, st, se, P, rule, 1.0, 0.5)
+        one = lambda x, st, se: seg(x, [st], [se])
+    elif model == "lozenge":
+        t = rt.LozengeTiler.MaxTiling(rt.LozengeTiler.AlmostHexDomain(7, 1))
+        P = rt.synthetic_params(len(t), 42)
+        seg = lambda x, st, se: o.lozenge_walk_segments(x, st, se, P)
+        one = lambda x, st, se: o.lozenge_walk(x, st, se, P)
+    else:
+        t = rt.RectTriangleTiler.maxHex(3)
+        P = rt.synthetic_params(len(t), 43)
+        seg = lambda x, st, se: o.recttri_walk_segments(x, st, se, P)
+        one = lambda x, st, se: o.recttri_walk(x, st, se, P)
+    steps, seeds = [37, 5, 64], [9, 123456, 2]
+    want = t
+    for st, se in zip(steps, seeds):
+        want = one(want, st, se)
+    got = seg(t, steps, seeds)
+    assert np.array_equal(got, want)
+    assert not np.array_equal(got, np.asarray(t))
+    # a different seed order is a different walk
+    assert not np.array_equal(seg(t, steps, seeds[::-1]), got)
+    if model.startswith("domino"):
+        # the plain domino Walk is not a one-segment list with the same seed
+        assert not np.array_equal(o.domino_walk(t, 64, 2, P, rule, 1.0, 0.5), seg(t, [64], [2]))
