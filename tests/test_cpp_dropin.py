@@ -1,6 +1,8 @@
 """The C++ drop-in header (include/rtiling_tilers.hpp) compiles against
 librtiling.so and its static helpers reproduce the reference's outputs;
-with a GPU present the same program also runs a Walk and a CFTP through it."""
+with a GPU present the same program also runs a Walk, a CFTP and a bibone
+Walk through it.  The program writes every tiling it makes as raw int32
+(`dump`), and the tests compare the whole arrays."""
 import os
 import subprocess
 
@@ -14,17 +16,24 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PROGRAM = r'''
 #include <cstdio>
 #include "rtiling_tilers.hpp"
+static void dump(const tiling& t, const char* path) {
+  FILE* f = std::fopen(path, "wb");
+  std::fwrite(t.data(), sizeof(int), t.size(), f);
+  std::fclose(f);
+}
 int main(int argc, char** argv) {
   domain d = DominoTiler::AztecDiamond(16);
   tiling t = DominoTiler::MinTiling(d);
   long s = 0;
   for (size_t k = 0; k < t.size(); k++) s += (long)(k % 97 + 1) * t[k];
   std::printf("min %zu %ld\n", t.size(), s);
+  dump(t, "min.bin");
   domain dl = LozengeTiler::AlmostHexDomain(7, 2);
   tiling l = LozengeTiler::MaxTiling(dl);
   s = 0;
   for (size_t k = 0; k < l.size(); k++) s += (long)(k % 89 + 1) * l[k];
   std::printf("loz %zu %ld\n", l.size(), s);
+  dump(l, "loz.bin");
   SaveMatrix(t, "saved.txt");
   tiling back = LoadMatrix("saved.txt");
   std::printf("io %d\n", back == t ? 1 : 0);
@@ -40,6 +49,7 @@ int main(int argc, char** argv) {
   s = 0;
   for (size_t k = 0; k < cone.size(); k++) s += (long)(k % 83 + 1) * cone[k];
   std::printf("cone %zu %ld\n", cone.size(), s);
+  dump(cone, "cone.bin");
   if (argc > 1) {  // device part
     DominoTiler D(0, "dominokernelCFTP.cl");
     std::vector<uint32_t> p(3 * t.size() / 2);
@@ -50,10 +60,12 @@ int main(int argc, char** argv) {
     s = 0;
     for (size_t k = 0; k < w.size(); k++) s += (long)(k % 97 + 1) * w[k];
     std::printf("walk %ld\n", s);
+    dump(w, "walk.bin");
     tiling c = D.CFTP(DominoTiler::MaxTiling(d), t, 345903);
     s = 0;
     for (size_t k = 0; k < c.size(); k++) s += (long)(k % 97 + 1) * c[k];
     std::printf("cftp %ld\n", s);
+    dump(c, "cftp.bin");
     TriangleDimerTiler B(0);
     std::vector<uint32_t> pb(3 * cone.size() / 4);
     for (size_t k = 0; k < pb.size(); k++) pb[k] = (uint32_t)(k * 2246822519u + 777u);
@@ -63,6 +75,7 @@ int main(int argc, char** argv) {
     s = 0;
     for (size_t k = 0; k < bw.size(); k++) s += (long)(k % 83 + 1) * bw[k];
     std::printf("bibone %ld\n", s);
+    dump(bw, "bibone.bin");
   }
   return 0;
 }
@@ -72,6 +85,10 @@ int main(int argc, char** argv) {
 def _checksum(t, m):
     k = np.arange(len(t))
     return int(((k % m + 1) * t.astype(np.int64)).sum())
+
+
+def _load(tmp_path, name):
+    return np.fromfile(tmp_path / name, dtype=np.int32)
 
 
 def _build(tmp_path):
@@ -91,13 +108,16 @@ def test_cpp_dropin_statics(tmp_path):
     d = rt.DominoTiler.AztecDiamond(16)
     t = rt.DominoTiler.MinTiling(d)
     assert out[0] == f"min {len(t)} {_checksum(t, 97)}"
+    assert np.array_equal(_load(tmp_path, "min.bin"), t)
     dl = rt.LozengeTiler.AlmostHexDomain(7, 2)
     l = rt.LozengeTiler.MaxTiling(dl)
     assert out[1] == f"loz {len(l)} {_checksum(l, 89)}"
+    assert np.array_equal(_load(tmp_path, "loz.bin"), l)
     assert out[2] == "io 1"
     assert out[3] == "untileable -7"
     cone = rt.TriangleDimerTiler.IceCreamCone(9, 2)
     assert out[4] == f"cone {len(cone)} {_checksum(cone, 83)}"
+    assert np.array_equal(_load(tmp_path, "cone.bin"), cone)
 
 
 @pytest.mark.gpu
@@ -113,14 +133,17 @@ def test_cpp_dropin_walk_and_cftp(tmp_path):
     p = ((np.arange(3 * n, dtype=np.uint64) * np.uint64(2654435761) + np.uint64(12345))
          & np.uint64(0xFFFFFFFF)).astype(np.uint32)
     w = o.domino_walk(t, 1000, 1, p, 1)
+    assert np.array_equal(_load(tmp_path, "walk.bin"), w)
     assert out[5] == f"walk {_checksum(w, 97)}"
     c, _ = o.domino_cftp(rt.DominoTiler.MaxTiling(d), t, 345903, p, 0, 128, 64, 1)
+    assert np.array_equal(_load(tmp_path, "cftp.bin"), c)
     assert out[6] == f"cftp {_checksum(c, 97)}"
     cone = rt.TriangleDimerTiler.IceCreamCone(9, 2)
     nb = len(cone) // 4
     pb = ((np.arange(3 * nb, dtype=np.uint64) * np.uint64(2246822519) + np.uint64(777))
           & np.uint64(0xFFFFFFFF)).astype(np.uint32)
     bw = o.bibone_walk(cone, 300, 5, pb)
+    assert np.array_equal(_load(tmp_path, "bibone.bin"), bw)
     assert out[7] == f"bibone {_checksum(bw, 83)}"
 
 
