@@ -298,14 +298,15 @@ def bibone_section(rt, shape=0, steps=10 ** 5):
 
 
 def served_stream(rt, seconds=40.0, seed=2024, dom=(64, 256), loz=(64, 160), t0=1024, burst_n=32,
-                  plain_n=6):
+                  plain_n=6, load=0.8):
     """A Poisson stream of CFTP requests through the continuous-batching
     server (rt_server), then the same requests one call at a time.  Request
     mix, drawn from `seed`: domino Aztec diamonds of order dom[0]..dom[1]
     (heat bath) and lozenge hexagons of side loz[0]..loz[1], doubling from
     t0 (rectangle-triangle has no CFTP: its coupling is not monotone).  The
-    arrival rate is 70 % of the capacity measured on a burst of burst_n
-    requests.  Reports samples/s, p50/p99 latency (submit to result) and
+    arrival rate is `load` times the capacity measured on a burst of burst_n
+    requests (the served rate follows the offered one; latency is the
+    price of a higher load).  Reports samples/s, p50/p99 latency (submit to result) and
     the plain path's samples/s on the first plain_n requests."""
     rng = np.random.default_rng(seed)
     cache = {}
@@ -358,7 +359,7 @@ def served_stream(rt, seconds=40.0, seed=2024, dom=(64, 256), loz=(64, 160), t0=
         for n, i in enumerate(ids):
             finish(i, f"burst request {n}")
         cap = len(burst) / (time.time() - tb)
-        rate = 0.7 * cap
+        rate = load * cap
         print(f"[bench] served: burst capacity {cap:.1f} samples/s, stream at {rate:.1f}/s",
               file=sys.stderr, flush=True)
         # Poisson stream
@@ -425,7 +426,8 @@ def served_stream(rt, seconds=40.0, seed=2024, dom=(64, 256), loz=(64, 160), t0=
     lat = np.array(lat)
     return {"workload": f"Poisson stream of CFTP requests: domino Aztec order {dom[0]}-{dom[1]} "
                         f"(heat bath) and lozenge hexagon side {loz[0]}-{loz[1]}, doubling from "
-                        f"2^{int(np.log2(t0))}, rate 70 % of burst capacity, continuous batching",
+                        f"2^{int(np.log2(t0))}, rate {round(100 * load)} % of burst capacity, "
+                        "continuous batching",
             "requests": len(stream), "span_s": round(span, 2),
             "samples_per_s": round(served, 4), "burst_capacity_per_s": round(cap, 4),
             "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
@@ -941,6 +943,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=40.0)
     ap.add_argument("--serve-plain", type=int, default=6,
                     help="served stream: requests timed on the plain paths")
+    ap.add_argument("--serve-load", type=float, default=0.8,
+                    help="served stream: arrival rate as a fraction of the burst capacity")
     ap.add_argument("--serve-burst", type=int, default=32,
                     help="served stream: requests of the burst that measures capacity")
     ap.add_argument("--bibone-steps", type=int, default=10 ** 5,
@@ -990,7 +994,7 @@ def main():
     def serve():
         return served_stream(rt, seconds=args.serve_seconds, dom=tuple(args.serve_dom),
                              loz=tuple(args.serve_loz), plain_n=args.serve_plain,
-                             burst_n=args.serve_burst)
+                             burst_n=args.serve_burst, load=args.serve_load)
 
     if args.profile_section:
         fn = {"sweep": lambda: plain_sweep(rt), "walks": lambda: batched_walks(rt),
