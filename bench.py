@@ -298,12 +298,15 @@ def bibone_section(rt, shape=0, steps=10 ** 5):
 
 
 def served_stream(rt, seconds=40.0, seed=2024, dom=(64, 256), loz=(64, 160), t0=1024, burst_n=32,
-                  plain_n=6, load=0.8):
+                  plain_n=6, load=0.8, rtri=(32, 256), rtri_steps=10 ** 4):
     """A Poisson stream of CFTP requests through the continuous-batching
     server (rt_server), then the same requests one call at a time.  Request
     mix, drawn from `seed`: domino Aztec diamonds of order dom[0]..dom[1]
     (heat bath) and lozenge hexagons of side loz[0]..loz[1], doubling from
-    t0 (rectangle-triangle has no CFTP: its coupling is not monotone).  The
+    t0, and rectangle-triangle hexagons of side rtri[0]..rtri[1] as walks of
+    rtri_steps steps from minHex (that model has no CFTP: its coupling is
+    not monotone; the reference's own example walks from minHex), each
+    model with probability 1/3 (1/2 each without rtri).  The
     arrival rate is `load` times the capacity measured on a burst of burst_n
     requests (the served rate follows the offered one; latency is the
     price of a higher load).  Reports samples/s, p50/p99 latency (submit to result) and
@@ -316,25 +319,32 @@ def served_stream(rt, seconds=40.0, seed=2024, dom=(64, 256), loz=(64, 160), t0=
             if model == rt.MODEL_DOMINO:
                 d = rt.DominoTiler.AztecDiamond(n)
                 cache[(model, n)] = (rt.DominoTiler.MaxTiling(d), rt.DominoTiler.MinTiling(d))
+            elif model == rt.MODEL_RECTTRI:
+                cache[(model, n)] = (rt.RectTriangleTiler.minHex(n), None)
             else:
                 d = rt.LozengeTiler.AlmostHexDomain(n, 0)
                 cache[(model, n)] = (rt.LozengeTiler.MaxTiling(d), rt.LozengeTiler.MinTiling(d))
         return cache[(model, n)]
 
+    sizes = {rt.MODEL_DOMINO: dom, rt.MODEL_LOZENGE: loz}
+    if rtri:
+        sizes[rt.MODEL_RECTTRI] = rtri
+
     def draw(k):
-        model = int(rng.integers(0, 2))
-        n = (int(rng.integers(dom[0], dom[1] + 1)) if model == rt.MODEL_DOMINO
-             else int(rng.integers(loz[0], loz[1] + 1)))
+        model = int(rng.integers(0, len(sizes)))
+        lo, hi = sizes[model]
+        n = int(rng.integers(lo, hi + 1))
         tmax, tmin = domain(model, n)
         return dict(model=model, n=n, tmax=tmax, tmin=tmin, master_seed=10000 + k)
 
-    big = max(rt.side(domain(rt.MODEL_DOMINO, dom[1])[0]),
-              rt.side(domain(rt.MODEL_LOZENGE, loz[1])[0]))
+    big = max(rt.side(domain(m, hi)[0]) for m, (lo, hi) in sizes.items())
     ctx = rt.Context(0)
     ctx.set_tinymt_params(rt.synthetic_params(big * big, None))
     srv = rt.Server(ctx, 8 << 30, max_active=256)
     try:
         def submit(q):
+            if q["model"] == rt.MODEL_RECTTRI:
+                return srv.submit_walk(q["model"], q["tmax"], rtri_steps, q["master_seed"])
             return srv.submit(q["model"], q["tmax"], q["tmin"], q["master_seed"],
                               rt.SCHED_DOUBLING, t0, 32, rt.DOMINO_HEATBATH)
 
@@ -382,7 +392,7 @@ def served_stream(rt, seconds=40.0, seed=2024, dom=(64, 256), loz=(64, 160), t0=
                 r = srv.poll(i)
                 if r is not None:
                     assert r[0] == 0
-                    lat.append(r[3])
+                    lat.append((pending[i]["model"], r[3]))
                     del pending[i]
                     print(f"[bench] served: a request completed after {r[3]:.1f} s",
                           file=sys.stderr, flush=True)
@@ -394,7 +404,7 @@ def served_stream(rt, seconds=40.0, seed=2024, dom=(64, 256), loz=(64, 160), t0=
             time.sleep(0.002)
         for n, i in enumerate(list(pending)):
             r = finish(i, f"stream request {n} of {len(pending)} left")
-            lat.append(r[3])
+            lat.append((pending[i]["model"], r[3]))
             print(f"[bench] served: drained {n + 1} of {len(pending)}", file=sys.stderr,
                   flush=True)
         span = time.time() - start
@@ -413,6 +423,8 @@ def served_stream(rt, seconds=40.0, seed=2024, dom=(64, 256), loz=(64, 160), t0=
             if q["model"] == rt.MODEL_DOMINO:
                 ctx.domino_cftp(q["tmax"], q["tmin"], q["master_seed"], rt.SCHED_DOUBLING, t0,
                                 32, rt.DOMINO_HEATBATH)
+            elif q["model"] == rt.MODEL_RECTTRI:
+                ctx.recttri_walk(q["tmax"], rtri_steps, q["master_seed"])
             else:
                 ctx.lozenge_cftp(q["tmax"], q["tmin"], q["master_seed"], rt.SCHED_DOUBLING, t0,
                                  32)
@@ -423,15 +435,24 @@ def served_stream(rt, seconds=40.0, seed=2024, dom=(64, 256), loz=(64, 160), t0=
     plain_fused = plain_rate(0, sub) if sub else None
     ctx.set_engine(0)
     ctx.close()
-    lat = np.array(lat)
-    return {"workload": f"Poisson stream of CFTP requests: domino Aztec order {dom[0]}-{dom[1]} "
-                        f"(heat bath) and lozenge hexagon side {loz[0]}-{loz[1]}, doubling from "
-                        f"2^{int(np.log2(t0))}, rate {round(100 * load)} % of burst capacity, "
-                        "continuous batching",
+    lat_cftp = np.array([v for m, v in lat if m != rt.MODEL_RECTTRI])
+    lat = np.array([v for _, v in lat])
+    mix = (f"CFTP requests of domino Aztec order {dom[0]}-{dom[1]} (heat bath) and lozenge "
+           f"hexagon side {loz[0]}-{loz[1]}, doubling from 2^{int(np.log2(t0))}")
+    if rtri:
+        mix += (f", and walks of {rtri_steps} steps from minHex of rectangle-triangle hexagon "
+                f"side {rtri[0]}-{rtri[1]} (no CFTP for that model), 1/3 each")
+    return {"workload": f"Poisson stream of {mix}; rate {round(100 * load)} % of burst "
+                        "capacity, continuous batching",
             "requests": len(stream), "span_s": round(span, 2),
             "samples_per_s": round(served, 4), "burst_capacity_per_s": round(cap, 4),
             "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
             "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
+            "cftp_requests": int(lat_cftp.size),
+            "p50_cftp_latency_s": round(float(np.percentile(lat_cftp, 50)), 4)
+            if lat_cftp.size else None,
+            "p99_cftp_latency_s": round(float(np.percentile(lat_cftp, 99)), 4)
+            if lat_cftp.size else None,
             "plain_samples_per_s": round(plain, 4) if plain else None,
             "plain_fused_samples_per_s": round(plain_fused, 4) if plain_fused else None,
             "plain_requests": len(sub),
@@ -940,6 +961,11 @@ def main():
                     help="served stream: domino Aztec orders")
     ap.add_argument("--serve-loz", type=int, nargs=2, default=[64, 160], metavar=("LO", "HI"),
                     help="served stream: lozenge hexagon sides")
+    ap.add_argument("--serve-rt", type=int, nargs=2, default=[32, 256], metavar=("LO", "HI"),
+                    help="served stream: rectangle-triangle hexagon sides of the walk requests "
+                         "(0 0: none)")
+    ap.add_argument("--serve-rt-steps", type=int, default=10 ** 4,
+                    help="served stream: steps of each rectangle-triangle walk request")
     ap.add_argument("--serve-seconds", type=float, default=40.0)
     ap.add_argument("--serve-plain", type=int, default=6,
                     help="served stream: requests timed on the plain paths")
@@ -994,7 +1020,9 @@ def main():
     def serve():
         return served_stream(rt, seconds=args.serve_seconds, dom=tuple(args.serve_dom),
                              loz=tuple(args.serve_loz), plain_n=args.serve_plain,
-                             burst_n=args.serve_burst, load=args.serve_load)
+                             burst_n=args.serve_burst, load=args.serve_load,
+                             rtri=tuple(args.serve_rt) if args.serve_rt[1] > 0 else None,
+                             rtri_steps=args.serve_rt_steps)
 
     if args.profile_section:
         fn = {"sweep": lambda: plain_sweep(rt), "walks": lambda: batched_walks(rt),

@@ -290,21 +290,31 @@ int rt_domino_walk_batch_heights(rt_ctx* ctx, int32_t* const* tilings, const int
  * `hbm_budget` bytes (at most `max_active` at once), keeps their chains
  * resident and advances all of them in shared launches, each at its own
  * point of its own schedule.  Request k's sample equals rt_*_cftp on the
- * same inputs, alone or mixed with any others.  While a server exists, its
- * context must not be used by other calls.  No reference counterpart. */
+ * same inputs, alone or mixed with any others.  Walk requests (kind
+ * RT_REQ_WALK) run `steps` steps of the model's plain Walk from tmax with
+ * seed master_seed and return rt_domino_walk / rt_lozenge_walk /
+ * rt_recttri_walk (the reference kernel's weights) of the same inputs; they
+ * are how rectangle-triangle tilings are served, since that model has no
+ * monotone coupling for CFTP.  While a server exists, its context must not
+ * be used by other calls.  No reference counterpart. */
 #define RT_MODEL_DOMINO 0
 #define RT_MODEL_LOZENGE 1
+#define RT_MODEL_RECTTRI 2
+#define RT_REQ_CFTP 0
+#define RT_REQ_WALK 1
 
 typedef struct {
-  int model;               /* RT_MODEL_DOMINO or RT_MODEL_LOZENGE */
+  int model;               /* RT_MODEL_*; RT_MODEL_RECTTRI for walks only */
   int N;                   /* tiling side */
-  const int32_t* tmax;     /* N*N, copied at submit */
-  const int32_t* tmin;
-  uint32_t master_seed;
-  int sched;               /* RT_SCHED_* */
-  long t0;
-  int max_rounds;
-  rt_domino_rule rule;     /* domino only */
+  const int32_t* tmax;     /* N*N, copied at submit; a walk's start */
+  const int32_t* tmin;     /* CFTP only (may be NULL for walks) */
+  uint32_t master_seed;    /* CFTP master seed; a walk's seed */
+  int sched;               /* RT_SCHED_* (CFTP) */
+  long t0;                 /* CFTP */
+  int max_rounds;          /* CFTP */
+  rt_domino_rule rule;     /* domino only: heat-bath rules for CFTP, any for walks */
+  int kind;                /* RT_REQ_CFTP (0) or RT_REQ_WALK */
+  long steps;              /* walk length (>= 1) */
 } rt_request;
 
 typedef struct {

@@ -54,7 +54,8 @@ class CftpStats(C.Structure):
 class Request(C.Structure):
     _fields_ = [("model", C.c_int), ("N", C.c_int), ("tmax", C.c_void_p), ("tmin", C.c_void_p),
                 ("master_seed", C.c_uint32), ("sched", C.c_int), ("t0", C.c_long),
-                ("max_rounds", C.c_int), ("rule", DominoRule)]
+                ("max_rounds", C.c_int), ("rule", DominoRule), ("kind", C.c_int),
+                ("steps", C.c_long)]
 
 
 class ServerStats(C.Structure):
@@ -63,7 +64,8 @@ class ServerStats(C.Structure):
                 ("span_s", C.c_double), ("samples_per_s", C.c_double), ("status", C.c_int)]
 
 
-MODEL_DOMINO, MODEL_LOZENGE = 0, 1
+MODEL_DOMINO, MODEL_LOZENGE, MODEL_RECTTRI = 0, 1, 2
+REQ_CFTP, REQ_WALK = 0, 1
 
 _lib = None
 _I32P = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
@@ -524,9 +526,22 @@ class Server:
 
     def submit(self, model, tmax, tmin, master_seed, sched=SCHED_DOUBLING, t0=1024,
                max_rounds=32, rule=DOMINO_HEATBATH, a=1.0, b=1.0):
+        """A CFTP request; returns its id."""
         tmax, tmin = _i32(tmax), _i32(tmin)
         q = Request(model, side(tmax), tmax.ctypes.data, tmin.ctypes.data, master_seed, sched, t0,
-                    max_rounds, DominoRule(rule, a, b))
+                    max_rounds, DominoRule(rule, a, b), REQ_CFTP, 0)
+        return self._submit(q, tmax)
+
+    def submit_walk(self, model, start, steps, seed, rule=DOMINO_TOGGLE08, a=1.0, b=1.0):
+        """A walk request: `steps` steps of the model's plain Walk from
+        `start` with `seed` (rectangle-triangle: the reference kernel's
+        weights); returns its id."""
+        start = _i32(start)
+        q = Request(model, side(start), start.ctypes.data, None, seed, SCHED_DOUBLING, 1024, 1,
+                    DominoRule(rule, a, b), REQ_WALK, steps)
+        return self._submit(q, start)
+
+    def _submit(self, q, tmax):
         rid = C.c_uint64()
         rc = self._L.rt_server_submit(self._h, C.byref(q), C.byref(rid))
         if rc != RT_OK:

@@ -954,6 +954,8 @@ int fused_launch_windows(rt_ctx* ctx, int model, int nc, int shape, const ModelP
   return RT_OK;
 }
 
+int fused_shape_halo(int shape) { return shape_geo(shape).H; }
+
 void fused_tiles(int shape, int rows, int cols, int* tiles_r, int* tiles_c) {
   const Geo g = shape_geo(shape);
   UnitDesc u{};

@@ -212,6 +212,8 @@ int fused_launch_windows(rt_ctx* ctx, int model, int nc, int shape, const ModelP
                          const WindowRec* d_wins);
 // Tile counts of a unit under a tile shape's geometry.
 void fused_tiles(int shape, int rows, int cols, int* tiles_r, int* tiles_c);
+// Halo depth of a tile shape (the most pulling steps one window may take).
+int fused_shape_halo(int shape);
 // Bring every colour plane of every unit up to date (end of a walk).
 int fused_sync(rt_ctx* ctx, const FusedJob& job);
 

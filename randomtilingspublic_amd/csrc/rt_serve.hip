@@ -1,6 +1,7 @@
-// rt_serve.cpp -- continuous-batching CFTP server.
+// rt_serve.cpp -- continuous-batching CFTP (and walk) server.
 //
-// Requests (domino or lozenge CFTP samples of any size) arrive at any time.
+// Requests (domino or lozenge CFTP samples of any size, and fixed-length
+// walks of any model, rectangle-triangle included) arrive at any time.
 // A device loop thread admits them by HBM footprint into slots of one
 // pre-allocated arena, keeps every slot's chain state resident, and advances
 // ALL active slots together: one fused launch per model per tick, each slot
@@ -9,7 +10,11 @@
 // test overlaps the next tick's sweep); it then either returns its sample
 // or starts its next round while the other slots keep running.  A result
 // depends only on its request (RNG streams are keyed by the request's seeds
-// and site indices), so it is the same alone and in any mix.
+// and site indices), so it is the same alone and in any mix.  A walk
+// request is one round of one chain whose op stream is the model's plain
+// Walk schedule; it ends after that round, with no coalescence test.
+// Launches are grouped by (model, chains): CFTP pairs of domino and
+// lozenge, one-chain walks of domino, lozenge and rectangle-triangle.
 //
 // Synchronisation: per-tick host tables are double-buffered in pinned
 // memory; tick t waits for the event of tick t-2 before reusing its buffer.
@@ -48,6 +53,8 @@ size_t round_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 struct Req {
   uint64_t id = 0;
   int model = M_LOZENGE;
+  bool walk = false;     // a fixed-length walk of tmax, not a CFTP sample
+  long walk_steps = 0;
   int N = 0;
   std::vector<int32_t> tmax, tmin;
   uint32_t master = 0;
@@ -75,6 +82,8 @@ struct Slot {
   uint32_t* state_b = nullptr;
   int par = 0;         // buffer set holding the current cells and states
   int ncol = 0;
+  int cb = 1;          // bytes per cell (int32 for rectangle-triangle)
+  int group = 0;       // launch group (kGroups)
   long plane = 0, cols = 0, nstream = 0;
   std::mt19937 master;
   std::vector<long> steps, seeds;             // newest first
@@ -125,6 +134,15 @@ struct Arena {
 };
 
 constexpr int kMaxSlots = 256;
+// Launch groups: CFTP pairs (domino, lozenge), one-chain walks (domino,
+// lozenge, rectangle-triangle).
+constexpr int kGroups = 5;
+int group_of(int model, bool walk) {
+  if (!walk) return model == M_DOMINO ? 0 : 1;
+  return model == M_DOMINO ? 2 : (model == M_LOZENGE ? 3 : 4);
+}
+int group_model(int g) { return g == 0 || g == 2 ? M_DOMINO : (g == 4 ? M_RECTTRI : M_LOZENGE); }
+int group_nc(int g) { return g < 2 ? 2 : 1; }
 constexpr int SH_SERVE_COUNT = 1;  // tile shape whose tile count picks the launch shape (64 rows)
 constexpr int kOpsPerUnit = 64;  // = the fused kernel's window limit
 // Windows per tick: each tick launches kSub sweep kernels back to back from
@@ -132,15 +150,18 @@ constexpr int kOpsPerUnit = 64;  // = the fused kernel's window limit
 // flag reads) is spread over kSub windows of every live request.
 constexpr int kSub = 4;
 
-// Offset of unit u's op bytes for (buffer b, sub-window k, model m).
-static inline size_t ops_at(int b, int k, int m, size_t u) {
-  return ((((size_t)b * kSub + k) * 2 + m) * kMaxSlots + u) * kOpsPerUnit;
+// Offset of unit u's op bytes for (buffer b, sub-window k, group g).
+static inline size_t ops_at(int b, int k, int g, size_t u) {
+  return ((((size_t)b * kSub + k) * kGroups + g) * kMaxSlots + u) * kOpsPerUnit;
 }
-static inline size_t wins_at(int b, int k, int m) {
-  return (((size_t)b * kSub + k) * 2 + m) * kMaxSlots;
+static inline size_t wins_at(int b, int k, int g) {
+  return (((size_t)b * kSub + k) * kGroups + g) * kMaxSlots;
 }
 
-size_t footprint(int model, int N, int* ncol, long* plane, long* nstream, long* cols) {
+// Device bytes of a slot: two TinyMT state sets and, per chain (two for
+// CFTP, one for a walk), the start planes and two working sets.
+size_t footprint(int model, int N, int chains, int* ncol, long* plane, long* nstream,
+                 long* cols) {
   if (model == M_DOMINO) {
     *ncol = 2;
     *cols = N / 2;
@@ -151,8 +172,19 @@ size_t footprint(int model, int N, int* ncol, long* plane, long* nstream, long* 
     *nstream = (long)N * (N / 3);
   }
   *plane = (long)N * *cols;
-  const size_t pb = round_up((size_t)*ncol * (size_t)*plane, 256);
-  return 2 * round_up(16 * (size_t)*nstream, 256) + 6 * pb;
+  const size_t cb = model == M_RECTTRI ? 4 : 1;
+  const size_t pb = round_up((size_t)*ncol * (size_t)*plane * cb, 256);
+  return 2 * round_up(16 * (size_t)*nstream, 256) + 3 * (size_t)chains * pb;
+}
+
+// Index of tiling cell (i, j) in the model's colour planes: domino black /
+// white by (i+j) % 2 (DominoTiler.cpp:24-33), lozenge by (j-i) mod 3
+// (LozengeTiler.cpp:38-48), rectangle-triangle by (i-j) mod 3
+// (RectTriangleTiler.cpp:40-50).
+size_t plane_index(int model, int i, int j, long plane, long cols) {
+  if (model == M_DOMINO) return (size_t)((i + j) % 2) * plane + (size_t)i * cols + j / 2;
+  const int c = model == M_LOZENGE ? ((j - i) % 3 + 3) % 3 : ((i - j) % 3 + 3) % 3;
+  return (size_t)c * plane + (size_t)i * cols + j / 3;
 }
 
 }  // namespace
@@ -179,14 +211,14 @@ struct Server {
   std::vector<Slot*> active;
   std::vector<int> free_ids;
   bool dirty = true;
-  UnitDesc* d_units[2] = {nullptr, nullptr};  // [model]
-  WorkItem* d_work[2] = {nullptr, nullptr};
-  size_t work_cap[2] = {0, 0};
-  unsigned nwork[2] = {0, 0};
-  int shape[2] = {0, 0};            // tile shape of each model's launches
-  std::vector<Slot*> unit_slot[2];  // unit index -> slot, per model
-  WindowRec* d_wins = nullptr;      // [buf][sub][model][kMaxSlots]
-  uint8_t* d_ops = nullptr;         // [buf][sub][model][kMaxSlots][kOpsPerUnit]
+  UnitDesc* d_units[kGroups] = {};  // [group]
+  WorkItem* d_work[kGroups] = {};
+  size_t work_cap[kGroups] = {};
+  unsigned nwork[kGroups] = {};
+  int shape[kGroups] = {};                // tile shape of each group's launches
+  std::vector<Slot*> unit_slot[kGroups];  // unit index -> slot, per group
+  WindowRec* d_wins = nullptr;            // [buf][sub][group][kMaxSlots]
+  uint8_t* d_ops = nullptr;               // [buf][sub][group][kMaxSlots][kOpsPerUnit]
   uint32_t* d_flags = nullptr;      // [buf][kMaxSlots]
   WindowRec* h_wins = nullptr;
   uint8_t* h_ops = nullptr;
@@ -237,7 +269,8 @@ static void build_round(Slot& s) {
 static int admit(Server* S, Req* r) {
   int ncol;
   long plane, nstream, cols;
-  const size_t bytes = footprint(r->model, r->N, &ncol, &plane, &nstream, &cols);
+  const int chains = r->walk ? 1 : 2;
+  const size_t bytes = footprint(r->model, r->N, chains, &ncol, &plane, &nstream, &cols);
   size_t off;
   if (S->free_ids.empty() || !S->arena.alloc(bytes, &off)) return RT_EAGAIN;
   Slot* s = new Slot();
@@ -247,6 +280,8 @@ static int admit(Server* S, Req* r) {
   s->off = off;
   s->bytes = bytes;
   s->ncol = ncol;
+  s->cb = r->model == M_RECTTRI ? 4 : 1;
+  s->group = group_of(r->model, r->walk);
   s->plane = plane;
   s->cols = cols;
   s->nstream = nstream;
@@ -255,34 +290,52 @@ static int admit(Server* S, Req* r) {
   base += round_up(16 * (size_t)nstream, 256);
   s->state_b = (uint32_t*)base;
   base += round_up(16 * (size_t)nstream, 256);
-  const size_t pb = round_up((size_t)ncol * (size_t)plane, 256);
-  s->init_top = base;
-  s->init_bot = base + pb;
-  s->top = base + 2 * pb;
-  s->bot = base + 3 * pb;
-  s->top_b = base + 4 * pb;
-  s->bot_b = base + 5 * pb;
-  std::vector<uint8_t> h((size_t)ncol * (size_t)plane);
+  const size_t pb = round_up((size_t)ncol * (size_t)plane * s->cb, 256);
+  auto take = [&]() {
+    uint8_t* q = base;
+    base += pb;
+    return q;
+  };
+  s->init_top = take();
+  if (!r->walk) s->init_bot = take();
+  s->top = take();
+  if (!r->walk) s->bot = take();
+  s->top_b = take();
+  if (!r->walk) s->bot_b = take();
+  std::vector<uint8_t> h((size_t)ncol * (size_t)plane * s->cb);
   const int N = r->N;
-  for (int which = 0; which < 2; which++) {
+  for (int which = 0; which < chains; which++) {
     const std::vector<int32_t>& t = which ? r->tmin : r->tmax;
     std::fill(h.begin(), h.end(), 0);
     for (int i = 0; i < N; i++)
       for (int j = 0; j < N; j++) {
-        const size_t idx = r->model == M_DOMINO
-                               ? (size_t)((i + j) % 2) * plane + (size_t)i * cols + j / 2
-                               : (size_t)(((j - i) % 3 + 3) % 3) * plane + (size_t)i * cols + j / 3;
-        h[idx] = (uint8_t)t[(size_t)i * N + j];
+        const size_t idx = plane_index(r->model, i, j, plane, cols);
+        if (s->cb == 4) std::memcpy(&h[idx * 4], &t[(size_t)i * N + j], 4);
+        else h[idx] = (uint8_t)t[(size_t)i * N + j];
       }
     if (hipMemcpy(which ? s->init_bot : s->init_top, h.data(), h.size(), hipMemcpyHostToDevice) !=
         hipSuccess)
       return fail(S, "upload of start tilings failed", RT_EDEVICE);
   }
-  s->master.seed(r->master);
-  s->steps = {r->t0};
-  s->seeds = {(long)s->master()};
-  gen_colours(*s, s->seeds[0], r->t0);
   s->round = 1;
+  if (r->walk) {
+    // the model's plain Walk: domino seeds TinyMT with the first mt19937
+    // output and draws its colours after it (DominoTiler.cpp:51-70);
+    // lozenge and rectangle-triangle seed TinyMT with the seed itself
+    // (LozengeTiler.cpp:86-91, RectTriangleTiler.cpp:69-71)
+    std::vector<uint8_t> c;
+    uint32_t tseed = r->master;
+    if (r->model == M_DOMINO) domino_schedule(r->master, r->walk_steps, true, &tseed, c);
+    else tri_schedule((long)r->master, r->walk_steps, c);
+    s->steps = {r->walk_steps};
+    s->seeds = {(long)tseed};
+    s->colours.push_back(std::move(c));
+  } else {
+    s->master.seed(r->master);
+    s->steps = {r->t0};
+    s->seeds = {(long)s->master()};
+    gen_colours(*s, s->seeds[0], r->t0);
+  }
   build_round(*s);
   r->t_start = now_s();
   r->active = true;
@@ -317,28 +370,31 @@ static void retire(Server* S, Slot* s, int status) {
 
 static int upload_tables(Server* S) {
   hipStreamSynchronize(S->ctx->stream);
-  for (int m = 0; m < 2; m++) {
-    const int model = m == 0 ? M_DOMINO : M_LOZENGE;
+  for (int m = 0; m < kGroups; m++) {
+    const int model = group_model(m), nc = group_nc(m);
     std::vector<UnitDesc> units;
     std::vector<WorkItem> work;
     S->unit_slot[m].clear();
     // tile shape from the live tile count (64-row tiles)
     long t64 = 0;
     for (Slot* s : S->active) {
-      if (s->req->model != model) continue;
+      if (s->group != m) continue;
       int tr, tc;
       fused_tiles(SH_SERVE_COUNT, s->req->N, (int)s->cols, &tr, &tc);
       t64 += (long)tr * tc;
     }
-    S->shape[m] = fused_serve_shape(S->ctx, model, 2, t64);
+    S->shape[m] = fused_serve_shape(S->ctx, model, nc, t64);
     for (Slot* s : S->active) {
-      if (s->req->model != model) continue;
+      if (s->group != m) continue;
       UnitDesc d{};
+      const size_t pc = (size_t)s->plane * s->cb;  // bytes of one colour plane
       for (int c = 0; c < s->ncol; c++) {
-        d.plane[0][c] = s->top + (size_t)c * s->plane;
-        d.plane[1][c] = s->bot + (size_t)c * s->plane;
-        d.plane_b[0][c] = s->top_b + (size_t)c * s->plane;
-        d.plane_b[1][c] = s->bot_b + (size_t)c * s->plane;
+        d.plane[0][c] = s->top + c * pc;
+        d.plane_b[0][c] = s->top_b + c * pc;
+        if (nc == 2) {
+          d.plane[1][c] = s->bot + c * pc;
+          d.plane_b[1][c] = s->bot_b + c * pc;
+        }
       }
       d.state = s->state;
       d.state_b = s->state_b;
@@ -380,18 +436,18 @@ static int upload_tables(Server* S) {
 // Host merge of a finished slot's top planes into the request's tiling.
 static int collect(Server* S, Slot* s) {
   Req* r = s->req;
-  std::vector<uint8_t> h((size_t)s->ncol * (size_t)s->plane);
+  std::vector<uint8_t> h((size_t)s->ncol * (size_t)s->plane * s->cb);
   if (hipMemcpy(h.data(), s->top, h.size(), hipMemcpyDeviceToHost) != hipSuccess)
     return fail(S, "download of a sample failed", RT_EDEVICE);
   const int N = r->N;
   r->out.assign((size_t)N * N, 0);
   for (int i = 0; i < N; i++)
     for (int j = 0; j < N; j++) {
-      const size_t idx = r->model == M_DOMINO
-                             ? (size_t)((i + j) % 2) * s->plane + (size_t)i * s->cols + j / 2
-                             : (size_t)(((j - i) % 3 + 3) % 3) * s->plane + (size_t)i * s->cols +
-                                   j / 3;
-      r->out[(size_t)i * N + j] = (int8_t)h[idx];
+      const size_t idx = plane_index(r->model, i, j, s->plane, s->cols);
+      int32_t v;
+      if (s->cb == 4) std::memcpy(&v, &h[idx * 4], 4);
+      else v = (int8_t)h[idx];
+      r->out[(size_t)i * N + j] = v;
     }
   r->st.rounds = s->round;
   r->st.total_steps = s->total;
@@ -404,7 +460,8 @@ static int process_flags(Server* S, int b) {
   std::vector<Slot*> done;
   for (Slot* s : std::vector<Slot*>(S->active)) {
     if (s->phase != 1 || (s->flag_tick & 1) != b || s->flag_tick > S->tick - 2) continue;
-    const uint32_t differs = S->h_flags[(size_t)b * kMaxSlots + s->id];
+    // a walk has no coalescence test: its one round is its result
+    const uint32_t differs = s->req->walk ? 0 : S->h_flags[(size_t)b * kMaxSlots + s->id];
     s->run += s->total;
     if (!differs) {
       int rc = collect(S, s);
@@ -447,11 +504,11 @@ static int run_tick(Server* S) {
   // resets of slots starting a round
   for (Slot* s : S->active)
     if (s->phase == 0 && s->fresh) {
-      const size_t pb = (size_t)s->ncol * (size_t)s->plane;
+      const size_t pb = (size_t)s->ncol * (size_t)s->plane * s->cb;
       if (hipMemcpyAsync(s->top, s->init_top, pb, hipMemcpyDeviceToDevice, ctx->stream) !=
               hipSuccess ||
-          hipMemcpyAsync(s->bot, s->init_bot, pb, hipMemcpyDeviceToDevice, ctx->stream) !=
-              hipSuccess)
+          (!s->req->walk && hipMemcpyAsync(s->bot, s->init_bot, pb, hipMemcpyDeviceToDevice,
+                                           ctx->stream) != hipSuccess))
         return fail(S, "plane reset", RT_EDEVICE);
       s->fresh = false;
       s->par = 0;
@@ -459,8 +516,8 @@ static int run_tick(Server* S) {
   // windows: kSub per tick for every live unit
   std::vector<Slot*> ended;
   for (int k = 0; k < kSub; k++) {
-    for (int m = 0; m < 2; m++) {
-      const int H = 6;  // fused halo of byte-cell models (rt_fused.hip tile shapes)
+    for (int m = 0; m < kGroups; m++) {
+      const int H = fused_shape_halo(S->shape[m]);  // pulling steps per window
       for (size_t u = 0; u < S->unit_slot[m].size(); u++) {
         Slot* s = S->unit_slot[m][u];
         WindowRec& w = S->h_wins[wins_at(b, k, m) + u];
@@ -489,7 +546,7 @@ static int run_tick(Server* S) {
       }
     }
   }
-  const size_t wcount = (size_t)kSub * 2 * kMaxSlots;
+  const size_t wcount = (size_t)kSub * kGroups * kMaxSlots;
   const size_t obytes = wcount * kOpsPerUnit;
   if (hipMemcpyAsync(S->d_wins + wins_at(b, 0, 0), S->h_wins + wins_at(b, 0, 0),
                      wcount * sizeof(WindowRec), hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
@@ -497,42 +554,48 @@ static int run_tick(Server* S) {
                      hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
     return fail(S, "tick upload", RT_EDEVICE);
   for (int k = 0; k < kSub; k++)
-    for (int m = 0; m < 2; m++) {
+    for (int m = 0; m < kGroups; m++) {
       if (S->unit_slot[m].empty()) continue;
-      const int model = m == 0 ? M_DOMINO : M_LOZENGE;
+      // per-unit rule and weights live in the unit table; the launch
+      // parameters carry the rectangle-triangle weights (the reference
+      // kernel's table for every served walk)
       ModelParams mp = S->unit_slot[m][0]->req->mp;
-      int rc = fused_launch_windows(ctx, model, 2, S->shape[m], mp, S->d_units[m], S->d_work[m],
-                                    S->nwork[m],
+      int rc = fused_launch_windows(ctx, group_model(m), group_nc(m), S->shape[m], mp,
+                                    S->d_units[m], S->d_work[m], S->nwork[m],
                                     S->d_wins + wins_at(b, k, m));
       if (rc) return fail(S, ctx->err, rc);
     }
   // slots whose round ended: cells back into the first buffer set, lazy
-  // sync of their planes, then coalescence flag
+  // sync of their planes, then (CFTP) coalescence flag
   for (Slot* s : ended) {
+    const bool pair = !s->req->walk;
+    const size_t pb = (size_t)s->ncol * (size_t)s->plane * s->cb;
     if (s->par) {
-      const size_t pb = (size_t)s->ncol * (size_t)s->plane;
       if (hipMemcpyAsync(s->top, s->top_b, pb, hipMemcpyDeviceToDevice, ctx->stream) !=
               hipSuccess ||
-          hipMemcpyAsync(s->bot, s->bot_b, pb, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess)
+          (pair && hipMemcpyAsync(s->bot, s->bot_b, pb, hipMemcpyDeviceToDevice, ctx->stream) !=
+                       hipSuccess))
         return fail(S, "plane copy-back", RT_EDEVICE);
       s->par = 0;
     }
     FusedJob one;
     one.model = s->req->model;
-    one.nc = 2;
+    one.nc = pair ? 2 : 1;
     UnitDesc d{};
     for (int c = 0; c < s->ncol; c++) {
-      d.plane[0][c] = s->top + (size_t)c * s->plane;
-      d.plane[1][c] = s->bot + (size_t)c * s->plane;
+      d.plane[0][c] = s->top + (size_t)c * s->plane * s->cb;
+      if (pair) d.plane[1][c] = s->bot + (size_t)c * s->plane * s->cb;
     }
     d.rows = s->req->N;
     one.units.push_back(d);
     one.hist.push_back(s->hist);
     int rc = fused_sync(ctx, one);
     if (rc) return fail(S, ctx->err, rc);
-    rc = launch_differ_pair(ctx, s->top, s->bot, (long)s->ncol * s->plane,
-                            S->d_flags + (size_t)b * kMaxSlots + s->id);
-    if (rc) return fail(S, ctx->err, rc);
+    if (pair) {
+      rc = launch_differ_pair(ctx, s->top, s->bot, (long)pb,
+                              S->d_flags + (size_t)b * kMaxSlots + s->id);
+      if (rc) return fail(S, ctx->err, rc);
+    }
     s->phase = 1;
     s->flag_tick = S->tick;
   }
@@ -615,13 +678,14 @@ extern "C" int rt_server_create(rt_ctx* ctx, size_t hbm_budget, int max_active, 
   S->max_active = std::min(max_active, kMaxSlots);
   hipSetDevice(ctx->device);
   bool ok = hipMalloc(&S->d_arena, hbm_budget) == hipSuccess;
-  for (int m = 0; m < 2 && ok; m++)
+  for (int m = 0; m < kGroups && ok; m++)
     ok = hipMalloc(&S->d_units[m], kMaxSlots * sizeof(UnitDesc)) == hipSuccess;
-  ok = ok && hipMalloc(&S->d_wins, 2 * kSub * 2 * kMaxSlots * sizeof(WindowRec)) == hipSuccess;
-  ok = ok && hipMalloc(&S->d_ops, 2 * kSub * 2 * (size_t)kMaxSlots * kOpsPerUnit) == hipSuccess;
+  const size_t nwin = 2 * kSub * kGroups * (size_t)kMaxSlots;  // [buf][sub][group][slot]
+  ok = ok && hipMalloc(&S->d_wins, nwin * sizeof(WindowRec)) == hipSuccess;
+  ok = ok && hipMalloc(&S->d_ops, nwin * kOpsPerUnit) == hipSuccess;
   ok = ok && hipMalloc(&S->d_flags, 2 * kMaxSlots * sizeof(uint32_t)) == hipSuccess;
-  ok = ok && hipHostMalloc(&S->h_wins, 2 * kSub * 2 * kMaxSlots * sizeof(WindowRec)) == hipSuccess;
-  ok = ok && hipHostMalloc(&S->h_ops, 2 * kSub * 2 * (size_t)kMaxSlots * kOpsPerUnit) == hipSuccess;
+  ok = ok && hipHostMalloc(&S->h_wins, nwin * sizeof(WindowRec)) == hipSuccess;
+  ok = ok && hipHostMalloc(&S->h_ops, nwin * kOpsPerUnit) == hipSuccess;
   ok = ok && hipHostMalloc(&S->h_flags, 2 * kMaxSlots * sizeof(uint32_t)) == hipSuccess;
   ok = ok && hipEventCreate(&S->ev[0]) == hipSuccess && hipEventCreate(&S->ev[1]) == hipSuccess;
   if (!ok) {
@@ -650,7 +714,7 @@ extern "C" void rt_server_destroy(rt_server* sv) {
   for (Slot* s : S->active) delete s;
   for (auto& kv : S->reqs) delete kv.second;
   if (S->d_arena) hipFree(S->d_arena);
-  for (int m = 0; m < 2; m++) {
+  for (int m = 0; m < kGroups; m++) {
     if (S->d_units[m]) hipFree(S->d_units[m]);
     if (S->d_work[m]) hipFree(S->d_work[m]);
   }
@@ -666,38 +730,53 @@ extern "C" void rt_server_destroy(rt_server* sv) {
 }
 
 extern "C" int rt_server_submit(rt_server* sv, const rt_request* q, uint64_t* id) {
-  if (!sv || !q || !id || !q->tmax || !q->tmin) return RT_EINVAL;
+  if (!sv || !q || !id || !q->tmax) return RT_EINVAL;
   Server* S = &sv->s;
-  const int model = q->model == RT_MODEL_DOMINO ? M_DOMINO
-                    : q->model == RT_MODEL_LOZENGE ? M_LOZENGE : -1;
-  if (model < 0) return RT_EINVAL;  // CFTP is served for the monotone models only
+  const bool walk = q->kind == RT_REQ_WALK;
+  if (!walk && q->kind != RT_REQ_CFTP) return RT_EINVAL;
+  const int model = q->model == RT_MODEL_DOMINO    ? M_DOMINO
+                    : q->model == RT_MODEL_LOZENGE ? M_LOZENGE
+                    : q->model == RT_MODEL_RECTTRI ? M_RECTTRI
+                                                   : -1;
+  // CFTP is served for the monotone models only (no rectangle-triangle)
+  if (model < 0 || (!walk && model == M_RECTTRI)) return RT_EINVAL;
   const int N = q->N;
   if (model == M_DOMINO ? (N < 6 || N % 2) : (N < 9 || N % 3)) return RT_EINVAL;
-  if (q->t0 <= 0 || q->max_rounds <= 0 ||
-      (q->sched != RT_SCHED_REFERENCE && q->sched != RT_SCHED_DOUBLING))
-    return RT_EINVAL;
-  if (model == M_DOMINO && !domino_cftp_rule(&q->rule)) return RT_EINVAL;
+  if (walk) {
+    if (q->steps <= 0) return RT_EINVAL;
+    if (model == M_DOMINO && (q->rule.kind < RT_DOMINO_TOGGLE08 ||
+                              q->rule.kind > RT_DOMINO_HEATBATH2P))
+      return RT_EINVAL;
+  } else {
+    if (!q->tmin || q->t0 <= 0 || q->max_rounds <= 0 ||
+        (q->sched != RT_SCHED_REFERENCE && q->sched != RT_SCHED_DOUBLING))
+      return RT_EINVAL;
+    if (model == M_DOMINO && !domino_cftp_rule(&q->rule)) return RT_EINVAL;
+  }
   int ncol;
   long plane, nstream, cols;
-  const size_t bytes = footprint(model, N, &ncol, &plane, &nstream, &cols);
+  const size_t bytes = footprint(model, N, walk ? 1 : 2, &ncol, &plane, &nstream, &cols);
   if (bytes > S->budget) return RT_EADMIT;
   const size_t need_params = model == M_DOMINO ? (size_t)N * N / 2 : (size_t)N * (N / 3);
   if (S->ctx->nparam < need_params) return RT_EPARAMS;
   Req* r = new Req();
   r->model = model;
+  r->walk = walk;
+  r->walk_steps = walk ? q->steps : 0;
   r->N = N;
   r->tmax.assign(q->tmax, q->tmax + (size_t)N * N);
-  r->tmin.assign(q->tmin, q->tmin + (size_t)N * N);
+  if (!walk) r->tmin.assign(q->tmin, q->tmin + (size_t)N * N);
   r->master = q->master_seed;
   r->sched = q->sched;
   r->t0 = q->t0;
-  r->max_rounds = q->max_rounds;
+  r->max_rounds = walk ? 1 : q->max_rounds;
   r->bytes = bytes;
   if (model == M_DOMINO) {
     r->mp.rule = q->rule.kind;
     r->mp.dw.wa2 = (float)(q->rule.a * q->rule.a);
     r->mp.dw.wb2 = (float)(q->rule.b * q->rule.b);
   }
+  if (model == M_RECTTRI) r->mp.rw = recttri_weights(nullptr);
   {
     std::lock_guard<std::mutex> g(S->mu);
     if (S->loop_rc != RT_OK) {

@@ -156,3 +156,64 @@ def test_submitters_on_two_threads(gpu_ctx):
         assert s["completed"] == len(reqs) and s["queued"] == 0 and s["status"] == 0
     finally:
         srv.close()
+
+
+def test_walk_requests_in_the_mix(gpu_ctx):
+    """Walk requests (RT_REQ_WALK) of all three models, rectangle-triangle
+    included, served together with CFTP requests: each equals the model's
+    plain Walk of the same start, steps and seed (rt_*_walk), and the small
+    rectangle-triangle ones the oracle."""
+    cftp = _mix()[:5]
+    R = rt.RectTriangleTiler
+    walks = [
+        dict(model=rt.MODEL_DOMINO, start=_domino(12)[1], steps=700, seed=5,
+             rule=rt.DOMINO_TOGGLE08),
+        dict(model=rt.MODEL_DOMINO, start=_domino(9)[0], steps=333, seed=6,
+             rule=rt.DOMINO_METRO2P, a=1.0, b=0.5),
+        dict(model=rt.MODEL_LOZENGE, start=_lozenge(6, 1)[0], steps=501, seed=7),
+        dict(model=rt.MODEL_RECTTRI, start=R.maxHex(3), steps=800, seed=8),
+        dict(model=rt.MODEL_RECTTRI, start=R.minHex(5), steps=301, seed=9),
+        dict(model=rt.MODEL_RECTTRI, start=R.maxHex(12), steps=1500, seed=10),
+    ]
+    big = max([rt.side(q["tmax"]) for q in cftp] + [rt.side(w["start"]) for w in walks])
+    P = rt.synthetic_params(big * big, 34)
+    gpu_ctx.set_tinymt_params(P)
+    want_c = [_alone(gpu_ctx, q)[0] for q in cftp]
+    want_w = []
+    for w in walks:
+        if w["model"] == rt.MODEL_DOMINO:
+            want_w.append(gpu_ctx.domino_walk(w["start"], w["steps"], w["seed"], w["rule"],
+                                              w.get("a", 1.0), w.get("b", 1.0)))
+        elif w["model"] == rt.MODEL_LOZENGE:
+            want_w.append(gpu_ctx.lozenge_walk(w["start"], w["steps"], w["seed"]))
+        else:
+            want_w.append(gpu_ctx.recttri_walk(w["start"], w["steps"], w["seed"]))
+    _say("server: walks of three models among CFTP requests")
+    srv = rt.Server(gpu_ctx, 256 << 20, 64)
+    try:
+        with pytest.raises(rt.RTError):  # rectangle-triangle has no CFTP
+            srv.submit(rt.MODEL_RECTTRI, R.maxHex(3), R.minHex(3), 1)
+        with pytest.raises(rt.RTError):
+            srv.submit_walk(rt.MODEL_LOZENGE, walks[2]["start"], 0, 1)
+        ids = []
+        for k in range(max(len(cftp), len(walks))):
+            if k < len(walks):
+                w = walks[k]
+                ids.append(("w", k, srv.submit_walk(w["model"], w["start"], w["steps"], w["seed"],
+                                                    w.get("rule", rt.DOMINO_TOGGLE08),
+                                                    w.get("a", 1.0), w.get("b", 1.0))))
+            if k < len(cftp):
+                ids.append(("c", k, srv.submit(max_rounds=32, **cftp[k])))
+            time.sleep(0.005)
+        for kind, k, rid in ids:
+            assert srv.wait(rid, 120.0)
+            rc, got, st, _ = srv.poll(rid)
+            assert rc == 0, (kind, k)
+            want = want_w[k] if kind == "w" else want_c[k]
+            assert np.array_equal(got, want), (kind, k)
+        assert srv.stats()["completed"] == len(ids)
+    finally:
+        srv.close()
+    for k in (3, 4):
+        w = walks[k]
+        assert np.array_equal(want_w[k], o.recttri_walk(w["start"], w["steps"], w["seed"], P)), k
