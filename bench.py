@@ -264,6 +264,42 @@ def weighted_and_bibone(rt, cftp_seeds=8, cftp_order=24, steps=10 ** 5, rounds51
     return out
 
 
+def two_periodic_scan(rt, orders=(16, 20, 24, 28, 32), seeds=8, max_rounds=15):
+    """Coalescence of the two-periodic heat-bath CFTP (a = 1.0, b = 0.5,
+    dominokernelCFTPtwoperiodic.cl) against the Aztec order: master seeds
+    1..seeds batched, doubling from 2^10, at most max_rounds rounds
+    (look-back up to 2^(9 + max_rounds)); per order the rounds each seed
+    needed (max_rounds + 1: not within the cap) and the device time."""
+    D = rt.DominoTiler
+    out = {"workload": f"two-periodic heat-bath CFTP (a=1.0, b=0.5), Aztec orders {list(orders)}, "
+                       f"master seeds 1..{seeds}, doubling from 2^10, at most {max_rounds} "
+                       "rounds", "orders": {}}
+    ctx = rt.Context(0)
+    ctx.set_round_callback(lambda r, n, fin, tot: print(
+        f"[bench] two-periodic round {r} ({n} steps): {fin} of {tot} coalesced", file=sys.stderr,
+        flush=True))
+    try:
+        for n in orders:
+            d = D.AztecDiamond(n)
+            tmin, tmax = D.MinTiling(d), D.MaxTiling(d)
+            N = rt.side(tmin)
+            ctx.set_tinymt_params(rt.synthetic_params(N * N // 2, None))
+            masters = list(range(1, seeds + 1))
+            t1 = time.time()
+            _, stats = ctx.domino_cftp_batch(tmax, tmin, masters, rt.SCHED_DOUBLING, 1024,
+                                             max_rounds, rt.DOMINO_HEATBATH2P, 1.0, 0.5,
+                                             partial=True)
+            rounds = [st["rounds"] if st["total_steps"] else max_rounds + 1 for st in stats]
+            wall = time.time() - t1
+            out["orders"][n] = {"rounds": rounds, "wall_s": round(wall, 2),
+                                "median_lookback_log2": float(np.median(rounds)) + 9}
+            print(f"[bench] two-periodic order {n}: rounds {rounds} in {wall:.1f} s",
+                  file=sys.stderr, flush=True)
+    finally:
+        ctx.close()
+    return out
+
+
 def bibone_walk(rt, ctx, steps=10 ** 5):
     """The bibone walk on the hexagon with sides 200, 201, 200 (tiling side
     812, 406 x 406 plane cells), `steps` steps from the hexagon's tiling, on
@@ -952,7 +988,8 @@ def main():
     ap.add_argument("--profile-rounds", type=int, default=0,
                     help="profiling run: the same batch capped at R doubling rounds (no sample "
                          "needs to finish); prints the sweep kernel's timing only")
-    ap.add_argument("--profile-section", choices=["sweep", "walks", "c3", "bibone", "served"],
+    ap.add_argument("--profile-section", choices=["sweep", "walks", "c3", "c3scan", "bibone",
+                                                  "served"],
                     default=None,
                     help="profiling run: only one section (plain Aztec sweep, 256-chain batch "
                          "walks, weighted walk / bibone, served stream), printed as one JSON "
@@ -1027,6 +1064,7 @@ def main():
     if args.profile_section:
         fn = {"sweep": lambda: plain_sweep(rt), "walks": lambda: batched_walks(rt),
               "c3": lambda: weighted_and_bibone(rt), "bibone": lambda: bibone_section(rt, args.tile_shape, args.bibone_steps),
+              "c3scan": lambda: two_periodic_scan(rt),
               "served": serve}[args.profile_section]
         print(json.dumps({"profile_section": args.profile_section, **fn()}), flush=True)
         return

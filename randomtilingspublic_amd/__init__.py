@@ -380,7 +380,7 @@ class Context:
                      "steps_run": st.steps_run, "device_ms": st.device_ms}
 
     # ---- batching ----
-    def _cftp_batch(self, fn, tmax, tmin, masters, sched, t0, max_rounds, *rule):
+    def _cftp_batch(self, fn, tmax, tmin, masters, sched, t0, max_rounds, *rule, partial=False):
         tmax, tmin = _i32(tmax), _i32(tmin)
         masters = np.ascontiguousarray(masters, dtype=np.uint32)
         n = len(masters)
@@ -392,7 +392,8 @@ class Context:
         rc = fn(*args)
         stats = [{"rounds": s.rounds, "total_steps": s.total_steps, "steps_run": s.steps_run,
                   "device_ms": s.device_ms} for s in st]
-        self._chk(rc)
+        if not (partial and rc == -4):  # partial: keep the stats of an ENOCOUPLE batch
+            self._chk(rc)
         return out.reshape(n, len(tmax)), stats
 
     def lozenge_cftp_batch(self, tmax, tmin, masters, sched=SCHED_DOUBLING, t0=1024,
@@ -429,10 +430,12 @@ class Context:
                                  sched, t0, rounds, C.byref(r))
 
     def domino_cftp_batch(self, tmax, tmin, masters, sched=SCHED_DOUBLING, t0=128,
-                          max_rounds=40, rule=DOMINO_HEATBATH, a=1.0, b=1.0):
+                          max_rounds=40, rule=DOMINO_HEATBATH, a=1.0, b=1.0, partial=False):
+        """partial=True: when some samples do not coalesce within max_rounds,
+        return anyway; those samples have stats total_steps == 0."""
         r = DominoRule(rule, a, b)
         return self._cftp_batch(self._L.rt_domino_cftp_batch, tmax, tmin, masters, sched, t0,
-                                max_rounds, C.byref(r))
+                                max_rounds, C.byref(r), partial=partial)
 
     def domino_walk_batch(self, tilings, steps, seeds, rule=DOMINO_TOGGLE08, a=1.0, b=1.0,
                           use_graph=False):
