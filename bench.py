@@ -33,6 +33,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+XCD_ORDER = None  # --xcd-order: rt_set_xcd_order of the measured contexts (None: the library's)
 # Sweep-block kernel instantiation per launch-shape slot (rt_timing.shape_*[k]):
 # k_fused<model 1 = lozenge, chains per workgroup, waves, rows per lane>; slots
 # 4..6 run CFTP pairs split, one chain per workgroup; slot 7 is the 48-row
@@ -154,6 +155,8 @@ def batched_walks(rt, seed=4, nchains=256, steps=10 ** 4):
     big = max(b["N"] for b in batch)
     sites = sum((b["N"] - 2) * (b["N"] // 2 - 2) for b in batch)
     ctx = rt.Context(0)
+    if XCD_ORDER is not None:
+        ctx.set_xcd_order(XCD_ORDER)
     try:
         ctx.set_tinymt_params(rt.synthetic_params(big * big // 2, None))
         seeds = [1 + k for k in range(nchains)]
@@ -1012,6 +1015,9 @@ def main():
                     help="served stream: requests of the burst that measures capacity")
     ap.add_argument("--bibone-steps", type=int, default=10 ** 5,
                     help="bibone profiling section: walk steps")
+    ap.add_argument("--xcd-order", type=int, default=None, choices=[0, 1, 2],
+                    help="rt_set_xcd_order for the headline batch and the batch walks "
+                         "(default: the library's)")
     ap.add_argument("--halo", type=int, default=6,
                     help="sweep-block halo depth (rt_set_halo; results never depend on it)")
     ap.add_argument("--tile-shape", type=int, default=0,
@@ -1039,6 +1045,8 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
+    global XCD_ORDER
+    XCD_ORDER = args.xcd_order
     if args.rocprof_profile:
         rocprof_profile(*args.rocprof_profile, summarize_only=args.summarize_only)
         return
@@ -1080,6 +1088,8 @@ def main():
     ctx.set_tinymt_params(params)
     ctx.set_tile_shape(args.tile_shape)
     ctx.set_halo(args.halo)
+    if XCD_ORDER is not None:
+        ctx.set_xcd_order(XCD_ORDER)
     print(f"[bench] device {ctx.device_name} hexagon {side} N={N} sites/step={P}", file=sys.stderr,
           flush=True)
     for w in range(args.warmup):

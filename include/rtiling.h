@@ -118,11 +118,12 @@ int rt_set_tile_shape(rt_ctx* ctx, int shape);
  * earlier choices. */
 int rt_set_autotune(rt_ctx* ctx, int mode);
 
-/* Sweep-block launches with no more workgroups than CUs hand each of the
+/* Sweep-block launches with no more workgroups than CUs (and lozenge-pair
+ * launches on tile shape 7 with more than four per CU) hand each of the
  * chip's 8 XCDs (which receive workgroups round robin) a contiguous share of
  * the tile list, so tiles that read each other's halo rows share an L2
- * (on = 1, the default; no reference counterpart; results never depend on
- * it). */
+ * (on = 1, the default; 2: every launch; 0: none; no reference counterpart;
+ * results never depend on it). */
 int rt_set_xcd_order(rt_ctx* ctx, int on);
 
 /* Halo depth H of the sweep-block tiles (no reference counterpart; default

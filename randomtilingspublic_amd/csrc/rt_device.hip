@@ -381,7 +381,7 @@ extern "C" int rt_set_round_callback(rt_ctx* ctx, rt_round_fn fn, void* user) {
 }
 
 extern "C" int rt_set_xcd_order(rt_ctx* ctx, int on) {
-  if (!ctx || on < 0 || on > 1) return RT_EINVAL;
+  if (!ctx || on < 0 || on > 2) return RT_EINVAL;
   ctx->xcd_order = on;
   return RT_OK;
 }

@@ -595,7 +595,7 @@ size_t fused_scratch_bytes(rt_ctx* ctx, FusedJob& job) {
 // from 15.3 to 13.3 us and 4 samples (64-row tiles) from 33.2 to 30.1 us;
 // with more workgroups than CUs it was as often slower (+2..5 %) as faster.
 static int xcd_of(const rt_ctx* ctx, unsigned nwork) {
-  return ctx->xcd_order && nwork <= (unsigned)ctx->num_cu ? 1 : 0;
+  return ctx->xcd_order == 2 || (ctx->xcd_order == 1 && nwork <= (unsigned)ctx->num_cu) ? 1 : 0;
 }
 static unsigned grid_of(int xcd, unsigned nwork) { return xcd ? (nwork + 7u) & ~7u : nwork; }
 
@@ -625,10 +625,19 @@ static void launch_o8(rt_ctx* ctx, unsigned nblocks, const UnitDesc* units, cons
 // Lozenge chain pairs on tile shape 7: both chains packed into one LDS word
 // per cell (k_pair_lozenge) when the halo is a multiple of 3, which that
 // kernel's compile-time row residues need; k_fused_o8<1> otherwise.
+// XCD order also pays for large grids of this kernel, whose tile loads
+// re-read their neighbours' halo: per launch on the side-300 hexagon CFTP
+// (6 rounds), plain -> XCD order, 32 samples (2176 workgroups) 100.9 ->
+// 91.7 us, 24 samples 80.9 -> 74.0, 16 samples 59.0 -> 56.4, but 8
+// samples (544, about one workgroup per slot) 37.0 -> 38.8; so it is on up
+// to the CU count and above four workgroups per CU.
 static void launch_pair_lozenge(rt_ctx* ctx, unsigned nblocks, const UnitDesc* units,
                                 const WorkItem* work, int H, const ModelParams& mp,
                                 const WindowRec* wins) {
-  const int xcd = xcd_of(ctx, nblocks);
+  const int xcd = xcd_of(ctx, nblocks) ||
+                          (ctx->xcd_order == 1 && nblocks > 4u * (unsigned)ctx->num_cu)
+                      ? 1
+                      : 0;
   if (H % 3 == 0)
     hipLaunchKernelGGL(k_pair_lozenge, dim3(grid_of(xcd, nblocks)), dim3(1024), 0, ctx->stream,
                        units, work, (int)nblocks, xcd, H, mp, wins);
