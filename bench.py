@@ -331,6 +331,8 @@ def bibone_section(rt, shape=0, steps=10 ** 5):
     ctx = rt.Context(0)
     try:
         ctx.set_tile_shape(shape)
+        if XCD_ORDER is not None:
+            ctx.set_xcd_order(XCD_ORDER)
         return bibone_walk(rt, ctx, steps)
     finally:
         ctx.close()

@@ -433,7 +433,8 @@ constexpr int kBiMaxK = 16;  // steps per window
 // against 2.74 for 48 x 48 / halo 8 (shape 2), 2.75 for 48 x 48 / halo 12
 // (6, 289 tiles against 169), 3.22 for 64 x 64 / halo 8 (3), 3.31 for
 // 40 x 40 / halo 8 (5), 3.95 for 48 x 48 / halo 12 at 576 threads (7) and
-// 4.39 for 32 x 32 / halo 4 (1).  VALU issue is a third of the chip's
+// 4.39 for 32 x 32 / halo 4 (1), all in plain tile order; the XCD order
+// of the default takes it to 2.28 us.  VALU issue is a third of the chip's
 // SIMD cycles: one 1024-thread workgroup per CU, two barriers per step.
 constexpr int kBiEdgeDefaultShape = 4;
 // Tile geometry of the walk: 48 x 48 loaded cells, halo 8 (32 x 32
@@ -712,12 +713,18 @@ __global__ __launch_bounds__(T) void k_bi_edge(const uint8_t* __restrict__ ein,
                                                uint32_t* __restrict__ sout,
                                                const uint32_t* __restrict__ param, long pstride,
                                                long sstride, int M, int tiles_c,
-                                               const BiMove* __restrict__ moves, int K) {
+                                               const BiMove* __restrict__ moves, int K, int xcd) {
   using G = EdgeTile<LW, H, T>;
   __shared__ uint32_t ew[G::kBytes / 4];
   __shared__ BiMove smv[kBiMaxK];
   uint8_t* e = (uint8_t*)ew;
-  const int tr = blockIdx.x / tiles_c, tc = blockIdx.x % tiles_c;
+  // XCD order: block b runs on XCD b % 8; XCD x takes the x-th contiguous
+  // eighth of the tiles, so tiles that read each other's halo share an L2
+  // (the grid is rounded up to a multiple of 8; surplus blocks leave).
+  unsigned tile = blockIdx.x;
+  if (xcd) tile = (tile & 7u) * (gridDim.x >> 3) + (tile >> 3);
+  if (tile >= (unsigned)(tiles_c * tiles_c)) return;
+  const int tr = (int)tile / tiles_c, tc = (int)tile % tiles_c;
   const int r0 = tr * G::TH - H, c0 = tc * G::TH - H;
   for (int k = threadIdx.x; k < G::kBytes / 4; k += T) ew[k] = 0;
   if (threadIdx.x < K) smv[threadIdx.x] = moves[threadIdx.x];
@@ -914,15 +921,17 @@ int bibone_edge_sweep(rt_ctx* ctx, int M, uint8_t* a, uint8_t* b, size_t soff, B
   using G = EdgeTile<LW, H, T>;
   const long n = (long)M * M;
   const int tiles = (M + G::TH - 1) / G::TH;
+  const unsigned ntiles = (unsigned)(tiles * tiles);
+  const int xcd = ctx->xcd_order == 2 || (ctx->xcd_order == 1 && ntiles <= (unsigned)ctx->num_cu);
+  const unsigned grid = xcd ? (ntiles + 7u) & ~7u : ntiles;
   uint8_t* cur = a;
   uint8_t* nxt = b;
   long launches = 0;
   for (long s0 = 0; s0 < steps; s0 += G::K) {
     const int K = (int)std::min<long>(G::K, steps - s0);
-    hipLaunchKernelGGL((k_bi_edge<LW, H, T>), dim3((unsigned)(tiles * tiles)), dim3(T), 0,
-                       ctx->stream, cur, nxt, (const uint32_t*)(cur + soff),
-                       (uint32_t*)(nxt + soff), ctx->d_param, (long)ctx->nparam, n, M, tiles,
-                       d_moves + s0, K);
+    hipLaunchKernelGGL((k_bi_edge<LW, H, T>), dim3(grid), dim3(T), 0, ctx->stream, cur, nxt,
+                       (const uint32_t*)(cur + soff), (uint32_t*)(nxt + soff), ctx->d_param,
+                       (long)ctx->nparam, n, M, tiles, d_moves + s0, K, xcd);
     launches++;
     std::swap(cur, nxt);
   }
