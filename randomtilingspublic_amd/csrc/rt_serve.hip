@@ -1,10 +1,10 @@
-// rt_serve.cpp -- continuous-batching CFTP (and walk) server.
+// rt_serve.hip -- continuous-batching CFTP (and walk) server.
 //
 // Requests (domino or lozenge CFTP samples of any size, and fixed-length
 // walks of any model, rectangle-triangle included) arrive at any time.
 // A device loop thread admits them by HBM footprint into slots of one
 // pre-allocated arena, keeps every slot's chain state resident, and advances
-// ALL active slots together: one fused launch per model per tick, each slot
+// ALL active slots together: one fused launch per group per tick, each slot
 // at its own position in its own schedule (per-unit windows).  A slot whose
 // round ends is synced and tested for coalescence two ticks later (the
 // test overlaps the next tick's sweep); it then either returns its sample
@@ -314,8 +314,12 @@ static int admit(Server* S, Req* r) {
         else h[idx] = (uint8_t)t[(size_t)i * N + j];
       }
     if (hipMemcpy(which ? s->init_bot : s->init_top, h.data(), h.size(), hipMemcpyHostToDevice) !=
-        hipSuccess)
+        hipSuccess) {
+      S->arena.release(off, bytes);
+      S->free_ids.push_back(s->id);
+      delete s;
       return fail(S, "upload of start tilings failed", RT_EDEVICE);
+    }
   }
   s->round = 1;
   if (r->walk) {
