@@ -76,6 +76,47 @@ def test_xcd_order_never_changes_results(gpu_ctx):
     assert np.array_equal(outs[1][2], o.domino_walk(ts[2], 500, 5, P, rt.DOMINO_HEATBATH))
 
 
+def test_xcd_order_never_changes_pair_or_bibone_results(gpu_ctx):
+    """Mode 2 (XCD order on every launch) against mode 0 on the lozenge pair
+    kernel (tile shape 7, more workgroups than CUs, the remap of large
+    benchmark grids) and on the bibone edge kernel (a tile shape with more
+    tiles than CUs)."""
+    d = rt.LozengeTiler.AlmostHexDomain(60, 0)
+    tmin, tmax = rt.LozengeTiler.MinTiling(d), rt.LozengeTiler.MaxTiling(d)
+    N = rt.side(tmin)
+    gpu_ctx.set_tinymt_params(rt.synthetic_params(N * N, 35))
+    masters = list(range(1, 13))
+    outs = []
+    try:
+        gpu_ctx.set_tile_shape(7)
+        for on in (0, 2):
+            _say("pair cftp rounds xcd order", on)
+            gpu_ctx.set_xcd_order(on)
+            tops, bots, _ = gpu_ctx.lozenge_cftp_batch_rounds(tmax, tmin, masters,
+                                                              rt.SCHED_DOUBLING, 256, 3)
+            outs.append((tops, bots))
+    finally:
+        gpu_ctx.set_tile_shape(0)
+        gpu_ctx.set_xcd_order(1)
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+    T = rt.TriangleDimerTiler
+    t = T.Hexagon(70, 71, 70)
+    Nb = rt.side(t)
+    gpu_ctx.set_tinymt_params(rt.synthetic_params((Nb // 2) ** 2, 36))
+    got = []
+    try:
+        gpu_ctx.set_tile_shape(6)  # 48 x 48 tiles, halo 12: 7 x 7 tiles here
+        for on in (0, 2):
+            _say("bibone walk xcd order", on)
+            gpu_ctx.set_xcd_order(on)
+            got.append(gpu_ctx.bibone_walk(t, 700, 3))
+    finally:
+        gpu_ctx.set_tile_shape(0)
+        gpu_ctx.set_xcd_order(1)
+    assert np.array_equal(got[0], got[1])
+    assert not np.array_equal(got[0], np.asarray(t))
+
+
 def test_round_callback_reports_every_round(gpu_ctx):
     d = rt.LozengeTiler.AlmostHexDomain(12, 0)
     tmin, tmax = rt.LozengeTiler.MinTiling(d), rt.LozengeTiler.MaxTiling(d)
