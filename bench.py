@@ -40,13 +40,14 @@ XCD_ORDER = None  # --xcd-order: rt_set_xcd_order of the measured contexts (None
 # pair kernel at eight waves per SIMD.
 KERNEL_BY_SHAPE = ["k_fused<1, 2, 8, 8>", "k_fused<1, 2, 16, 4>", "k_fused<1, 2, 16, 2>",
                    "k_fused<2, 2, 8, 4>", "k_fused<1, 1, 8, 8>", "k_fused<1, 1, 16, 4>",
-                   "k_fused<1, 1, 16, 2>", "k_pair_lozenge"]
+                   "k_fused<1, 1, 16, 2>", "k_pair_lozenge<32>"]
 
 
 def set_halo_kernels(halo):
     """Slot 7 runs k_pair_lozenge (both chains packed per LDS word) when the
-    halo is a multiple of 3, else k_fused_o8<1>."""
-    KERNEL_BY_SHAPE[7] = "k_pair_lozenge" if halo % 3 == 0 else "k_fused_o8<1>"
+    halo is the default (0: 9 for these batches) or a multiple of 3, else
+    k_fused_o8<1>."""
+    KERNEL_BY_SHAPE[7] = "k_pair_lozenge<32>" if halo % 3 == 0 else "k_fused_o8<1>"
 # Algorithmic bytes per site per launch: a pair workgroup reads the TinyMT
 # state (16 B) and parameters (12 B) once, writes the state (16 B), reads and
 # writes 3 colour planes x 2 chains (12 B) = 56 B; split pairs read state and
@@ -542,7 +543,7 @@ def copy_probe(mib):
 # per dispatch of that kernel).
 PROFILE_WORKLOADS = {
     "hex300_cftp_32seeds": (["--profile-rounds", "6", "--samples", "32", "--tile-shape", "7"],
-                            "k_pair_lozenge", 56),
+                            "k_pair_lozenge<32>", 56),
     "aztec3000_step": (["--profile-section", "sweep"], "k_domino_step<1>", 48),
     "batch256_walk": (["--profile-section", "walks"], "k_walk_o8<0>", 48),
     "bibone_hex200": (["--profile-section", "bibone", "--bibone-steps", "20000"],
@@ -551,8 +552,9 @@ PROFILE_WORKLOADS = {
 PROFILE_TEXT = {
     "hex300_cftp_32seeds": "lozenge hexagon a=b=c=300 (N=609, 123 627 sites per plane), batched "
                            "CFTP of master seeds 1..32, doubling from 2^10, stopped after 6 "
-                           "doubling rounds, tile shape 7 (48-row chain-pair tiles at eight waves "
-                           "per SIMD); 56 B per site: TinyMT state 16 B read + 16 B written, "
+                           "doubling rounds, tile shape 7 (k_pair_lozenge: 96 x 32 chain-pair "
+                           "tiles, halo 9 rows and 3 columns, eight waves per SIMD); 56 B per "
+                           "site: TinyMT state 16 B read + 16 B written, "
                            "parameters 12 B, 3 colour planes x 2 chains read and written",
     "aztec3000_step": "plain heat-bath domino sweep of the Aztec diamond of order 3000 "
                       "(N=6006), 1000 steps, one launch per colour step; 48 B per attempt: "
@@ -722,6 +724,19 @@ def rocprof_sq(name, outdir, summarize_only=False):
     md = f"{kernel.replace('<', '_').replace('>', '').replace(', ', '_')}_{name}_sq.md"
     with open(os.path.join(ROOT, "profiles", md), "w") as f:
         f.write("\n".join(lines) + "\n")
+    # wave-level VALU instructions per site per dispatch, which the headline
+    # line's roofline.valu scales by its own sites and launch time
+    jpath = os.path.join(ROOT, "profiles", "kernels_sq.json")
+    table = {}
+    if os.path.exists(jpath):
+        with open(jpath) as f:
+            table = json.load(f)
+    table[kernel] = {"workload": name, "valu_wave_insts_per_site": round(valu / sites, 4),
+                     "sites_per_dispatch": sites, "avg_launch_us": round(avg_us, 2),
+                     "valu_issue_frac": round(valu / (SIMD_WAVE_INSTS_PER_US * avg_us), 4),
+                     "profile": md}
+    with open(jpath, "w") as f:
+        json.dump(table, f, indent=1)
     print(json.dumps({"rocprof_sq": name, "kernel": kernel, "avg_us": avg_us,
                       "valu_per_dispatch": valu, "summary": md}), flush=True)
 
@@ -1020,8 +1035,9 @@ def main():
     ap.add_argument("--xcd-order", type=int, default=None, choices=[0, 1, 2],
                     help="rt_set_xcd_order for the headline batch and the batch walks "
                          "(default: the library's)")
-    ap.add_argument("--halo", type=int, default=6,
-                    help="sweep-block halo depth (rt_set_halo; results never depend on it)")
+    ap.add_argument("--halo", type=int, default=0,
+                    help="sweep-block halo depth (rt_set_halo; 0 = the library's default; "
+                         "results never depend on it)")
     ap.add_argument("--tile-shape", type=int, default=0,
                     help="force a sweep-block tile shape (rt_set_tile_shape; 0 = autotuned), "
                          "for profiling one kernel")
@@ -1170,6 +1186,22 @@ def main():
             per_kernel = json.load(f)
         if kname in per_kernel:
             traffic = int(per_kernel[kname]["hbm_bytes_per_site"] * sites_per_launch)
+    # The fused kernel runs about 14 colour steps per launch from LDS, so
+    # HBM does not bind it: beside the HBM figure, its VALU issue -- wave
+    # instructions per site per launch from the committed SQ-counter profile
+    # of the same kernel and workload, times this run's sites per launch,
+    # over this run's launch time and the chip's issue capacity.
+    valu = None
+    sqfile = os.path.join(ROOT, "profiles", "kernels_sq.json")
+    if os.path.exists(sqfile) and side == 300:
+        with open(sqfile) as f:
+            per_sq = json.load(f)
+        if kname in per_sq and avg_launch_s > 0:
+            ins = per_sq[kname]["valu_wave_insts_per_site"] * sites_per_launch
+            valu = {"issue_frac": round(ins / (SIMD_WAVE_INSTS_PER_US * avg_launch_s * 1e6), 4),
+                    "wave_insts_per_launch": int(ins),
+                    "peak_wave_insts_per_us": SIMD_WAVE_INSTS_PER_US,
+                    "source": "profiles/kernels_sq.json (" + per_sq[kname]["profile"] + ")"}
     roof = {"bound": "hbm", "achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
             "kernel": kname, "alg_bytes_per_site": ALG_BYTES_BY_SHAPE[shape],
@@ -1178,7 +1210,11 @@ def main():
             "launches": klaunch, "kernel_time_share": round(kms / max(tm["kernel_ms"], 1e-9), 3),
             "sites_per_launch": int(sites_per_launch), "alg_bytes_per_launch": int(alg_bytes),
             "flip_attempts_per_s_in_kernels": round(2 * draws / (tm["kernel_ms"] / 1e3), 1)
-            if tm["kernel_ms"] else None}
+            if tm["kernel_ms"] else None,
+            # colour steps per launch and site (TinyMT draws over launched sites)
+            "steps_per_launch": round(draws / tm["kernel_sites"], 2)
+            if tm["kernel_sites"] else None,
+            "valu": valu}
     sweep = None if args.no_sweep else plain_sweep(rt)
     if sweep:
         print(f"[bench] plain sweep {sweep}", file=sys.stderr, flush=True)

@@ -1,10 +1,9 @@
-// pair_lozenge.hpp -- lozenge CFTP chain pairs on 48-row tiles at eight
-// waves per SIMD, the two chains of a pair packed into one LDS word per
-// cell (included by rt_fused.hip).
+// pair_lozenge.hpp -- lozenge CFTP chain pairs at eight waves per SIMD,
+// the two chains of a pair packed into one LDS word per cell (included by
+// rt_fused.hip).
 //
-// Tiles, halo, windows, op bytes, buffer sets and TinyMT handling are those
-// of k_fused (one 64-column tile per workgroup, one lane per column).  What
-// differs:
+// Windows, op bytes, buffer sets and TinyMT handling are those of k_fused
+// (one tile per workgroup, one lane per column).  What differs:
 //
 // * Cell layout in LDS: cell (i, j) of colour c is one 32-bit word holding
 //   the top chain's byte in bits 0-7 and the bottom chain's in bits 16-23.
@@ -13,20 +12,23 @@
 //   one bit-field insert per neighbour refresh both chains; the cube move
 //   (lozengekernel.cl:20-25: 25 -> 38 when rd < .4, 38 -> 25 when
 //   .4 <= rd < .8) tests both halves against the draw's target at once.
-// * Row bands: wave wv owns tile rows 3wv .. 3wv+2.  With the halo H a
-//   multiple of 3 every tile starts on a global row that is a multiple of 3
-//   (tiles step by 48 - 2H rows from -H), so the row's residue mod 3, which
-//   selects the pull's neighbour columns, is the row's index in the band:
-//   a compile-time constant.  With the colour as a template parameter too
-//   (twelve step bodies: 3 colours x 4 pull patterns), every LDS address is
-//   the lane's base plus an immediate offset.
+// * Tiles of 3072 cells, 96 rows x 32 columns (LW = 32, the one the host
+//   launches) or 48 x 64, each lane 3 rows of one column: row bands of 3.
+//   With the halo H a multiple of 3 every tile starts on a global row that
+//   is a multiple of 3 (tiles step by LH - 2H rows from -H), so the row's
+//   residue mod 3, which selects the pull's neighbour columns, is the row's
+//   index in the band: a compile-time constant.  With the colour as a
+//   template parameter too (twelve step bodies: 3 colours x 4 pull
+//   patterns), every LDS address is the lane's base plus an immediate
+//   offset.
+// * Halo of H rows but only H / 3 columns (kernel comment below).
 // * Stale bands: after k pulling steps of a window the top and bottom k
 //   rows of the loaded tile are stale (their neighbours outside the tile
 //   were not loaded); halo rows are never written back, so a wave whose
-//   whole band is stale skips the rest of the window, draws included.
+//   rows are all stale skips the rest of the window, draws included.
 //
-// The host uses this kernel for tile shape 7 when H % 3 == 0 and
-// k_fused_o8<1> otherwise; results are identical.
+// The host uses this kernel for lozenge chain pairs on tile shape 7 when
+// H % 3 == 0 and k_fused_o8<1> otherwise; results are identical.
 #pragma once
 
 namespace rt {
@@ -112,6 +114,19 @@ __device__ __forceinline__ void pk_step(uint32_t* p, const uint32_t* interior,
 
 // Kernel arguments, work items, windows and buffer sets as k_fused; mp is
 // unused (the lozenge move has no parameters).  Needs H % 3 == 0.
+//
+// LW = 64: 48 x 64 tiles, wave wv owns rows 3wv .. 3wv+2.  LW = 32: 96 x 32
+// tiles, lanes 0-31 of wave wv own rows 3wv .. 3wv+2 and lanes 32-63 rows
+// 93-3wv .. 95-3wv, so the two bands of a wave go stale together (both at
+// distance 3wv from a tile edge).  Both load a 3072-cell tile.
+//
+// Halo: H rows, H / 3 columns.  A pulling step reads rows i-1 and i+1, so
+// rows go stale one deeper per pull; its column offsets (p2, p4, p8, p16)
+// are 1 only for one row residue each, and over every colour sequence the
+// stale columns at either side number at most ceil(pulls / 3) (checked
+// exhaustively by the host tests on the lozenge pull rule), so a window of
+// at most H pulls leaves columns H / 3 .. LW - 1 - H / 3 exact.
+template <int LW>
 __global__ __launch_bounds__(1024, 8) void k_pair_lozenge(const UnitDesc* __restrict__ units,
                                                           const WorkItem* __restrict__ work,
                                                           int nwork, int xcd, int H,
@@ -119,8 +134,9 @@ __global__ __launch_bounds__(1024, 8) void k_pair_lozenge(const UnitDesc* __rest
                                                           const WindowRec* __restrict__ wins) {
   constexpr int W = 16, R = kPkRows;
   constexpr int kThreads = 64 * W;
-  constexpr int LH = W * R;
-  constexpr int RS = kLW + 2;
+  constexpr int HALVES = 64 / LW;
+  constexpr int LH = W * R * HALVES;
+  constexpr int RS = LW + 2;
   constexpr int PL = (LH + 2) * RS;
   __shared__ __align__(16) uint32_t lds[3 * PL];
   __shared__ uint8_t sops[kKMax];
@@ -136,13 +152,16 @@ __global__ __launch_bounds__(1024, 8) void k_pair_lozenge(const UnitDesc* __rest
   UnitDesc U = units[wi.unit];
   window_buffers(U, w.flags);
   const int rows = U.rows, cols = U.cols;
-  const int TH = LH - 2 * H, TW = kLW - 2 * H;
+  const int HC = H / 3;
+  const int TH = LH - 2 * H, TW = LW - 2 * HC;
   const int tr = wi.tile / U.tiles_c, tc = wi.tile % U.tiles_c;
-  const int r0 = tr * TH - H, c0 = tc * TW - H;
+  const int r0 = tr * TH - H, c0 = tc * TW - HC;
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int band = R * wv;  // tile row of the wave's row 0
-  const int gc = c0 + lane;
+  const int col = lane & (LW - 1);
+  const int bw = R * wv;  // the wave's band row at the top of the tile
+  const int band = (HALVES == 2 && lane >= LW) ? LH - R - bw : bw;  // tile row of the lane's row 0
+  const int gc = c0 + col;
   const bool col_in = gc >= 0 && gc < cols;
 
   for (int k = threadIdx.x; k < 3 * PL; k += kThreads) lds[k] = 0;
@@ -151,7 +170,7 @@ __global__ __launch_bounds__(1024, 8) void k_pair_lozenge(const UnitDesc* __rest
 
   Tiny st[R];
   uint32_t interior[R];
-  uint32_t* const p = lds + (band + 1) * RS + lane + 1;
+  uint32_t* const p = lds + (band + 1) * RS + col + 1;
 #pragma unroll
   for (int r = 0; r < R; r++) {
     const int gr = r0 + band + r;
@@ -191,10 +210,12 @@ __global__ __launch_bounds__(1024, 8) void k_pair_lozenge(const UnitDesc* __rest
     for (int s = 0; s < K; s++) {
       pulls += (op & (OP_PULL_A | OP_PULL_B)) ? 1 : 0;
       // After k pulls, tile rows < k and > LH - 1 - k are stale: a row that
-      // pulls is exact when the rows above and below it were.  A band that
-      // is stale after this step skips it; every row is still computed for
-      // as long as it is exact, so the rows that exact rows read are too.
-      const bool stale = band + R - 1 < pulls || band > LH - 1 - pulls;
+      // pulls is exact when the rows above and below it were.  A wave whose
+      // rows are all stale after this step skips it; every row is still
+      // computed for as long as it is exact, so the rows that exact rows
+      // read are too.  (With LW = 32 the mirrored band is stale exactly
+      // when the top one is.)
+      const bool stale = bw + R - 1 < pulls || (HALVES == 1 && bw > LH - 1 - pulls);
       if (!stale) {
         switch (op & 15) {
 #define RT_PK_CASE(c, pl) \
@@ -222,8 +243,8 @@ __global__ __launch_bounds__(1024, 8) void k_pair_lozenge(const UnitDesc* __rest
 #pragma unroll
   for (int r = 0; r < R; r++) {
     const int lr = band + r, gr = r0 + lr;
-    const bool inner = col_in && gr >= 0 && gr < rows && lr >= H && lr < H + TH && lane >= H &&
-                       lane < H + TW;
+    const bool inner = col_in && gr >= 0 && gr < rows && lr >= H && lr < H + TH && col >= HC &&
+                       col < HC + TW;
     if (!inner) continue;
     const long idx = (long)gr * cols + gc;
 #pragma unroll

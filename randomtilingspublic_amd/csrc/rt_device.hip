@@ -367,9 +367,12 @@ extern "C" int rt_device_limits(const rt_ctx* ctx, int* num_cu, size_t* lds_per_
 }
 
 extern "C" int rt_set_halo(rt_ctx* ctx, int halo) {
-  if (!ctx || halo < 1 || halo > 12) return RT_EINVAL;
-  if (halo != ctx->halo) ctx->tuned.clear();
+  if (!ctx || halo < 0 || halo > 12) return RT_EINVAL;
+  const bool set = halo != 0;
+  if (!set) halo = 6;
+  if (halo != ctx->halo || set != ctx->halo_set) ctx->tuned.clear();
   ctx->halo = halo;
+  ctx->halo_set = set;
   return RT_OK;
 }
 

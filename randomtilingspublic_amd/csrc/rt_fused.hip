@@ -383,8 +383,45 @@ static Geo shape_geo(int shape) {
   }
 }
 
-static void tile_counts(const UnitDesc& u, int LH, int H, int* tr, int* tc) {
-  const int TH = LH - 2 * H, TW = kLW - 2 * H;
+// Loaded tile of a launch: LH x LW cells, a halo of Hr rows and Hc columns.
+struct TileGeo {
+  int LH, LW, Hr, Hc;
+};
+
+// Lozenge chain pairs on tile shape 7 run k_pair_lozenge<32>: 96 x 32
+// tiles (mirrored row bands), a halo of H rows and H / 3 columns, H = 9
+// unless rt_set_halo chose another.  Per launch window set on the side-300
+// hexagon CFTP (6 doubling rounds, kernel time), against the best of the
+// other shapes at H = 6 (autotuned): 32 samples 516 against 632 ms, 16
+// samples 316 / 430, 8 samples 179 / 229, 4 samples 108 / 135, one sample
+// 93 / 88; the 48 x 64 form of the same kernel (k_pair_lozenge<64>, same
+// column halo) 643 ms at 32 samples and H = 9, 671 at H = 6.  A taller
+// tile needs fewer of its rows as halo and, with the narrow column halo,
+// a deeper row halo costs little, so windows are half again as long and
+// the per-tile-load cost (HBM traffic, tile set-up) falls by a third.
+constexpr int kPairLW = 32;
+constexpr int kPairHalo = 9;
+
+// Launches that run k_pair_lozenge: lozenge chain pairs (not split) on tile
+// shape 7 with a halo divisible by 3, on the byte-cell engine.
+static bool packed_pair(const rt_ctx* ctx, int model, int nc, int shape, int H) {
+  return shape == SH_16x3_O8 && model == M_LOZENGE && nc == 2 && H % 3 == 0 &&
+         ctx->engine != RT_ENGINE_FUSED_SWAR;
+}
+
+// A lone small job (64-row tiles for under a third of the CUs) does better
+// on the rule of thumb / autotuner at H = 6 (one hexagon-300 sample: 88
+// against 93 ms), so k_pair_lozenge takes over from that size up.
+static bool pair_job_is_large(const rt_ctx* ctx, long tiles64) { return 3 * tiles64 > ctx->num_cu; }
+
+static TileGeo tile_geo(const rt_ctx* ctx, int model, int nc, int shape, int H) {
+  if (packed_pair(ctx, model, nc, shape, H)) return TileGeo{3072 / kPairLW, kPairLW, H, H / 3};
+  const Geo g = shape_geo(shape);
+  return TileGeo{g.W * g.R, kLW, H, H};
+}
+
+static void tile_counts(const UnitDesc& u, const TileGeo& g, int* tr, int* tc) {
+  const int TH = g.LH - 2 * g.Hr, TW = g.LW - 2 * g.Hc;
   *tr = (u.rows + TH - 1) / TH;
   *tc = (u.cols + TW - 1) / TW;
 }
@@ -393,7 +430,7 @@ static long tiles64(const FusedJob& job) {
   long t64 = 0;
   for (const UnitDesc& u : job.units) {
     int tr, tc;
-    tile_counts(u, 64, 6, &tr, &tc);
+    tile_counts(u, TileGeo{64, kLW, 6, 6}, &tr, &tc);
     t64 += (long)tr * tc;
   }
   return t64;
@@ -463,6 +500,11 @@ static void choose_shape(const rt_ctx* ctx, FusedJob& job) {
     job.shape = swar ? SH_16x4 : SH_16x3_O8;
     return;
   }
+  if (ctx->tile_shape == SH_AUTO && job.halo == kPairHalo &&
+      packed_pair(ctx, job.model, job.nc, SH_16x3_O8, job.halo)) {
+    job.shape = SH_16x3_O8;  // fastest at every batch size measured: no trials
+    return;
+  }
   if (ctx->tile_shape != SH_AUTO) {  // 1..3, or 4..6 = the same with pairs split
     job.shape = ctx->tile_shape > 3 ? ctx->tile_shape - 3 : ctx->tile_shape;
     job.split = ctx->tile_shape > 3 && job.nc == 2;
@@ -514,13 +556,13 @@ static void cut_windows(FusedJob& job, int H) {
   }
 }
 
-static size_t tiles_bytes(const FusedJob& job, int shape, bool split) {
-  const Geo g = shape_geo(shape);
+static size_t tiles_bytes(const rt_ctx* ctx, const FusedJob& job, int shape, bool split) {
+  const TileGeo g = tile_geo(ctx, job.model, split ? 1 : job.nc, shape, job.halo);
   const size_t per = split ? 2 : 1;
   size_t work = 0;
   for (const UnitDesc& u : job.units) {
     int tr, tc;
-    tile_counts(u, g.W * g.R, job.halo, &tr, &tc);
+    tile_counts(u, g, &tr, &tc);
     work += (size_t)tr * tc * per;
   }
   size_t b = ((job.units.size() * per * sizeof(UnitDesc) + 255) & ~(size_t)255) +
@@ -565,10 +607,18 @@ __global__ void k_copy_list(const CopyDesc* __restrict__ list, unsigned n) {
 }
 
 // Halo of a job's tiles: the context's setting, except rect-triangle pairs
-// (32-row int32 tiles, H = 4).  Every shape of a job has the same halo, so
-// one window list serves all.
+// (32-row int32 tiles, H = 4) and, unless rt_set_halo was called, lozenge
+// chain pairs that may run k_pair_lozenge (H = 9).  Every shape of a job has
+// the same halo, so one window list serves all.
 static void plan_windows(rt_ctx* ctx, FusedJob& job) {
-  job.halo = job.model == M_RECTTRI && job.nc == 2 ? shape_geo(SH_RT8x4).H : ctx->halo;
+  if (job.model == M_RECTTRI && job.nc == 2)
+    job.halo = shape_geo(SH_RT8x4).H;
+  else if (!ctx->halo_set && (ctx->tile_shape == SH_AUTO || ctx->tile_shape == SH_16x3_O8) &&
+           packed_pair(ctx, job.model, job.nc, SH_16x3_O8, kPairHalo) &&
+           (ctx->tile_shape == SH_16x3_O8 || pair_job_is_large(ctx, tiles64(job))))
+    job.halo = kPairHalo;
+  else
+    job.halo = ctx->halo;
   cut_windows(job, job.halo);
   choose_shape(ctx, job);
 }
@@ -582,9 +632,9 @@ size_t fused_scratch_bytes(rt_ctx* ctx, FusedJob& job) {
                  ((job.nlaunch * nu * sizeof(WindowRec) + 255) & ~(size_t)255) + 1024;
   if (job.tune)
     for (int c = 0; c < tune_count(job); c++)
-      bytes += tiles_bytes(job, tune_shape(job, c).shape, tune_shape(job, c).split);
+      bytes += tiles_bytes(ctx, job, tune_shape(job, c).shape, tune_shape(job, c).split);
   else
-    bytes += tiles_bytes(job, job.shape, job.split);
+    bytes += tiles_bytes(ctx, job, job.shape, job.split);
   for (const UnitDesc& u : job.units) bytes += second_set_bytes(job, u);
   bytes += ((nu * 7 * sizeof(CopyDesc)) + 255) & ~(size_t)255;
   return bytes;
@@ -639,8 +689,8 @@ static void launch_pair_lozenge(rt_ctx* ctx, unsigned nblocks, const UnitDesc* u
                       ? 1
                       : 0;
   if (H % 3 == 0)
-    hipLaunchKernelGGL(k_pair_lozenge, dim3(grid_of(xcd, nblocks)), dim3(1024), 0, ctx->stream,
-                       units, work, (int)nblocks, xcd, H, mp, wins);
+    hipLaunchKernelGGL(k_pair_lozenge<kPairLW>, dim3(grid_of(xcd, nblocks)), dim3(1024), 0,
+                       ctx->stream, units, work, (int)nblocks, xcd, H, mp, wins);
   else
     hipLaunchKernelGGL((k_fused_o8<M_LOZENGE>), dim3(grid_of(xcd, nblocks)), dim3(1024), 0,
                        ctx->stream, units, work, (int)nblocks, xcd, H, mp, wins);
@@ -686,7 +736,7 @@ static LaunchFn pick(const rt_ctx* ctx, int model, int nc, int shape) {
 static int upload_tiles(rt_ctx* ctx, const FusedJob& job, LaunchShape ls, uint8_t*& p,
                         const WindowRec* d_wins, const std::vector<WindowRec>& wins,
                         FusedPlan::Tiles* out) {
-  const Geo geo = shape_geo(ls.shape);
+  const TileGeo geo = tile_geo(ctx, job.model, ls.split ? 1 : job.nc, ls.shape, job.halo);
   const size_t nu = job.units.size();
   std::vector<UnitDesc> units;
   if (ls.split) {
@@ -708,7 +758,7 @@ static int upload_tiles(rt_ctx* ctx, const FusedJob& job, LaunchShape ls, uint8_
   std::vector<WorkItem> work;
   for (size_t u = 0; u < units.size(); u += ls.split ? 2 : 1) {
     int tr, tc;
-    tile_counts(units[u], geo.W * geo.R, job.halo, &tr, &tc);
+    tile_counts(units[u], geo, &tr, &tc);
     units[u].tiles_c = tc;
     if (ls.split) units[u + 1].tiles_c = tc;
     // Split pairs: the two chains' workgroups of a tile sit next to each
@@ -942,6 +992,9 @@ int fused_launch(rt_ctx* ctx, const FusedJob& job, const FusedPlan& plan) {
 
 int fused_serve_shape(const rt_ctx* ctx, int model, int nc, long tiles64) {
   if (model == M_RECTTRI) return nc == 2 ? SH_RT8x4 : SH_8x8;
+  // lozenge chain pairs: k_pair_lozenge from a third of the CUs up (see kPairLW)
+  if (packed_pair(ctx, model, nc, SH_16x3_O8, kPairHalo) && pair_job_is_large(ctx, tiles64))
+    return SH_16x3_O8;
   // Few tiles: one 16-wave workgroup per CU on 64-row tiles; more tiles
   // than CUs: the 48-row tiles at eight waves per SIMD (two workgroups per
   // CU), which the batch tuner picks from about 16 hexagon-300 samples up.
@@ -957,20 +1010,23 @@ int fused_launch_windows(rt_ctx* ctx, int model, int nc, int shape, const ModelP
     ctx->err = "no sweep-block kernel for this tile shape";
     return RT_EINVAL;
   }
-  fn(ctx, nwork, d_units, d_work, shape_geo(shape).H, mp, d_wins);
+  fn(ctx, nwork, d_units, d_work, fused_shape_halo(ctx, model, nc, shape), mp, d_wins);
   HIPCHK_RET(ctx, hipGetLastError());
   ctx->last.launches += 1;
   return RT_OK;
 }
 
-int fused_shape_halo(int shape) { return shape_geo(shape).H; }
+int fused_shape_halo(const rt_ctx* ctx, int model, int nc, int shape) {
+  return packed_pair(ctx, model, nc, shape, kPairHalo) ? kPairHalo : shape_geo(shape).H;
+}
 
-void fused_tiles(int shape, int rows, int cols, int* tiles_r, int* tiles_c) {
-  const Geo g = shape_geo(shape);
+void fused_tiles(const rt_ctx* ctx, int model, int nc, int shape, int rows, int cols,
+                 int* tiles_r, int* tiles_c) {
   UnitDesc u{};
   u.rows = rows;
   u.cols = cols;
-  tile_counts(u, g.W * g.R, g.H, tiles_r, tiles_c);
+  tile_counts(u, tile_geo(ctx, model, nc, shape, fused_shape_halo(ctx, model, nc, shape)), tiles_r,
+              tiles_c);
 }
 
 int fused_run(rt_ctx* ctx, FusedJob& job, void* scratch) {

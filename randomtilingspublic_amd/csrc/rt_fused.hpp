@@ -203,7 +203,8 @@ inline int shape_slot(int shape, bool split) {
 void fused_add_trials(const FusedPlan& plan, rt_timing* acc);
 
 // Tile shape for launches over `tiles64` 64-row tiles of (model, nc)
-// units whose windows are cut by the caller (the serving loop, halo 6).
+// units whose windows are cut by the caller (the serving loop, with the
+// halo of fused_shape_halo).
 int fused_serve_shape(const rt_ctx* ctx, int model, int nc, long tiles64);
 // One launch over `nwork` work items of tile shape `shape` where each unit
 // follows its own window (wins[unit]); model / chains / params as in a job.
@@ -211,9 +212,12 @@ int fused_launch_windows(rt_ctx* ctx, int model, int nc, int shape, const ModelP
                          const UnitDesc* d_units, const WorkItem* d_work, unsigned nwork,
                          const WindowRec* d_wins);
 // Tile counts of a unit under a tile shape's geometry.
-void fused_tiles(int shape, int rows, int cols, int* tiles_r, int* tiles_c);
-// Halo depth of a tile shape (the most pulling steps one window may take).
-int fused_shape_halo(int shape);
+void fused_tiles(const rt_ctx* ctx, int model, int nc, int shape, int rows, int cols,
+                 int* tiles_r, int* tiles_c);
+// Halo depth of a serving launch group on a tile shape (the most pulling
+// steps one window may take): 9 for lozenge chain pairs on shape 7
+// (k_pair_lozenge), else the shape's own (6; rect-triangle pairs 4).
+int fused_shape_halo(const rt_ctx* ctx, int model, int nc, int shape);
 // Bring every colour plane of every unit up to date (end of a walk).
 int fused_sync(rt_ctx* ctx, const FusedJob& job);
 

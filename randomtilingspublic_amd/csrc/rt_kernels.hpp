@@ -45,6 +45,7 @@ struct rt_ctx {
   int autotune = 1;    // time candidate tile shapes on a job's first windows
   int xcd_order = 1;   // each XCD takes a contiguous share of a launch's tiles
   int halo = 6;        // sweep-block halo depth (byte-cell models)
+  bool halo_set = false;  // rt_set_halo was called (else lozenge pairs take 9)
   size_t lds_cu = 0;   // LDS bytes per CU
   size_t lds_block = 0;  // LDS bytes one workgroup may hold
   std::map<uint32_t, int> tuned;  // tuning key (model, chains, tile count) -> shape

@@ -384,7 +384,7 @@ static int upload_tables(Server* S) {
     for (Slot* s : S->active) {
       if (s->group != m) continue;
       int tr, tc;
-      fused_tiles(SH_SERVE_COUNT, s->req->N, (int)s->cols, &tr, &tc);
+      fused_tiles(S->ctx, model, nc, SH_SERVE_COUNT, s->req->N, (int)s->cols, &tr, &tc);
       t64 += (long)tr * tc;
     }
     S->shape[m] = fused_serve_shape(S->ctx, model, nc, t64);
@@ -412,7 +412,7 @@ static int upload_tables(Server* S) {
       const int u = (int)units.size();
       d.ops = S->d_ops;  // window s0 = ops_at(buffer, sub, model, unit)
       int tr, tc;
-      fused_tiles(S->shape[m], d.rows, d.cols, &tr, &tc);
+      fused_tiles(S->ctx, model, nc, S->shape[m], d.rows, d.cols, &tr, &tc);
       d.tiles_c = tc;
       for (int t = 0; t < tr * tc; t++) work.push_back(WorkItem{u, t});
       units.push_back(d);
@@ -521,7 +521,7 @@ static int run_tick(Server* S) {
   std::vector<Slot*> ended;
   for (int k = 0; k < kSub; k++) {
     for (int m = 0; m < kGroups; m++) {
-      const int H = fused_shape_halo(S->shape[m]);  // pulling steps per window
+      const int H = fused_shape_halo(S->ctx, group_model(m), group_nc(m), S->shape[m]);
       for (size_t u = 0; u < S->unit_slot[m].size(); u++) {
         Slot* s = S->unit_slot[m][u];
         WindowRec& w = S->h_wins[wins_at(b, k, m) + u];

@@ -147,7 +147,7 @@ def test_round_callback_reports_every_round(gpu_ctx):
         gpu_ctx.set_round_callback(None)
 
 
-@pytest.mark.parametrize("halo", [2, 9])
+@pytest.mark.parametrize("halo", [2, 6, 9, 12])
 def test_halo_depth_never_changes_results(gpu_ctx, halo):
     # Domains wide enough for two tile columns at either halo (64-column
     # tiles overlap by 2 * halo): the hexagon of side 100 (69 lozenge
@@ -180,10 +180,10 @@ def test_halo_depth_never_changes_results(gpu_ctx, halo):
             for got, w in zip(outs, want_d):
                 assert np.array_equal(got, w)
     finally:
-        gpu_ctx.set_halo(6)
+        gpu_ctx.set_halo(0)
         gpu_ctx.set_tile_shape(0)
     with pytest.raises(rt.RTError):
-        gpu_ctx.set_halo(0)
+        gpu_ctx.set_halo(-1)
     with pytest.raises(rt.RTError):
         gpu_ctx.set_halo(13)
 
@@ -209,7 +209,7 @@ def test_swar_engine_halo_depth_never_changes_results(gpu_ctx, halo):
             assert np.array_equal(gpu_ctx.lozenge_walk(tmin, 300, 8), want)
             assert np.array_equal(gpu_ctx.lozenge_walk_segments(tmax, steps, seeds), want_seg)
     finally:
-        gpu_ctx.set_halo(6)
+        gpu_ctx.set_halo(0)
         gpu_ctx.set_engine(rt.ENGINE_AUTO)
         gpu_ctx.set_tile_shape(0)
 

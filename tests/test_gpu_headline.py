@@ -1,17 +1,18 @@
 """The benchmark's kernel at the benchmark's shape, against the oracle.
 
 The batched hexagon CFTP (lozenge hexagon a = b = c = 300, N = 609) runs
-its chain pairs on 48-row tiles at eight waves per SIMD (tile shape 7,
-k_fused_o8<1>) with far more workgroups than the chip holds at once: 68
-tiles per sample, 16 samples = 1088 workgroups against 2 per CU.  Here both
+its chain pairs on tile shape 7 (k_pair_lozenge: both chains in one LDS
+word per cell, 96 x 32 tiles, halo 9 rows and 3 columns, eight waves per
+SIMD) with far more workgroups than the chip holds at once: 64 tiles per
+sample, 16 samples = 1024 workgroups against 2 per CU.  Here both
 chains of every sample after exactly three doubling rounds (look-back
 1024 + 1024 + 2048 = 4096 steps in the last round, 7168 per sample in all)
 come back through rt_lozenge_cftp_batch_rounds and are compared bit for
 bit with the oracle's segmented walks over that round's segment list
 (LozengeTiler.cpp:27-159 per segment, seeds newest first as the CFTP
 driver prepends them, AztecDiamondCftpEx.cpp:57-76), from MaxTiling and
-MinTiling.  The other pair shapes (64-row pairs, 64-row split pairs) must
-give the same chains.
+MinTiling.  The other pair shapes (64-row pairs, 64-row split pairs, at
+halo 6) must give the same chains.
 """
 import os
 
