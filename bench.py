@@ -542,8 +542,8 @@ def copy_probe(mib):
 # kernel whose dispatches the summary is about, algorithmic bytes per site
 # per dispatch of that kernel).
 PROFILE_WORKLOADS = {
-    "hex300_cftp_32seeds": (["--profile-rounds", "6", "--samples", "32", "--tile-shape", "7"],
-                            "k_pair_lozenge<32>", 56),
+    "hex300_cftp_32seeds": (["--profile-rounds", "6", "--samples", "32", "--tile-shape", "7",
+                             "--streams", "1"], "k_pair_lozenge<32>", 56),
     "aztec3000_step": (["--profile-section", "sweep"], "k_domino_step<1>", 48),
     "batch256_walk": (["--profile-section", "walks"], "k_walk_o8<0>", 48),
     "bibone_hex200": (["--profile-section", "bibone", "--bibone-steps", "20000"],
@@ -553,7 +553,9 @@ PROFILE_TEXT = {
     "hex300_cftp_32seeds": "lozenge hexagon a=b=c=300 (N=609, 123 627 sites per plane), batched "
                            "CFTP of master seeds 1..32, doubling from 2^10, stopped after 6 "
                            "doubling rounds, tile shape 7 (k_pair_lozenge: 96 x 32 chain-pair "
-                           "tiles, halo 9 rows and 3 columns, eight waves per SIMD); 56 B per "
+                           "tiles, halo 9 rows and 3 columns, eight waves per SIMD), one "
+                           "stream, so one dispatch per launch window covers every sample (the "
+                           "benchmark splits them over two); 56 B per "
                            "site: TinyMT state 16 B read + 16 B written, "
                            "parameters 12 B, 3 colour planes x 2 chains read and written",
     "aztec3000_step": "plain heat-bath domino sweep of the Aztec diamond of order 3000 "
@@ -767,15 +769,27 @@ def rocprof_bench(outdir, summarize_only=False):
         with open(path, newline="") as f:
             for r in csv.DictReader(f):
                 trace.append((int(r["Dispatch_Id"]), _short_kernel(r["Kernel_Name"]),
-                              int(r["End_Timestamp"]) - int(r["Start_Timestamp"])))
+                              int(r["Start_Timestamp"]), int(r["End_Timestamp"])))
     trace.sort()
     head = []
-    for _, k, ns in trace:
+    for _, k, t0, t1 in trace:
         if k.startswith("k_lozenge_step"):
             break
         if k == kernel:
-            head.append(ns)
-    avg_head = sum(head) / len(head) / 1e3
+            head.append((t0, t1))
+    avg_head = sum(t1 - t0 for t0, t1 in head) / len(head) / 1e3
+    # the batch's units run in parts on concurrent streams: the time the
+    # kernel occupies the chip (union of the dispatch intervals) per launch
+    # window, which is what the JSON line's HIP events time
+    busy, end = 0, None
+    for t0, t1 in sorted(head):
+        if end is None or t0 > end:
+            busy += t1 - t0
+            end = t1
+        elif t1 > end:
+            busy += t1 - end
+            end = t1
+    busy_per_window = busy / roof["launches"] / 1e3
     lines = [f"# {kernel} -- the default benchmark command, kernel trace", "",
              f"Workload: `python3 bench.py {' '.join(BENCH_DEFAULT_ARGS)}` (the whole default "
              f"run: headline batch `{out['config']['workload']}`, then the plain paths, C1, "
@@ -791,10 +805,14 @@ def rocprof_bench(outdir, summarize_only=False):
                      f"{v['min_us']:.2f} | {v['max_us']:.2f} | {v['pct']:.2f} |")
     lines += ["",
               f"Headline section (dispatches before the plain path's first `k_lozenge_step`): "
-              f"{len(head)} dispatches of `{kernel}`, average {avg_head:.2f} us by the trace; "
-              f"the same run's JSON line: {roof['launches']} launches, avg_launch_us "
-              f"{roof['avg_launch_us']} (HIP events), value {out['value']} {out['unit']}, "
-              f"roofline frac {roof['frac']}.",
+              f"{len(head)} dispatches of `{kernel}` for {roof['launches']} launch windows "
+              f"(the batch's samples in parts on concurrent streams, one dispatch per part "
+              f"and window), average dispatch {avg_head:.2f} us; the kernel occupies the chip "
+              f"(union of the dispatch intervals) {busy / 1e6:.1f} ms = "
+              f"**{busy_per_window:.2f} us per launch window** by the trace, against the same "
+              f"run's JSON line: avg_launch_us {roof['avg_launch_us']} per window (HIP events "
+              f"around the windows), value {out['value']} {out['unit']}, roofline frac "
+              f"{roof['frac']}.",
               f"Over the whole run the tracer counts {stats.get(kernel, {}).get('calls', 0)} "
               f"dispatches of `{kernel}`, average "
               f"{stats.get(kernel, {}).get('avg_us', 0.0):.2f} us (the served stream launches "
@@ -802,8 +820,8 @@ def rocprof_bench(outdir, summarize_only=False):
     with open(os.path.join(ROOT, "profiles", "bench_default_kernel_trace.md"), "w") as f:
         f.write("\n".join(lines) + "\n")
     print(json.dumps({"rocprof_bench": kernel, "headline_dispatches": len(head),
-                      "trace_avg_us": avg_head, "events_avg_us": roof["avg_launch_us"]}),
-          flush=True)
+                      "trace_avg_dispatch_us": avg_head, "trace_us_per_window": busy_per_window,
+                      "events_avg_us": roof["avg_launch_us"]}), flush=True)
 
 
 def rocprof_profile(name, outdir, summarize_only=False):
@@ -1038,6 +1056,9 @@ def main():
     ap.add_argument("--halo", type=int, default=0,
                     help="sweep-block halo depth (rt_set_halo; 0 = the library's default; "
                          "results never depend on it)")
+    ap.add_argument("--streams", type=int, default=0,
+                    help="streams the headline batch splits its samples over (rt_set_streams; "
+                         "0 = the library's default, 2); the per-dispatch profiles use 1")
     ap.add_argument("--tile-shape", type=int, default=0,
                     help="force a sweep-block tile shape (rt_set_tile_shape; 0 = autotuned), "
                          "for profiling one kernel")
@@ -1106,6 +1127,8 @@ def main():
     ctx.set_tinymt_params(params)
     ctx.set_tile_shape(args.tile_shape)
     ctx.set_halo(args.halo)
+    if args.streams:
+        ctx.set_streams(args.streams)
     if XCD_ORDER is not None:
         ctx.set_xcd_order(XCD_ORDER)
     print(f"[bench] device {ctx.device_name} hexagon {side} N={N} sites/step={P}", file=sys.stderr,
@@ -1214,6 +1237,9 @@ def main():
             # colour steps per launch and site (TinyMT draws over launched sites)
             "steps_per_launch": round(draws / tm["kernel_sites"], 2)
             if tm["kernel_sites"] else None,
+            "launch_note": "a launch is one window of every unfinished sample; the samples run "
+                           "in two parts on concurrent streams (rt_set_streams), one dispatch "
+                           "per part, so avg_launch_us is HIP-event time per window",
             "valu": valu}
     sweep = None if args.no_sweep else plain_sweep(rt)
     if sweep:

@@ -140,6 +140,14 @@ int rt_set_xcd_order(rt_ctx* ctx, int on);
  * rectangle-triangle pairs 4. */
 int rt_set_halo(rt_ctx* ctx, int halo);
 
+/* Streams of a batched CFTP or (eager) batched walk job (no reference
+ * counterpart; default 2, 1..4): a job of at least two workgroups per CU
+ * splits its units (samples, chains) into that many parts of about equal
+ * tile count, each launched on its own stream, so one part's tile loads
+ * and stores overlap another's colour steps; the parts join before the
+ * call returns.  Results never depend on it. */
+int rt_set_streams(rt_ctx* ctx, int n);
+
 /* Round callback of the CFTP entry points (no reference counterpart: the
  * reference's driver loop, AztecDiamondCftpEx.cpp:57-77, reports nothing
  * until it has coupled).  After each round's coalescence test, in the

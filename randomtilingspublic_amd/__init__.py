@@ -112,6 +112,7 @@ def lib():
     L.rt_set_xcd_order.argtypes = [vp, C.c_int]
     L.rt_set_round_callback.argtypes = [vp, C.c_void_p, C.c_void_p]
     L.rt_set_halo.argtypes = [vp, C.c_int]
+    L.rt_set_streams.argtypes = [vp, C.c_int]
     L.rt_device_limits.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_size_t),
                                    C.POINTER(C.c_size_t)]
     L.rt_lozenge_cftp_batch.argtypes = [vp, _I32P, _I32P, C.c_int, _U32P, C.c_int, C.c_int,
@@ -295,6 +296,11 @@ class Context:
         over a third of the CUs): windows of at most H pulling steps; never
         changes results."""
         self._chk(self._L.rt_set_halo(self._h, halo))
+
+    def set_streams(self, n):
+        """Streams a batched CFTP / eager batched walk job is split over
+        (default 2, 1..4); never changes results."""
+        self._chk(self._L.rt_set_streams(self._h, n))
 
     def set_round_callback(self, fn):
         """fn(round, round_steps, finished, total) after each CFTP round's

@@ -258,7 +258,7 @@ static int cftp_batch(rt_ctx* ctx, int model, const int32_t* tmax, const int32_t
     if (!rc) rc = fused_tune(ctx, job, &plan);
     if (rc) return rc;
     HIPCHK_RET(ctx, hipEventRecord(ctx->evk0, ctx->stream));
-    rc = fused_launch(ctx, job, plan);
+    rc = fused_launch(ctx, job, plan, true);
     if (rc) return rc;
     HIPCHK_RET(ctx, hipEventRecord(ctx->evk1, ctx->stream));
     rc = fused_sync(ctx, job);
@@ -732,7 +732,7 @@ static int walk_batch(rt_ctx* ctx, int32_t* const* tilings, const int* Ns, int n
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     HIPCHK_RET(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    int lrc = fused_launch(ctx, job, plan);
+    int lrc = fused_launch(ctx, job, plan, false);
     if (!lrc) lrc = fused_sync(ctx, job);
     hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
     if (lrc) {
@@ -751,7 +751,7 @@ static int walk_batch(rt_ctx* ctx, int32_t* const* tilings, const int* Ns, int n
     ctx->last.kernel_sites = plan.sites(plan.first, plan.nlaunch);
   } else {
     HIPCHK_RET(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    rc = fused_launch(ctx, job, plan);
+    rc = fused_launch(ctx, job, plan, true);
     if (rc) return rc;
     HIPCHK_RET(ctx, hipEventRecord(ctx->evk1, ctx->stream));
     rc = fused_sync(ctx, job);

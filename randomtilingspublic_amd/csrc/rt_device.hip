@@ -313,7 +313,23 @@ extern "C" int rt_create(rt_ctx** out, int device_ordinal) {
   return RT_OK;
 }
 
+int rt_ctx::ensure_streams(int n) {
+  for (int k = 0; k < n && k < 4; k++) {
+    if (k && !sx[k] && hipStreamCreate(&sx[k]) != hipSuccess) {
+      err = "extra stream";
+      return RT_EDEVICE;
+    }
+    if (!evj[k] && hipEventCreateWithFlags(&evj[k], hipEventDisableTiming) != hipSuccess) {
+      err = "stream join event";
+      return RT_EDEVICE;
+    }
+  }
+  return RT_OK;
+}
+
 rt_ctx::~rt_ctx() {
+  for (hipStream_t s : sx)
+    if (s) hipStreamSynchronize(s);
   if (stream) hipStreamSynchronize(stream);
   if (arena) hipFree(arena);
   if (aux) hipFree(aux);
@@ -326,6 +342,10 @@ rt_ctx::~rt_ctx() {
   if (evk1) hipEventDestroy(evk1);
   for (hipEvent_t e : evt)
     if (e) hipEventDestroy(e);
+  for (hipEvent_t e : evj)
+    if (e) hipEventDestroy(e);
+  for (hipStream_t s : sx)
+    if (s) hipStreamDestroy(s);
   if (stream) hipStreamDestroy(stream);
 }
 
@@ -373,6 +393,12 @@ extern "C" int rt_set_halo(rt_ctx* ctx, int halo) {
   if (halo != ctx->halo || set != ctx->halo_set) ctx->tuned.clear();
   ctx->halo = halo;
   ctx->halo_set = set;
+  return RT_OK;
+}
+
+extern "C" int rt_set_streams(rt_ctx* ctx, int n) {
+  if (!ctx || n < 1 || n > 4) return RT_EINVAL;
+  ctx->nstreams = n;
   return RT_OK;
 }
 

@@ -650,50 +650,52 @@ static int xcd_of(const rt_ctx* ctx, unsigned nwork) {
 static unsigned grid_of(int xcd, unsigned nwork) { return xcd ? (nwork + 7u) & ~7u : nwork; }
 
 template <int MODEL, int NC, int W, int R>
-static void launch(rt_ctx* ctx, unsigned nblocks, const UnitDesc* units, const WorkItem* work,
-                   int H, const ModelParams& mp, const WindowRec* wins) {
+static void launch(rt_ctx* ctx, hipStream_t st, unsigned nblocks, const UnitDesc* units,
+                   const WorkItem* work, int H, const ModelParams& mp, const WindowRec* wins) {
   const int xcd = xcd_of(ctx, nblocks);
   hipLaunchKernelGGL((k_fused<MODEL, NC, W, R>), dim3(grid_of(xcd, nblocks)), dim3(64 * W), 0,
-                     ctx->stream, units, work, (int)nblocks, xcd, H, mp, wins);
+                     st, units, work, (int)nblocks, xcd, H, mp, wins);
 }
 
-typedef void (*LaunchFn)(rt_ctx*, unsigned, const UnitDesc*, const WorkItem*, int,
+typedef void (*LaunchFn)(rt_ctx*, hipStream_t, unsigned, const UnitDesc*, const WorkItem*, int,
                          const ModelParams&, const WindowRec*);
 
 template <int MODEL, int NC>
-static void launch_o8(rt_ctx* ctx, unsigned nblocks, const UnitDesc* units, const WorkItem* work,
-                      int H, const ModelParams& mp, const WindowRec* wins) {
+static void launch_o8(rt_ctx* ctx, hipStream_t st, unsigned nblocks, const UnitDesc* units,
+                      const WorkItem* work, int H, const ModelParams& mp, const WindowRec* wins) {
   const int xcd = xcd_of(ctx, nblocks);
   if (NC == 2)
-    hipLaunchKernelGGL((k_fused_o8<MODEL>), dim3(grid_of(xcd, nblocks)), dim3(1024), 0,
-                       ctx->stream, units, work, (int)nblocks, xcd, H, mp, wins);
+    hipLaunchKernelGGL((k_fused_o8<MODEL>), dim3(grid_of(xcd, nblocks)), dim3(1024), 0, st,
+                       units, work, (int)nblocks, xcd, H, mp, wins);
   else
-    hipLaunchKernelGGL((k_walk_o8<MODEL>), dim3(grid_of(xcd, nblocks)), dim3(1024), 0,
-                       ctx->stream, units, work, (int)nblocks, xcd, H, mp, wins);
+    hipLaunchKernelGGL((k_walk_o8<MODEL>), dim3(grid_of(xcd, nblocks)), dim3(1024), 0, st,
+                       units, work, (int)nblocks, xcd, H, mp, wins);
 }
 
 // Lozenge chain pairs on tile shape 7: both chains packed into one LDS word
 // per cell (k_pair_lozenge) when the halo is a multiple of 3, which that
 // kernel's compile-time row residues need; k_fused_o8<1> otherwise.
-// XCD order also pays for large grids of this kernel, whose tile loads
-// re-read their neighbours' halo: per launch on the side-300 hexagon CFTP
-// (6 rounds), plain -> XCD order, 32 samples (2176 workgroups) 100.9 ->
-// 91.7 us, 24 samples 80.9 -> 74.0, 16 samples 59.0 -> 56.4, but 8
-// samples (544, about one workgroup per slot) 37.0 -> 38.8; so it is on up
-// to the CU count and above four workgroups per CU.
-static void launch_pair_lozenge(rt_ctx* ctx, unsigned nblocks, const UnitDesc* units,
-                                const WorkItem* work, int H, const ModelParams& mp,
-                                const WindowRec* wins) {
+// XCD order pays for every grid of k_pair_lozenge, whose tile loads
+// re-read their neighbours' halo: 6 doubling rounds of the side-300 hexagon
+// CFTP, kernel ms plain -> XCD order, one stream: 8 samples 179 -> 171, 16
+// samples 316 -> 290, 32 samples 516 -> 516; two streams: 8 samples 140 ->
+// 139, 16 samples 247 -> 231, 32 samples 471 -> 451.  (The 48 x 64
+// byte-cell kernel k_fused_o8<1> keeps the rule of the other shapes above
+// four workgroups per CU, where it was measured.)
+static void launch_pair_lozenge(rt_ctx* ctx, hipStream_t st, unsigned nblocks,
+                                const UnitDesc* units, const WorkItem* work, int H,
+                                const ModelParams& mp, const WindowRec* wins) {
+  const bool pair = H % 3 == 0;
   const int xcd = xcd_of(ctx, nblocks) ||
-                          (ctx->xcd_order == 1 && nblocks > 4u * (unsigned)ctx->num_cu)
+                          (ctx->xcd_order == 1 && (pair || nblocks > 4u * (unsigned)ctx->num_cu))
                       ? 1
                       : 0;
-  if (H % 3 == 0)
-    hipLaunchKernelGGL(k_pair_lozenge<kPairLW>, dim3(grid_of(xcd, nblocks)), dim3(1024), 0,
-                       ctx->stream, units, work, (int)nblocks, xcd, H, mp, wins);
+  if (pair)
+    hipLaunchKernelGGL(k_pair_lozenge<kPairLW>, dim3(grid_of(xcd, nblocks)), dim3(1024), 0, st,
+                       units, work, (int)nblocks, xcd, H, mp, wins);
   else
-    hipLaunchKernelGGL((k_fused_o8<M_LOZENGE>), dim3(grid_of(xcd, nblocks)), dim3(1024), 0,
-                       ctx->stream, units, work, (int)nblocks, xcd, H, mp, wins);
+    hipLaunchKernelGGL((k_fused_o8<M_LOZENGE>), dim3(grid_of(xcd, nblocks)), dim3(1024), 0, st,
+                       units, work, (int)nblocks, xcd, H, mp, wins);
 }
 
 template <int MODEL, int NC>
@@ -706,11 +708,12 @@ static LaunchFn pick_byte(int shape) {
 }
 
 template <int NC, int W, int R>
-static void launch_swar(rt_ctx* ctx, unsigned nblocks, const UnitDesc* units, const WorkItem* work,
-                        int H, const ModelParams& mp, const WindowRec* wins) {
+static void launch_swar(rt_ctx* ctx, hipStream_t st, unsigned nblocks, const UnitDesc* units,
+                        const WorkItem* work, int H, const ModelParams& mp,
+                        const WindowRec* wins) {
   const int xcd = xcd_of(ctx, nblocks);
   hipLaunchKernelGGL((k_swar_lozenge<NC, W, R>), dim3(grid_of(xcd, nblocks)), dim3(64 * W), 0,
-                     ctx->stream, units, work, (int)nblocks, xcd, H, mp, wins);
+                     st, units, work, (int)nblocks, xcd, H, mp, wins);
 }
 
 // Lozenge: four cells per lane-word (k_swar_lozenge), same tiles: 64-row
@@ -756,6 +759,7 @@ static int upload_tiles(rt_ctx* ctx, const FusedJob& job, LaunchShape ls, uint8_
     units = job.units;
   }
   std::vector<WorkItem> work;
+  std::vector<size_t> unit_end;  // work-list size after each (pair) unit
   for (size_t u = 0; u < units.size(); u += ls.split ? 2 : 1) {
     int tr, tc;
     tile_counts(units[u], geo, &tr, &tc);
@@ -768,7 +772,9 @@ static int upload_tiles(rt_ctx* ctx, const FusedJob& job, LaunchShape ls, uint8_
       work.push_back(WorkItem{(int)u, t});
       if (ls.split) work.push_back(WorkItem{(int)u + 1, t});
     }
+    unit_end.push_back(work.size());
   }
+
   UnitDesc* d_units = (UnitDesc*)p;
   p += (units.size() * sizeof(UnitDesc) + 255) & ~(size_t)255;
   WorkItem* d_work = (WorkItem*)p;
@@ -793,7 +799,7 @@ static int upload_tiles(rt_ctx* ctx, const FusedJob& job, LaunchShape ls, uint8_
   // return, so wait before the vectors go out of scope.
   HIPCHK_RET(ctx, hipStreamSynchronize(ctx->stream));
   *out = FusedPlan::Tiles{d_units, d_work, (unsigned)work.size(), ls.shape, ls.split, d_wins,
-                          units.size()};
+                          units.size(), std::vector<unsigned>(unit_end.begin(), unit_end.end())};
   return RT_OK;
 }
 
@@ -920,7 +926,7 @@ int fused_tune(rt_ctx* ctx, FusedJob& job, FusedPlan* plan) {
       return RT_EINVAL;
     }
     for (size_t x = trials[k].l0; x < trials[k].l1; x++)
-      fn(ctx, t.nwork, t.d_units, t.d_work, H, job.mp, t.d_wins + x * t.nunits);
+      fn(ctx, ctx->stream, t.nwork, t.d_units, t.d_work, H, job.mp, t.d_wins + x * t.nunits);
     HIPCHK_RET(ctx, hipGetLastError());
     HIPCHK_RET(ctx, hipEventRecord(ctx->evt[k + 1], ctx->stream));
   }
@@ -968,7 +974,7 @@ void fused_add_trials(const FusedPlan& plan, rt_timing* acc) {
   }
 }
 
-int fused_launch(rt_ctx* ctx, const FusedJob& job, const FusedPlan& plan) {
+int fused_launch(rt_ctx* ctx, const FusedJob& job, const FusedPlan& plan, bool split_streams) {
   ctx->last.shape = job.shape;
   ctx->last.shape_source = job.shape_source;
   ctx->last.split = job.split ? 1 : 0;
@@ -980,8 +986,50 @@ int fused_launch(rt_ctx* ctx, const FusedJob& job, const FusedPlan& plan) {
     return RT_EINVAL;
   }
   const int H = job.halo;
-  for (size_t l = plan.first; l < plan.nlaunch; l++)
-    fn(ctx, t.nwork, t.d_units, t.d_work, H, job.mp, t.d_wins + l * t.nunits);
+  // The units in two halves on two streams: a half's next window waits only
+  // for its own last one, so one half's tile loads and stores (HBM-bound,
+  // a third of a launch) overlap the other half's steps (VALU-bound) and
+  // launch boundaries.  Kernel ms, 6 doubling rounds of the side-300
+  // hexagon CFTP on k_pair_lozenge (XCD order), streams 1 / 2 / 3 / 4:
+  // 8 samples 171 / 139 / 150 / 234, 16 samples 290 / 231 / 234 / 293, 32
+  // samples 516 / 451 / 456 / 524.
+  // parts: contiguous work-list ranges at unit boundaries, about equal
+  std::vector<unsigned> cut{0};
+  // (only grids of two workgroups per CU or more: below that a part's
+  // launches are latency-bound and splitting only doubles the dispatches,
+  // 8 order-24 two-periodic domino samples 2.2 -> 1.2 samples/s)
+  const int want = split_streams && plan.nlaunch > plan.first + 1 &&
+                           t.nwork >= 2u * (unsigned)ctx->num_cu
+                       ? ctx->nstreams
+                       : 1;
+  for (int k = 1; k < want; k++) {
+    const unsigned target = (unsigned)((size_t)t.nwork * k / want);
+    unsigned best = 0;
+    for (unsigned e : t.ends)
+      if (e > cut.back() && e < t.nwork &&
+          (best == 0 || std::labs((long)e - (long)target) < std::labs((long)best - (long)target)))
+        best = e;
+    if (best) cut.push_back(best);
+  }
+  cut.push_back(t.nwork);
+  const int parts = (int)cut.size() - 1;
+  if (parts > 1) {
+    int rc = ctx->ensure_streams(parts);
+    if (rc) return rc;
+    HIPCHK_RET(ctx, hipEventRecord(ctx->evj[0], ctx->stream));
+    for (int k = 1; k < parts; k++)
+      HIPCHK_RET(ctx, hipStreamWaitEvent(ctx->sx[k], ctx->evj[0], 0));
+  }
+  for (size_t l = plan.first; l < plan.nlaunch; l++) {
+    const WindowRec* w = t.d_wins + l * t.nunits;
+    for (int k = 0; k < parts; k++)
+      fn(ctx, k ? ctx->sx[k] : ctx->stream, cut[k + 1] - cut[k], t.d_units, t.d_work + cut[k], H,
+         job.mp, w);
+  }
+  for (int k = 1; k < parts; k++) {
+    HIPCHK_RET(ctx, hipEventRecord(ctx->evj[k], ctx->sx[k]));
+    HIPCHK_RET(ctx, hipStreamWaitEvent(ctx->stream, ctx->evj[k], 0));
+  }
   if (plan.ncopies)
     hipLaunchKernelGGL(k_copy_list, dim3(64, std::min(plan.ncopies, 4096u)), dim3(256), 0,
                        ctx->stream, plan.d_copies, plan.ncopies);
@@ -1010,7 +1058,8 @@ int fused_launch_windows(rt_ctx* ctx, int model, int nc, int shape, const ModelP
     ctx->err = "no sweep-block kernel for this tile shape";
     return RT_EINVAL;
   }
-  fn(ctx, nwork, d_units, d_work, fused_shape_halo(ctx, model, nc, shape), mp, d_wins);
+  fn(ctx, ctx->stream, nwork, d_units, d_work, fused_shape_halo(ctx, model, nc, shape), mp,
+     d_wins);
   HIPCHK_RET(ctx, hipGetLastError());
   ctx->last.launches += 1;
   return RT_OK;
@@ -1034,7 +1083,7 @@ int fused_run(rt_ctx* ctx, FusedJob& job, void* scratch) {
   int rc = fused_upload(ctx, job, scratch, &plan);
   if (!rc) rc = fused_tune(ctx, job, &plan);
   if (rc) return rc;
-  return fused_launch(ctx, job, plan);
+  return fused_launch(ctx, job, plan, false);
 }
 
 int fused_sync(rt_ctx* ctx, const FusedJob& job) {

@@ -167,6 +167,7 @@ struct FusedPlan {
     bool split;
     const WindowRec* d_wins;  // [launch][unit of this launch shape]
     size_t nunits;
+    std::vector<unsigned> ends;  // work-list size after each unit (stream split points)
   };
   const UnitDesc* d_units = nullptr;  // the tiles of job.shape
   const WorkItem* d_work = nullptr;
@@ -193,7 +194,10 @@ struct FusedPlan {
 };
 int fused_upload(rt_ctx* ctx, FusedJob& job, void* scratch, FusedPlan* plan);
 int fused_tune(rt_ctx* ctx, FusedJob& job, FusedPlan* plan);
-int fused_launch(rt_ctx* ctx, const FusedJob& job, const FusedPlan& plan);
+// The plan's launches after the tuner's; with split_streams (callers that
+// are not under graph capture) the units are split into rt_set_streams
+// parts run on that many streams, joined before it returns.
+int fused_launch(rt_ctx* ctx, const FusedJob& job, const FusedPlan& plan, bool split_streams);
 // Slot of a launch shape in rt_timing.shape_*: shape - 1, +4 when split;
 // 7 for 16x4 pairs at eight waves per SIMD (shape 7).
 inline int shape_slot(int shape, bool split) {

@@ -27,6 +27,12 @@ struct rt_ctx {
   int device = 0;
   int num_cu = 256;
   hipStream_t stream = nullptr;
+  // second stream (default priority, created on first use) and its fork /
+  // join events, for batch launches split in two halves
+  hipStream_t sx[4] = {};          // sx[1..3]; part 0 runs on `stream`
+  hipEvent_t evj[4] = {};
+  int nstreams = 2;                // streams a batch job's launches are split over (1..4)
+  int ensure_streams(int n);
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipEvent_t evk0 = nullptr, evk1 = nullptr;  // around sweep-kernel sequences
   std::string err;

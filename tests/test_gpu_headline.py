@@ -93,6 +93,34 @@ def test_headline_pair_kernel_matches_oracle_at_hexagon_300(gpu_ctx, hexagon):
         o.set_threads(1)
 
 
+def test_stream_split_never_changes_results(gpu_ctx, hexagon):
+    """The batch's units split over 1, 2 or 3 streams (the default is 2):
+    the same chains after two doubling rounds of 9 hexagon-300 samples
+    (576 workgroups, enough to split; odd, so the parts differ in size)."""
+    tmin, tmax, P = hexagon
+    gpu_ctx.set_tinymt_params(P)
+    runs = []
+    try:
+        for n in (1, 2, 3):
+            _say("hexagon 300 cftp pairs, 9 samples, 2 rounds, streams", n)
+            gpu_ctx.set_streams(n)
+            tops, bots, _ = gpu_ctx.lozenge_cftp_batch_rounds(
+                tmax, tmin, [3, 1, 4, 1, 5, 9, 2, 6, 5], rt.SCHED_DOUBLING, T0, 2)
+            runs.append((tops, bots))
+    finally:
+        gpu_ctx.set_streams(2)
+    for tops, bots in runs[1:]:
+        assert all(np.array_equal(a, b) for a, b in zip(tops, runs[0][0]))
+        assert all(np.array_equal(a, b) for a, b in zip(bots, runs[0][1]))
+    # a master seed twice (1 at 1 and 3, 5 at 4 and 8): the same chains
+    assert np.array_equal(runs[0][0][1], runs[0][0][3])
+    assert np.array_equal(runs[0][1][4], runs[0][1][8])
+    with pytest.raises(rt.RTError):
+        gpu_ctx.set_streams(0)
+    with pytest.raises(rt.RTError):
+        gpu_ctx.set_streams(5)
+
+
 def test_rounds_entry_point_agrees_with_cftp_batch(gpu_ctx):
     """On a small hexagon: where the CFTP batch says a sample coalesced in
     round r, the fixed-rounds call at r returns two equal chains equal to
