@@ -209,9 +209,10 @@ __device__ __forceinline__ void fused_body(const UnitDesc* __restrict__ units,
   UnitDesc U = units[wi.unit];
   window_buffers(U, w.flags);
   const int rows = U.rows, cols = U.cols;
-  const int TH = LH - 2 * H, TW = kLW - 2 * H;
+  const int HC = col_halo(MODEL, H);
+  const int TH = LH - 2 * H, TW = kLW - 2 * HC;
   const int tr = wi.tile / U.tiles_c, tc = wi.tile % U.tiles_c;
-  const int r0 = tr * TH - H, c0 = tc * TW - H;
+  const int r0 = tr * TH - H, c0 = tc * TW - HC;
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform row offset
   const int gc = c0 + lane;
@@ -282,8 +283,8 @@ __device__ __forceinline__ void fused_body(const UnitDesc* __restrict__ units,
 #pragma unroll
   for (int r = 0; r < R; r++) {
     const int lr = wv + W * r, gr = r0 + lr;
-    const bool inner = col_in && gr >= 0 && gr < rows && lr >= H && lr < H + TH && lane >= H &&
-                       lane < H + TW;
+    const bool inner = col_in && gr >= 0 && gr < rows && lr >= H && lr < H + TH && lane >= HC &&
+                       lane < HC + TW;
     if (!inner) continue;
 #pragma unroll
     for (int ch = 0; ch < NC; ch++)
@@ -417,7 +418,8 @@ static bool pair_job_is_large(const rt_ctx* ctx, long tiles64) { return 3 * tile
 static TileGeo tile_geo(const rt_ctx* ctx, int model, int nc, int shape, int H) {
   if (packed_pair(ctx, model, nc, shape, H)) return TileGeo{3072 / kPairLW, kPairLW, H, H / 3};
   const Geo g = shape_geo(shape);
-  return TileGeo{g.W * g.R, kLW, H, H};
+  if (model == M_LOZENGE && ctx->engine == RT_ENGINE_FUSED_SWAR) return TileGeo{g.W * g.R, kLW, H, H};
+  return TileGeo{g.W * g.R, kLW, H, col_halo(model, H)};
 }
 
 static void tile_counts(const UnitDesc& u, const TileGeo& g, int* tr, int* tc) {

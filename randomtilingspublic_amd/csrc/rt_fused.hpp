@@ -24,6 +24,18 @@ namespace rt {
 
 enum ModelKind { M_DOMINO = 0, M_LOZENGE = 1, M_RECTTRI = 2 };
 
+// Columns of halo a tile needs for windows of at most H pulling steps.  A
+// pull reads rows i-1 and i+1, so rows go stale one deeper per pull; its
+// column offsets are 1 only in some rows: domino (dominokernel*.cl
+// UpdateTiles, j -/+ (i+t) parity) in one row parity per side, so at most
+// ceil(H/2) stale columns per side; lozenge (lozengekernel.cl:38-53, p2 p4
+// p8 p16) in one row residue mod 3 each, at most ceil(H/3).  Both bounds
+// hold over every colour sequence (host tests, exhaustively).
+// Rectangle-triangle keeps H.
+__host__ __device__ constexpr int col_halo(int model, int H) {
+  return model == M_DOMINO ? (H + 1) / 2 : model == M_LOZENGE ? (H + 2) / 3 : H;
+}
+
 // One op byte per colour step:
 //   bits 0-1 colour rotated;
 //   bit 2    domino: pull the rotated colour from the other one;

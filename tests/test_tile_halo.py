@@ -1,11 +1,13 @@
-"""Halo of the lozenge chain-pair tiles: rows H, columns H / 3.
+"""Halo of the sweep-block tiles: H rows, ceil(H/3) lozenge and ceil(H/2)
+domino columns.
 
-The packed-pair kernel loads a tile with a halo of H rows and only H / 3
-columns, and runs a window of at most H pulling colour steps on it
-(a step of colour c pulls the bits facing c from the colours rotated since
-c last was, LozengeTiler.cpp:86-91 picks the colours, lozengekernel.cl:
-38-53 gives the neighbour offsets).  Two checks that the tile interior stays
-exact:
+The sweep-block kernels load a tile with a halo of H rows but fewer
+columns, and run a window of at most H pulling colour steps on it (a step
+of colour c pulls the bits facing c from the colours rotated since c last
+was; LozengeTiler.cpp:86-91 and DominoTiler.cpp:69-97 pick the colours,
+lozengekernel.cl:38-53 and dominokernel*.cl UpdateTiles give the
+neighbour offsets).  Checks that the tile interior stays exact, for the
+lozenge model:
 
 * the dependency frontier of the pull rule, over every colour sequence and
   every order of the colours' last rotations: after p pulls at most p rows
@@ -13,9 +15,15 @@ exact:
   bounds are reached;
 * the reference's own step (the oracle's literal Rotate-then-Update walk):
   with garbage in every cell outside a region, a walk of a few steps
-  leaves the region shrunk by p rows and ceil(p / 3) plane columns equal to
-  the clean walk, for p the walk's pull count; shrinking by one plane
-  column less fails for some seed, so the column bound is not slack.
+  leaves the region shrunk by q rows and ceil(q / 3) plane columns equal to
+  the clean walk, for q the walk's pull count plus one (the reference's
+  UpdateTiles after the last rotation spreads once more; the lazy engine
+  makes that refresh once per walk, over the whole grid); shrinking by one
+  plane column less fails for some seed, so the column bound is not slack;
+
+and the same two for the domino model (every colour change is a pull; a
+pull reads rows i-1, i+1 and column j - 1 or j + 1 by row parity): p rows
+and ceil(p / 2) columns.
 """
 import itertools
 import math
@@ -121,11 +129,78 @@ def test_reference_step_interior_survives_garbage_outside_the_tile():
         bad[mask] = rng.integers(0, 64, size=int(mask.sum()))
         want = o.lozenge_walk(t0.ravel(), steps, seed, P).reshape(N, N)
         got = o.lozenge_walk(bad.ravel(), steps, seed, P).reshape(N, N)
+        p += 1  # the final UpdateTiles
         h = math.ceil(p / 3)
         rows = slice(r0 + p, r1 - p)
         assert np.array_equal(got[rows, 3 * (q0 + h):3 * (q1 - h)],
                               want[rows, 3 * (q0 + h):3 * (q1 - h)]), (seed, steps, p)
         if h >= 1 and not np.array_equal(got[rows, 3 * (q0 + h - 1):3 * (q1 - h + 1)],
                                          want[rows, 3 * (q0 + h - 1):3 * (q1 - h + 1)]):
+            slack_fails += 1
+    assert slack_fails > 0
+
+
+def _domino_frontier(p):
+    """Stale columns at the right / left edge and stale rows at the top after
+    p pulls, the colours alternating (UpdateTiles(tiling, reftiling, N, t):
+    cell (i, j) of the other colour reads reftiling at (i-1, j), (i+1, j),
+    (i, j - (i+t+1)%2) and (i, j + (i+t)%2))."""
+    worst = (0, 0, 0)
+    for first in (0, 1):
+        R = [[0, 0], [0, 0]]
+        L = [[0, 0], [0, 0]]
+        T = [0, 0]
+        c = first
+        for _ in range(p):
+            t = 1 - c  # the colour rotated last, pulled from
+            nR, nL = list(R[c]), list(L[c])
+            for r in range(2):
+                right, left = (r + t) % 2, (r + t + 1) % 2
+                for rr, dj in (((r - 1) % 2, 0), ((r + 1) % 2, 0), (r, -left), (r, right)):
+                    nR[r] = max(nR[r], R[t][rr] + dj)
+                    nL[r] = max(nL[r], L[t][rr] - dj)
+            R[c], L[c], T[c] = nR, nL, max(T[c], T[t] + 1)
+            c = t
+        worst = tuple(max(a, b) for a, b in zip(worst, (max(map(max, R)), max(map(max, L)),
+                                                         max(T))))
+    return worst
+
+
+def test_domino_pull_frontier_rows_p_columns_ceil_p_over_2():
+    for p in range(1, 13):
+        right, left, top = _domino_frontier(p)
+        assert top == p
+        assert right == left == math.ceil(p / 2), (p, right, left)
+
+
+def _domino_pulls(seed, steps):
+    cols = [int(x) % 2 for x in o.mt19937(seed, steps + 1)[1:]]  # the first output seeds TinyMT
+    return sum(1 for a, b in zip(cols, cols[1:]) if a != b)
+
+
+def test_domino_reference_step_interior_survives_garbage_outside_the_tile():
+    t0 = np.asarray(rt.DominoTiler.MinTiling(rt.DominoTiler.AztecDiamond(40)), dtype=np.int32)
+    N = rt.side(t0)
+    P = rt.synthetic_params(N * N // 2, 92)
+    t0 = o.domino_walk(t0, 4000, 5, P, rt.DOMINO_HEATBATH).reshape(N, N)
+    r0, r1, q0, q1 = 20, 62, 10, 32  # rows and plane columns (2 tiling columns each)
+    rng = np.random.default_rng(4)
+    slack_fails = 0
+    for seed in range(1, 41):
+        steps = 4 + seed % 11
+        p = _domino_pulls(seed, steps)
+        bad = t0.copy()
+        mask = np.ones((N, N), dtype=bool)
+        mask[r0:r1, 2 * q0:2 * q1] = False
+        bad[mask] = rng.integers(0, 16, size=int(mask.sum()))
+        want = o.domino_walk(t0.ravel(), steps, seed, P, rt.DOMINO_HEATBATH).reshape(N, N)
+        got = o.domino_walk(bad.ravel(), steps, seed, P, rt.DOMINO_HEATBATH).reshape(N, N)
+        p += 1  # the final UpdateTiles
+        h = math.ceil(p / 2)
+        rows = slice(r0 + p, r1 - p)
+        assert np.array_equal(got[rows, 2 * (q0 + h):2 * (q1 - h)],
+                              want[rows, 2 * (q0 + h):2 * (q1 - h)]), (seed, steps, p)
+        if h >= 1 and not np.array_equal(got[rows, 2 * (q0 + h - 1):2 * (q1 - h + 1)],
+                                         want[rows, 2 * (q0 + h - 1):2 * (q1 - h + 1)]):
             slack_fails += 1
     assert slack_fails > 0
