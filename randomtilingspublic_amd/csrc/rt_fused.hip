@@ -1051,17 +1051,16 @@ int fused_serve_shape(const rt_ctx* ctx, int model, int nc, long tiles64) {
   return tiles64 <= ctx->num_cu ? SH_16x4 : SH_16x3_O8;
 }
 
-int fused_launch_windows(rt_ctx* ctx, int model, int nc, int shape, const ModelParams& mp,
-                         const UnitDesc* d_units, const WorkItem* d_work, unsigned nwork,
-                         const WindowRec* d_wins) {
+int fused_launch_windows(rt_ctx* ctx, hipStream_t st, int model, int nc, int shape,
+                         const ModelParams& mp, const UnitDesc* d_units, const WorkItem* d_work,
+                         unsigned nwork, const WindowRec* d_wins) {
   if (nwork == 0) return RT_OK;
   LaunchFn fn = pick(ctx, model, nc, shape);
   if (!fn) {
     ctx->err = "no sweep-block kernel for this tile shape";
     return RT_EINVAL;
   }
-  fn(ctx, ctx->stream, nwork, d_units, d_work, fused_shape_halo(ctx, model, nc, shape), mp,
-     d_wins);
+  fn(ctx, st, nwork, d_units, d_work, fused_shape_halo(ctx, model, nc, shape), mp, d_wins);
   HIPCHK_RET(ctx, hipGetLastError());
   ctx->last.launches += 1;
   return RT_OK;

@@ -224,9 +224,9 @@ void fused_add_trials(const FusedPlan& plan, rt_timing* acc);
 int fused_serve_shape(const rt_ctx* ctx, int model, int nc, long tiles64);
 // One launch over `nwork` work items of tile shape `shape` where each unit
 // follows its own window (wins[unit]); model / chains / params as in a job.
-int fused_launch_windows(rt_ctx* ctx, int model, int nc, int shape, const ModelParams& mp,
-                         const UnitDesc* d_units, const WorkItem* d_work, unsigned nwork,
-                         const WindowRec* d_wins);
+int fused_launch_windows(rt_ctx* ctx, hipStream_t st, int model, int nc, int shape,
+                         const ModelParams& mp, const UnitDesc* d_units, const WorkItem* d_work,
+                         unsigned nwork, const WindowRec* d_wins);
 // Tile counts of a unit under a tile shape's geometry.
 void fused_tiles(const rt_ctx* ctx, int model, int nc, int shape, int rows, int cols,
                  int* tiles_r, int* tiles_c);

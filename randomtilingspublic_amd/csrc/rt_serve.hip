@@ -143,6 +143,8 @@ int group_of(int model, bool walk) {
 }
 int group_model(int g) { return g == 0 || g == 2 ? M_DOMINO : (g == 4 ? M_RECTTRI : M_LOZENGE); }
 int group_nc(int g) { return g < 2 ? 2 : 1; }
+// stream of a launch group within a tick: domino pairs, lozenge pairs, walks
+int group_stream(int g) { return g < 2 ? g : 2; }
 constexpr int SH_SERVE_COUNT = 1;  // tile shape whose tile count picks the launch shape (64 rows)
 constexpr int kOpsPerUnit = 64;  // = the fused kernel's window limit
 // Windows per tick: each tick launches kSub sweep kernels back to back from
@@ -557,6 +559,22 @@ static int run_tick(Server* S) {
       hipMemcpyAsync(S->d_ops + ops_at(b, 0, 0, 0), S->h_ops + ops_at(b, 0, 0, 0), obytes,
                      hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
     return fail(S, "tick upload", RT_EDEVICE);
+  // Launch groups are independent: domino pairs, lozenge pairs and the
+  // one-chain walks each get a stream (forked after the uploads above and
+  // joined before the ended slots below), so one group's tile loads
+  // overlap another's steps.
+  int live[3] = {0, 0, 0};
+  for (int m = 0; m < kGroups; m++)
+    if (!S->unit_slot[m].empty()) live[group_stream(m)] = 1;
+  const bool fork = live[0] + live[1] + live[2] > 1;
+  if (fork) {
+    int rc = ctx->ensure_streams(3);
+    if (rc) return fail(S, ctx->err, rc);
+    if (hipEventRecord(ctx->evj[0], ctx->stream) != hipSuccess ||
+        hipStreamWaitEvent(ctx->sx[1], ctx->evj[0], 0) != hipSuccess ||
+        hipStreamWaitEvent(ctx->sx[2], ctx->evj[0], 0) != hipSuccess)
+      return fail(S, "stream fork", RT_EDEVICE);
+  }
   for (int k = 0; k < kSub; k++)
     for (int m = 0; m < kGroups; m++) {
       if (S->unit_slot[m].empty()) continue;
@@ -564,11 +582,17 @@ static int run_tick(Server* S) {
       // parameters carry the rectangle-triangle weights (the reference
       // kernel's table for every served walk)
       ModelParams mp = S->unit_slot[m][0]->req->mp;
-      int rc = fused_launch_windows(ctx, group_model(m), group_nc(m), S->shape[m], mp,
-                                    S->d_units[m], S->d_work[m], S->nwork[m],
-                                    S->d_wins + wins_at(b, k, m));
+      const int gs = fork ? group_stream(m) : 0;
+      int rc = fused_launch_windows(ctx, gs ? ctx->sx[gs] : ctx->stream, group_model(m),
+                                    group_nc(m), S->shape[m], mp, S->d_units[m], S->d_work[m],
+                                    S->nwork[m], S->d_wins + wins_at(b, k, m));
       if (rc) return fail(S, ctx->err, rc);
     }
+  if (fork)
+    for (int g = 1; g < 3; g++)
+      if (hipEventRecord(ctx->evj[g], ctx->sx[g]) != hipSuccess ||
+          hipStreamWaitEvent(ctx->stream, ctx->evj[g], 0) != hipSuccess)
+        return fail(S, "stream join", RT_EDEVICE);
   // slots whose round ended: cells back into the first buffer set, lazy
   // sync of their planes, then (CFTP) coalescence flag
   for (Slot* s : ended) {
