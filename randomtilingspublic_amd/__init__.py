@@ -291,10 +291,10 @@ class Context:
         self._chk(self._L.rt_set_autotune(self._h, mode))
 
     def set_halo(self, halo):
-        """Sweep-block halo depth H (1..12; 0 = the default: 6, and 9 for
-        lozenge chain-pair jobs on shape 7 whose 64-row tiles would fill
-        over a third of the CUs): windows of at most H pulling steps; never
-        changes results."""
+        """Sweep-block halo depth H (1..12; 0 = the default: 6, and 9 / 8
+        for lozenge / domino chain-pair jobs on shape 7 whose 64-row tiles
+        would fill over a third of the CUs): windows of at most H pulling
+        steps; never changes results."""
         self._chk(self._L.rt_set_halo(self._h, halo))
 
     def set_streams(self, n):

@@ -337,6 +337,7 @@ __global__ __launch_bounds__(1024, 8) void k_walk_o8(const UnitDesc* __restrict_
 
 #include "swar_lozenge.hpp"
 #include "pair_lozenge.hpp"
+#include "pair_domino.hpp"
 
 namespace rt {
 
@@ -401,13 +402,21 @@ struct TileGeo {
 // a deeper row halo costs little, so windows are half again as long and
 // the per-tile-load cost (HBM traffic, tile set-up) falls by a third.
 constexpr int kPairLW = 32;
-constexpr int kPairHalo = 9;
+// Default halo of the packed-pair kernels: lozenge 9 (above), domino 8
+// (k_pair_domino, Aztec diamond of order 300, 5 doubling rounds, kernel ms
+// at H = 6 / 8 / 10: 8 samples 88 / 83 / 81, 32 samples 309 / 304 / 308;
+// against 620 ms for the byte-cell k_fused_o8<0> at 32 samples and 140
+// against 69 at 4).
+static int pair_halo(int model) { return model == M_DOMINO ? 8 : 9; }
 
-// Launches that run k_pair_lozenge: lozenge chain pairs (not split) on tile
-// shape 7 with a halo divisible by 3, on the byte-cell engine.
+// Launches that run the packed-pair kernels on 96 x 32 tiles: chain pairs
+// (not split) on tile shape 7, lozenge with a halo divisible by 3 on the
+// byte-cell engine (k_pair_lozenge), domino with an even halo
+// (k_pair_domino).
 static bool packed_pair(const rt_ctx* ctx, int model, int nc, int shape, int H) {
-  return shape == SH_16x3_O8 && model == M_LOZENGE && nc == 2 && H % 3 == 0 &&
-         ctx->engine != RT_ENGINE_FUSED_SWAR;
+  if (shape != SH_16x3_O8 || nc != 2) return false;
+  if (model == M_LOZENGE) return H % 3 == 0 && ctx->engine != RT_ENGINE_FUSED_SWAR;
+  return model == M_DOMINO && H % 2 == 0;
 }
 
 // A lone small job (64-row tiles for under a third of the CUs) does better
@@ -416,7 +425,8 @@ static bool packed_pair(const rt_ctx* ctx, int model, int nc, int shape, int H) 
 static bool pair_job_is_large(const rt_ctx* ctx, long tiles64) { return 3 * tiles64 > ctx->num_cu; }
 
 static TileGeo tile_geo(const rt_ctx* ctx, int model, int nc, int shape, int H) {
-  if (packed_pair(ctx, model, nc, shape, H)) return TileGeo{3072 / kPairLW, kPairLW, H, H / 3};
+  if (packed_pair(ctx, model, nc, shape, H))
+    return TileGeo{3072 / kPairLW, kPairLW, H, col_halo(model, H)};
   const Geo g = shape_geo(shape);
   if (model == M_LOZENGE && ctx->engine == RT_ENGINE_FUSED_SWAR) return TileGeo{g.W * g.R, kLW, H, H};
   return TileGeo{g.W * g.R, kLW, H, col_halo(model, H)};
@@ -502,7 +512,7 @@ static void choose_shape(const rt_ctx* ctx, FusedJob& job) {
     job.shape = swar ? SH_16x4 : SH_16x3_O8;
     return;
   }
-  if (ctx->tile_shape == SH_AUTO && job.halo == kPairHalo &&
+  if (ctx->tile_shape == SH_AUTO && job.halo == pair_halo(job.model) &&
       packed_pair(ctx, job.model, job.nc, SH_16x3_O8, job.halo)) {
     job.shape = SH_16x3_O8;  // fastest at every batch size measured: no trials
     return;
@@ -610,15 +620,16 @@ __global__ void k_copy_list(const CopyDesc* __restrict__ list, unsigned n) {
 
 // Halo of a job's tiles: the context's setting, except rect-triangle pairs
 // (32-row int32 tiles, H = 4) and, unless rt_set_halo was called, lozenge
-// chain pairs that may run k_pair_lozenge (H = 9).  Every shape of a job has
-// the same halo, so one window list serves all.
+// and domino chain pairs that may run the packed-pair kernels (H = 9 and
+// 8).  Every shape of a job has the same halo, so one window list serves
+// all.
 static void plan_windows(rt_ctx* ctx, FusedJob& job) {
   if (job.model == M_RECTTRI && job.nc == 2)
     job.halo = shape_geo(SH_RT8x4).H;
   else if (!ctx->halo_set && (ctx->tile_shape == SH_AUTO || ctx->tile_shape == SH_16x3_O8) &&
-           packed_pair(ctx, job.model, job.nc, SH_16x3_O8, kPairHalo) &&
+           packed_pair(ctx, job.model, job.nc, SH_16x3_O8, pair_halo(job.model)) &&
            (ctx->tile_shape == SH_16x3_O8 || pair_job_is_large(ctx, tiles64(job))))
-    job.halo = kPairHalo;
+    job.halo = pair_halo(job.model);
   else
     job.halo = ctx->halo;
   cut_windows(job, job.halo);
@@ -700,9 +711,25 @@ static void launch_pair_lozenge(rt_ctx* ctx, hipStream_t st, unsigned nblocks,
                        units, work, (int)nblocks, xcd, H, mp, wins);
 }
 
+// Domino chain pairs on tile shape 7: k_pair_domino when the halo is even
+// (its row parities), k_fused_o8<0> otherwise.
+static void launch_pair_domino(rt_ctx* ctx, hipStream_t st, unsigned nblocks,
+                               const UnitDesc* units, const WorkItem* work, int H,
+                               const ModelParams& mp, const WindowRec* wins) {
+  const bool pair = H % 2 == 0;
+  const int xcd = xcd_of(ctx, nblocks) || (ctx->xcd_order == 1 && pair) ? 1 : 0;
+  if (pair)
+    hipLaunchKernelGGL(k_pair_domino, dim3(grid_of(xcd, nblocks)), dim3(1024), 0, st, units,
+                       work, (int)nblocks, xcd, H, mp, wins);
+  else
+    hipLaunchKernelGGL((k_fused_o8<M_DOMINO>), dim3(grid_of(xcd, nblocks)), dim3(1024), 0, st,
+                       units, work, (int)nblocks, xcd, H, mp, wins);
+}
+
 template <int MODEL, int NC>
 static LaunchFn pick_byte(int shape) {
   if (shape == SH_16x3_O8 && MODEL == M_LOZENGE && NC == 2) return launch_pair_lozenge;
+  if (shape == SH_16x3_O8 && MODEL == M_DOMINO && NC == 2) return launch_pair_domino;
   if (shape == SH_16x3_O8) return launch_o8<MODEL, NC>;
   if (shape == SH_16x4) return launch<MODEL, NC, 16, 4>;
   if (shape == SH_16x2) return launch<MODEL, NC, 16, 2>;
@@ -1042,8 +1069,8 @@ int fused_launch(rt_ctx* ctx, const FusedJob& job, const FusedPlan& plan, bool s
 
 int fused_serve_shape(const rt_ctx* ctx, int model, int nc, long tiles64) {
   if (model == M_RECTTRI) return nc == 2 ? SH_RT8x4 : SH_8x8;
-  // lozenge chain pairs: k_pair_lozenge from a third of the CUs up (see kPairLW)
-  if (packed_pair(ctx, model, nc, SH_16x3_O8, kPairHalo) && pair_job_is_large(ctx, tiles64))
+  // chain pairs: the packed-pair kernels from a third of the CUs up (see kPairLW)
+  if (packed_pair(ctx, model, nc, SH_16x3_O8, pair_halo(model)) && pair_job_is_large(ctx, tiles64))
     return SH_16x3_O8;
   // Few tiles: one 16-wave workgroup per CU on 64-row tiles; more tiles
   // than CUs: the 48-row tiles at eight waves per SIMD (two workgroups per
@@ -1067,7 +1094,8 @@ int fused_launch_windows(rt_ctx* ctx, hipStream_t st, int model, int nc, int sha
 }
 
 int fused_shape_halo(const rt_ctx* ctx, int model, int nc, int shape) {
-  return packed_pair(ctx, model, nc, shape, kPairHalo) ? kPairHalo : shape_geo(shape).H;
+  return packed_pair(ctx, model, nc, shape, pair_halo(model)) ? pair_halo(model)
+                                                              : shape_geo(shape).H;
 }
 
 void fused_tiles(const rt_ctx* ctx, int model, int nc, int shape, int rows, int cols,

@@ -231,8 +231,9 @@ int fused_launch_windows(rt_ctx* ctx, hipStream_t st, int model, int nc, int sha
 void fused_tiles(const rt_ctx* ctx, int model, int nc, int shape, int rows, int cols,
                  int* tiles_r, int* tiles_c);
 // Halo depth of a serving launch group on a tile shape (the most pulling
-// steps one window may take): 9 for lozenge chain pairs on shape 7
-// (k_pair_lozenge), else the shape's own (6; rect-triangle pairs 4).
+// steps one window may take): 9 for lozenge and 8 for domino chain pairs on
+// shape 7 (the packed-pair kernels), else the shape's own (6;
+// rect-triangle pairs 4).
 int fused_shape_halo(const rt_ctx* ctx, int model, int nc, int shape);
 // Bring every colour plane of every unit up to date (end of a walk).
 int fused_sync(rt_ctx* ctx, const FusedJob& job);
