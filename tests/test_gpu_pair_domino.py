@@ -84,7 +84,8 @@ def test_pair_domino_matches_oracle_at_aztec_250(gpu_ctx, aztec250, rule):
 @pytest.mark.parametrize("halo", [2, 4, 6, 8, 12, 5])
 def test_pair_domino_halo_never_changes_results(gpu_ctx, halo):
     """Aztec order 70 (142 rows, 71 plane columns: several tiles at every
-    halo), 3 samples, both rules, tile shape 7 against 16 x 4 pairs at the
+    halo), 3 samples, both rules (two-periodic also with weights whose
+    ratio is no power of two), tile shape 7 against 16 x 4 pairs at the
     default halo; the oracle for one sample."""
     d = rt.DominoTiler.AztecDiamond(70)
     tmin, tmax = rt.DominoTiler.MinTiling(d), rt.DominoTiler.MaxTiling(d)
@@ -93,7 +94,7 @@ def test_pair_domino_halo_never_changes_results(gpu_ctx, halo):
     gpu_ctx.set_tinymt_params(P)
     masters = [4, 9, 16]
     try:
-        for rule in RULES:
+        for rule in RULES + [(rt.DOMINO_HEATBATH2P, 0.9, 1.3)]:
             gpu_ctx.set_halo(0)
             gpu_ctx.set_tile_shape(2)
             want_t, want_b, _ = gpu_ctx.domino_cftp_batch_rounds(tmax, tmin, masters,
