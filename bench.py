@@ -1241,6 +1241,9 @@ def main():
                            "in two parts on concurrent streams (rt_set_streams), one dispatch "
                            "per part, so avg_launch_us is HIP-event time per window",
             "valu": valu}
+    # the later sections make contexts of their own; this one's streams
+    # would share the process's hardware queues with theirs
+    ctx.close()
     sweep = None if args.no_sweep else plain_sweep(rt)
     if sweep:
         print(f"[bench] plain sweep {sweep}", file=sys.stderr, flush=True)

@@ -146,7 +146,12 @@ int rt_set_halo(rt_ctx* ctx, int halo);
  * splits its units (samples, chains) into that many parts of about equal
  * tile count, each launched on its own stream, so one part's tile loads
  * and stores overlap another's colour steps; the parts join before the
- * call returns.  Results never depend on it. */
+ * call returns.  Results never depend on it.  Streams run concurrently only
+ * on distinct hardware queues, of which a process has GPU_MAX_HW_QUEUES
+ * (4 by default): every context holds one stream, up to three more once it
+ * splits a job or serves, so keep few contexts open at once (a server's
+ * stream groups measured 27 % slower with another context's two streams
+ * open in the process). */
 int rt_set_streams(rt_ctx* ctx, int n);
 
 /* Round callback of the CFTP entry points (no reference counterpart: the
