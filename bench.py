@@ -545,7 +545,7 @@ PROFILE_WORKLOADS = {
     "hex300_cftp_32seeds": (["--profile-rounds", "6", "--samples", "32", "--tile-shape", "7",
                              "--streams", "1"], "k_pair_lozenge<32>", 56),
     "aztec3000_step": (["--profile-section", "sweep"], "k_domino_step<1>", 48),
-    "batch256_walk": (["--profile-section", "walks"], "k_walk_o8<0>", 48),
+    "batch256_walk": (["--profile-section", "walks"], "k_pair_domino", 48),
     "bibone_hex200": (["--profile-section", "bibone", "--bibone-steps", "20000"],
                       "k_bi_edge<64, 16, 1024>", 46),
 }
@@ -562,8 +562,10 @@ PROFILE_TEXT = {
                       "(N=6006), 1000 steps, one launch per colour step; 48 B per attempt: "
                       "TinyMT state 16 + 16 B, parameters 12 B, cells 4 B",
     "batch256_walk": "256 heat-bath domino chains on uneven domains (seed 4), 10^4 steps, one "
-                     "hipGraph, autotuned tile shape; 48 B per site per launch: TinyMT state "
-                     "16 + 16 B, parameters 12 B, 2 colour planes read and written",
+                     "hipGraph (two parallel chains of launches), tile shape 7 (k_pair_domino, "
+                     "one chain in the low halves, 96 x 32 tiles, halo 8 rows and 4 columns); "
+                     "48 B per site per launch: TinyMT state 16 + 16 B, parameters 12 B, 2 "
+                     "colour planes read and written",
     "bibone_hex200": "bibone walk on the hexagon with sides 200, 201, 200 (tiling side 812, "
                      "406 x 406 plane cells), 2 x 10^4 steps on the edge form (one bit per "
                      "lattice edge), 64 x 64-cell tiles with halo 16, 8 steps per launch; 46 B "

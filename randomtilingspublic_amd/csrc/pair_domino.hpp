@@ -34,8 +34,10 @@
 //   or the ratio itself, scaled by a power of two).  Cells without a TinyMT
 //   stream (the border ring) get a target no cell equals.
 //
-// The host uses this kernel for domino chain pairs (heat-bath rules, which
-// CFTP requires) on tile shape 7 when H is even.
+// The host uses this kernel on tile shape 7 when H is even, for domino
+// chain pairs (heat-bath rules, which CFTP requires) and for one-chain walks
+// with a heat-bath rule (the chain in the low halves; a packed operation
+// costs the same for one live half).
 #pragma once
 
 namespace rt {
@@ -141,6 +143,9 @@ __global__ __launch_bounds__(1024, 8) void k_pair_domino(const UnitDesc* __restr
   const int q = band & 1;                    // its parity (r0 is even)
   const int gc = c0 + col;
   const bool col_in = gc >= 0 && gc < cols;
+  // a CFTP pair, or one chain (a walk) in the low halves, the high halves
+  // then zero: never a target, never stored
+  const bool pair = U.plane[1][0] != nullptr;
 
   for (int k = threadIdx.x; k < 2 * PL; k += kThreads) lds[k] = 0;
   if (threadIdx.x < K) sops[threadIdx.x] = U.ops[s0 + threadIdx.x];
@@ -164,8 +169,9 @@ __global__ __launch_bounds__(1024, 8) void k_pair_domino(const UnitDesc* __restr
       const long idx = (long)gr * cols + gc;
 #pragma unroll
       for (int k = 0; k < 2; k++)
-        p[k * PL + r * RS] = (uint32_t)((const uint8_t*)U.plane[0][k])[idx] |
-                             ((uint32_t)((const uint8_t*)U.plane[1][k])[idx] << 16);
+        p[k * PL + r * RS] =
+            (uint32_t)((const uint8_t*)U.plane[0][k])[idx] |
+            (pair ? (uint32_t)((const uint8_t*)U.plane[1][k])[idx] << 16 : 0u);
     }
     if (hs) {
       const long id = stream_id<M_DOMINO>(gr, gc, rows, cols);
@@ -247,7 +253,7 @@ __global__ __launch_bounds__(1024, 8) void k_pair_domino(const UnitDesc* __restr
     for (int k = 0; k < 2; k++) {
       const uint32_t x = p[k * PL + r * RS];
       ((uint8_t*)U.plane_b[0][k])[idx] = (uint8_t)x;
-      ((uint8_t*)U.plane_b[1][k])[idx] = (uint8_t)(x >> 16);
+      if (pair) ((uint8_t*)U.plane_b[1][k])[idx] = (uint8_t)(x >> 16);
     }
     if (((interior[r] & 1u) != 0) && !U.no_state_write) {
       const long id = stream_id<M_DOMINO>(gr, gc, rows, cols);

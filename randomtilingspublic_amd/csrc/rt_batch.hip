@@ -732,7 +732,9 @@ static int walk_batch(rt_ctx* ctx, int32_t* const* tilings, const int* Ns, int n
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     HIPCHK_RET(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    int lrc = fused_launch(ctx, job, plan, false);
+    // the parts' streams join the capture through the fork event and leave
+    // it through the join: the graph holds one chain of launches per part
+    int lrc = fused_launch(ctx, job, plan, true);
     if (!lrc) lrc = fused_sync(ctx, job);
     hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
     if (lrc) {

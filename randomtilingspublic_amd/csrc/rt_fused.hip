@@ -409,14 +409,18 @@ constexpr int kPairLW = 32;
 // against 69 at 4).
 static int pair_halo(int model) { return model == M_DOMINO ? 8 : 9; }
 
-// Launches that run the packed-pair kernels on 96 x 32 tiles: chain pairs
-// (not split) on tile shape 7, lozenge with a halo divisible by 3 on the
-// byte-cell engine (k_pair_lozenge), domino with an even halo
-// (k_pair_domino).
-static bool packed_pair(const rt_ctx* ctx, int model, int nc, int shape, int H) {
-  if (shape != SH_16x3_O8 || nc != 2) return false;
-  if (model == M_LOZENGE) return H % 3 == 0 && ctx->engine != RT_ENGINE_FUSED_SWAR;
-  return model == M_DOMINO && H % 2 == 0;
+// Launches that run the packed kernels on 96 x 32 tiles, on tile shape 7:
+// lozenge chain pairs with a halo divisible by 3 on the byte-cell engine
+// (k_pair_lozenge); domino units with an even halo (k_pair_domino), chain
+// pairs (CFTP pairs: every CFTP entry point and the server take heat-bath
+// rules only) and one-chain walks whose rule is a heat-bath one, the only
+// moves that kernel has (rule: the job's, or -1 when its units differ).
+// The tile geometry and the kernel a launch function picks both follow it.
+static bool packed_pair(const rt_ctx* ctx, int model, int nc, int shape, int H, int rule) {
+  if (shape != SH_16x3_O8) return false;
+  if (model == M_LOZENGE) return nc == 2 && H % 3 == 0 && ctx->engine != RT_ENGINE_FUSED_SWAR;
+  if (model != M_DOMINO || H % 2 != 0) return false;
+  return nc == 2 || (nc == 1 && (rule == D_HEATBATH || rule == D_HEATBATH2P));
 }
 
 // A lone small job (64-row tiles for under a third of the CUs) does better
@@ -424,8 +428,8 @@ static bool packed_pair(const rt_ctx* ctx, int model, int nc, int shape, int H) 
 // against 93 ms), so k_pair_lozenge takes over from that size up.
 static bool pair_job_is_large(const rt_ctx* ctx, long tiles64) { return 3 * tiles64 > ctx->num_cu; }
 
-static TileGeo tile_geo(const rt_ctx* ctx, int model, int nc, int shape, int H) {
-  if (packed_pair(ctx, model, nc, shape, H))
+static TileGeo tile_geo(const rt_ctx* ctx, int model, int nc, int shape, int H, int rule) {
+  if (packed_pair(ctx, model, nc, shape, H, rule))
     return TileGeo{3072 / kPairLW, kPairLW, H, col_halo(model, H)};
   const Geo g = shape_geo(shape);
   if (model == M_LOZENGE && ctx->engine == RT_ENGINE_FUSED_SWAR) return TileGeo{g.W * g.R, kLW, H, H};
@@ -513,7 +517,7 @@ static void choose_shape(const rt_ctx* ctx, FusedJob& job) {
     return;
   }
   if (ctx->tile_shape == SH_AUTO && job.halo == pair_halo(job.model) &&
-      packed_pair(ctx, job.model, job.nc, SH_16x3_O8, job.halo)) {
+      packed_pair(ctx, job.model, job.nc, SH_16x3_O8, job.halo, job.mp.rule)) {
     job.shape = SH_16x3_O8;  // fastest at every batch size measured: no trials
     return;
   }
@@ -569,7 +573,7 @@ static void cut_windows(FusedJob& job, int H) {
 }
 
 static size_t tiles_bytes(const rt_ctx* ctx, const FusedJob& job, int shape, bool split) {
-  const TileGeo g = tile_geo(ctx, job.model, split ? 1 : job.nc, shape, job.halo);
+  const TileGeo g = tile_geo(ctx, job.model, split ? 1 : job.nc, shape, job.halo, job.mp.rule);
   const size_t per = split ? 2 : 1;
   size_t work = 0;
   for (const UnitDesc& u : job.units) {
@@ -627,7 +631,7 @@ static void plan_windows(rt_ctx* ctx, FusedJob& job) {
   if (job.model == M_RECTTRI && job.nc == 2)
     job.halo = shape_geo(SH_RT8x4).H;
   else if (!ctx->halo_set && (ctx->tile_shape == SH_AUTO || ctx->tile_shape == SH_16x3_O8) &&
-           packed_pair(ctx, job.model, job.nc, SH_16x3_O8, pair_halo(job.model)) &&
+           packed_pair(ctx, job.model, job.nc, SH_16x3_O8, pair_halo(job.model), job.mp.rule) &&
            (ctx->tile_shape == SH_16x3_O8 || pair_job_is_large(ctx, tiles64(job))))
     job.halo = pair_halo(job.model);
   else
@@ -712,24 +716,29 @@ static void launch_pair_lozenge(rt_ctx* ctx, hipStream_t st, unsigned nblocks,
 }
 
 // Domino chain pairs on tile shape 7: k_pair_domino when the halo is even
-// (its row parities), k_fused_o8<0> otherwise.
+// (its row parities), k_fused_o8<0> otherwise (the same test as
+// packed_pair, which fixes the tiles).
+template <int NC>
 static void launch_pair_domino(rt_ctx* ctx, hipStream_t st, unsigned nblocks,
                                const UnitDesc* units, const WorkItem* work, int H,
                                const ModelParams& mp, const WindowRec* wins) {
-  const bool pair = H % 2 == 0;
+  const bool pair = packed_pair(ctx, M_DOMINO, NC, SH_16x3_O8, H, mp.rule);
   const int xcd = xcd_of(ctx, nblocks) || (ctx->xcd_order == 1 && pair) ? 1 : 0;
   if (pair)
     hipLaunchKernelGGL(k_pair_domino, dim3(grid_of(xcd, nblocks)), dim3(1024), 0, st, units,
                        work, (int)nblocks, xcd, H, mp, wins);
-  else
+  else if (NC == 2)
     hipLaunchKernelGGL((k_fused_o8<M_DOMINO>), dim3(grid_of(xcd, nblocks)), dim3(1024), 0, st,
+                       units, work, (int)nblocks, xcd, H, mp, wins);
+  else
+    hipLaunchKernelGGL((k_walk_o8<M_DOMINO>), dim3(grid_of(xcd, nblocks)), dim3(1024), 0, st,
                        units, work, (int)nblocks, xcd, H, mp, wins);
 }
 
 template <int MODEL, int NC>
 static LaunchFn pick_byte(int shape) {
   if (shape == SH_16x3_O8 && MODEL == M_LOZENGE && NC == 2) return launch_pair_lozenge;
-  if (shape == SH_16x3_O8 && MODEL == M_DOMINO && NC == 2) return launch_pair_domino;
+  if (shape == SH_16x3_O8 && MODEL == M_DOMINO) return launch_pair_domino<NC>;
   if (shape == SH_16x3_O8) return launch_o8<MODEL, NC>;
   if (shape == SH_16x4) return launch<MODEL, NC, 16, 4>;
   if (shape == SH_16x2) return launch<MODEL, NC, 16, 2>;
@@ -768,7 +777,8 @@ static LaunchFn pick(const rt_ctx* ctx, int model, int nc, int shape) {
 static int upload_tiles(rt_ctx* ctx, const FusedJob& job, LaunchShape ls, uint8_t*& p,
                         const WindowRec* d_wins, const std::vector<WindowRec>& wins,
                         FusedPlan::Tiles* out) {
-  const TileGeo geo = tile_geo(ctx, job.model, ls.split ? 1 : job.nc, ls.shape, job.halo);
+  const TileGeo geo =
+      tile_geo(ctx, job.model, ls.split ? 1 : job.nc, ls.shape, job.halo, job.mp.rule);
   const size_t nu = job.units.size();
   std::vector<UnitDesc> units;
   if (ls.split) {
@@ -1067,10 +1077,11 @@ int fused_launch(rt_ctx* ctx, const FusedJob& job, const FusedPlan& plan, bool s
   return RT_OK;
 }
 
-int fused_serve_shape(const rt_ctx* ctx, int model, int nc, long tiles64) {
+int fused_serve_shape(const rt_ctx* ctx, int model, int nc, int rule, long tiles64) {
   if (model == M_RECTTRI) return nc == 2 ? SH_RT8x4 : SH_8x8;
   // chain pairs: the packed-pair kernels from a third of the CUs up (see kPairLW)
-  if (packed_pair(ctx, model, nc, SH_16x3_O8, pair_halo(model)) && pair_job_is_large(ctx, tiles64))
+  if (packed_pair(ctx, model, nc, SH_16x3_O8, pair_halo(model), rule) &&
+      pair_job_is_large(ctx, tiles64))
     return SH_16x3_O8;
   // Few tiles: one 16-wave workgroup per CU on 64-row tiles; more tiles
   // than CUs: the 48-row tiles at eight waves per SIMD (two workgroups per
@@ -1087,24 +1098,26 @@ int fused_launch_windows(rt_ctx* ctx, hipStream_t st, int model, int nc, int sha
     ctx->err = "no sweep-block kernel for this tile shape";
     return RT_EINVAL;
   }
-  fn(ctx, st, nwork, d_units, d_work, fused_shape_halo(ctx, model, nc, shape), mp, d_wins);
+  fn(ctx, st, nwork, d_units, d_work, fused_shape_halo(ctx, model, nc, mp.rule, shape), mp,
+     d_wins);
   HIPCHK_RET(ctx, hipGetLastError());
   ctx->last.launches += 1;
   return RT_OK;
 }
 
-int fused_shape_halo(const rt_ctx* ctx, int model, int nc, int shape) {
-  return packed_pair(ctx, model, nc, shape, pair_halo(model)) ? pair_halo(model)
-                                                              : shape_geo(shape).H;
+int fused_shape_halo(const rt_ctx* ctx, int model, int nc, int rule, int shape) {
+  return packed_pair(ctx, model, nc, shape, pair_halo(model), rule) ? pair_halo(model)
+                                                                    : shape_geo(shape).H;
 }
 
-void fused_tiles(const rt_ctx* ctx, int model, int nc, int shape, int rows, int cols,
+void fused_tiles(const rt_ctx* ctx, int model, int nc, int rule, int shape, int rows, int cols,
                  int* tiles_r, int* tiles_c) {
   UnitDesc u{};
   u.rows = rows;
   u.cols = cols;
-  tile_counts(u, tile_geo(ctx, model, nc, shape, fused_shape_halo(ctx, model, nc, shape)), tiles_r,
-              tiles_c);
+  tile_counts(u,
+              tile_geo(ctx, model, nc, shape, fused_shape_halo(ctx, model, nc, rule, shape), rule),
+              tiles_r, tiles_c);
 }
 
 int fused_run(rt_ctx* ctx, FusedJob& job, void* scratch) {

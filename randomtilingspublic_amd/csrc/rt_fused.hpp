@@ -206,9 +206,10 @@ struct FusedPlan {
 };
 int fused_upload(rt_ctx* ctx, FusedJob& job, void* scratch, FusedPlan* plan);
 int fused_tune(rt_ctx* ctx, FusedJob& job, FusedPlan* plan);
-// The plan's launches after the tuner's; with split_streams (callers that
-// are not under graph capture) the units are split into rt_set_streams
-// parts run on that many streams, joined before it returns.
+// The plan's launches after the tuner's; with split_streams the units are
+// split into rt_set_streams parts run on that many streams, forked from and
+// joined into the context's stream with events (under graph capture the
+// parts become parallel chains of the graph).
 int fused_launch(rt_ctx* ctx, const FusedJob& job, const FusedPlan& plan, bool split_streams);
 // Slot of a launch shape in rt_timing.shape_*: shape - 1, +4 when split;
 // 7 for 16x4 pairs at eight waves per SIMD (shape 7).
@@ -221,20 +222,20 @@ void fused_add_trials(const FusedPlan& plan, rt_timing* acc);
 // Tile shape for launches over `tiles64` 64-row tiles of (model, nc)
 // units whose windows are cut by the caller (the serving loop, with the
 // halo of fused_shape_halo).
-int fused_serve_shape(const rt_ctx* ctx, int model, int nc, long tiles64);
+int fused_serve_shape(const rt_ctx* ctx, int model, int nc, int rule, long tiles64);
 // One launch over `nwork` work items of tile shape `shape` where each unit
 // follows its own window (wins[unit]); model / chains / params as in a job.
 int fused_launch_windows(rt_ctx* ctx, hipStream_t st, int model, int nc, int shape,
                          const ModelParams& mp, const UnitDesc* d_units, const WorkItem* d_work,
                          unsigned nwork, const WindowRec* d_wins);
 // Tile counts of a unit under a tile shape's geometry.
-void fused_tiles(const rt_ctx* ctx, int model, int nc, int shape, int rows, int cols,
+void fused_tiles(const rt_ctx* ctx, int model, int nc, int rule, int shape, int rows, int cols,
                  int* tiles_r, int* tiles_c);
 // Halo depth of a serving launch group on a tile shape (the most pulling
 // steps one window may take): 9 for lozenge and 8 for domino chain pairs on
 // shape 7 (the packed-pair kernels), else the shape's own (6;
 // rect-triangle pairs 4).
-int fused_shape_halo(const rt_ctx* ctx, int model, int nc, int shape);
+int fused_shape_halo(const rt_ctx* ctx, int model, int nc, int rule, int shape);
 // Bring every colour plane of every unit up to date (end of a walk).
 int fused_sync(rt_ctx* ctx, const FusedJob& job);
 

@@ -218,6 +218,9 @@ struct Server {
   size_t work_cap[kGroups] = {};
   unsigned nwork[kGroups] = {};
   int shape[kGroups] = {};                // tile shape of each group's launches
+  // flip rule shared by every live unit of a group, -1 when they differ
+  // (the kernels read each unit's own; this picks the group's kernel)
+  int group_rule[kGroups] = {};
   std::vector<Slot*> unit_slot[kGroups];  // unit index -> slot, per group
   WindowRec* d_wins = nullptr;            // [buf][sub][group][kMaxSlots]
   uint8_t* d_ops = nullptr;               // [buf][sub][group][kMaxSlots][kOpsPerUnit]
@@ -383,13 +386,16 @@ static int upload_tables(Server* S) {
     S->unit_slot[m].clear();
     // tile shape from the live tile count (64-row tiles)
     long t64 = 0;
+    int rule = -2;
     for (Slot* s : S->active) {
       if (s->group != m) continue;
       int tr, tc;
-      fused_tiles(S->ctx, model, nc, SH_SERVE_COUNT, s->req->N, (int)s->cols, &tr, &tc);
+      fused_tiles(S->ctx, model, nc, -1, SH_SERVE_COUNT, s->req->N, (int)s->cols, &tr, &tc);
       t64 += (long)tr * tc;
+      rule = rule == -2 || rule == s->req->mp.rule ? s->req->mp.rule : -1;
     }
-    S->shape[m] = fused_serve_shape(S->ctx, model, nc, t64);
+    S->group_rule[m] = rule < 0 ? -1 : rule;
+    S->shape[m] = fused_serve_shape(S->ctx, model, nc, S->group_rule[m], t64);
     for (Slot* s : S->active) {
       if (s->group != m) continue;
       UnitDesc d{};
@@ -414,7 +420,7 @@ static int upload_tables(Server* S) {
       const int u = (int)units.size();
       d.ops = S->d_ops;  // window s0 = ops_at(buffer, sub, model, unit)
       int tr, tc;
-      fused_tiles(S->ctx, model, nc, S->shape[m], d.rows, d.cols, &tr, &tc);
+      fused_tiles(S->ctx, model, nc, S->group_rule[m], S->shape[m], d.rows, d.cols, &tr, &tc);
       d.tiles_c = tc;
       for (int t = 0; t < tr * tc; t++) work.push_back(WorkItem{u, t});
       units.push_back(d);
@@ -523,7 +529,8 @@ static int run_tick(Server* S) {
   std::vector<Slot*> ended;
   for (int k = 0; k < kSub; k++) {
     for (int m = 0; m < kGroups; m++) {
-      const int H = fused_shape_halo(S->ctx, group_model(m), group_nc(m), S->shape[m]);
+      const int H = fused_shape_halo(S->ctx, group_model(m), group_nc(m), S->group_rule[m],
+                                     S->shape[m]);
       for (size_t u = 0; u < S->unit_slot[m].size(); u++) {
         Slot* s = S->unit_slot[m][u];
         WindowRec& w = S->h_wins[wins_at(b, k, m) + u];
@@ -582,6 +589,7 @@ static int run_tick(Server* S) {
       // parameters carry the rectangle-triangle weights (the reference
       // kernel's table for every served walk)
       ModelParams mp = S->unit_slot[m][0]->req->mp;
+      mp.rule = S->group_rule[m];  // the group's kernel choice (units keep their own rules)
       const int gs = fork ? group_stream(m) : 0;
       int rc = fused_launch_windows(ctx, gs ? ctx->sx[gs] : ctx->stream, group_model(m),
                                     group_nc(m), S->shape[m], mp, S->d_units[m], S->d_work[m],

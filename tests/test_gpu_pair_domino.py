@@ -110,3 +110,35 @@ def test_pair_domino_halo_never_changes_results(gpu_ctx, halo):
     finally:
         gpu_ctx.set_halo(0)
         gpu_ctx.set_tile_shape(0)
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_packed_domino_walk_batch_matches_oracle(gpu_ctx, use_graph):
+    """One-chain domino walks with a heat-bath rule on k_pair_domino (the
+    chain in the low halves of the words): 40 Aztec diamonds of order
+    60..160 in one batch (about 700 workgroups of 96 x 32 tiles, more than
+    the chip holds), 600 steps, eager (two streams) and as a hipGraph (two
+    parallel chains), against 64-row tiles and, for three chains, the
+    oracle's Walk (DominoTiler.cpp:18-122)."""
+    orders = [60 + (37 * k) % 101 for k in range(40)]
+    tilings = [rt.DominoTiler.MinTiling(rt.DominoTiler.AztecDiamond(n)) for n in orders]
+    big = max(rt.side(t) for t in tilings)
+    P = rt.synthetic_params(big * big // 2, 160)
+    gpu_ctx.set_tinymt_params(P)
+    seeds = [13 * k + 5 for k in range(len(tilings))]
+    try:
+        for rule in RULES:
+            runs = {}
+            for shape in (7, 2):
+                _say("domino walk batch, 40 aztec diamonds, rule", rule, "shape", shape,
+                     "graph", use_graph)
+                gpu_ctx.set_tile_shape(shape)
+                runs[shape] = gpu_ctx.domino_walk_batch(tilings, 600, seeds, rule[0], rule[1],
+                                                        rule[2], use_graph=use_graph)
+            for a, b in zip(runs[7], runs[2]):
+                assert np.array_equal(a, b)
+            for k in (0, 17, 39):
+                assert np.array_equal(runs[7][k],
+                                      o.domino_walk(tilings[k], 600, seeds[k], P, *rule)), k
+    finally:
+        gpu_ctx.set_tile_shape(0)
