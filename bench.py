@@ -34,6 +34,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 XCD_ORDER = None  # --xcd-order: rt_set_xcd_order of the measured contexts (None: the library's)
+STREAMS = 0  # --streams: rt_set_streams of the headline and batch-walk contexts (0: the library's)
 # Sweep-block kernel instantiation per launch-shape slot (rt_timing.shape_*[k]):
 # k_fused<model 1 = lozenge, chains per workgroup, waves, rows per lane>; slots
 # 4..6 run CFTP pairs split, one chain per workgroup; slot 7 is the 48-row
@@ -158,6 +159,8 @@ def batched_walks(rt, seed=4, nchains=256, steps=10 ** 4):
     ctx = rt.Context(0)
     if XCD_ORDER is not None:
         ctx.set_xcd_order(XCD_ORDER)
+    if STREAMS:
+        ctx.set_streams(STREAMS)
     try:
         ctx.set_tinymt_params(rt.synthetic_params(big * big // 2, None))
         seeds = [1 + k for k in range(nchains)]
@@ -175,6 +178,17 @@ def batched_walks(rt, seed=4, nchains=256, steps=10 ** 4):
     alg = 48.0 * tm["kernel_sites"]  # sites of the chains each launch advances
     gbs = alg / dev_s / 1e9
     foot = sum(b["N"] * b["N"] // 2 * (16 + 2) for b in batch)
+    # VALU issue over the device time: wave instructions per site per launch
+    # of the packed kernel (committed SQ profile of this workload) x sites
+    valu = None
+    sqfile = os.path.join(ROOT, "profiles", "kernels_sq.json")
+    if os.path.exists(sqfile) and tm["shape"] == 7:
+        with open(sqfile) as f:
+            sq = json.load(f).get("k_pair_domino")
+        if sq and sq["workload"] == "batch256_walk":
+            ins = sq["valu_wave_insts_per_site"] * tm["kernel_sites"]
+            valu = {"issue_frac": round(ins / (SIMD_WAVE_INSTS_PER_US * dev_s * 1e6), 4),
+                    "source": "profiles/kernels_sq.json (" + sq["profile"] + ")"}
     return {"workload": f"{nchains} heat-bath domino chains on uneven domains from seed {seed} "
                         f"(Aztec order U[32,512] and rectangle unions), {steps} steps each, "
                         "one hipGraph, autotuned tile shape",
@@ -189,7 +203,8 @@ def batched_walks(rt, seed=4, nchains=256, steps=10 ** 4):
             "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
                          "alg_bytes_per_site_per_launch": 48,
-                         "note": "device time includes the final colour-plane sync"}}
+                         "note": "device time includes the final colour-plane sync",
+                         "valu": valu}}
 
 
 def weighted_and_bibone(rt, cftp_seeds=8, cftp_order=24, steps=10 ** 5, rounds512=7):
@@ -545,7 +560,7 @@ PROFILE_WORKLOADS = {
     "hex300_cftp_32seeds": (["--profile-rounds", "6", "--samples", "32", "--tile-shape", "7",
                              "--streams", "1"], "k_pair_lozenge<32>", 56),
     "aztec3000_step": (["--profile-section", "sweep"], "k_domino_step<1>", 48),
-    "batch256_walk": (["--profile-section", "walks"], "k_pair_domino", 48),
+    "batch256_walk": (["--profile-section", "walks", "--streams", "1"], "k_pair_domino", 48),
     "bibone_hex200": (["--profile-section", "bibone", "--bibone-steps", "20000"],
                       "k_bi_edge<64, 16, 1024>", 46),
 }
@@ -562,7 +577,8 @@ PROFILE_TEXT = {
                       "(N=6006), 1000 steps, one launch per colour step; 48 B per attempt: "
                       "TinyMT state 16 + 16 B, parameters 12 B, cells 4 B",
     "batch256_walk": "256 heat-bath domino chains on uneven domains (seed 4), 10^4 steps, one "
-                     "hipGraph (two parallel chains of launches), tile shape 7 (k_pair_domino, "
+                     "hipGraph (one stream here; the benchmark's graph has two parallel "
+                     "chains of launches), tile shape 7 (k_pair_domino, "
                      "one chain in the low halves, 96 x 32 tiles, halo 8 rows and 4 columns); "
                      "48 B per site per launch: TinyMT state 16 + 16 B, parameters 12 B, 2 "
                      "colour planes read and written",
@@ -735,6 +751,8 @@ def rocprof_sq(name, outdir, summarize_only=False):
     if os.path.exists(jpath):
         with open(jpath) as f:
             table = json.load(f)
+    profiled = {k for _, k, _ in PROFILE_WORKLOADS.values()}
+    table = {k: v for k, v in table.items() if k in profiled}
     table[kernel] = {"workload": name, "valu_wave_insts_per_site": round(valu / sites, 4),
                      "sites_per_dispatch": sites, "avg_launch_us": round(avg_us, 2),
                      "valu_issue_frac": round(valu / (SIMD_WAVE_INSTS_PER_US * avg_us), 4),
@@ -1059,8 +1077,9 @@ def main():
                     help="sweep-block halo depth (rt_set_halo; 0 = the library's default; "
                          "results never depend on it)")
     ap.add_argument("--streams", type=int, default=0,
-                    help="streams the headline batch splits its samples over (rt_set_streams; "
-                         "0 = the library's default, 2); the per-dispatch profiles use 1")
+                    help="streams the headline batch and the batch walks split their units "
+                         "over (rt_set_streams; 0 = the library's default, 2); the per-dispatch "
+                         "profiles use 1")
     ap.add_argument("--tile-shape", type=int, default=0,
                     help="force a sweep-block tile shape (rt_set_tile_shape; 0 = autotuned), "
                          "for profiling one kernel")
@@ -1086,8 +1105,9 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
-    global XCD_ORDER
+    global XCD_ORDER, STREAMS
     XCD_ORDER = args.xcd_order
+    STREAMS = args.streams
     if args.rocprof_profile:
         rocprof_profile(*args.rocprof_profile, summarize_only=args.summarize_only)
         return
@@ -1129,8 +1149,8 @@ def main():
     ctx.set_tinymt_params(params)
     ctx.set_tile_shape(args.tile_shape)
     ctx.set_halo(args.halo)
-    if args.streams:
-        ctx.set_streams(args.streams)
+    if STREAMS:
+        ctx.set_streams(STREAMS)
     if XCD_ORDER is not None:
         ctx.set_xcd_order(XCD_ORDER)
     print(f"[bench] device {ctx.device_name} hexagon {side} N={N} sites/step={P}", file=sys.stderr,
