@@ -104,11 +104,12 @@ int rt_set_engine(rt_ctx* ctx, int engine);
  * by 16 waves x 3 rows at eight waves per SIMD (two workgroups per CU,
  * VGPRs capped at 64; lozenge and domino chain pairs there hold both
  * chains in one LDS word per cell, on 96-row x 32-column tiles, when the
- * halo is a multiple of 3 (lozenge) or even (domino)); 0 = chosen per job
- * (autotuner, else a rule of thumb on the tile count; chain pairs at their
- * packed-pair halo take shape 7 directly).  The serving loop picks 64-row
- * tiles while its live tiles fit the CUs, shape 7 beyond (chain pairs:
- * from a third of the CUs up). */
+ * halo is a multiple of 3 (lozenge) or even (domino), and so do one-chain
+ * domino walks with a heat-bath rule, in the low halves); 0 = chosen per
+ * job (autotuner, else a rule of thumb on the tile count; jobs that the
+ * packed kernels take, at their halo, go to shape 7 directly).  The
+ * serving loop picks 64-row tiles while its live tiles fit the CUs, shape
+ * 7 beyond (packed-kernel groups: from a third of the CUs up). */
 int rt_set_tile_shape(rt_ctx* ctx, int shape);
 
 /* Tile-shape autotuner (no reference counterpart; on by default).  With
@@ -134,9 +135,9 @@ int rt_set_xcd_order(rt_ctx* ctx, int on);
  * each unit, tiles overlap by 2H rows and by 2H/3 (lozenge) or H (domino)
  * columns.  Larger H means fewer launches and more redundant halo work.
  * Results never depend on it.  Until it is called, lozenge and domino
- * chain-pair jobs whose 64-row tiles would fill over a third of the CUs
- * take H = 9 and 8 (the packed-pair kernels); halo 0 returns to that
- * default.  Setting it forgets the autotuner's earlier choices; the
+ * chain-pair jobs (and one-chain domino heat-bath walks) whose 64-row tiles
+ * would fill over a third of the CUs take H = 9 and 8 (the packed
+ * kernels); halo 0 returns to that default.  Setting it forgets the autotuner's earlier choices; the
  * serving loop keeps 6 (9 and 8 for lozenge and domino chain pairs on
  * shape 7) and rectangle-triangle pairs 4. */
 int rt_set_halo(rt_ctx* ctx, int halo);
