@@ -112,10 +112,16 @@ def test_pair_domino_halo_never_changes_results(gpu_ctx, halo):
         gpu_ctx.set_tile_shape(0)
 
 
+WALK_RULES = [(rt.DOMINO_TOGGLE08, 1.0, 1.0), (rt.DOMINO_HEATBATH, 1.0, 1.0),
+              (rt.DOMINO_METRO2P, 1.0, 0.5), (rt.DOMINO_HEATBATH2P, 1.0, 0.5),
+              (rt.DOMINO_METRO2P, 0.9, 1.3)]
+
+
 @pytest.mark.parametrize("use_graph", [False, True])
 def test_packed_domino_walk_batch_matches_oracle(gpu_ctx, use_graph):
-    """One-chain domino walks with a heat-bath rule on k_pair_domino (the
-    chain in the low halves of the words): 40 Aztec diamonds of order
+    """One-chain domino walks on tile shape 7 for every rule (heat-bath
+    rules on k_pair_domino, the chain in the low halves of the words; the
+    others on k_walk_o8<0>): 40 Aztec diamonds of order
     60..160 in one batch (about 700 workgroups of 96 x 32 tiles, more than
     the chip holds), 600 steps, eager (two streams) and as a hipGraph (two
     parallel chains), against 64-row tiles and, for three chains, the
@@ -127,7 +133,7 @@ def test_packed_domino_walk_batch_matches_oracle(gpu_ctx, use_graph):
     gpu_ctx.set_tinymt_params(P)
     seeds = [13 * k + 5 for k in range(len(tilings))]
     try:
-        for rule in RULES:
+        for rule in WALK_RULES:
             runs = {}
             for shape in (7, 2):
                 _say("domino walk batch, 40 aztec diamonds, rule", rule, "shape", shape,
