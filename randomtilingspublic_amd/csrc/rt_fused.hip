@@ -425,7 +425,10 @@ static bool packed_pair(const rt_ctx* ctx, int model, int nc, int shape, int H, 
 
 // A lone small job (64-row tiles for under a third of the CUs) does better
 // on the rule of thumb / autotuner at H = 6 (one hexagon-300 sample: 88
-// against 93 ms), so k_pair_lozenge takes over from that size up.
+// against 93 ms), so the packed pair kernels take over from that size up;
+// one-chain domino heat-bath walks take k_pair_domino at every size (us
+// per step, packed at H = 8 against the tuned choice at H = 6: Aztec order
+// 100 1.16 / 1.31, 300 1.17 / 1.31, 512 1.79 / 2.26).
 static bool pair_job_is_large(const rt_ctx* ctx, long tiles64) { return 3 * tiles64 > ctx->num_cu; }
 
 static TileGeo tile_geo(const rt_ctx* ctx, int model, int nc, int shape, int H, int rule) {
@@ -632,7 +635,8 @@ static void plan_windows(rt_ctx* ctx, FusedJob& job) {
     job.halo = shape_geo(SH_RT8x4).H;
   else if (!ctx->halo_set && (ctx->tile_shape == SH_AUTO || ctx->tile_shape == SH_16x3_O8) &&
            packed_pair(ctx, job.model, job.nc, SH_16x3_O8, pair_halo(job.model), job.mp.rule) &&
-           (ctx->tile_shape == SH_16x3_O8 || pair_job_is_large(ctx, tiles64(job))))
+           (ctx->tile_shape == SH_16x3_O8 || pair_job_is_large(ctx, tiles64(job)) ||
+            (job.model == M_DOMINO && job.nc == 1)))
     job.halo = pair_halo(job.model);
   else
     job.halo = ctx->halo;

@@ -108,12 +108,14 @@ def test_domino_walk_batch_rect_unions(gpu_ctx):
     P = rt.synthetic_params(big * big // 2, 24)
     gpu_ctx.set_tinymt_params(P)
     seeds = [7 * k + 3 for k in range(len(tilings))]
-    want = [o.domino_walk(t, 1800, s, P, rt.DOMINO_HEATBATH) for t, s in zip(tilings, seeds)]
+    # the 0.8 toggle: the autotuner picks its tile shape (heat-bath walks go
+    # to the packed kernel directly, checked below)
+    want = [o.domino_walk(t, 1800, s, P, rt.DOMINO_TOGGLE08) for t, s in zip(tilings, seeds)]
     gpu_ctx.set_autotune(2)
     try:
         for use_graph in (False, True):
             _say("domino walk batch rect unions graph=", use_graph)
-            outs = gpu_ctx.domino_walk_batch(tilings, 1800, seeds, rt.DOMINO_HEATBATH,
+            outs = gpu_ctx.domino_walk_batch(tilings, 1800, seeds, rt.DOMINO_TOGGLE08,
                                              use_graph=use_graph)
             tm = gpu_ctx.last_timing()
             # first call tunes (1800 steps: ~150 launches), the second reuses it
@@ -121,6 +123,12 @@ def test_domino_walk_batch_rect_unions(gpu_ctx):
             assert tm["shape"] in (1, 2, 3, 7)
             for got, w in zip(outs, want):
                 assert np.array_equal(got, w)
+        _say("domino walk batch rect unions, heat bath")
+        outs = gpu_ctx.domino_walk_batch(tilings, 1800, seeds, rt.DOMINO_HEATBATH)
+        tm = gpu_ctx.last_timing()
+        assert tm["shape"] == 7 and tm["shape_source"] == 0, tm
+        for got, t, s in zip(outs, tilings, seeds):
+            assert np.array_equal(got, o.domino_walk(t, 1800, s, P, rt.DOMINO_HEATBATH))
     finally:
         gpu_ctx.set_autotune(1)
 
