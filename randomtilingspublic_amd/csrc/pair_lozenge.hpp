@@ -27,8 +27,10 @@
 //   were not loaded); halo rows are never written back, so a wave whose
 //   rows are all stale skips the rest of the window, draws included.
 //
-// The host uses this kernel for lozenge chain pairs on tile shape 7 when
-// H % 3 == 0 and k_fused_o8<1> otherwise; results are identical.
+// The host uses this kernel on tile shape 7 when H % 3 == 0, for lozenge
+// chain pairs and for one-chain walks (the chain in the low halves, the
+// high halves zero: a packed operation costs the same for one live half);
+// k_fused_o8<1> / k_walk_o8<1> otherwise; results are identical.
 #pragma once
 
 namespace rt {
@@ -163,6 +165,9 @@ __global__ __launch_bounds__(1024, 8) void k_pair_lozenge(const UnitDesc* __rest
   const int band = (HALVES == 2 && lane >= LW) ? LH - R - bw : bw;  // tile row of the lane's row 0
   const int gc = c0 + col;
   const bool col_in = gc >= 0 && gc < cols;
+  // a CFTP pair, or one chain (a walk) in the low halves, the high halves
+  // then zero: never a target, never stored
+  const bool pair = U.plane[1][0] != nullptr;
 
   for (int k = threadIdx.x; k < 3 * PL; k += kThreads) lds[k] = 0;
   if (threadIdx.x < K) sops[threadIdx.x] = U.ops[s0 + threadIdx.x];
@@ -181,8 +186,9 @@ __global__ __launch_bounds__(1024, 8) void k_pair_lozenge(const UnitDesc* __rest
       const long idx = (long)gr * cols + gc;
 #pragma unroll
       for (int k = 0; k < 3; k++)
-        p[k * PL + r * RS] = (uint32_t)((const uint8_t*)U.plane[0][k])[idx] |
-                             ((uint32_t)((const uint8_t*)U.plane[1][k])[idx] << 16);
+        p[k * PL + r * RS] =
+            (uint32_t)((const uint8_t*)U.plane[0][k])[idx] |
+            (pair ? (uint32_t)((const uint8_t*)U.plane[1][k])[idx] << 16 : 0u);
       // every lozenge cell of the grid owns a stream: id = j + (N/3) i
       st[r].mat1 = U.param[idx];
       st[r].mat2 = U.param[U.pstride + idx];
@@ -251,7 +257,7 @@ __global__ __launch_bounds__(1024, 8) void k_pair_lozenge(const UnitDesc* __rest
     for (int k = 0; k < 3; k++) {
       const uint32_t x = p[k * PL + r * RS];
       ((uint8_t*)U.plane_b[0][k])[idx] = (uint8_t)x;
-      ((uint8_t*)U.plane_b[1][k])[idx] = (uint8_t)(x >> 16);
+      if (pair) ((uint8_t*)U.plane_b[1][k])[idx] = (uint8_t)(x >> 16);
     }
     if (!U.no_state_write) {
       U.state_b[idx] = st[r].s0;

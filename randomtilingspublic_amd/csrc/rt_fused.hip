@@ -410,15 +410,16 @@ constexpr int kPairLW = 32;
 static int pair_halo(int model) { return model == M_DOMINO ? 8 : 9; }
 
 // Launches that run the packed kernels on 96 x 32 tiles, on tile shape 7:
-// lozenge chain pairs with a halo divisible by 3 on the byte-cell engine
-// (k_pair_lozenge); domino units with an even halo (k_pair_domino), chain
-// pairs (CFTP pairs: every CFTP entry point and the server take heat-bath
-// rules only) and one-chain walks whose rule is a heat-bath one, the only
-// moves that kernel has (rule: the job's, or -1 when its units differ).
+// lozenge units (chain pairs and one-chain walks) with a halo divisible by
+// 3 on the byte-cell engine (k_pair_lozenge); domino units with an even
+// halo (k_pair_domino), chain pairs (CFTP pairs: every CFTP entry point and
+// the server take heat-bath rules only) and one-chain walks whose rule is a
+// heat-bath one, the only moves that kernel has (rule: the job's, or -1
+// when its units differ).
 // The tile geometry and the kernel a launch function picks both follow it.
 static bool packed_pair(const rt_ctx* ctx, int model, int nc, int shape, int H, int rule) {
   if (shape != SH_16x3_O8) return false;
-  if (model == M_LOZENGE) return nc == 2 && H % 3 == 0 && ctx->engine != RT_ENGINE_FUSED_SWAR;
+  if (model == M_LOZENGE) return H % 3 == 0 && ctx->engine != RT_ENGINE_FUSED_SWAR;
   if (model != M_DOMINO || H % 2 != 0) return false;
   return nc == 2 || (nc == 1 && (rule == D_HEATBATH || rule == D_HEATBATH2P));
 }
@@ -703,19 +704,24 @@ static void launch_o8(rt_ctx* ctx, hipStream_t st, unsigned nblocks, const UnitD
 // 139, 16 samples 247 -> 231, 32 samples 471 -> 451.  (The 48 x 64
 // byte-cell kernel k_fused_o8<1> keeps the rule of the other shapes above
 // four workgroups per CU, where it was measured.)
+template <int NC>
 static void launch_pair_lozenge(rt_ctx* ctx, hipStream_t st, unsigned nblocks,
                                 const UnitDesc* units, const WorkItem* work, int H,
                                 const ModelParams& mp, const WindowRec* wins) {
-  const bool pair = H % 3 == 0;
-  const int xcd = xcd_of(ctx, nblocks) ||
-                          (ctx->xcd_order == 1 && (pair || nblocks > 4u * (unsigned)ctx->num_cu))
-                      ? 1
-                      : 0;
+  const bool pair = packed_pair(ctx, M_LOZENGE, NC, SH_16x3_O8, H, mp.rule);
+  const int xcd =
+      xcd_of(ctx, nblocks) ||
+              (ctx->xcd_order == 1 && (pair || (NC == 2 && nblocks > 4u * (unsigned)ctx->num_cu)))
+          ? 1
+          : 0;
   if (pair)
     hipLaunchKernelGGL(k_pair_lozenge<kPairLW>, dim3(grid_of(xcd, nblocks)), dim3(1024), 0, st,
                        units, work, (int)nblocks, xcd, H, mp, wins);
-  else
+  else if (NC == 2)
     hipLaunchKernelGGL((k_fused_o8<M_LOZENGE>), dim3(grid_of(xcd, nblocks)), dim3(1024), 0, st,
+                       units, work, (int)nblocks, xcd, H, mp, wins);
+  else
+    hipLaunchKernelGGL((k_walk_o8<M_LOZENGE>), dim3(grid_of(xcd, nblocks)), dim3(1024), 0, st,
                        units, work, (int)nblocks, xcd, H, mp, wins);
 }
 
@@ -741,7 +747,7 @@ static void launch_pair_domino(rt_ctx* ctx, hipStream_t st, unsigned nblocks,
 
 template <int MODEL, int NC>
 static LaunchFn pick_byte(int shape) {
-  if (shape == SH_16x3_O8 && MODEL == M_LOZENGE && NC == 2) return launch_pair_lozenge;
+  if (shape == SH_16x3_O8 && MODEL == M_LOZENGE) return launch_pair_lozenge<NC>;
   if (shape == SH_16x3_O8 && MODEL == M_DOMINO) return launch_pair_domino<NC>;
   if (shape == SH_16x3_O8) return launch_o8<MODEL, NC>;
   if (shape == SH_16x4) return launch<MODEL, NC, 16, 4>;

@@ -144,3 +144,29 @@ def test_rounds_entry_point_agrees_with_cftp_batch(gpu_ctx):
             assert not np.array_equal(tops[0], bots[0])
     with pytest.raises(rt.RTError):
         gpu_ctx.lozenge_cftp_batch_rounds(tmax, tmin, [1], rt.SCHED_DOUBLING, 64, 0)
+
+
+@pytest.mark.parametrize("halo", [9, 6, 5])
+def test_lozenge_walk_on_shape_7_matches_oracle(gpu_ctx, halo):
+    """A one-chain lozenge Walk (LozengeTiler.cpp:27-159) forced onto tile
+    shape 7: at halos divisible by 3 k_pair_lozenge runs it with the chain
+    in the low halves of its words, otherwise k_walk_o8<1>; the hexagon of
+    side 100 (201 rows, 67 plane columns: several 96 x 32 tiles)."""
+    d = rt.LozengeTiler.AlmostHexDomain(100, 0)
+    tmin = rt.LozengeTiler.MinTiling(d)
+    N = rt.side(tmin)
+    P = rt.synthetic_params(N * N, 101)
+    gpu_ctx.set_tinymt_params(P)
+    want = o.lozenge_walk(tmin, 500, 3, P)
+    steps, seeds = [200, 100], [5, 9]
+    want_seg = o.lozenge_walk_segments(tmin, steps, seeds, P)
+    try:
+        gpu_ctx.set_halo(halo)
+        gpu_ctx.set_tile_shape(7)
+        _say("lozenge walk hexagon 100, shape 7, halo", halo)
+        assert np.array_equal(gpu_ctx.lozenge_walk(tmin, 500, 3), want)
+        assert gpu_ctx.last_timing()["shape"] == 7
+        assert np.array_equal(gpu_ctx.lozenge_walk_segments(tmin, steps, seeds), want_seg)
+    finally:
+        gpu_ctx.set_halo(0)
+        gpu_ctx.set_tile_shape(0)
