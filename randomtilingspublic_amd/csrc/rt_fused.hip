@@ -424,12 +424,15 @@ static bool packed_pair(const rt_ctx* ctx, int model, int nc, int shape, int H, 
   return nc == 2 || (nc == 1 && (rule == D_HEATBATH || rule == D_HEATBATH2P));
 }
 
-// A lone small job (64-row tiles for under a third of the CUs) does better
-// on the rule of thumb / autotuner at H = 6 (one hexagon-300 sample: 88
-// against 93 ms), so the packed pair kernels take over from that size up;
-// one-chain domino heat-bath walks take k_pair_domino at every size (us
-// per step, packed at H = 8 against the tuned choice at H = 6: Aztec order
-// 100 1.16 / 1.31, 300 1.17 / 1.31, 512 1.79 / 2.26).
+// A lone small lozenge job (64-row tiles for under a third of the CUs)
+// does better on the rule of thumb / autotuner at H = 6 (one hexagon-300
+// sample: 88 against 93 ms; hexagon 64..160, 1 or 3 samples, 4 doubling
+// rounds from 1024: 20 / 22 ms), so k_pair_lozenge takes over from that
+// size up.  Domino units take k_pair_domino at H = 8 at every size:
+// one-chain heat-bath walks (us per step against the tuned choice at
+// H = 6: Aztec order 100 1.16 / 1.31, 300 1.17 / 1.31, 512 1.79 / 2.26)
+// and chain pairs (one Aztec sample, 4 doubling rounds from 1024, ms
+// against the same kernel at H = 6: order 64 18.1 / 20.5, 256 19.0 / 21.7).
 static bool pair_job_is_large(const rt_ctx* ctx, long tiles64) { return 3 * tiles64 > ctx->num_cu; }
 
 static TileGeo tile_geo(const rt_ctx* ctx, int model, int nc, int shape, int H, int rule) {
@@ -637,7 +640,7 @@ static void plan_windows(rt_ctx* ctx, FusedJob& job) {
   else if (!ctx->halo_set && (ctx->tile_shape == SH_AUTO || ctx->tile_shape == SH_16x3_O8) &&
            packed_pair(ctx, job.model, job.nc, SH_16x3_O8, pair_halo(job.model), job.mp.rule) &&
            (ctx->tile_shape == SH_16x3_O8 || pair_job_is_large(ctx, tiles64(job)) ||
-            (job.model == M_DOMINO && job.nc == 1)))
+            job.model == M_DOMINO))
     job.halo = pair_halo(job.model);
   else
     job.halo = ctx->halo;
@@ -1089,9 +1092,10 @@ int fused_launch(rt_ctx* ctx, const FusedJob& job, const FusedPlan& plan, bool s
 
 int fused_serve_shape(const rt_ctx* ctx, int model, int nc, int rule, long tiles64) {
   if (model == M_RECTTRI) return nc == 2 ? SH_RT8x4 : SH_8x8;
-  // chain pairs: the packed-pair kernels from a third of the CUs up (see kPairLW)
+  // the packed kernels: domino at every size, lozenge from a third of the
+  // CUs up (as plan_windows)
   if (packed_pair(ctx, model, nc, SH_16x3_O8, pair_halo(model), rule) &&
-      pair_job_is_large(ctx, tiles64))
+      (model == M_DOMINO || pair_job_is_large(ctx, tiles64)))
     return SH_16x3_O8;
   // Few tiles: one 16-wave workgroup per CU on 64-row tiles; more tiles
   // than CUs: the 48-row tiles at eight waves per SIMD (two workgroups per

@@ -152,13 +152,14 @@ constexpr int kOpsPerUnit = 64;  // = the fused kernel's window limit
 // flag reads) is spread over kSub windows of every live request.
 constexpr int kSub = 4;
 
-// Offset of unit u's op bytes for (buffer b, sub-window k, group g).
-static inline size_t ops_at(int b, int k, int g, size_t u) {
-  return ((((size_t)b * kSub + k) * kGroups + g) * kMaxSlots + u) * kOpsPerUnit;
-}
-static inline size_t wins_at(int b, int k, int g) {
-  return (((size_t)b * kSub + k) * kGroups + g) * kMaxSlots;
-}
+// A tick's tables travel to the device in one copy from one pinned block
+// per tick buffer: the window records of the live units, [sub][group][unit]
+// packed with no gaps, then (from the next 256-byte boundary) the op bytes
+// of those windows, K bytes each back to back.  Only the bytes used are
+// copied, so the upload shrinks with the number of live requests.
+constexpr size_t kTickWinBytes = (size_t)kSub * kGroups * kMaxSlots * sizeof(WindowRec);
+constexpr size_t kTickBytes = kTickWinBytes + (size_t)kSub * kGroups * kMaxSlots * kOpsPerUnit;
+static inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // Device bytes of a slot: two TinyMT state sets and, per chain (two for
 // CFTP, one for a walk), the start planes and two working sets.
@@ -222,11 +223,9 @@ struct Server {
   // (the kernels read each unit's own; this picks the group's kernel)
   int group_rule[kGroups] = {};
   std::vector<Slot*> unit_slot[kGroups];  // unit index -> slot, per group
-  WindowRec* d_wins = nullptr;            // [buf][sub][group][kMaxSlots]
-  uint8_t* d_ops = nullptr;               // [buf][sub][group][kMaxSlots][kOpsPerUnit]
+  uint8_t* d_tick = nullptr;              // [buf][kTickBytes]: window records, op bytes
   uint32_t* d_flags = nullptr;      // [buf][kMaxSlots]
-  WindowRec* h_wins = nullptr;
-  uint8_t* h_ops = nullptr;
+  uint8_t* h_tick = nullptr;              // pinned twin of d_tick
   uint32_t* h_flags = nullptr;
   hipEvent_t ev[2] = {nullptr, nullptr};
   bool ev_used[2] = {false, false};
@@ -418,7 +417,7 @@ static int upload_tables(Server* S) {
       d.rule = s->req->mp.rule;
       d.dw = s->req->mp.dw;
       const int u = (int)units.size();
-      d.ops = S->d_ops;  // window s0 = ops_at(buffer, sub, model, unit)
+      d.ops = S->d_tick;  // a window's s0 is its op bytes' offset in d_tick
       int tr, tc;
       fused_tiles(S->ctx, model, nc, S->group_rule[m], S->shape[m], d.rows, d.cols, &tr, &tc);
       d.tiles_c = tc;
@@ -527,13 +526,23 @@ static int run_tick(Server* S) {
     }
   // windows: kSub per tick for every live unit
   std::vector<Slot*> ended;
+  const size_t base = (size_t)b * kTickBytes;
+  WindowRec* hw = reinterpret_cast<WindowRec*>(S->h_tick + base);
+  size_t wbase[kSub][kGroups];
+  size_t nw = 0;
+  for (int k = 0; k < kSub; k++)
+    for (int m = 0; m < kGroups; m++) {
+      wbase[k][m] = nw;
+      nw += S->unit_slot[m].size();
+    }
+  size_t op = base + align256(nw * sizeof(WindowRec));  // next op byte, offset in the block
   for (int k = 0; k < kSub; k++) {
     for (int m = 0; m < kGroups; m++) {
       const int H = fused_shape_halo(S->ctx, group_model(m), group_nc(m), S->group_rule[m],
                                      S->shape[m]);
       for (size_t u = 0; u < S->unit_slot[m].size(); u++) {
         Slot* s = S->unit_slot[m][u];
-        WindowRec& w = S->h_wins[wins_at(b, k, m) + u];
+        WindowRec& w = hw[wbase[k][m] + u];
         w = WindowRec{0, 0, 0, 0, 0};
         if (s->phase != 0 || s->cursor >= s->total) continue;
         // segment containing the cursor
@@ -547,8 +556,9 @@ static int run_tick(Server* S) {
           pulls += p;
           K++;
         }
-        std::memcpy(S->h_ops + ops_at(b, k, m, u), s->ops.data() + s->cursor, (size_t)K);
-        w.s0 = (long)ops_at(b, k, m, u);
+        std::memcpy(S->h_tick + op, s->ops.data() + s->cursor, (size_t)K);
+        w.s0 = (long)op;
+        op += (size_t)K;
         w.K = K;
         w.init = s->cursor == s->seg_start[seg] ? 1 : 0;
         w.seed = (uint32_t)s->seeds[seg];
@@ -559,13 +569,10 @@ static int run_tick(Server* S) {
       }
     }
   }
-  const size_t wcount = (size_t)kSub * kGroups * kMaxSlots;
-  const size_t obytes = wcount * kOpsPerUnit;
-  if (hipMemcpyAsync(S->d_wins + wins_at(b, 0, 0), S->h_wins + wins_at(b, 0, 0),
-                     wcount * sizeof(WindowRec), hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
-      hipMemcpyAsync(S->d_ops + ops_at(b, 0, 0, 0), S->h_ops + ops_at(b, 0, 0, 0), obytes,
-                     hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+  if (hipMemcpyAsync(S->d_tick + base, S->h_tick + base, op - base, hipMemcpyHostToDevice,
+                     ctx->stream) != hipSuccess)
     return fail(S, "tick upload", RT_EDEVICE);
+  const WindowRec* dw = reinterpret_cast<const WindowRec*>(S->d_tick + base);
   // Launch groups are independent: domino pairs, lozenge pairs and the
   // one-chain walks each get a stream (forked after the uploads above and
   // joined before the ended slots below), so one group's tile loads
@@ -593,7 +600,7 @@ static int run_tick(Server* S) {
       const int gs = fork ? group_stream(m) : 0;
       int rc = fused_launch_windows(ctx, gs ? ctx->sx[gs] : ctx->stream, group_model(m),
                                     group_nc(m), S->shape[m], mp, S->d_units[m], S->d_work[m],
-                                    S->nwork[m], S->d_wins + wins_at(b, k, m));
+                                    S->nwork[m], dw + wbase[k][m]);
       if (rc) return fail(S, ctx->err, rc);
     }
   if (fork)
@@ -716,12 +723,9 @@ extern "C" int rt_server_create(rt_ctx* ctx, size_t hbm_budget, int max_active, 
   bool ok = hipMalloc(&S->d_arena, hbm_budget) == hipSuccess;
   for (int m = 0; m < kGroups && ok; m++)
     ok = hipMalloc(&S->d_units[m], kMaxSlots * sizeof(UnitDesc)) == hipSuccess;
-  const size_t nwin = 2 * kSub * kGroups * (size_t)kMaxSlots;  // [buf][sub][group][slot]
-  ok = ok && hipMalloc(&S->d_wins, nwin * sizeof(WindowRec)) == hipSuccess;
-  ok = ok && hipMalloc(&S->d_ops, nwin * kOpsPerUnit) == hipSuccess;
+  ok = ok && hipMalloc(&S->d_tick, 2 * kTickBytes) == hipSuccess;
   ok = ok && hipMalloc(&S->d_flags, 2 * kMaxSlots * sizeof(uint32_t)) == hipSuccess;
-  ok = ok && hipHostMalloc(&S->h_wins, nwin * sizeof(WindowRec)) == hipSuccess;
-  ok = ok && hipHostMalloc(&S->h_ops, nwin * kOpsPerUnit) == hipSuccess;
+  ok = ok && hipHostMalloc(&S->h_tick, 2 * kTickBytes) == hipSuccess;
   ok = ok && hipHostMalloc(&S->h_flags, 2 * kMaxSlots * sizeof(uint32_t)) == hipSuccess;
   ok = ok && hipEventCreate(&S->ev[0]) == hipSuccess && hipEventCreate(&S->ev[1]) == hipSuccess;
   if (!ok) {
@@ -754,11 +758,9 @@ extern "C" void rt_server_destroy(rt_server* sv) {
     if (S->d_units[m]) hipFree(S->d_units[m]);
     if (S->d_work[m]) hipFree(S->d_work[m]);
   }
-  if (S->d_wins) hipFree(S->d_wins);
-  if (S->d_ops) hipFree(S->d_ops);
+  if (S->d_tick) hipFree(S->d_tick);
   if (S->d_flags) hipFree(S->d_flags);
-  if (S->h_wins) hipHostFree(S->h_wins);
-  if (S->h_ops) hipHostFree(S->h_ops);
+  if (S->h_tick) hipHostFree(S->h_tick);
   if (S->h_flags) hipHostFree(S->h_flags);
   for (int b = 0; b < 2; b++)
     if (S->ev[b]) hipEventDestroy(S->ev[b]);
