@@ -181,13 +181,19 @@ static int cftp_batch(rt_ctx* ctx, int model, const int32_t* tmax, const int32_t
     bot[k] = cur + sbytes + pbytes;
     cur += (per + 255) & ~(size_t)255;
   }
+  std::vector<int> spans;  // shared by every sample (FusedJob::spans)
   {
-    std::vector<uint8_t> h(pbytes, 0);
+    std::vector<uint8_t> h(pbytes, 0), h2(pbytes, 0);
     split_planes(model, tmax, N, h.data());
     HIPCHK_RET(ctx, hipMemcpy(init_top, h.data(), pbytes, hipMemcpyHostToDevice));
-    std::fill(h.begin(), h.end(), 0);
-    split_planes(model, tmin, N, h.data());
-    HIPCHK_RET(ctx, hipMemcpy(init_bot, h.data(), pbytes, hipMemcpyHostToDevice));
+    split_planes(model, tmin, N, h2.data());
+    HIPCHK_RET(ctx, hipMemcpy(init_bot, h2.data(), pbytes, hipMemcpyHostToDevice));
+    std::vector<const uint8_t*> pl;
+    for (int c = 0; c < g.ncol; c++) {
+      pl.push_back(h.data() + (size_t)c * g.plane);
+      pl.push_back(h2.data() + (size_t)c * g.plane);
+    }
+    spans = fused_live_spans(pl.data(), (int)pl.size(), N, (int)g.cols);
   }
   std::vector<Sample> S(ns);
   for (int k = 0; k < ns; k++) {
@@ -222,6 +228,7 @@ static int cftp_batch(rt_ctx* ctx, int model, const int32_t* tmax, const int32_t
     job.seg_seeds.resize(act.size());
     job.ops.resize(act.size());
     job.hist.resize(act.size());
+    job.spans.assign(act.size(), spans);
     parallel_for((int)act.size(), [&](int u) {
       const int k = act[u];
       UnitDesc d{};
@@ -715,11 +722,14 @@ static int walk_batch(rt_ctx* ctx, int32_t* const* tilings, const int* Ns, int n
     job.hist[k].model = M_DOMINO;
     job.hist[k].append(colours, job.ops[k]);
   });
+  job.spans.resize(nchains);
   for (int k = 0; k < nchains; k++) {
     const Geometry g = geometry_of(M_DOMINO, Ns[k]);
     std::vector<uint8_t> h(2 * (size_t)g.plane);
     split_planes(M_DOMINO, tilings[k], Ns[k], h.data());
     HIPCHK_RET(ctx, hipMemcpy(planes[k], h.data(), h.size(), hipMemcpyHostToDevice));
+    const uint8_t* pl[2] = {h.data(), h.data() + g.plane};
+    job.spans[k] = fused_live_spans(pl, 2, Ns[k], (int)g.cols);
   }
   rc = ctx->reserve_aux(fused_scratch_bytes(ctx, job));
   if (rc) return rc;

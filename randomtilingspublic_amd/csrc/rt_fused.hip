@@ -661,7 +661,7 @@ size_t fused_scratch_bytes(rt_ctx* ctx, FusedJob& job) {
   else
     bytes += tiles_bytes(ctx, job, job.shape, job.split);
   for (const UnitDesc& u : job.units) bytes += second_set_bytes(job, u);
-  bytes += ((nu * 7 * sizeof(CopyDesc)) + 255) & ~(size_t)255;
+  bytes += 2 * (((nu * 7 * sizeof(CopyDesc)) + 255) & ~(size_t)255);
   return bytes;
 }
 
@@ -785,6 +785,69 @@ static LaunchFn pick(const rt_ctx* ctx, int model, int nc, int shape) {
   return nc == 1 ? launch<M_RECTTRI, 1, 8, 8> : launch<M_RECTTRI, 2, 8, 4>;
 }
 
+std::vector<int> fused_live_spans(const uint8_t* const* planes, int nplanes, int rows,
+                                  int cols) {
+  std::vector<int> sp(2 * (size_t)rows);
+  for (int r = 0; r < rows; r++) {
+    int lo = cols, hi = -1;
+    for (int k = 0; k < nplanes; k++) {
+      const uint8_t* row = planes[k] + (size_t)r * cols;
+      int a = 0, b = cols - 1;
+      while (a < cols && !row[a]) a++;
+      while (b > a && !row[b]) b--;
+      if (a < cols) {
+        lo = std::min(lo, a);
+        hi = std::max(hi, b);
+      }
+    }
+    sp[2 * r] = lo;
+    sp[2 * r + 1] = hi;
+  }
+  return sp;
+}
+
+// Whether tile t of unit U (geometry g) has a loaded cell within one row
+// and one column of a cell the spans cover; sites += its computed cells.
+static bool tile_live(const std::vector<int>& sp, const UnitDesc& U, const TileGeo& g, int t,
+                      long* cells) {
+  const int TH = g.LH - 2 * g.Hr, TW = g.LW - 2 * g.Hc;
+  const int tr = t / U.tiles_c, tc = t % U.tiles_c;
+  const int r0 = tr * TH - g.Hr, c0 = tc * TW - g.Hc;
+  const int ra = std::max(r0 - 1, 0), rb = std::min(r0 + g.LH, U.rows - 1);
+  const int ca = std::max(c0 - 1, 0), cb = std::min(c0 + g.LW, U.cols - 1);
+  for (int r = ra; r <= rb; r++)
+    if (sp[2 * r] <= cb && sp[2 * r + 1] >= ca) {
+      *cells += (long)(std::min(tr * TH + TH, U.rows) - tr * TH) *
+                (std::min(tc * TW + TW, U.cols) - tc * TW);
+      return true;
+    }
+  return false;
+}
+
+// Tiles may be left out (FusedJob::spans): domino and byte-cell lozenge
+// units that carry spans.
+static bool skips_tiles(const rt_ctx* ctx, const FusedJob& job, size_t u) {
+  return (job.model == M_DOMINO ||
+          (job.model == M_LOZENGE && ctx->engine != RT_ENGINE_FUSED_SWAR)) &&
+         u < job.spans.size() && (int)job.spans[u].size() == 2 * job.units[u].rows;
+}
+
+// Per launch, the sites its windows compute under the tiles t.
+static void set_launch_sites(const FusedJob& job, const FusedPlan::Tiles& t,
+                             std::vector<long>& out) {
+  out.assign(job.nlaunch, 0);
+  for (size_t u = 0; u < job.units.size(); u++)
+    for (size_t l = 0; l < job.windows[u].size(); l++) out[l] += t.unit_sites[u];
+}
+static long sites_of(const FusedJob& job, const FusedPlan::Tiles& t, size_t l0, size_t l1) {
+  long n = 0;
+  for (size_t u = 0; u < job.units.size(); u++) {
+    const size_t e = std::min(l1, job.windows[u].size());
+    if (e > l0) n += (long)(e - l0) * t.unit_sites[u];
+  }
+  return n;
+}
+
 // Unit table, work list (and, split, window records) of one launch shape,
 // at p (advanced past them).  `wins` are the job's [launch][unit] records.
 static int upload_tiles(rt_ctx* ctx, const FusedJob& job, LaunchShape ls, uint8_t*& p,
@@ -812,18 +875,26 @@ static int upload_tiles(rt_ctx* ctx, const FusedJob& job, LaunchShape ls, uint8_
   }
   std::vector<WorkItem> work;
   std::vector<size_t> unit_end;  // work-list size after each (pair) unit
+  std::vector<long> unit_sites(nu);
   for (size_t u = 0; u < units.size(); u += ls.split ? 2 : 1) {
+    const size_t ju = ls.split ? u / 2 : u;  // the job's unit
     int tr, tc;
     tile_counts(units[u], geo, &tr, &tc);
     units[u].tiles_c = tc;
     if (ls.split) units[u + 1].tiles_c = tc;
+    const bool skip = skips_tiles(ctx, job, ju);
+    long cells = 0;
     // Split pairs: the two chains' workgroups of a tile sit next to each
     // other in the list, so they land in the same XCD share and the second
     // one finds the shared TinyMT states in L2.
     for (int t = 0; t < tr * tc; t++) {
+      if (skip && !tile_live(job.spans[ju], units[u], geo, t, &cells)) continue;
       work.push_back(WorkItem{(int)u, t});
       if (ls.split) work.push_back(WorkItem{(int)u + 1, t});
     }
+    const long all = (long)units[u].rows * units[u].cols;
+    unit_sites[ju] = skip ? (long)((double)job.units[ju].nstream * cells / all + 0.5)
+                          : job.units[ju].nstream;
     unit_end.push_back(work.size());
   }
 
@@ -851,7 +922,8 @@ static int upload_tiles(rt_ctx* ctx, const FusedJob& job, LaunchShape ls, uint8_
   // return, so wait before the vectors go out of scope.
   HIPCHK_RET(ctx, hipStreamSynchronize(ctx->stream));
   *out = FusedPlan::Tiles{d_units, d_work, (unsigned)work.size(), ls.shape, ls.split, d_wins,
-                          units.size(), std::vector<unsigned>(unit_end.begin(), unit_end.end())};
+                          units.size(), std::vector<unsigned>(unit_end.begin(), unit_end.end()),
+                          unit_sites};
   return RT_OK;
 }
 
@@ -865,7 +937,8 @@ int fused_upload(rt_ctx* ctx, FusedJob& job, void* scratch, FusedPlan* plan) {
   for (const UnitDesc& u : job.units) plan->streams += u.nstream;
   if (nu == 0 || T == 0) return RT_OK;
   // Device layout of the scratch block: ops | window records | second
-  // buffer sets | copy-back list | (units | work) per tile shape.
+  // buffer sets | copy-back list | first-set copy list | (units | work) per
+  // tile shape.
   uint8_t* p = (uint8_t*)scratch;
   uint8_t* d_ops = p;
   p += (nu * (size_t)T + 255) & ~(size_t)255;
@@ -873,7 +946,7 @@ int fused_upload(rt_ctx* ctx, FusedJob& job, void* scratch, FusedPlan* plan) {
   const size_t L = job.nlaunch;
   p += (L * nu * sizeof(WindowRec) + 255) & ~(size_t)255;
   const int ncol = job.model == M_DOMINO ? 2 : 3;
-  std::vector<CopyDesc> copies;
+  std::vector<CopyDesc> copies, seed_b;
   for (size_t u = 0; u < nu; u++) {
     UnitDesc& U = job.units[u];
     const size_t pb = plane_bytes(job, U);
@@ -881,6 +954,10 @@ int fused_upload(rt_ctx* ctx, FusedJob& job, void* scratch, FusedPlan* plan) {
       for (int k = 0; k < ncol; k++) {
         U.plane_b[ch][k] = p;
         p += (pb + 255) & ~(size_t)255;
+        // the cells of left-out tiles are never written: both sets start
+        // as the start planes (zero there)
+        if (skips_tiles(ctx, job, u))
+          seed_b.push_back(CopyDesc{U.plane[ch][k], U.plane_b[ch][k], (long)pb});
       }
     U.state_b = (uint32_t*)p;
     p += (16 * (size_t)U.nstream + 255) & ~(size_t)255;
@@ -893,8 +970,16 @@ int fused_upload(rt_ctx* ctx, FusedJob& job, void* scratch, FusedPlan* plan) {
   }
   CopyDesc* d_copies = (CopyDesc*)p;
   p += ((nu * 7 * sizeof(CopyDesc)) + 255) & ~(size_t)255;
+  CopyDesc* d_seed_b = (CopyDesc*)p;
+  p += ((nu * 7 * sizeof(CopyDesc)) + 255) & ~(size_t)255;
+  if (!seed_b.empty()) {
+    HIPCHK_RET(ctx, hipMemcpyAsync(d_seed_b, seed_b.data(), seed_b.size() * sizeof(CopyDesc),
+                                   hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_copy_list, dim3(64, std::min((unsigned)seed_b.size(), 4096u)), dim3(256),
+                       0, ctx->stream, d_seed_b, (unsigned)seed_b.size());
+    HIPCHK_RET(ctx, hipGetLastError());
+  }
   std::vector<WindowRec> wins(L * nu, WindowRec{0, 0, 0, 0, 0});
-  plan->launch_sites.assign(L, 0);
   plan->launch_work.assign(L, 0);
   for (size_t u = 0; u < nu; u++) {
     job.units[u].ops = d_ops + u * (size_t)T;
@@ -911,7 +996,6 @@ int fused_upload(rt_ctx* ctx, FusedJob& job, void* scratch, FusedPlan* plan) {
     for (size_t l = 0; l < win.size(); l++) {
       wins[l * nu + u] = WindowRec{win[l].s0, win[l].K, win[l].init,
                                    job.seg_seeds[u][win[l].seg], (int)(l & 1)};
-      plan->launch_sites[l] += job.units[u].nstream;
       plan->launch_work[l] += (long)win[l].K * job.units[u].nstream;
     }
   }
@@ -940,6 +1024,7 @@ int fused_upload(rt_ctx* ctx, FusedJob& job, void* scratch, FusedPlan* plan) {
   plan->nwork = plan->cand[0].nwork;
   plan->d_wins = plan->cand[0].d_wins;
   plan->nlaunch = L;
+  set_launch_sites(job, plan->cand[0], plan->launch_sites);
   return RT_OK;
 }
 
@@ -990,7 +1075,7 @@ int fused_tune(rt_ctx* ctx, FusedJob& job, FusedPlan* plan) {
     const int c = trials[k].cand, sl = shape_slot(plan->cand[c].shape, plan->cand[c].split);
     plan->trial_ms[sl] += e;
     plan->trial_launches[sl] += (long)(trials[k].l1 - trials[k].l0);
-    plan->trial_sites[sl] += plan->sites(trials[k].l0, trials[k].l1);
+    plan->trial_sites[sl] += sites_of(job, plan->cand[c], trials[k].l0, trials[k].l1);
     if (k <= nc) continue;  // the seeding launch and the first round are not fair trials
     ms[c] += e;
     for (size_t x = trials[k].l0; x < trials[k].l1; x++) work[c] += (double)plan->launch_work[x];
@@ -1011,6 +1096,7 @@ int fused_tune(rt_ctx* ctx, FusedJob& job, FusedPlan* plan) {
   plan->d_work = plan->cand[0].d_work;
   plan->nwork = plan->cand[0].nwork;
   plan->d_wins = plan->cand[0].d_wins;
+  set_launch_sites(job, plan->cand[0], plan->launch_sites);
   ctx->tuned[tune_key(job, tiles64(job))] = job.shape | (job.split ? 16 : 0);
   return RT_OK;
 }

@@ -148,7 +148,25 @@ struct FusedJob {
   // not shorten each other's windows.
   std::vector<std::vector<FusedWindow>> windows;
   size_t nlaunch = 0;                 // longest window list
+  // Per unit (or empty): fused_live_spans of the unit's start planes.
+  // Domino and lozenge tiles whose loaded cells all lie farther than one
+  // row and one column from every cell those spans cover are left out of
+  // the work lists: their cells are zero in every tiling the chain can
+  // reach, so leaving them at zero (both buffer sets, their TinyMT streams
+  // never advanced) gives the same planes as running them.
+  std::vector<std::vector<int>> spans;
 };
+
+// Row extents of the nonzero cells of a unit's start planes: `nplanes`
+// compressed planes of rows x cols bytes (every colour plane of every
+// chain's start tiling); entry 2r / 2r+1 is the first / last column of row
+// r holding a nonzero cell in any of them (first > last: none).  A cell
+// value's bits are lattice edges between two faces of the domain (a
+// domino's or lozenge's inner edge), and every face of the domain is
+// covered in every tiling, so a cell that is nonzero at any time has a
+// lattice neighbour (one row and one compressed column away at most) that
+// is nonzero in any valid tiling of the domain.
+std::vector<int> fused_live_spans(const uint8_t* const* planes, int nplanes, int rows, int cols);
 
 // Run all windows of the job on ctx->stream (no final sync).  `scratch`
 // must hold fused_scratch_bytes(ctx, job) bytes of device memory; that call
@@ -180,6 +198,9 @@ struct FusedPlan {
     const WindowRec* d_wins;  // [launch][unit of this launch shape]
     size_t nunits;
     std::vector<unsigned> ends;  // work-list size after each unit (stream split points)
+    // per job unit: the unit's sites scaled by the share of its cells that
+    // the tiles of this shape compute (FusedJob::spans)
+    std::vector<long> unit_sites;
   };
   const UnitDesc* d_units = nullptr;  // the tiles of job.shape
   const WorkItem* d_work = nullptr;
@@ -192,7 +213,7 @@ struct FusedPlan {
   size_t first = 0;                   // launches before this one ran in fused_tune
   long total_steps = 0;               // steps per chain
   long streams = 0;                   // TinyMT sites over all units
-  std::vector<long> launch_sites;     // per launch: sites of the units it advances
+  std::vector<long> launch_sites;     // per launch: sites it computes (Tiles::unit_sites)
   std::vector<long> launch_work;      // per launch: sum over units of K x sites
   long sites(size_t l0, size_t l1) const {
     long n = 0;
