@@ -1205,6 +1205,18 @@ int fused_launch_windows(rt_ctx* ctx, hipStream_t st, int model, int nc, int sha
   return RT_OK;
 }
 
+bool fused_tile_live(const rt_ctx* ctx, int model, int nc, int rule, int shape,
+                     const UnitDesc& u, const std::vector<int>& spans, int t) {
+  if (!(model == M_DOMINO || (model == M_LOZENGE && ctx->engine != RT_ENGINE_FUSED_SWAR)) ||
+      (int)spans.size() != 2 * u.rows)
+    return true;
+  long cells = 0;
+  return tile_live(spans, u,
+                   tile_geo(ctx, model, nc, shape, fused_shape_halo(ctx, model, nc, rule, shape),
+                            rule),
+                   t, &cells);
+}
+
 int fused_shape_halo(const rt_ctx* ctx, int model, int nc, int rule, int shape) {
   return packed_pair(ctx, model, nc, shape, pair_halo(model), rule) ? pair_halo(model)
                                                                     : shape_geo(shape).H;

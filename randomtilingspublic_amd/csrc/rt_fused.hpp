@@ -167,6 +167,12 @@ struct FusedJob {
 // lattice neighbour (one row and one compressed column away at most) that
 // is nonzero in any valid tiling of the domain.
 std::vector<int> fused_live_spans(const uint8_t* const* planes, int nplanes, int rows, int cols);
+// Whether tile t of unit u (rows, cols and tiles_c set) must run under the
+// launch geometry of (model, nc, rule, shape) given the unit's
+// fused_live_spans; always true for other models and for spans of another
+// size.
+bool fused_tile_live(const rt_ctx* ctx, int model, int nc, int rule, int shape,
+                     const UnitDesc& u, const std::vector<int>& spans, int t);
 
 // Run all windows of the job on ctx->stream (no final sync).  `scratch`
 // must hold fused_scratch_bytes(ctx, job) bytes of device memory; that call

@@ -96,6 +96,7 @@ struct Slot {
   int phase = 0;       // 0 sweeping, 1 coalescence flag pending
   bool fresh = true;   // planes must be reset from the initial ones
   long flag_tick = -1;
+  std::vector<int> spans;  // fused_live_spans of the start planes (tiles left out)
 };
 
 // First-fit allocator over [0, cap).
@@ -317,6 +318,19 @@ static int admit(Server* S, Req* r) {
         if (s->cb == 4) std::memcpy(&h[idx * 4], &t[(size_t)i * N + j], 4);
         else h[idx] = (uint8_t)t[(size_t)i * N + j];
       }
+    if (s->cb == 1) {  // the union of both chains' extents
+      std::vector<const uint8_t*> pl;
+      for (int c = 0; c < ncol; c++) pl.push_back(h.data() + (size_t)c * plane);
+      std::vector<int> sp = fused_live_spans(pl.data(), ncol, N, (int)cols);
+      if (s->spans.empty()) {
+        s->spans = std::move(sp);
+      } else {
+        for (int i = 0; i < N; i++) {
+          s->spans[2 * i] = std::min(s->spans[2 * i], sp[2 * i]);
+          s->spans[2 * i + 1] = std::max(s->spans[2 * i + 1], sp[2 * i + 1]);
+        }
+      }
+    }
     if (hipMemcpy(which ? s->init_bot : s->init_top, h.data(), h.size(), hipMemcpyHostToDevice) !=
         hipSuccess) {
       S->arena.release(off, bytes);
@@ -421,7 +435,9 @@ static int upload_tables(Server* S) {
       int tr, tc;
       fused_tiles(S->ctx, model, nc, S->group_rule[m], S->shape[m], d.rows, d.cols, &tr, &tc);
       d.tiles_c = tc;
-      for (int t = 0; t < tr * tc; t++) work.push_back(WorkItem{u, t});
+      for (int t = 0; t < tr * tc; t++)
+        if (fused_tile_live(S->ctx, model, nc, S->group_rule[m], S->shape[m], d, s->spans, t))
+          work.push_back(WorkItem{u, t});
       units.push_back(d);
       S->unit_slot[m].push_back(s);
     }
@@ -512,14 +528,20 @@ static int run_tick(Server* S) {
     if (rc) return rc;
   }
   if (S->active.empty()) return RT_OK;
-  // resets of slots starting a round
+  // resets of slots starting a round (both buffer sets: left-out tiles'
+  // cells are never written)
   for (Slot* s : S->active)
     if (s->phase == 0 && s->fresh) {
       const size_t pb = (size_t)s->ncol * (size_t)s->plane * s->cb;
+      const bool pair = !s->req->walk;
       if (hipMemcpyAsync(s->top, s->init_top, pb, hipMemcpyDeviceToDevice, ctx->stream) !=
               hipSuccess ||
-          (!s->req->walk && hipMemcpyAsync(s->bot, s->init_bot, pb, hipMemcpyDeviceToDevice,
-                                           ctx->stream) != hipSuccess))
+          hipMemcpyAsync(s->top_b, s->init_top, pb, hipMemcpyDeviceToDevice, ctx->stream) !=
+              hipSuccess ||
+          (pair && hipMemcpyAsync(s->bot, s->init_bot, pb, hipMemcpyDeviceToDevice,
+                                  ctx->stream) != hipSuccess) ||
+          (pair && hipMemcpyAsync(s->bot_b, s->init_bot, pb, hipMemcpyDeviceToDevice,
+                                  ctx->stream) != hipSuccess))
         return fail(S, "plane reset", RT_EDEVICE);
       s->fresh = false;
       s->par = 0;
