@@ -133,7 +133,7 @@ __global__ __launch_bounds__(1024, 8) void k_pair_domino(const UnitDesc* __restr
   const int HC = col_halo(M_DOMINO, H);
   const int TH = LH - 2 * H, TW = LW - 2 * HC;
   const int tr = wi.tile / U.tiles_c, tc = wi.tile % U.tiles_c;
-  const int r0 = tr * TH - H, c0 = tc * TW - HC;
+  const int r0 = tr * TH - H, c0 = tc * TW - HC + wi.cs;
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int col = lane & (LW - 1);

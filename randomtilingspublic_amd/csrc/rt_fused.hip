@@ -212,7 +212,7 @@ __device__ __forceinline__ void fused_body(const UnitDesc* __restrict__ units,
   const int HC = col_halo(MODEL, H);
   const int TH = LH - 2 * H, TW = kLW - 2 * HC;
   const int tr = wi.tile / U.tiles_c, tc = wi.tile % U.tiles_c;
-  const int r0 = tr * TH - H, c0 = tc * TW - HC;
+  const int r0 = tr * TH - H, c0 = tc * TW - HC + wi.cs;
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform row offset
   const int gc = c0 + lane;
@@ -806,22 +806,35 @@ std::vector<int> fused_live_spans(const uint8_t* const* planes, int nplanes, int
   return sp;
 }
 
-// Whether tile t of unit U (geometry g) has a loaded cell within one row
-// and one column of a cell the spans cover; sites += its computed cells.
-static bool tile_live(const std::vector<int>& sp, const UnitDesc& U, const TileGeo& g, int t,
-                      long* cells) {
+// Work items of one unit under geometry g (fused_unit_work), `copies`
+// consecutive unit indices per tile (split pairs); cells += the cells the
+// items' interiors cover inside the grid.
+static void unit_work(const std::vector<int>* sp, const UnitDesc& U, const TileGeo& g, int unit,
+                      int copies, std::vector<WorkItem>& out, long* cells) {
   const int TH = g.LH - 2 * g.Hr, TW = g.LW - 2 * g.Hc;
-  const int tr = t / U.tiles_c, tc = t % U.tiles_c;
-  const int r0 = tr * TH - g.Hr, c0 = tc * TW - g.Hc;
-  const int ra = std::max(r0 - 1, 0), rb = std::min(r0 + g.LH, U.rows - 1);
-  const int ca = std::max(c0 - 1, 0), cb = std::min(c0 + g.LW, U.cols - 1);
-  for (int r = ra; r <= rb; r++)
-    if (sp[2 * r] <= cb && sp[2 * r + 1] >= ca) {
-      *cells += (long)(std::min(tr * TH + TH, U.rows) - tr * TH) *
-                (std::min(tc * TW + TW, U.cols) - tc * TW);
-      return true;
+  int tr, tc;
+  tile_counts(U, g, &tr, &tc);
+  for (int r = 0; r < tr; r++) {
+    const int ra = r * TH, rb = std::min(ra + TH, U.rows);  // interior rows [ra, rb)
+    int lo = 0, hi = U.cols - 1;
+    if (sp) {
+      lo = U.cols;
+      hi = -1;
+      for (int i = std::max(ra - 1, 0); i <= std::min(rb, U.rows - 1); i++)
+        if ((*sp)[2 * i] <= (*sp)[2 * i + 1]) {
+          lo = std::min(lo, (*sp)[2 * i] - 1);
+          hi = std::max(hi, (*sp)[2 * i + 1] + 1);
+        }
+      lo = std::max(lo, 0);
+      hi = std::min(hi, U.cols - 1);
+      if (lo > hi) continue;  // no cell of this row band can turn nonzero
     }
-  return false;
+    const int n = (hi - lo + TW) / TW;  // <= tc
+    for (int k = 0; k < n; k++) {
+      for (int c = 0; c < copies; c++) out.push_back(WorkItem{unit + c, r * tc + k, lo});
+      *cells += (long)(rb - ra) * (std::min(lo + (k + 1) * TW, U.cols) - (lo + k * TW));
+    }
+  }
 }
 
 // Tiles may be left out (FusedJob::spans): domino and byte-cell lozenge
@@ -887,11 +900,8 @@ static int upload_tiles(rt_ctx* ctx, const FusedJob& job, LaunchShape ls, uint8_
     // Split pairs: the two chains' workgroups of a tile sit next to each
     // other in the list, so they land in the same XCD share and the second
     // one finds the shared TinyMT states in L2.
-    for (int t = 0; t < tr * tc; t++) {
-      if (skip && !tile_live(job.spans[ju], units[u], geo, t, &cells)) continue;
-      work.push_back(WorkItem{(int)u, t});
-      if (ls.split) work.push_back(WorkItem{(int)u + 1, t});
-    }
+    unit_work(skip ? &job.spans[ju] : nullptr, units[u], geo, (int)u, ls.split ? 2 : 1, work,
+              &cells);
     const long all = (long)units[u].rows * units[u].cols;
     unit_sites[ju] = skip ? (long)((double)job.units[ju].nstream * cells / all + 0.5)
                           : job.units[ju].nstream;
@@ -1205,16 +1215,16 @@ int fused_launch_windows(rt_ctx* ctx, hipStream_t st, int model, int nc, int sha
   return RT_OK;
 }
 
-bool fused_tile_live(const rt_ctx* ctx, int model, int nc, int rule, int shape,
-                     const UnitDesc& u, const std::vector<int>& spans, int t) {
-  if (!(model == M_DOMINO || (model == M_LOZENGE && ctx->engine != RT_ENGINE_FUSED_SWAR)) ||
-      (int)spans.size() != 2 * u.rows)
-    return true;
+void fused_unit_work(const rt_ctx* ctx, int model, int nc, int rule, int shape,
+                     const UnitDesc& u, const std::vector<int>& spans, int unit,
+                     std::vector<WorkItem>& out) {
+  const bool skip = (model == M_DOMINO ||
+                     (model == M_LOZENGE && ctx->engine != RT_ENGINE_FUSED_SWAR)) &&
+                    (int)spans.size() == 2 * u.rows;
   long cells = 0;
-  return tile_live(spans, u,
-                   tile_geo(ctx, model, nc, shape, fused_shape_halo(ctx, model, nc, rule, shape),
-                            rule),
-                   t, &cells);
+  unit_work(skip ? &spans : nullptr, u,
+            tile_geo(ctx, model, nc, shape, fused_shape_halo(ctx, model, nc, rule, shape), rule),
+            unit, 1, out, &cells);
 }
 
 int fused_shape_halo(const rt_ctx* ctx, int model, int nc, int rule, int shape) {

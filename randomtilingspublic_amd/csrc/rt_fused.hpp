@@ -100,6 +100,7 @@ __device__ __forceinline__ void window_buffers(UnitDesc& U, int parity) {
 struct WorkItem {
   int unit;
   int tile;  // tile index within the unit (row-major over tiles_c)
+  int cs;    // column shift of the tile: its interior starts at column tc * TW + cs
 };
 
 struct ModelParams {
@@ -149,11 +150,12 @@ struct FusedJob {
   std::vector<std::vector<FusedWindow>> windows;
   size_t nlaunch = 0;                 // longest window list
   // Per unit (or empty): fused_live_spans of the unit's start planes.
-  // Domino and lozenge tiles whose loaded cells all lie farther than one
-  // row and one column from every cell those spans cover are left out of
-  // the work lists: their cells are zero in every tiling the chain can
-  // reach, so leaving them at zero (both buffer sets, their TinyMT streams
-  // never advanced) gives the same planes as running them.
+  // Domino and byte-cell lozenge work lists then hold, per row of tiles,
+  // only the tiles whose interiors cover the cells within one row and one
+  // column of those spans, shifted to start at the first such column
+  // (WorkItem::cs).  The cells no tile covers are zero in every tiling the
+  // chain can reach, so leaving them at zero in both buffer sets (their
+  // TinyMT streams never advanced) gives the same planes as running them.
   std::vector<std::vector<int>> spans;
 };
 
@@ -167,12 +169,14 @@ struct FusedJob {
 // lattice neighbour (one row and one compressed column away at most) that
 // is nonzero in any valid tiling of the domain.
 std::vector<int> fused_live_spans(const uint8_t* const* planes, int nplanes, int rows, int cols);
-// Whether tile t of unit u (rows, cols and tiles_c set) must run under the
-// launch geometry of (model, nc, rule, shape) given the unit's
-// fused_live_spans; always true for other models and for spans of another
-// size.
-bool fused_tile_live(const rt_ctx* ctx, int model, int nc, int rule, int shape,
-                     const UnitDesc& u, const std::vector<int>& spans, int t);
+// Work items of unit index `unit` (rows, cols and tiles_c set) under the
+// launch geometry of (model, nc, rule, shape): every tile, or with the
+// unit's fused_live_spans (domino and byte-cell lozenge units) per row of
+// tiles only those covering the cells within one row and one column of the
+// spans, shifted to start at the first such column (FusedJob::spans).
+void fused_unit_work(const rt_ctx* ctx, int model, int nc, int rule, int shape,
+                     const UnitDesc& u, const std::vector<int>& spans, int unit,
+                     std::vector<WorkItem>& out);
 
 // Run all windows of the job on ctx->stream (no final sync).  `scratch`
 // must hold fused_scratch_bytes(ctx, job) bytes of device memory; that call

@@ -435,9 +435,7 @@ static int upload_tables(Server* S) {
       int tr, tc;
       fused_tiles(S->ctx, model, nc, S->group_rule[m], S->shape[m], d.rows, d.cols, &tr, &tc);
       d.tiles_c = tc;
-      for (int t = 0; t < tr * tc; t++)
-        if (fused_tile_live(S->ctx, model, nc, S->group_rule[m], S->shape[m], d, s->spans, t))
-          work.push_back(WorkItem{u, t});
+      fused_unit_work(S->ctx, model, nc, S->group_rule[m], S->shape[m], d, s->spans, u, work);
       units.push_back(d);
       S->unit_slot[m].push_back(s);
     }

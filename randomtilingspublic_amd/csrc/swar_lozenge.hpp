@@ -132,7 +132,7 @@ __global__ __launch_bounds__(64 * W, W >= 16 ? 1 : 2) void k_swar_lozenge(
   const int rows = U.rows, cols = U.cols;
   const int TH = LH - 2 * H, TW = kLW - 2 * H;
   const int tr = wi.tile / U.tiles_c, tc = wi.tile % U.tiles_c;
-  const int r0 = tr * TH - H, c0 = tc * TW - H;
+  const int r0 = tr * TH - H, c0 = tc * TW - H + wi.cs;
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int q = lane >> 4, w = lane & 15;
