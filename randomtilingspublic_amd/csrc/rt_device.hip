@@ -604,6 +604,8 @@ int domino_walk_impl(rt_ctx* ctx, int32_t* tiling, int N, const Segments& seg,
   std::vector<uint8_t> host(2 * half);
   domino_split(tiling, N, host.data());
   HIPCHK(ctx, hipMemcpyAsync(arr, host.data(), 2 * half, hipMemcpyHostToDevice, ctx->stream));
+  const uint8_t* pl[2] = {host.data(), host.data() + half};
+  const std::vector<int> spans = fused_live_spans(pl, 2, N, N / 2);
   DominoRun R;
   R.ctx = ctx;
   R.N = N;
@@ -616,7 +618,7 @@ int domino_walk_impl(rt_ctx* ctx, int32_t* tiling, int N, const Segments& seg,
   begin_timing(ctx);
   rc = ctx->engine == 1 ? domino_device_walk(R, seg, state)
                         : domino_fused_walk(ctx, R.a, 1, N, R.rule, R.w, seg.steps, seg.seeds,
-                                            seg.single, state);
+                                            seg.single, state, &spans);
   if (rc) return rc;
   rc = end_timing(ctx);
   if (rc) return rc;
@@ -656,7 +658,7 @@ int rt::launch_recttri_sync(rt_ctx* ctx, TriArrays32* a, int nc, int N, int last
 
 int rt::domino_fused_walk(rt_ctx* ctx, DominoArrays* a, int nc, int N, int rule, DominoWeights w,
                           const std::vector<long>& steps, const std::vector<long>& seeds,
-                          bool single, uint32_t* state) {
+                          bool single, uint32_t* state, const std::vector<int>* spans) {
   if (ctx->engine == 1) {
     DominoRun R;
     R.ctx = ctx;
@@ -687,6 +689,7 @@ int rt::domino_fused_walk(rt_ctx* ctx, DominoArrays* a, int nc, int N, int rule,
   u.rows = N;
   u.cols = N / 2;
   job.units.push_back(u);
+  if (spans) job.spans.push_back(*spans);
   job.seg_steps = steps;
   job.seg_seeds.resize(1);
   job.ops.resize(1);
@@ -830,7 +833,7 @@ void tri_merge8(int32_t* t, int N, const uint8_t* host) {
 
 int rt::lozenge_device_walk(rt_ctx* ctx, TriArrays8* a, int nc, int N,
                             const std::vector<long>& steps, const std::vector<long>& seeds,
-                            uint32_t* state) {
+                            uint32_t* state, const std::vector<int>* spans) {
   if (ctx->engine != 1) {
     FusedJob job;
     job.model = M_LOZENGE;
@@ -845,6 +848,7 @@ int rt::lozenge_device_walk(rt_ctx* ctx, TriArrays8* a, int nc, int N,
     u.rows = N;
     u.cols = N / 3;
     job.units.push_back(u);
+    if (spans) job.spans.push_back(*spans);
     job.seg_steps = steps;
     job.seg_seeds.resize(1);
     job.ops.resize(1);
@@ -928,10 +932,12 @@ static int lozenge_walk_impl(rt_ctx* ctx, int32_t* tiling, int N, const std::vec
   std::vector<uint8_t> host(3 * sz);
   tri_split8(tiling, N, host.data(), true);
   HIPCHK(ctx, hipMemcpyAsync(arr, host.data(), 3 * sz, hipMemcpyHostToDevice, ctx->stream));
+  const uint8_t* pl[3] = {host.data(), host.data() + sz, host.data() + 2 * sz};
+  const std::vector<int> spans = fused_live_spans(pl, 3, N, N / 3);
   TriArrays8 a;
   for (int c = 0; c < 3; c++) a.col[c] = arr + c * sz;
   begin_timing(ctx);
-  rc = lozenge_device_walk(ctx, &a, 1, N, steps, seeds, state);
+  rc = lozenge_device_walk(ctx, &a, 1, N, steps, seeds, state, &spans);
   if (rc) return rc;
   rc = end_timing(ctx);
   if (rc) return rc;
@@ -1197,11 +1203,13 @@ extern "C" int rt_domino_cftp(rt_ctx* ctx, const int32_t* tmax, const int32_t* t
   uint8_t* init_bot = base + 2 * half;
   uint8_t* top = base + 4 * half;
   uint8_t* bot = base + 6 * half;
-  std::vector<uint8_t> host(2 * half);
+  std::vector<uint8_t> host(2 * half), host2(2 * half);
   domino_split(tmax, N, host.data());
   HIPCHK(ctx, hipMemcpy(init_top, host.data(), 2 * half, hipMemcpyHostToDevice));
-  domino_split(tmin, N, host.data());
-  HIPCHK(ctx, hipMemcpy(init_bot, host.data(), 2 * half, hipMemcpyHostToDevice));
+  domino_split(tmin, N, host2.data());
+  HIPCHK(ctx, hipMemcpy(init_bot, host2.data(), 2 * half, hipMemcpyHostToDevice));
+  const uint8_t* pl[4] = {host.data(), host.data() + half, host2.data(), host2.data() + half};
+  const std::vector<int> spans = fused_live_spans(pl, 4, N, N / 2);
   DominoArrays a[2];
   a[0].col[0] = top;
   a[0].col[1] = top + half;
@@ -1217,7 +1225,8 @@ extern "C" int rt_domino_cftp(rt_ctx* ctx, const int32_t* tmax, const int32_t* t
                                               ctx->stream));
                    HIPCHK(ctx, hipMemcpyAsync(bot, init_bot, 2 * half, hipMemcpyDeviceToDevice,
                                               ctx->stream));
-                   int r = domino_fused_walk(ctx, a, 2, N, kind, w, steps, seeds, false, state);
+                   int r = domino_fused_walk(ctx, a, 2, N, kind, w, steps, seeds, false, state,
+                                             &spans);
                    if (r) return r;
                    r = differ(ctx, top, bot, 2 * half, differs);
                    if (r) return r;
@@ -1250,11 +1259,17 @@ extern "C" int rt_lozenge_cftp(rt_ctx* ctx, const int32_t* tmax, const int32_t* 
   uint8_t* init_bot = base + 3 * sz;
   uint8_t* top = base + 6 * sz;
   uint8_t* bot = base + 9 * sz;
-  std::vector<uint8_t> host(3 * sz);
+  std::vector<uint8_t> host(3 * sz), host2(3 * sz);
   tri_split8(tmax, N, host.data(), true);
   HIPCHK(ctx, hipMemcpy(init_top, host.data(), 3 * sz, hipMemcpyHostToDevice));
-  tri_split8(tmin, N, host.data(), true);
-  HIPCHK(ctx, hipMemcpy(init_bot, host.data(), 3 * sz, hipMemcpyHostToDevice));
+  tri_split8(tmin, N, host2.data(), true);
+  HIPCHK(ctx, hipMemcpy(init_bot, host2.data(), 3 * sz, hipMemcpyHostToDevice));
+  std::vector<const uint8_t*> pl;
+  for (int c = 0; c < 3; c++) {
+    pl.push_back(host.data() + c * sz);
+    pl.push_back(host2.data() + c * sz);
+  }
+  const std::vector<int> spans = fused_live_spans(pl.data(), 6, N, N / 3);
   TriArrays8 a[2];
   for (int c = 0; c < 3; c++) {
     a[0].col[c] = top + c * sz;
@@ -1268,7 +1283,7 @@ extern "C" int rt_lozenge_cftp(rt_ctx* ctx, const int32_t* tmax, const int32_t* 
                                               ctx->stream));
                    HIPCHK(ctx, hipMemcpyAsync(bot, init_bot, 3 * sz, hipMemcpyDeviceToDevice,
                                               ctx->stream));
-                   int r = lozenge_device_walk(ctx, a, 2, N, steps, seeds, state);
+                   int r = lozenge_device_walk(ctx, a, 2, N, steps, seeds, state, &spans);
                    if (r) return r;
                    r = differ(ctx, top, bot, 3 * sz, differs);
                    if (r) return r;

@@ -78,11 +78,14 @@ struct TriArrays32 {
   int32_t* col[3];  // rectangle-triangle R, B, G; each N x N/3
 };
 
+// spans: fused_live_spans of the start planes (tiles that stay zero are
+// left out), or null.
 int domino_fused_walk(rt_ctx* ctx, DominoArrays* a, int nc, int N, int rule, DominoWeights w,
                       const std::vector<long>& steps, const std::vector<long>& seeds, bool single,
-                      uint32_t* state);
+                      uint32_t* state, const std::vector<int>* spans = nullptr);
 int lozenge_device_walk(rt_ctx* ctx, TriArrays8* a, int nc, int N, const std::vector<long>& steps,
-                        const std::vector<long>& seeds, uint32_t* state);
+                        const std::vector<long>& seeds, uint32_t* state,
+                        const std::vector<int>* spans = nullptr);
 int lozenge_device_sync(rt_ctx* ctx, TriArrays8* a, int nc, int N, const long* last);
 int recttri_device_walk(rt_ctx* ctx, TriArrays32* a, int nc, int N,
                         const std::vector<long>& steps, const std::vector<long>& seeds,

@@ -79,3 +79,35 @@ def test_lozenge_cftp_pairs_on_a_slanted_hexagon_match_oracle(gpu_ctx):
         steps = [512, 256, 256]
         assert np.array_equal(tops[k], o.lozenge_walk_segments(tmax, steps, seeds, P)), k
         assert np.array_equal(bots[k], o.lozenge_walk_segments(tmin, steps, seeds, P)), k
+
+
+def test_single_walks_and_cftp_on_sparse_domains_match_oracle(gpu_ctx):
+    """The one-sample entry points (rt_domino_walk, rt_lozenge_walk,
+    rt_domino_cftp) leave out the same tiles: an L-shaped domino domain
+    (heat bath, the 0.8 toggle), the slanted hexagon, and an Aztec diamond's
+    CFTP sample."""
+    D = rt.DominoTiler
+    t = D.MinTiling(D.RectangleUnion([(0, 0, 200, 12), (188, 0, 12, 200)]))
+    N = rt.side(t)
+    P = rt.synthetic_params(N * N // 2, 33)
+    gpu_ctx.set_tinymt_params(P)
+    for rule in (rt.DOMINO_HEATBATH, rt.DOMINO_TOGGLE08):
+        _say("L-shaped domino walk, rule", rule)
+        got = gpu_ctx.domino_walk(t, 700, 21, rule)
+        assert np.array_equal(got, o.domino_walk(t, 700, 21, P, rule))
+    d = rt.LozengeTiler.AlmostHexDomain(90, 30)
+    tl = rt.LozengeTiler.MinTiling(d)
+    NL = rt.side(tl)
+    PL = rt.synthetic_params(NL * NL // 3, 34)
+    gpu_ctx.set_tinymt_params(PL)
+    _say("slanted hexagon lozenge walk")
+    assert np.array_equal(gpu_ctx.lozenge_walk(tl, 600, 8), o.lozenge_walk(tl, 600, 8, PL))
+    az = D.AztecDiamond(40)
+    tmin, tmax = D.MinTiling(az), D.MaxTiling(az)
+    NA = rt.side(tmin)
+    PA = rt.synthetic_params(NA * NA // 2, 35)
+    gpu_ctx.set_tinymt_params(PA)
+    _say("Aztec 40 domino CFTP, one sample")
+    got, st = gpu_ctx.domino_cftp(tmax, tmin, 6, rt.SCHED_DOUBLING, 128, 12)
+    want, wst = o.domino_cftp(tmax, tmin, 6, PA, rt.SCHED_DOUBLING, 128, 12)
+    assert np.array_equal(got, want) and st["rounds"] == wst["rounds"]
