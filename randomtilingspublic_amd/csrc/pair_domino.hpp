@@ -46,9 +46,11 @@ namespace rt {
 // (pk_apply with the 4-bit mask).
 __device__ __forceinline__ uint32_t pkd_apply(uint32_t v, uint32_t t2) {
   const uint32_t d = v ^ t2;
-  uint32_t m;
+  uint32_t m, r;
   asm("v_pk_min_u16 %0, %1, %2" : "=v"(m) : "v"(d), "s"(0x00010001u));
-  return v ^ __umul24(m, 15u) ^ 0x000F000Fu;
+  asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0x96" : "=v"(r) : "v"(v), "v"(__umul24(m, 15u)),
+      "s"(0x000F000Fu));
+  return r;
 }
 
 constexpr uint32_t kPkdNone = 0x01000100u;  // a target no cell equals
@@ -90,13 +92,12 @@ __device__ __forceinline__ void pkd_step(uint32_t* p, const uint32_t* pq, const 
       const uint32_t* P = ((r + C) & 1) ? pn : pq;
       const uint32_t up = p[S * PL + o - RS], dn = p[S * PL + o + RS];
       const uint32_t lf = P[S * PL + o - 1], rg = P[S * PL + o];
-      // up bit 2 -> 1 and left bit 8 -> 4 (shift right); down bit 1 -> 2
-      // and right bit 4 -> 8 (shift left)
-      const uint32_t a = pk_bfi(0x00080008u, lf, up) >> 1;
+      // up bit 2 -> 1 and left bit 8 -> 4 (shift right, per half: a holds
+      // nothing but those bits and bit 2 of each half); down bit 1 -> 2 and
+      // right bit 4 -> 8 (shift left), of which bits 1 and 3 are taken
+      const uint32_t a = as_u32(as_u16x2(pk_bfi(0x00080008u, lf, up)) >> (u16x2)(1));
       const uint32_t b = pk_bfi(0x00040004u, rg, dn) << 1;
-      uint32_t w;
-      asm("v_and_or_b32 %0, %1, %2, %3" : "=v"(w) : "v"(b), "s"(0x000A000Au), "v"(a & 0x00050005u));
-      v = pk_bfi_v(interior[r], w, v);
+      v = pk_bfi_v(interior[r], pk_bfi(0x000A000Au, b, a), v);
     }
     out[r] = pkd_apply(v, t2[r]);
   }

@@ -45,6 +45,12 @@ __device__ __forceinline__ uint32_t pk_bfi(uint32_t m, uint32_t x, uint32_t y) {
   return r;
 }
 
+// Two 16-bit halves of a word, for the per-half shifts that keep one
+// chain's bits out of the other's.
+typedef uint16_t u16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ u16x2 as_u16x2(uint32_t x) { return __builtin_bit_cast(u16x2, x); }
+__device__ __forceinline__ uint32_t as_u32(u16x2 x) { return __builtin_bit_cast(uint32_t, x); }
+
 // The same with a per-lane mask.
 __device__ __forceinline__ uint32_t pk_bfi_v(uint32_t m, uint32_t x, uint32_t y) {
   uint32_t r;
@@ -62,12 +68,15 @@ __device__ __forceinline__ uint32_t pk_target(uint32_t b) {
 // d = v ^ t2 has zero halves where a cell equals the target; the packed
 // unsigned minimum with 1 turns each half into 0 (equal) or 1, times 63 is
 // 0 or 63 per half (below 2^24, so the 24-bit multiply is exact), and
-// v ^ that ^ 0x003F003F flips exactly the equal halves.
+// v ^ that ^ 0x003F003F (one three-input XOR, the constant in an SGPR)
+// flips exactly the equal halves.
 __device__ __forceinline__ uint32_t pk_apply(uint32_t v, uint32_t t2) {
   const uint32_t d = v ^ t2;
-  uint32_t m;
+  uint32_t m, r;
   asm("v_pk_min_u16 %0, %1, %2" : "=v"(m) : "v"(d), "s"(0x00010001u));
-  return v ^ __umul24(m, 63u) ^ 0x003F003Fu;
+  asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0x96" : "=v"(r) : "v"(v), "v"(__umul24(m, 63u)),
+      "s"(0x003F003Fu));
+  return r;
 }
 
 // One colour step C with pull pattern PULL on the packed tile.  `p` is the
