@@ -1188,11 +1188,12 @@ int fused_launch(rt_ctx* ctx, const FusedJob& job, const FusedPlan& plan, bool s
 
 int fused_serve_shape(const rt_ctx* ctx, int model, int nc, int rule, long tiles64) {
   if (model == M_RECTTRI) return nc == 2 ? SH_RT8x4 : SH_8x8;
-  // the packed kernels: domino at every size, lozenge from a third of the
-  // CUs up (as plan_windows)
-  if (packed_pair(ctx, model, nc, SH_16x3_O8, pair_halo(model), rule) &&
-      (model == M_DOMINO || pair_job_is_large(ctx, tiles64)))
-    return SH_16x3_O8;
+  // The packed kernels at every size: a lone small lozenge job is faster on
+  // 64-row tiles (plan_windows), but the server's groups share the chip
+  // with each other and run many windows of many requests (the default
+  // served stream's burst capacity 3.31 -> 4.04 requests/s, p99 17.6 ->
+  // 11.5 s, with lozenge pairs packed at every size too).
+  if (packed_pair(ctx, model, nc, SH_16x3_O8, pair_halo(model), rule)) return SH_16x3_O8;
   // Few tiles: one 16-wave workgroup per CU on 64-row tiles; more tiles
   // than CUs: the 48-row tiles at eight waves per SIMD (two workgroups per
   // CU), which the batch tuner picks from about 16 hexagon-300 samples up.
