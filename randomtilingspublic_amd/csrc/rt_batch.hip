@@ -50,21 +50,6 @@ int launch_differ_pair(rt_ctx* ctx, const uint8_t* a, const uint8_t* b, long byt
   return RT_OK;
 }
 
-template <class F>
-static void parallel_for(int n, F f) {
-  const int nt = std::max(1, std::min<int>(n, (int)std::thread::hardware_concurrency()));
-  if (nt <= 1) {
-    for (int i = 0; i < n; i++) f(i);
-    return;
-  }
-  std::vector<std::thread> th;
-  for (int t = 0; t < nt; t++)
-    th.emplace_back([&, t] {
-      for (int i = t; i < n; i += nt) f(i);
-    });
-  for (auto& x : th) x.join();
-}
-
 namespace {
 
 // Host split / merge between the int32 N x N tiling and the per-colour
@@ -208,6 +193,31 @@ static int cftp_batch(rt_ctx* ctx, int model, const int32_t* tmax, const int32_t
     S[k].colours.push_back(std::move(c));
   };
   parallel_for(ns, [&](int k) { gen(k, S[k].seeds[0], t0); });
+  // A sample's op stream over all its segments, newest first (colours are
+  // stored oldest first).
+  auto build_ops = [&](int k, std::vector<uint8_t>& ops, OpBuilder& h) {
+    h = OpBuilder{};
+    h.model = model;
+    size_t tot = 0;
+    for (const std::vector<uint8_t>& c : S[k].colours) tot += c.size();
+    ops.clear();
+    ops.reserve(tot);
+    const int nseg = (int)S[k].seeds.size();
+    for (int s = 0; s < nseg; s++) h.append(S[k].colours[nseg - 1 - s], ops);
+  };
+  // The next round's seeds, colours and op streams are made on a host
+  // thread while the device runs the current round (every sample still
+  // running draws its next seed then; one that coalesces in this round
+  // never uses it, and its master stream serves nobody else).
+  std::vector<std::vector<uint8_t>> next_ops(ns);
+  std::vector<OpBuilder> next_hist(ns);
+  std::vector<char> has_next(ns, 0);
+  struct Joiner {
+    std::thread t;
+    ~Joiner() {
+      if (t.joinable()) t.join();
+    }
+  };
 
   rt_timing acc{};
   double ms_total = 0;
@@ -243,13 +253,13 @@ static int cftp_batch(rt_ctx* ctx, int model, const int32_t* tmax, const int32_t
       d.rows = N;
       d.cols = (int)g.cols;
       job.units[u] = d;
-      job.hist[u].model = model;
-      job.ops[u].reserve(tot);
-      // segments newest first; colours are stored oldest first
-      const int nseg = (int)S[k].seeds.size();
-      for (int s = 0; s < nseg; s++) {
-        job.seg_seeds[u].push_back((uint32_t)S[k].seeds[s]);
-        job.hist[u].append(S[k].colours[nseg - 1 - s], job.ops[u]);
+      for (long s : S[k].seeds) job.seg_seeds[u].push_back((uint32_t)s);
+      if (has_next[k]) {
+        job.ops[u] = std::move(next_ops[k]);
+        job.hist[u] = next_hist[k];
+        has_next[k] = 0;
+      } else {
+        build_ops(k, job.ops[u], job.hist[u]);
       }
     });
     rc = ctx->reserve_aux(fused_scratch_bytes(ctx, job));
@@ -265,6 +275,19 @@ static int cftp_batch(rt_ctx* ctx, int model, const int32_t* tmax, const int32_t
     if (!rc) rc = fused_tune(ctx, job, &plan);
     if (rc) return rc;
     HIPCHK_RET(ctx, hipEventRecord(ctx->evk0, ctx->stream));
+    const long ns_len = sched == RT_SCHED_REFERENCE ? steps.back() * 2 : tot;
+    Joiner prep;
+    if (r < max_rounds)
+      prep.t = std::thread([&, ns_len] {
+        for (int k : act) S[k].seeds.insert(S[k].seeds.begin(), (long)S[k].master());
+        // a few threads: the launch loop shares the host
+        parallel_for((int)act.size(), [&](int i) {
+          const int k = act[i];
+          gen(k, S[k].seeds[0], ns_len);
+          build_ops(k, next_ops[k], next_hist[k]);
+          has_next[k] = 1;
+        }, 6);
+      });
     rc = fused_launch(ctx, job, plan, true);
     if (rc) return rc;
     HIPCHK_RET(ctx, hipEventRecord(ctx->evk1, ctx->stream));
@@ -290,6 +313,7 @@ static int cftp_batch(rt_ctx* ctx, int model, const int32_t* tmax, const int32_t
                                    ctx->stream));
     HIPCHK_RET(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     HIPCHK_RET(ctx, hipEventSynchronize(ctx->ev1));
+    if (prep.t.joinable()) prep.t.join();
     float ms = 0;
     HIPCHK_RET(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     ms_total += ms;
@@ -328,6 +352,8 @@ static int cftp_batch(rt_ctx* ctx, int model, const int32_t* tmax, const int32_t
       merge_planes(model, out + (size_t)k * N * N, N, h.data());
       S[k].done = true;
       remaining--;
+      std::vector<uint8_t>().swap(next_ops[k]);  // its speculative next round
+      has_next[k] = 0;
       if (stats) {
         stats[k].rounds = r;
         stats[k].total_steps = tot;
@@ -337,17 +363,7 @@ static int cftp_batch(rt_ctx* ctx, int model, const int32_t* tmax, const int32_t
     }
     if (ctx->round_fn) ctx->round_fn(ctx->round_user, r, tot, ns - remaining, ns);
     if (remaining == 0 || r == max_rounds) break;
-    const long ns_len = sched == RT_SCHED_REFERENCE ? steps.back() * 2 : tot;
     steps.insert(steps.begin(), ns_len);
-    for (int k = 0; k < ns; k++)
-      if (!S[k].done) S[k].seeds.insert(S[k].seeds.begin(), (long)S[k].master());
-    std::vector<int> still;
-    for (int k = 0; k < ns; k++)
-      if (!S[k].done) still.push_back(k);
-    parallel_for((int)still.size(), [&](int i) {
-      const int k = still[i];
-      gen(k, S[k].seeds[0], ns_len);
-    });
   }
   acc.device_ms = ms_total;
   ctx->last = acc;

@@ -342,26 +342,67 @@ __global__ __launch_bounds__(1024, 8) void k_walk_o8(const UnitDesc* __restrict_
 namespace rt {
 
 // ------------------------------------------------------------- host side
-long OpBuilder::append(const std::vector<uint8_t>& colours, std::vector<uint8_t>& ops) {
-  long pulls = 0;
-  for (uint8_t c : colours) {
-    uint8_t op = c;
-    if (model == M_DOMINO) {
-      if (prev >= 0 && prev != c) op |= OP_PULL_A;
-    } else if (model == M_LOZENGE) {
-      const int ca = (c + 2) % 3, cb = (c + 1) % 3;
-      if (last[ca] > last[c]) op |= OP_PULL_A;
-      if (last[cb] > last[c]) op |= OP_PULL_B;
-      last[c] = g;
-    } else {
-      if (prev >= 0 && prev != c) op |= OP_PULL_A | (uint8_t)(prev << 4);
-    }
-    if (op & (OP_PULL_A | OP_PULL_B)) pulls++;
-    prev = c;
-    g++;
-    ops.push_back(op);
+// Lozenge pull state: two bits per colour y, bit 2y set when colour y-1
+// (mod 3) rotated since y's last rotation (y refreshes bits {2, 4, 32} from
+// it: OP_PULL_A), bit 2y+1 when colour y+1 did (OP_PULL_B); the table maps
+// (state, colour) at index 4 * state + colour to the op byte and the next
+// state.  Equal to comparing the colours' last rotation steps, which is
+// what `last` keeps between calls.
+namespace {
+struct LozengeOps {
+  uint8_t op[256], next[256];
+  LozengeOps() {
+    for (int st = 0; st < 64; st++)
+      for (int c = 0; c < 3; c++) {
+        const int k = 4 * st + c, b = st >> (2 * c);
+        op[k] = (uint8_t)(c | ((b & 1) ? OP_PULL_A : 0) | ((b & 2) ? OP_PULL_B : 0));
+        int ns = st & ~(3 << (2 * c));
+        ns |= 1 << (2 * ((c + 1) % 3));
+        ns |= 2 << (2 * ((c + 2) % 3));
+        next[k] = (uint8_t)ns;
+      }
   }
-  return pulls;
+};
+}  // namespace
+
+void OpBuilder::append(const std::vector<uint8_t>& colours, std::vector<uint8_t>& ops) {
+  const size_t n = colours.size(), base = ops.size();
+  if (n == 0) return;
+  ops.resize(base + n);
+  uint8_t* o = ops.data() + base;
+  const uint8_t* c = colours.data();
+  if (model == M_LOZENGE) {
+    static const LozengeOps T;
+    int st = 0;
+    for (int y = 0; y < 3; y++) {
+      if (last[(y + 2) % 3] > last[y]) st |= 1 << (2 * y);
+      if (last[(y + 1) % 3] > last[y]) st |= 2 << (2 * y);
+    }
+    for (size_t i = 0; i < n; i++) {
+      const int k = 4 * st + c[i];
+      o[i] = T.op[k];
+      st = T.next[k];
+    }
+    int found = 0;  // each colour's last rotation in this segment
+    for (size_t i = n; i-- > 0 && found != 7;)
+      if (!(found & (1 << c[i]))) {
+        found |= 1 << c[i];
+        last[c[i]] = g + (long)i;
+      }
+  } else {
+    // domino: pull whenever the colour changes; rectangle-triangle also
+    // records the previous colour in the op's high bits
+    int p = prev;
+    for (size_t i = 0; i < n; i++) {
+      const int ci = c[i];
+      uint8_t op = (uint8_t)ci;
+      if (p >= 0 && p != ci) op |= model == M_DOMINO ? OP_PULL_A : (uint8_t)(OP_PULL_A | (p << 4));
+      o[i] = op;
+      p = ci;
+    }
+  }
+  prev = c[n - 1];
+  g += (long)n;
 }
 
 // Tile shapes: waves W, rows per lane R (tile height W*R), halo H.  Byte
@@ -555,7 +596,7 @@ static void cut_windows(FusedJob& job, int H) {
   job.windows.assign(job.units.size(), {});
   job.nlaunch = 0;
   if (job.units.empty() || T == 0) return;
-  for (size_t u = 0; u < job.units.size(); u++) {
+  parallel_for((int)job.units.size(), [&](int u) {
     const std::vector<uint8_t>& o = job.ops[u];
     std::vector<FusedWindow>& win = job.windows[u];
     long s0 = 0;
@@ -575,8 +616,9 @@ static void cut_windows(FusedJob& job, int H) {
         s0 += K;
       }
     }
+  });
+  for (const std::vector<FusedWindow>& win : job.windows)
     job.nlaunch = std::max(job.nlaunch, win.size());
-  }
 }
 
 static size_t tiles_bytes(const rt_ctx* ctx, const FusedJob& job, int shape, bool split) {

@@ -16,6 +16,8 @@
 // steps leaves the tile interior exact.  Halo cells also advance their
 // (owners') TinyMT streams, redundantly, so every draw is the reference's.
 #pragma once
+#include <algorithm>
+#include <thread>
 #include <vector>
 
 #include "rt_kernels.hpp"
@@ -115,8 +117,8 @@ struct OpBuilder {
   long last[3] = {-1, -1, -1};
   int prev = -1;
   long g = 0;
-  // Append ops for `colours`; returns the number of pull steps.
-  long append(const std::vector<uint8_t>& colours, std::vector<uint8_t>& ops);
+  // Append the op bytes of `colours` (colour | pull flags) to ops.
+  void append(const std::vector<uint8_t>& colours, std::vector<uint8_t>& ops);
 };
 
 // One launch's share of a job: K steps from global step s0; init re-seeds
@@ -127,6 +129,24 @@ struct FusedWindow {
   int init;
   int seg;
 };
+
+// f(0) .. f(n-1) on up to one host thread per hardware thread (at most
+// max_threads when that is positive).
+template <class F>
+void parallel_for(int n, F f, int max_threads = 0) {
+  int nt = std::max(1, std::min<int>(n, (int)std::thread::hardware_concurrency()));
+  if (max_threads > 0) nt = std::min(nt, max_threads);
+  if (nt <= 1) {
+    for (int i = 0; i < n; i++) f(i);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (int t = 0; t < nt; t++)
+    th.emplace_back([&, t] {
+      for (int i = t; i < n; i += nt) f(i);
+    });
+  for (auto& x : th) x.join();
+}
 
 // Host job: units on one device advancing through a shared segment list.
 struct FusedJob {
