@@ -1092,6 +1092,8 @@ int fused_tune(rt_ctx* ctx, FusedJob& job, FusedPlan* plan) {
   const size_t nc = plan->cand.size();
   const size_t need = 1 + (size_t)kTrialReps * nc * kTrialWindows;
   if (!job.tune || nc < 2 || plan->nlaunch < need) return RT_OK;
+  for (const FusedPlan::Tiles& t : plan->cand)
+    if (t.nwork == 0) return RT_OK;
   const int H = job.halo;
   // Launch 0 (TinyMT seeding) under the rule-of-thumb shape, then
   // kTrialReps rounds of kTrialWindows launches under each candidate;
@@ -1213,8 +1215,9 @@ int fused_launch(rt_ctx* ctx, const FusedJob& job, const FusedPlan& plan, bool s
   for (size_t l = plan.first; l < plan.nlaunch; l++) {
     const WindowRec* w = t.d_wins + l * t.nunits;
     for (int k = 0; k < parts; k++)
-      fn(ctx, k ? ctx->sx[k] : ctx->stream, cut[k + 1] - cut[k], t.d_units, t.d_work + cut[k], H,
-         job.mp, w);
+      if (cut[k + 1] > cut[k])  // no work items: every tile of the job stays zero
+        fn(ctx, k ? ctx->sx[k] : ctx->stream, cut[k + 1] - cut[k], t.d_units, t.d_work + cut[k],
+           H, job.mp, w);
   }
   for (int k = 1; k < parts; k++) {
     HIPCHK_RET(ctx, hipEventRecord(ctx->evj[k], ctx->sx[k]));

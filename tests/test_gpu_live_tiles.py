@@ -111,3 +111,29 @@ def test_single_walks_and_cftp_on_sparse_domains_match_oracle(gpu_ctx):
     got, st = gpu_ctx.domino_cftp(tmax, tmin, 6, rt.SCHED_DOUBLING, 128, 12)
     want, wst = o.domino_cftp(tmax, tmin, 6, PA, rt.SCHED_DOUBLING, 128, 12)
     assert np.array_equal(got, want) and st["rounds"] == wst["rounds"]
+
+
+def test_all_zero_tilings_stay_zero(gpu_ctx):
+    """A tiling with no nonzero cell leaves every tile out, so a job of such
+    units launches no sweep kernel: walks, batches of only such units and
+    with one among others, against the oracle (which runs every cell)."""
+    N = 40
+    z = np.zeros(N * N, dtype=np.int32)
+    P = rt.synthetic_params(N * N // 2, 9)
+    gpu_ctx.set_tinymt_params(P)
+    _say("all-zero domino tilings")
+    assert np.array_equal(o.domino_walk(z, 300, 4, P, rt.DOMINO_HEATBATH), z)
+    assert np.array_equal(gpu_ctx.domino_walk(z, 300, 4, rt.DOMINO_HEATBATH), z)
+    assert np.array_equal(gpu_ctx.domino_walk(z, 300, 4, rt.DOMINO_TOGGLE08), z)
+    for a, b in zip(gpu_ctx.domino_walk_batch([z, z], 300, [4, 5], rt.DOMINO_HEATBATH), [z, z]):
+        assert np.array_equal(a, b)
+    t = rt.DominoTiler.MinTiling(rt.DominoTiler.AztecDiamond(17))
+    assert rt.side(t) == N
+    outs = gpu_ctx.domino_walk_batch([z, t], 300, [4, 5], rt.DOMINO_HEATBATH)
+    assert np.array_equal(outs[0], z)
+    assert np.array_equal(outs[1], o.domino_walk(t, 300, 5, P, rt.DOMINO_HEATBATH))
+    NL = 27
+    zl = np.zeros(NL * NL, dtype=np.int32)
+    gpu_ctx.set_tinymt_params(rt.synthetic_params(NL * NL, 10))
+    _say("all-zero lozenge tiling")
+    assert np.array_equal(gpu_ctx.lozenge_walk(zl, 300, 4), zl)
