@@ -67,7 +67,8 @@ typedef struct {
   long attempts;      /* flip attempts (TinyMT draws) */
   double kernel_ms;   /* time inside the sweep kernels alone (batched calls) */
   long kernel_launches;
-  long kernel_sites;  /* sum over those launches of the TinyMT sites they own */
+  long kernel_sites;  /* sum over those launches of the TinyMT sites they compute
+                       * (the tiles left out below excluded) */
   /* The same three split by sweep-block launch shape: index 0, 1, 2 =
    * 64 rows / 8 waves, 64 rows / 16 waves, 32 rows / 16 waves, 3 = 32 rows /
    * 8 waves for rect-triangle pairs, 4, 5, 6 = shapes 0, 1, 2 with CFTP
@@ -243,6 +244,15 @@ int rt_lozenge_cftp(rt_ctx* ctx, const int32_t* tmax, const int32_t* tmin,
                     int max_rounds, int32_t* out, rt_cftp_stats* stats);
 
 /* ------------------------------------------------------------- batching */
+/* Domino and lozenge walks and CFTP on the sweep-block kernels (the batch
+ * entry points below, rt_domino_walk, rt_lozenge_walk and the one-sample
+ * CFTP, the server) leave out the tiles whose cells all lie more than one
+ * row and one column away from every nonzero cell of the start tilings:
+ * in a tiling of a domain a cell's bits are lattice edges between two of
+ * the domain's faces, so such cells stay zero in every tiling the chain
+ * reaches, and the results equal the reference's, which visits them.  The
+ * start tilings must be tilings of their domain, which the reference's
+ * kernels assume as well. */
 /* Batched CFTP: one exact sample per master seed, all samples of the batch
  * advanced by the same fused launches (one launch covers every tile of
  * every unfinished sample).  Sample k is bit-identical to
