@@ -217,3 +217,38 @@ def test_walk_requests_in_the_mix(gpu_ctx):
     for k in (3, 4):
         w = walks[k]
         assert np.array_equal(want_w[k], o.recttri_walk(w["start"], w["steps"], w["seed"], P)), k
+
+
+def test_sparse_domains_served_equal_alone(gpu_ctx):
+    """Requests of which the server leaves tiles out (the cells that stay
+    zero in every tiling): an L-shaped domino domain (most of its bounding
+    square empty), an Aztec diamond and a hexagon with unequal sides, each
+    equal to the same request run alone, and the L to the oracle."""
+    D = rt.DominoTiler
+    L = D.RectangleUnion([(0, 0, 110, 10), (100, 0, 10, 110)])
+    reqs = [dict(model=rt.MODEL_DOMINO, tmax=D.MaxTiling(L), tmin=D.MinTiling(L), master_seed=11,
+                 sched=rt.SCHED_DOUBLING, t0=128, rule=rt.DOMINO_HEATBATH)]
+    tmax, tmin = _domino(40)
+    reqs.append(dict(model=rt.MODEL_DOMINO, tmax=tmax, tmin=tmin, master_seed=12,
+                     sched=rt.SCHED_DOUBLING, t0=128, rule=rt.DOMINO_HEATBATH))
+    tmax, tmin = _lozenge(36, 12)
+    reqs.append(dict(model=rt.MODEL_LOZENGE, tmax=tmax, tmin=tmin, master_seed=13,
+                     sched=rt.SCHED_DOUBLING, t0=64))
+    big = max(rt.side(q["tmax"]) for q in reqs)
+    P = rt.synthetic_params(big * big, 32)
+    gpu_ctx.set_tinymt_params(P)
+    alone = [_alone(gpu_ctx, q) for q in reqs]
+    _say("server: sparse domains")
+    srv = rt.Server(gpu_ctx, 256 << 20, 64)
+    try:
+        ids = [srv.submit(max_rounds=32, **q) for q in reqs]
+        for rid, (want, wst) in zip(ids, alone):
+            assert srv.wait(rid, 120.0)
+            rc, got, st, lat = srv.poll(rid)
+            assert rc == 0 and np.array_equal(got, want) and st["rounds"] == wst["rounds"]
+    finally:
+        srv.close()
+    q = reqs[0]
+    w, _ = o.domino_cftp(q["tmax"], q["tmin"], q["master_seed"], P, q["sched"], q["t0"], 32,
+                         q["rule"])
+    assert np.array_equal(alone[0][0], w)
