@@ -150,8 +150,11 @@ constexpr int SH_SERVE_COUNT = 1;  // tile shape whose tile count picks the laun
 constexpr int kOpsPerUnit = 64;  // = the fused kernel's window limit
 // Windows per tick: each tick launches kSub sweep kernels back to back from
 // one upload of window tables, so the host work per tick (tables, event,
-// flag reads) is spread over kSub windows of every live request.
-constexpr int kSub = 4;
+// flag reads) is spread over kSub windows of every live request.  Burst
+// capacity of the default served stream (domino 64-256 and lozenge 64-160
+// CFTP, rectangle-triangle walks), requests/s at kSub 4 / 8 / 16 / 32:
+// 4.06 / 4.54 / 4.83 / 5.00, p99 latency 11.5 / 10.0 / 9.3 / 9.0 s.
+constexpr int kSub = 32;
 
 // A tick's tables travel to the device in one copy from one pinned block
 // per tick buffer: the window records of the live units, [sub][group][unit]
