@@ -4,7 +4,10 @@
  * Plain pointers and sizes only.  Every call returns an int status
  * (RT_OK = 0, negative on error) and never throws; rt_last_error() gives
  * the message of the last failing call on a context.  Calls on one context
- * are synchronous and must come from one host thread at a time.
+ * are synchronous and must come from one host thread at a time.  The
+ * batched calls use helper host threads of their own while they run (op
+ * schedules are built in parallel, and the next CFTP round's while the
+ * device runs the current one); none outlives the call.
  *
  * Tilings are int32 N*N row-major arrays, exactly the reference's `tiling`
  * (std::vector<int>, src/common/common.h:50-52).  Walk calls mutate the
