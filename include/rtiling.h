@@ -112,8 +112,9 @@ int rt_set_engine(rt_ctx* ctx, int engine);
  * domino walks with a heat-bath rule, in the low halves); 0 = chosen per
  * job (autotuner, else a rule of thumb on the tile count; jobs that the
  * packed kernels take, at their halo, go to shape 7 directly).  The
- * serving loop picks 64-row tiles while its live tiles fit the CUs, shape
- * 7 beyond (packed-kernel groups: from a third of the CUs up). */
+ * serving loop puts domino and lozenge units on the packed kernels at
+ * every size, and other groups on 64-row tiles while their live tiles fit
+ * the CUs, shape 7 beyond. */
 int rt_set_tile_shape(rt_ctx* ctx, int shape);
 
 /* Tile-shape autotuner (no reference counterpart; on by default).  With
@@ -138,12 +139,13 @@ int rt_set_xcd_order(rt_ctx* ctx, int on);
  * 6, 1..12): a launch runs a window of at most H pulling colour steps of
  * each unit, tiles overlap by 2H rows and by 2H/3 (lozenge) or H (domino)
  * columns.  Larger H means fewer launches and more redundant halo work.
- * Results never depend on it.  Until it is called, lozenge and domino
- * chain-pair jobs (and one-chain domino heat-bath walks) whose 64-row tiles
- * would fill over a third of the CUs take H = 9 and 8 (the packed
- * kernels); halo 0 returns to that default.  Setting it forgets the autotuner's earlier choices; the
- * serving loop keeps 6 (9 and 8 for lozenge and domino chain pairs on
- * shape 7) and rectangle-triangle pairs 4. */
+ * Results never depend on it.  Until it is called, lozenge chain-pair
+ * jobs whose 64-row tiles would fill over a third of the CUs take H = 9,
+ * and domino units on the packed kernel (chain pairs, heat-bath walks)
+ * H = 8 at every size; halo 0 returns to that default.  Setting it
+ * forgets the autotuner's earlier choices; the serving loop keeps 6 (9
+ * and 8 for lozenge and domino units on the packed kernels) and
+ * rectangle-triangle pairs 4. */
 int rt_set_halo(rt_ctx* ctx, int halo);
 
 /* Streams of a batched CFTP or (eager) batched walk job (no reference
