@@ -81,3 +81,18 @@ def test_domino_nonzero_cells_stay_near_the_start_support(dom, rule):
         # domino cells (lattice vertices) off both extremes' supports do turn
         # nonzero: the one-cell margin is needed
         assert margin
+
+
+@pytest.mark.parametrize("n", [6, 10, 16])
+def test_recttri_nonzero_cells_stay_near_the_start_support(n):
+    """The same fact for rectangle-triangle tilings of minHex/maxHex
+    (RectTriangleTiler.cpp:33-171 Walk, recttrianglekernel.cl:25-118 moves):
+    checked here ahead of the engine using it for that model (its sweep
+    blocks still run every tile)."""
+    R = rt.RectTriangleTiler
+    tmax, tmin = R.maxHex(n), R.minHex(n)
+    N = rt.side(tmin)
+    P = rt.synthetic_params(N * N, 3)
+    _check(lambda t, s, seed: o.recttri_walk(t, s, seed, P), tmax, tmin, 41, 15)
+    # the walks move (their cells change value, not necessarily support)
+    assert not np.array_equal(o.recttri_walk(tmin, 41 * 15, 7, P), tmin)
